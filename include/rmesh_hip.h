@@ -1,0 +1,170 @@
+/*
+ * rmesh_hip — C ABI of the MI355X-native differentiable radiance-mesh
+ * train step (projection + tile binning, depth radix sort, tiled compositing,
+ * reverse-order adjoint, loss and Adam).
+ *
+ * This is the drop-in boundary for rmeshvk's `RMeshRenderer` pyclass
+ * (crates/rmesh-python/src/lib.rs:138-2705): each entry point below names the
+ * method it replaces.  Plain pointers and sizes only; "host" pointers are
+ * caller-owned CPU memory, "_dev" pointers are device memory on the renderer's
+ * GPU.  All calls are synchronous at the boundary unless noted, one renderer
+ * handle is not thread-safe (same as the PyO3 `&mut self` borrow).
+ *
+ * Errors: every call returns rmesh_status; rmesh_last_error() gives the
+ * message of the last failure on the calling thread.  The Python shim maps
+ * RMESH_ERR_INVALID -> ValueError, RMESH_ERR_DEVICE -> RuntimeError,
+ * RMESH_ERR_UNSUPPORTED -> NotImplementedError (lib.rs:311-329,894-904,2381-2389).
+ */
+#ifndef RMESH_HIP_H
+#define RMESH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rmesh_renderer rmesh_renderer;
+typedef int32_t rmesh_status;
+
+#define RMESH_OK 0
+#define RMESH_ERR_INVALID 1
+#define RMESH_ERR_DEVICE 2
+#define RMESH_ERR_UNSUPPORTED 3
+#define RMESH_ERR_CAPACITY 4
+
+typedef struct rmesh_scene_desc {
+    const float* vertices;     /* [n_vertices*3] */
+    uint64_t n_vertices;
+    const uint32_t* indices;   /* [n_tets*4] */
+    uint64_t n_tets;
+    const float* base_colors;  /* [n_tets*3] colours used as given (reference training path) */
+    const float* densities;    /* [n_tets] */
+    const float* color_grads;  /* [n_tets*3] */
+    const float* circumdata;   /* [n_tets*4] (cx, cy, cz, r^2) */
+    uint32_t width, height;
+    uint32_t aux_dim;          /* custom aux channels: only 0 is supported */
+    uint32_t tile_size;        /* 0 -> 12 (lib.rs:442); 8, 12 and 16 are supported */
+    int32_t device_id;         /* -1 -> current device */
+    const float* sh_coeffs;    /* nullable [n_tets*3*(deg+1)^2], channel-major per tet */
+    uint32_t sh_degree;        /* 0..3, used when sh_coeffs != NULL */
+    uint32_t max_views;        /* views per batched call (0 -> 1) */
+} rmesh_scene_desc;
+
+/* RMeshRenderer.__new__ (lib.rs:265-870).  Uploads the scene and allocates
+ * per-handle device state. */
+rmesh_status rmesh_create(const rmesh_scene_desc* desc, rmesh_renderer** out);
+/* Drop of RMeshRenderer. */
+void rmesh_destroy(rmesh_renderer* h);
+/* Message of the last failing call on this thread ("" if none). */
+const char* rmesh_last_error(void);
+
+/* RMeshRenderer.set_min_t (lib.rs:873-875). */
+rmesh_status rmesh_set_min_t(rmesh_renderer* h, float min_t);
+/* Run on an external HIP stream (e.g. torch's current stream); NULL restores
+ * the handle's own stream. */
+rmesh_status rmesh_set_stream(rmesh_renderer* h, void* hip_stream);
+
+/* RMeshRenderer.forward_tiled (lib.rs:1040-1204).  cam[3], vp[16]
+ * (column-major), c2w_intrinsics[16]; out_rgba [H*W*4] host, out_aux
+ * [H*W*8] host or NULL (render_aux=True). */
+rmesh_status rmesh_forward_tiled(rmesh_renderer* h, const float* cam, const float* vp, const float* c2w_intrinsics,
+                                 float* out_rgba, float* out_aux);
+/* RMeshRenderer.forward (lib.rs:932-1025): the hardware-raster path is served
+ * by the tiled forward (same outputs, FP32 instead of an Rgba16Float target). */
+rmesh_status rmesh_forward(rmesh_renderer* h, const float* cam, const float* vp, const float* c2w_intrinsics,
+                           float* out_rgba);
+/* RMeshRenderer.read_instance_count (lib.rs:1879-1884): tets that passed
+ * culling in the last forward (first view). */
+rmesh_status rmesh_read_instance_count(rmesh_renderer* h, uint32_t* out);
+
+/* RMeshRenderer.backward (lib.rs:1894-2031).  Uses the sorted tile lists and
+ * image of the last forward_tiled.  dl_d_image [H*W*4] host; outputs host:
+ * d_vertices [N*3], d_densities [M], d_base_colors [M*3], d_color_grads [M*3]. */
+rmesh_status rmesh_backward(rmesh_renderer* h, const float* dl_d_image, float* d_vertices, float* d_densities,
+                            float* d_base_colors, float* d_color_grads);
+/* SH extension: d_sh [M*3*nc] of the last rmesh_backward (host). */
+rmesh_status rmesh_get_sh_grad(rmesh_renderer* h, float* d_sh);
+
+/* RMeshRenderer.train_step (lib.rs:2405-2636): forward + loss + backward +
+ * Adam (vertices, densities, color_grads; step += 1).  gt [H*W*3] host,
+ * loss_type 0 = L1, 1 = L2.  *out_loss receives the loss (mean over pixels of
+ * the per-pixel sum over RGB, matching the gradient's 1/(W*H) scaling); the
+ * reference never writes its loss buffer and returns 0.0. */
+rmesh_status rmesh_train_step(rmesh_renderer* h, const float* cam, const float* vp, const float* c2w_intrinsics,
+                              const float* gt, float lr_verts, float lr_dens, float lr_grads, uint32_t loss_type,
+                              float* out_loss);
+/* Extension learning rates for train steps: SH coefficients and base_colors
+ * (0 disables; the reference never optimises base_colors). */
+rmesh_status rmesh_set_extra_lrs(rmesh_renderer* h, float lr_sh, float lr_base);
+
+/* RMeshRenderer.update_params / get_params (lib.rs:2641-2686), host arrays. */
+rmesh_status rmesh_update_params(rmesh_renderer* h, const float* vertices, const float* base_colors,
+                                 const float* densities, const float* color_grads);
+rmesh_status rmesh_get_params(rmesh_renderer* h, float* vertices, float* base_colors, float* densities,
+                              float* color_grads);
+rmesh_status rmesh_update_sh(rmesh_renderer* h, const float* sh);
+rmesh_status rmesh_get_sh(rmesh_renderer* h, float* sh);
+
+/* RMeshRenderer.step / width / height getters (lib.rs:2689-2704). */
+uint32_t rmesh_get_step(const rmesh_renderer* h);
+uint32_t rmesh_get_width(const rmesh_renderer* h);
+uint32_t rmesh_get_height(const rmesh_renderer* h);
+
+/* ---- batched multi-view path (no counterpart in the reference: it runs one
+ * view per submit) ---- */
+
+/* Forward of n_views cameras (cams [n*3], vps [n*16], c2ws [n*16] host);
+ * images land in the renderer's device image buffer [n][H][W][4]
+ * (see rmesh_device_buffer); out_rgba (host, nullable) receives a copy. */
+rmesh_status rmesh_forward_batched(rmesh_renderer* h, uint32_t n_views, const float* cams, const float* vps,
+                                   const float* c2ws, float* out_rgba);
+/* One training step over n_views: forward + loss + backward of every view,
+ * gradients summed, then one Adam update of every enabled group (step += 1).
+ * gt_dev: device [n][H][W][3]; NULL means the renderer's own target buffer
+ * (RMESH_BUF_GT).  lr_sh applies when the renderer carries SH.  out_loss
+ * (host, nullable) receives the mean per-view loss.  When graph mode is on,
+ * the launch sequence is captured once into a hipGraph and replayed. */
+rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t n_views, const float* cams, const float* vps,
+                                      const float* c2ws, const float* gt_dev, uint32_t loss_type, float lr_verts,
+                                      float lr_dens, float lr_grads, float lr_sh, float* out_loss);
+rmesh_status rmesh_set_graph_mode(rmesh_renderer* h, int enable);
+
+#define RMESH_BUF_RGBA 0        /* float [max_views][H][W][4] */
+#define RMESH_BUF_GT 1          /* float [max_views][H][W][3] */
+#define RMESH_BUF_VERTICES 2    /* float [N*3] */
+#define RMESH_BUF_DENSITIES 3   /* float [M] */
+#define RMESH_BUF_COLOR_GRADS 4 /* float [M*3] */
+#define RMESH_BUF_BASE_COLORS 5 /* float [M*3] */
+#define RMESH_BUF_SH 6          /* float [M*3*nc] */
+#define RMESH_BUF_D_VERTICES 7
+#define RMESH_BUF_D_DENSITIES 8
+#define RMESH_BUF_D_COLOR_GRADS 9
+/* Device pointer of an internal buffer (zero-copy for torch callers). */
+void* rmesh_device_buffer(rmesh_renderer* h, int which);
+/* Synchronise the renderer's stream. */
+rmesh_status rmesh_synchronize(rmesh_renderer* h);
+
+/* ---- inspection of the binning state, for parity tests (the reference's
+ * kernel tests read the same buffers back: tiles_touched, indirect_args,
+ * tile_sort_keys/values, tile_ranges — kernel_tests.rs:96-167,1618-1671) ---- */
+
+/* Per-tet tile counts of view `view` of the last forward [M]. */
+rmesh_status rmesh_debug_tiles_touched(rmesh_renderer* h, uint32_t view, uint32_t* out);
+/* Pre-sort pairs of one camera: runs projection + pair generation only. */
+rmesh_status rmesh_debug_pairs(rmesh_renderer* h, const float* cam, const float* vp, const float* c2w_intrinsics,
+                               uint64_t* keys, uint32_t* vals, uint64_t cap, uint64_t* n_pairs);
+/* Sorted pairs and per-tile ranges of the last forward (all views). */
+rmesh_status rmesh_debug_sorted(rmesh_renderer* h, uint64_t* keys, uint32_t* vals, uint64_t cap, uint64_t* n_pairs,
+                                uint32_t* ranges, uint64_t ranges_cap);
+/* Standalone sort of host (key, value) pairs on the GPU with the pipeline's
+ * radix sort: stable on the low `sorting_bits` bits (multiple of 8). */
+rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t sorting_bits, int32_t device_id);
+/* Bits sorted for a given tile count (crates/rmesh-sort/src/drs.rs:23-26). */
+uint32_t rmesh_sorting_bits(uint32_t num_tiles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RMESH_HIP_H */

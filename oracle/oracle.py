@@ -1,0 +1,267 @@
+"""ctypes wrapper of the CPU oracle (liborc.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module; the product (rmeshvk_amd, rmesh_wgpu) never does.
+
+What pins each oracle function (checks live in tests/test_oracle_*.py):
+
+================  ==========================================================
+orc_project       regular-tet known answer (kernel_tests.rs:1618-1671:
+                  visible count 1, tiles > 0); brute-force tile coverage
+                  (every pixel whose centre a tet covers lies in a binned
+                  tile); sum(tiles_touched) == P.
+orc_pairs         same scanline as orc_project (count == generated pairs
+                  except at 1-ulp NDC disagreements); keys decode to
+                  (tile, ~bits(|c-cam|^2 - r^2)) recomputed in numpy f64.
+orc_sort          numpy ``argsort(kind="stable")`` of the masked key; the
+                  reference's 64-key KAT (kernel_tests.rs:96-167).
+orc_ranges        closed form from sorted tile ids (numpy searchsorted);
+                  the reference's sentinel quirk on a planted case.
+orc_forward       closed form for the regular tet with g = 0
+                  (RGB = c * alpha, alpha = 1 - exp(-sigma * chord), chord
+                  from an independent numpy slab test); an independent
+                  per-pixel numpy renderer (tests/test_oracle_render.py).
+orc_loss          KAT of kernel_tests.rs:175-305 (+-1/4096 at 64x64).
+orc_backward      f64 central finite differences of the f64 forward
+                  (micro-scenes and config-A crops); the reference's
+                  tolerances rel 0.10 (kernel_tests.rs:1067-1075).
+orc_adam          KAT of kernel_tests.rs:313-420 (m = 0.1 g, v = 0.001 g^2,
+                  update ~ -lr sign(g)) plus a closed-form multi-step check.
+orc_sh_colors     degree 0 reduces to softplus(C0 sh0 + 0.5 + g.(v0-c));
+                  basis orthonormality checked by quadrature.
+orc_sh_backward   f64 finite differences (no reference adjoint exists).
+================  ==========================================================
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_f32 = ctypes.c_float
+_vp = ctypes.c_void_p
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liborc.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        for sfx, rp in (("_f32", _f32p), ("_f64", _f64p)):
+            getattr(L, "orc_project" + sfx).argtypes = [rp, _u32p, rp, _u32, _f32p, _f32p, _f32p,
+                                                         _u32, _u32, _u32, _u32p, _u8p]
+            f = getattr(L, "orc_pairs" + sfx)
+            f.argtypes = [rp, _u32p, rp, _u32, _f32p, _f32p, _f32p, _u32, _u32, _u32, _u32p, _u64p, _u32p, _u64]
+            f.restype = _u64
+            getattr(L, "orc_forward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p, _f32p,
+                                                         _u32, _u32, _u32, _f32, _u32, _u32, rp, _vp, _vp]
+            f = getattr(L, "orc_loss" + sfx)
+            f.argtypes = [rp, rp, _u32, _u32, _u32, rp]
+            f.restype = ctypes.c_double
+            getattr(L, "orc_backward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p, _f32p,
+                                                          _u32, _u32, _u32, rp, rp, _u32, _u32, rp, rp, rp, rp]
+            getattr(L, "orc_sh_colors" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp]
+            getattr(L, "orc_sh_backward" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp, rp, rp, rp]
+        L.orc_sorting_bits.argtypes = [_u32]
+        L.orc_sorting_bits.restype = _u32
+        L.orc_sort.argtypes = [_u64p, _u32p, _u64, _u32]
+        L.orc_ranges.argtypes = [_u64p, _u64, _u32p]
+        L.orc_adam.argtypes = [_f32p, _f32p, _f32p, _f32p, _u64, _u32, _f32, _f32, _f32, _f32]
+        L.orc_circumspheres.argtypes = [_f32p, _u32p, _u64, _f32p]
+        _LIB = L
+    return _LIB
+
+
+def _sfx(dtype):
+    return "_f64" if np.dtype(dtype) == np.float64 else "_f32"
+
+
+def _a(x, dtype):
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+def tile_grid(w, h, ts):
+    tx = (w + ts - 1) // ts
+    ty = (h + ts - 1) // ts
+    return tx, ty, tx * ty
+
+
+def sorting_bits(num_tiles: int) -> int:
+    return int(lib().orc_sorting_bits(num_tiles))
+
+
+def project(scene, cam, w, h, ts=12, dtype=np.float32):
+    """K1: (tiles_touched[M], visible[M]) in tet-id order."""
+    cp, vp, ci = cam
+    M = len(scene["indices"]) // 4
+    tt = np.zeros(M, dtype=np.uint32)
+    vis = np.zeros(M, dtype=np.uint8)
+    getattr(lib(), "orc_project" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(scene["densities"], dtype), M,
+        _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32), w, h, ts, tt, vis)
+    return tt, vis
+
+
+def pairs(scene, cam, w, h, tiles_touched, ts=12, dtype=np.float32, cap=None):
+    """K6+K7: unsorted (keys u64, values u32, P)."""
+    cp, vp, ci = cam
+    M = len(scene["indices"]) // 4
+    P = int(tiles_touched.sum(dtype=np.uint64))
+    cap = P if cap is None else cap
+    keys = np.zeros(max(cap, 1), dtype=np.uint64)
+    vals = np.zeros(max(cap, 1), dtype=np.uint32)
+    n = getattr(lib(), "orc_pairs" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(scene["circumdata"], dtype), M,
+        _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32), w, h, ts,
+        _a(tiles_touched, np.uint32), keys, vals, cap)
+    return keys[:min(n, cap)], vals[:min(n, cap)], int(n)
+
+
+def sort(keys, vals, bits):
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    v = np.array(vals, dtype=np.uint32, copy=True)
+    lib().orc_sort(k, v, len(k), bits)
+    return k, v
+
+
+def ranges(sorted_keys, num_tiles):
+    r = np.zeros(2 * num_tiles, dtype=np.uint32)
+    lib().orc_ranges(_a(sorted_keys, np.uint64), len(sorted_keys), r)
+    return r
+
+
+def forward(scene, cam, w, h, sorted_vals, rng_, ts=12, min_t=0.0, colors=None, dtype=np.float32,
+            tiles=None, want_aux=False, want_stats=False):
+    cp, vp, ci = cam
+    _, _, T = tile_grid(w, h, ts)
+    t0, t1 = (0, T) if tiles is None else tiles
+    rgba = np.zeros(h * w * 4, dtype=dtype)
+    aux = np.zeros(h * w * 8, dtype=dtype) if want_aux else None
+    stats = np.zeros(h * w * 4, dtype=np.uint32) if want_stats else None
+    col = scene["base_colors"] if colors is None else colors
+    getattr(lib(), "orc_forward" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
+        _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
+        _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
+        w, h, ts, min_t, t0, t1, rgba,
+        None if aux is None else aux.ctypes.data, None if stats is None else stats.ctypes.data)
+    out = {"rgba": rgba.reshape(h, w, 4)}
+    if want_aux:
+        out["aux"] = aux.reshape(h, w, 8)
+    if want_stats:
+        out["stats"] = stats.reshape(h, w, 4)
+    return out
+
+
+def loss_grad(rgba, gt, loss_type, dtype=np.float32):
+    h, w = rgba.shape[:2]
+    dl = np.zeros(h * w * 4, dtype=dtype)
+    loss = getattr(lib(), "orc_loss" + _sfx(dtype))(_a(rgba, dtype).ravel(), _a(gt, dtype).ravel(),
+                                                     w, h, loss_type, dl)
+    return dl.reshape(h, w, 4), float(loss)
+
+
+def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, dtype=np.float32, tiles=None):
+    cp, vp, ci = cam
+    _, _, T = tile_grid(w, h, ts)
+    t0, t1 = (0, T) if tiles is None else tiles
+    N = len(scene["vertices"]) // 3
+    M = len(scene["indices"]) // 4
+    dV = np.zeros(N * 3, dtype=dtype)
+    dD = np.zeros(M, dtype=dtype)
+    dG = np.zeros(M * 3, dtype=dtype)
+    dC = np.zeros(M * 3, dtype=dtype)
+    col = scene["base_colors"] if colors is None else colors
+    getattr(lib(), "orc_backward" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
+        _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
+        _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
+        w, h, ts, _a(rgba, dtype).ravel(), _a(dl, dtype).ravel(), t0, t1, dV, dD, dG, dC)
+    return {"d_vertices": dV, "d_densities": dD, "d_color_grads": dG, "d_base_colors": dC}
+
+
+def adam(params, grads, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    p = np.array(params, dtype=np.float32, copy=True)
+    mm = np.array(m, dtype=np.float32, copy=True)
+    vv = np.array(v, dtype=np.float32, copy=True)
+    lib().orc_adam(p, _a(grads, np.float32), mm, vv, len(p), step, lr, beta1, beta2, eps)
+    return p, mm, vv
+
+
+def sh_colors(scene, cam_pos, dtype=np.float32):
+    M = len(scene["indices"]) // 4
+    out = np.zeros(M * 3, dtype=dtype)
+    getattr(lib(), "orc_sh_colors" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(scene["sh"], dtype),
+        _a(scene["color_grads"], dtype), M, int(scene["sh_degree"]), _a(cam_pos, np.float32), out)
+    return out
+
+
+def sh_backward(scene, cam_pos, d_colors, dtype=np.float32):
+    M = len(scene["indices"]) // 4
+    N = len(scene["vertices"]) // 3
+    d_sh = np.zeros(len(scene["sh"]), dtype=dtype)
+    dG = np.zeros(M * 3, dtype=dtype)
+    dV = np.zeros(N * 3, dtype=dtype)
+    getattr(lib(), "orc_sh_backward" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(scene["sh"], dtype),
+        _a(scene["color_grads"], dtype), M, int(scene["sh_degree"]), _a(cam_pos, np.float32),
+        _a(d_colors, dtype), d_sh, dG, dV)
+    return {"d_sh": d_sh, "d_color_grads": dG, "d_vertices": dV}
+
+
+def circumspheres(vertices, indices):
+    M = len(indices) // 4
+    out = np.zeros(M * 4, dtype=np.float32)
+    lib().orc_circumspheres(_a(vertices, np.float32), _a(indices, np.uint32), M, out)
+    return out
+
+
+def render(scene, cam, w, h, ts=12, min_t=0.0, dtype=np.float32, colors=None, want_aux=False,
+           want_stats=False):
+    """Full forward pipeline K1 -> K7 -> sort -> K11 -> K12; returns every intermediate."""
+    if colors is None and "sh" in scene:
+        colors = sh_colors(scene, cam[0], dtype)
+    tt, vis = project(scene, cam, w, h, ts, dtype)
+    keys, vals, P = pairs(scene, cam, w, h, tt, ts, dtype)
+    _, _, T = tile_grid(w, h, ts)
+    sk, sv = sort(keys, vals, sorting_bits(T))
+    rg = ranges(sk, T)
+    out = forward(scene, cam, w, h, sv, rg, ts, min_t, colors, dtype, want_aux=want_aux, want_stats=want_stats)
+    out.update(tiles_touched=tt, visible=vis, keys=keys, vals=vals, P=P, sorted_keys=sk,
+               sorted_vals=sv, ranges=rg, colors=colors)
+    return out
+
+
+def render_backward(scene, cam, w, h, gt, loss_type=1, ts=12, dtype=np.float32):
+    """Forward + loss + backward (+ SH adjoint when the scene carries SH)."""
+    fw = render(scene, cam, w, h, ts, dtype=dtype)
+    colors = fw["colors"]
+    dl, loss = loss_grad(fw["rgba"], gt, loss_type, dtype)
+    g = backward(scene, cam, w, h, fw["sorted_vals"], fw["ranges"], fw["rgba"], dl, ts, colors, dtype)
+    if "sh" in scene:
+        sg = sh_backward(scene, cam[0], g["d_base_colors"], dtype)
+        g["d_sh"] = sg["d_sh"]
+        g["d_color_grads"] = g["d_color_grads"] + sg["d_color_grads"]
+        g["d_vertices"] = g["d_vertices"] + sg["d_vertices"]
+    return fw, dl, loss, g

@@ -1,0 +1,836 @@
+/*
+ * CPU restatement of rmeshvk's tiled train-step kernels — TEST INFRASTRUCTURE ONLY.
+ *
+ * This header is instantiated twice by rmesh_oracle.c: once with R = float
+ * (the reference's own FP32 arithmetic, used as the bit-exact checker for
+ * binning/sort/ranges and the tolerance checker for images and gradients) and
+ * once with R = double (used for finite-difference gradient checks).
+ *
+ * Every function names the reference kernel and lines it follows.  Float
+ * semantics chosen where WGSL leaves them to the driver (documented in
+ * DESIGN.md, "oracle conventions"):
+ *   dot(a,b)      = (a.x*b.x + a.y*b.y) + a.z*b.z, no FMA contraction
+ *   cross(a,b)    = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
+ *   normalize(v)  = v / sqrt(dot(v,v))   (IEEE division and square root)
+ *   mat*vec       = column-major, summed column 0 → 3
+ *   min/max       = IEEE minNum/maxNum (fmin/fmax)
+ *   f32 → u32/i32 = truncation, saturating at the type range, NaN → 0
+ */
+
+#ifndef R
+#error "define R (float/double) and SFX before including oracle_impl.h"
+#endif
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+
+/* ------------------------------------------------------------------ */
+/* small vector helpers                                                 */
+/* ------------------------------------------------------------------ */
+typedef struct { R x, y, z; } FN(v3_);
+#define V3 FN(v3_)
+
+static inline V3 FN(mk)(R x, R y, R z) { V3 r = {x, y, z}; return r; }
+static inline V3 FN(sub)(V3 a, V3 b) { return FN(mk)(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 FN(add)(V3 a, V3 b) { return FN(mk)(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 FN(scl)(V3 a, R s) { return FN(mk)(a.x * s, a.y * s, a.z * s); }
+static inline V3 FN(neg)(V3 a) { return FN(mk)(-a.x, -a.y, -a.z); }
+static inline R FN(dot)(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline V3 FN(cross)(V3 a, V3 b) {
+    return FN(mk)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline R FN(rmin)(R a, R b) { return RFMIN(a, b); }
+static inline R FN(rmax)(R a, R b) { return RFMAX(a, b); }
+static inline R FN(clampr)(R v, R lo, R hi) { return FN(rmin)(FN(rmax)(v, lo), hi); }
+static inline V3 FN(normalize)(V3 v) {
+    R l = RSQRT(FN(dot)(v, v));
+    return FN(mk)(v.x / l, v.y / l, v.z / l);
+}
+static inline V3 FN(load3)(const R* p, uint32_t i) { return FN(mk)(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+
+/* safe math — crates/rmesh-util/src/wgsl/math.wgsl:3-65 */
+#define TINY_VAL_R ((R)1.0754944e-20)
+#define MIN_VAL_R ((R)-1e+20)
+#define MAX_VAL_R ((R)1e+20)
+#define LOG_MAX_VAL_R ((R)46.0517)
+
+static inline R FN(safe_clip)(R v, R lo, R hi) { return FN(rmax)(FN(rmin)(v, hi), lo); }
+static inline V3 FN(safe_clip3)(V3 v, R lo, R hi) {
+    return FN(mk)(FN(safe_clip)(v.x, lo, hi), FN(safe_clip)(v.y, lo, hi), FN(safe_clip)(v.z, lo, hi));
+}
+static inline R FN(safe_div)(R a, R b) {
+    if (RABS(b) < TINY_VAL_R) return FN(safe_clip)(a / TINY_VAL_R, MIN_VAL_R, MAX_VAL_R);
+    return FN(safe_clip)(a / b, MIN_VAL_R, MAX_VAL_R);
+}
+static inline R FN(safe_exp)(R v) { return REXP(FN(clampr)(v, (R)-88.0, LOG_MAX_VAL_R)); }
+/* math.wgsl:35-42 */
+static inline R FN(phi)(R x) {
+    if (RABS(x) < (R)0.02)
+        return (R)1.0 + x * ((R)-0.5 + x * ((R)(1.0 / 6.0) + x * ((R)(-1.0 / 24.0))));
+    return FN(safe_div)((R)1.0 - REXP(-x), x);
+}
+/* math.wgsl:44-51 */
+static inline R FN(dphi_dx)(R x) {
+    if (RABS(x) < (R)0.02)
+        return (R)-0.5 + x * ((R)(1.0 / 3.0) + x * ((R)(-1.0 / 8.0) + x * ((R)(1.0 / 30.0))));
+    R ex = REXP(-x);
+    return FN(safe_div)(ex * (x + (R)1.0) - (R)1.0, x * x);
+}
+/* math.wgsl:20-33 */
+static inline R FN(softplus)(R x) {
+    if (x * (R)10.0 > (R)20.0) return x;
+    return (R)0.1 * RLOG((R)1.0 + REXP((R)10.0 * x));
+}
+static inline R FN(dsoftplus)(R x) {
+    if (x * (R)10.0 > (R)20.0) return (R)1.0;
+    R e = REXP((R)10.0 * x);
+    return FN(safe_div)(e, (R)1.0 + e);
+}
+
+/* clip = vp * vec4(p, 1): column-major, columns summed in order (project_compute.wgsl:77-81) */
+static inline void FN(mul_vp)(const R vp[16], V3 p, R out[4]) {
+    for (int k = 0; k < 4; ++k)
+        out[k] = ((vp[0 * 4 + k] * p.x + vp[1 * 4 + k] * p.y) + vp[2 * 4 + k] * p.z) + vp[3 * 4 + k];
+}
+
+/* ------------------------------------------------------------------ */
+/* camera / view description                                            */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    R vp[16];    /* column-major view-projection */
+    R c2w[9];    /* camera-to-world rotation, columns */
+    R intr[4];   /* fx fy cx cy */
+    R cam[3];
+    R min_t;
+    uint32_t width, height, tile_size;
+} FN(orc_cam_);
+#define CAM FN(orc_cam_)
+
+static void FN(cam_from_flat)(CAM* c, const float* cam_pos, const float* vp, const float* c2w_intr,
+                              uint32_t w, uint32_t h, uint32_t ts, float min_t) {
+    /* c2w_from_flat / mat4_from_flat — crates/rmesh-python/src/lib.rs:117-136 */
+    for (int i = 0; i < 16; ++i) c->vp[i] = (R)vp[i];
+    for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) c->c2w[j * 3 + k] = (R)c2w_intr[j * 4 + k];
+    for (int i = 0; i < 4; ++i) c->intr[i] = (R)c2w_intr[12 + i];
+    for (int i = 0; i < 3; ++i) c->cam[i] = (R)cam_pos[i];
+    c->min_t = (R)min_t;
+    c->width = w; c->height = h; c->tile_size = ts;
+}
+
+/* tet face table (a, b, c, opposite) — crates/rmesh-util/src/wgsl/intersect.wgsl:4-9 */
+static const int FN(FACES)[4][4] = {{0, 2, 1, 3}, {1, 2, 3, 0}, {0, 3, 2, 1}, {3, 0, 1, 2}};
+
+/* edges scanned by the tile scanline — project_compute.wgsl:225-226 */
+static const int FN(EI)[6] = {0, 0, 0, 1, 1, 2};
+static const int FN(EJ)[6] = {1, 2, 3, 2, 3, 3};
+
+/* ------------------------------------------------------------------ */
+/* K1: per-tet projection, culling and conservative tile count          */
+/* crates/rmesh-render/src/wgsl/project_compute.wgsl:88-262             */
+/* ------------------------------------------------------------------ */
+/* Returns the tile count for one tet; *visible receives the vis_idx flag
+ * (tets with visible=1 get a compact slot even when the count is 0). */
+static uint32_t FN(project_one)(const R* verts, const uint32_t* idx, const R* dens, uint32_t t,
+                                const CAM* c, int* visible) {
+    V3 v[4];
+    for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+    R p[4][4]; /* ndc.xyz, clip.w */
+    for (int i = 0; i < 4; ++i) {
+        R clip[4];
+        FN(mul_vp)(c->vp, v[i], clip);
+        R inv_w = (R)1.0 / (clip[3] + (R)1e-6);
+        p[i][0] = clip[0] * inv_w; p[i][1] = clip[1] * inv_w; p[i][2] = clip[2] * inv_w; p[i][3] = clip[3];
+    }
+    int any_behind = (p[0][3] <= 0) || (p[1][3] <= 0) || (p[2][3] <= 0) || (p[3][3] <= 0);
+    R W = (R)c->width, H = (R)c->height;
+    int vis = 1;
+    if (!any_behind) {
+        R min_x = FN(rmin)(FN(rmin)(p[0][0], p[1][0]), FN(rmin)(p[2][0], p[3][0]));
+        R max_x = FN(rmax)(FN(rmax)(p[0][0], p[1][0]), FN(rmax)(p[2][0], p[3][0]));
+        R min_y = FN(rmin)(FN(rmin)(p[0][1], p[1][1]), FN(rmin)(p[2][1], p[3][1]));
+        R max_y = FN(rmax)(FN(rmax)(p[0][1], p[1][1]), FN(rmax)(p[2][1], p[3][1]));
+        R min_z = FN(rmin)(FN(rmin)(p[0][2], p[1][2]), FN(rmin)(p[2][2], p[3][2]));
+        if (max_x < (R)-1.0 || min_x > (R)1.0 || max_y < (R)-1.0 || min_y > (R)1.0 || min_z > (R)1.0) vis = 0;
+        R ext_x = (max_x - min_x) * W;
+        R ext_y = (max_y - min_y) * H;
+        if (ext_x * ext_y < (R)1.0) vis = 0;
+    }
+    if (!vis) { *visible = 0; return 0; }
+    /* optical-depth cull, :149-162 */
+    {
+        V3 e01 = FN(sub)(v[1], v[0]), e02 = FN(sub)(v[2], v[0]), e03 = FN(sub)(v[3], v[0]);
+        V3 e12 = FN(sub)(v[2], v[1]), e13 = FN(sub)(v[3], v[1]), e23 = FN(sub)(v[3], v[2]);
+        R m = FN(rmax)(FN(rmax)(FN(dot)(e01, e01), FN(rmax)(FN(dot)(e02, e02), FN(dot)(e03, e03))),
+                       FN(rmax)(FN(dot)(e12, e12), FN(rmax)(FN(dot)(e13, e13), FN(dot)(e23, e23))));
+        R max_od = dens[t] * RSQRT(m);
+        if (max_od < (R)0.01) { *visible = 0; return 0; }
+    }
+    *visible = 1;
+    if (any_behind) return 0;
+    R proj[4][2];
+    for (int i = 0; i < 4; ++i) {
+        proj[i][0] = ((p[i][0] + (R)1.0) * (R)0.5) * W;
+        proj[i][1] = (((R)1.0 - p[i][1]) * (R)0.5) * H;
+    }
+    R pix_min_x = FN(rmax)(FN(rmin)(FN(rmin)(proj[0][0], proj[1][0]), FN(rmin)(proj[2][0], proj[3][0])), (R)0.0);
+    R pix_max_x = FN(rmin)(FN(rmax)(FN(rmax)(proj[0][0], proj[1][0]), FN(rmax)(proj[2][0], proj[3][0])), W - (R)1.0);
+    R pix_min_y = FN(rmax)(FN(rmin)(FN(rmin)(proj[0][1], proj[1][1]), FN(rmin)(proj[2][1], proj[3][1])), (R)0.0);
+    R pix_max_y = FN(rmin)(FN(rmax)(FN(rmax)(proj[0][1], proj[1][1]), FN(rmax)(proj[2][1], proj[3][1])), H - (R)1.0);
+    if (pix_min_x > pix_max_x || pix_min_y > pix_max_y) return 0;
+    R T = (R)c->tile_size;
+    uint32_t tiles_x_total = sat_u32((double)RCEIL(W / T));
+    uint32_t tile_min_x = sat_u32((double)(pix_min_x / T));
+    uint32_t tile_max_x = umin32(sat_u32((double)(pix_max_x / T)), tiles_x_total - 1u);
+    uint32_t tile_min_y = sat_u32((double)(pix_min_y / T));
+    uint32_t tile_max_y = umin32(sat_u32((double)(pix_max_y / T)), sat_u32((double)RCEIL(H / T)) - 1u);
+    uint32_t count = 0;
+    for (uint32_t ty = tile_min_y; ty <= tile_max_y; ++ty) {
+        R xl = (R)1e10f, xr = (R)-1e10f;
+        for (int e = 0; e < 6; ++e) {
+            R yi = proj[FN(EI)[e]][1], yj = proj[FN(EJ)[e]][1];
+            R xi = proj[FN(EI)[e]][0], xj = proj[FN(EJ)[e]][0];
+            for (uint32_t b = 0; b < 2; ++b) {
+                R ytest = (R)(ty + b) * T;
+                if ((yi <= ytest && yj > ytest) || (yj <= ytest && yi > ytest)) {
+                    R tt = (ytest - yi) / (yj - yi);
+                    R x = xi + tt * (xj - xi);
+                    xl = FN(rmin)(xl, x); xr = FN(rmax)(xr, x);
+                }
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+            R vy = proj[k][1];
+            if (vy >= (R)ty * T && vy <= (R)(ty + 1u) * T) {
+                xl = FN(rmin)(xl, proj[k][0]); xr = FN(rmax)(xr, proj[k][0]);
+            }
+        }
+        if (xl <= xr) {
+            uint32_t tx0 = uclamp32(sat_u32((double)FN(rmax)(RFLOOR(xl / T), (R)0.0)), tile_min_x, tile_max_x);
+            uint32_t tx1 = uclamp32(sat_u32((double)FN(rmax)(RFLOOR(xr / T), (R)0.0)), tile_min_x, tile_max_x);
+            count += tx1 - tx0 + 1u;
+        }
+    }
+    return count;
+}
+
+void FN(orc_project)(const R* verts, const uint32_t* idx, const R* dens, uint32_t M,
+                     const float* cam_pos, const float* vp, const float* c2w_intr,
+                     uint32_t w, uint32_t h, uint32_t ts,
+                     uint32_t* tiles_touched, uint8_t* visible) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    for (uint32_t t = 0; t < M; ++t) {
+        int vis = 0;
+        tiles_touched[t] = FN(project_one)(verts, idx, dens, t, &c, &vis);
+        visible[t] = (uint8_t)vis;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* K7: pair generation with sentinel padding                             */
+/* crates/rmesh-tile/src/wgsl/tile_gen_scan_compute.wgsl:61-203         */
+/* tile_fill_compute.wgsl:9-19 (sentinel key 0xFFFFFFFF:0xFFFFFFFF,      */
+/* value 0) applied to every slot the generator leaves unwritten.       */
+/* Keys are u64 = (tile_id << 32) | ~bits(depth).                       */
+/* Compact order = tet-id order (the reference's vis_idx order comes    */
+/* from an atomic and is not deterministic).                            */
+/* ------------------------------------------------------------------ */
+uint64_t FN(orc_pairs)(const R* verts, const uint32_t* idx, const R* circum, uint32_t M,
+                       const float* cam_pos, const float* vp, const float* c2w_intr,
+                       uint32_t w, uint32_t h, uint32_t ts, const uint32_t* tiles_touched,
+                       uint64_t* keys, uint32_t* vals, uint64_t cap) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    R W = (R)w, H = (R)h, T = (R)ts;
+    uint32_t tiles_x = (w + ts - 1) / ts;
+    uint64_t total = 0;
+    for (uint32_t t = 0; t < M; ++t) total += tiles_touched[t];
+    for (uint64_t i = 0; i < total && i < cap; ++i) { keys[i] = ~(uint64_t)0; vals[i] = 0; }
+    uint64_t write_start = 0;
+    for (uint32_t t = 0; t < M; ++t) {
+        uint32_t max_write = tiles_touched[t];
+        uint64_t base = write_start;
+        write_start += max_write;
+        if (max_write == 0) continue;
+        V3 v[4];
+        for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+        R proj[4][2];
+        for (int i = 0; i < 4; ++i) {
+            R clip[4];
+            FN(mul_vp)(c.vp, v[i], clip);
+            R nx = clip[0] / (clip[3] + (R)1e-6);
+            R ny = clip[1] / (clip[3] + (R)1e-6);
+            proj[i][0] = ((nx + (R)1.0) * (R)0.5) * W;
+            proj[i][1] = (((R)1.0 - ny) * (R)0.5) * H;
+        }
+        R pix_min_x = FN(rmax)(FN(rmin)(FN(rmin)(proj[0][0], proj[1][0]), FN(rmin)(proj[2][0], proj[3][0])), (R)0.0);
+        R pix_max_x = FN(rmin)(FN(rmax)(FN(rmax)(proj[0][0], proj[1][0]), FN(rmax)(proj[2][0], proj[3][0])), W - (R)1.0);
+        R pix_min_y = FN(rmax)(FN(rmin)(FN(rmin)(proj[0][1], proj[1][1]), FN(rmin)(proj[2][1], proj[3][1])), (R)0.0);
+        R pix_max_y = FN(rmin)(FN(rmax)(FN(rmax)(proj[0][1], proj[1][1]), FN(rmax)(proj[2][1], proj[3][1])), H - (R)1.0);
+        if (pix_min_x > pix_max_x || pix_min_y > pix_max_y) continue;
+        uint32_t tiles_x_total = sat_u32((double)RCEIL(W / T));
+        uint32_t tile_min_x = sat_u32((double)(pix_min_x / T));
+        uint32_t tile_max_x = umin32(sat_u32((double)(pix_max_x / T)), tiles_x_total - 1u);
+        uint32_t tile_min_y = sat_u32((double)(pix_min_y / T));
+        uint32_t tile_max_y = umin32(sat_u32((double)(pix_max_y / T)), sat_u32((double)RCEIL(H / T)) - 1u);
+        V3 cc = FN(mk)(circum[4 * t], circum[4 * t + 1], circum[4 * t + 2]);
+        R r2 = circum[4 * t + 3];
+        V3 diff = FN(sub)(cc, FN(mk)(c.cam[0], c.cam[1], c.cam[2]));
+        R depth = FN(dot)(diff, diff) - r2;
+        uint32_t inv_depth = ~f32_bits((float)depth);
+        uint32_t local = 0;
+        for (uint32_t ty = tile_min_y; ty <= tile_max_y; ++ty) {
+            R xl = (R)1e10f, xr = (R)-1e10f;
+            for (int e = 0; e < 6; ++e) {
+                R yi = proj[FN(EI)[e]][1], yj = proj[FN(EJ)[e]][1];
+                R xi = proj[FN(EI)[e]][0], xj = proj[FN(EJ)[e]][0];
+                for (uint32_t b = 0; b < 2; ++b) {
+                    R ytest = (R)(ty + b) * T;
+                    if ((yi <= ytest && yj > ytest) || (yj <= ytest && yi > ytest)) {
+                        R tt = (ytest - yi) / (yj - yi);
+                        R x = xi + tt * (xj - xi);
+                        xl = FN(rmin)(xl, x); xr = FN(rmax)(xr, x);
+                    }
+                }
+            }
+            for (int k = 0; k < 4; ++k) {
+                R vy = proj[k][1];
+                if (vy >= (R)ty * T && vy <= (R)(ty + 1u) * T) {
+                    xl = FN(rmin)(xl, proj[k][0]); xr = FN(rmax)(xr, proj[k][0]);
+                }
+            }
+            if (xl <= xr) {
+                uint32_t tx0 = uclamp32(sat_u32((double)FN(rmax)(RFLOOR(xl / T), (R)0.0)), tile_min_x, tile_max_x);
+                uint32_t tx1 = uclamp32(sat_u32((double)FN(rmax)(RFLOOR(xr / T), (R)0.0)), tile_min_x, tile_max_x);
+                for (uint32_t tx = tx0; tx <= tx1; ++tx) {
+                    uint32_t tile_id = ty * tiles_x + tx;
+                    if (local < max_write) {
+                        uint64_t wi = base + local;
+                        if (wi < cap) {
+                            keys[wi] = ((uint64_t)tile_id << 32) | inv_depth;
+                            vals[wi] = t;
+                        }
+                        local++;
+                    }
+                }
+            }
+        }
+    }
+    return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* K12: tiled front-to-back compositing                                  */
+/* crates/rmesh-render/src/wgsl/rasterize_compute.wgsl:92-511           */
+/* ------------------------------------------------------------------ */
+/* Pixel-row span of a projected tet in tile-local pixel coordinates.
+ * forward=1 adds the vertex-in-row rule (rasterize_compute.wgsl:236-242);
+ * the backward kernel omits it (backward_tiled_compute.wgsl:245-283). */
+static void FN(row_span)(const R proj[4][2], uint32_t row, uint32_t TS, int forward, int32_t* xl_i, int32_t* xr_i) {
+    R xl_f = (R)1e10f, xr_f = (R)-1e10f;
+    R yc = (R)row + (R)0.5;
+    for (int e = 0; e < 6; ++e) {
+        int a = FN(EI)[e], b = FN(EJ)[e];
+        R yi = proj[a][1], yj = proj[b][1];
+        if ((yi <= yc && yj > yc) || (yj <= yc && yi > yc)) {
+            R tt = (yc - yi) / (yj - yi);
+            R x = proj[a][0] + tt * (proj[b][0] - proj[a][0]);
+            xl_f = FN(rmin)(xl_f, x); xr_f = FN(rmax)(xr_f, x);
+        }
+    }
+    if (forward) {
+        for (int k = 0; k < 4; ++k) {
+            R vy = proj[k][1];
+            if (vy >= yc - (R)0.5 && vy < yc + (R)0.5) {
+                xl_f = FN(rmin)(xl_f, proj[k][0]); xr_f = FN(rmax)(xr_f, proj[k][0]);
+            }
+        }
+    }
+    *xl_i = 0; *xr_i = -1;
+    if (xl_f <= xr_f) {
+        R eps = (R)0.001f;
+        int32_t l = imax32(sat_i32((double)RCEIL(xl_f - (R)0.5 - eps)), 0);
+        int32_t r = imin32(sat_i32((double)RFLOOR(xr_f - (R)0.5 + eps)), (int32_t)TS - 1);
+        if (l <= r) { *xl_i = l; *xr_i = r; }
+    }
+}
+
+/* Tile-local projection used by both tiled kernels (rasterize_compute.wgsl:153-167). */
+static int FN(tile_proj)(const CAM* c, const V3 v[4], R tile_ox, R tile_oy, R proj[4][2]) {
+    R W = (R)c->width, H = (R)c->height;
+    R clip[4][4];
+    for (int i = 0; i < 4; ++i) FN(mul_vp)(c->vp, v[i], clip[i]);
+    int any_behind = (clip[0][3] <= 0) || (clip[1][3] <= 0) || (clip[2][3] <= 0) || (clip[3][3] <= 0);
+    if (!any_behind) {
+        for (int i = 0; i < 4; ++i) {
+            proj[i][0] = ((clip[i][0] / clip[i][3] + (R)1.0) * (R)0.5) * W - tile_ox;
+            proj[i][1] = (((R)1.0 - clip[i][1] / clip[i][3]) * (R)0.5) * H - tile_oy;
+        }
+    }
+    return any_behind;
+}
+
+static V3 FN(pixel_ray)(const CAM* c, uint32_t px, uint32_t py) {
+    V3 dc = FN(mk)((((R)px + (R)0.5) - c->intr[2]) / c->intr[0], (((R)py + (R)0.5) - c->intr[3]) / c->intr[1], (R)1.0);
+    dc = FN(normalize)(dc);
+    V3 col0 = FN(mk)(c->c2w[0], c->c2w[1], c->c2w[2]);
+    V3 col1 = FN(mk)(c->c2w[3], c->c2w[4], c->c2w[5]);
+    V3 col2 = FN(mk)(c->c2w[6], c->c2w[7], c->c2w[8]);
+    V3 d = FN(mk)((col0.x * dc.x + col1.x * dc.y) + col2.x * dc.z,
+                  (col0.y * dc.x + col1.y * dc.y) + col2.y * dc.z,
+                  (col0.z * dc.x + col1.z * dc.y) + col2.z * dc.z);
+    return FN(normalize)(d);
+}
+
+/* colors: [M*3] per-tet colour (base_colors or SH-evaluated); aux: [H*W*8] or NULL. */
+void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                     const uint32_t* sorted_vals, const uint32_t* ranges,
+                     const float* cam_pos, const float* vp, const float* c2w_intr,
+                     uint32_t w, uint32_t h, uint32_t ts, float min_t,
+                     uint32_t tile_begin, uint32_t tile_end, R* rgba, R* aux, uint32_t* stats) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, min_t);
+    uint32_t tiles_x = (w + ts - 1) / ts;
+    uint32_t TS = ts;
+    R* st = (R*)calloc((size_t)TS * TS * 12, sizeof(R)); /* color3, log_t, nd4, ent4 */
+    uint32_t* ss = (uint32_t*)calloc((size_t)TS * TS * 4, sizeof(uint32_t));
+    V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
+    for (uint32_t tile = tile_begin; tile < tile_end; ++tile) {
+        uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
+        R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
+        memset(st, 0, sizeof(R) * TS * TS * 12);
+        memset(ss, 0, sizeof(uint32_t) * TS * TS * 4);
+        uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+        uint32_t cursor = re;
+        while (cursor > rs) {
+            cursor -= 1;
+            uint32_t t = sorted_vals[cursor];
+            V3 v[4];
+            for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+            R proj[4][2];
+            int any_behind = FN(tile_proj)(&c, v, tile_ox, tile_oy, proj);
+            R density = dens[t];
+            V3 col = FN(load3)(colors, t), g = FN(load3)(grads, t);
+            V3 fn[4];
+            for (int fi = 0; fi < 4; ++fi) {
+                const int* f = FN(FACES)[fi];
+                V3 n = FN(cross)(FN(sub)(v[f[2]], v[f[0]]), FN(sub)(v[f[1]], v[f[0]]));
+                if (FN(dot)(n, FN(sub)(v[f[3]], v[f[0]])) < 0) n = FN(neg)(n);
+                fn[fi] = n;
+            }
+            for (uint32_t row = 0; row < TS; ++row) {
+                int32_t xl_i = 0, xr_i = -1;
+                if (!any_behind) FN(row_span)(proj, row, TS, 1, &xl_i, &xr_i);
+                for (int32_t col_i = xl_i; col_i <= xr_i; ++col_i) {
+                    uint32_t colu = (uint32_t)col_i;
+                    uint32_t pl = row * TS + colu;
+                    uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+                    if (px >= w || py >= h) continue;
+                    R* s = st + 12 * pl;
+                    if (s[3] < (R)-13.8) { ss[4 * pl + 2]++; continue; }
+                    V3 rd = FN(pixel_ray)(&c, px, py);
+                    V3 ro = FN(add)(cam, FN(scl)(rd, c.min_t));
+                    R t_min = (R)-3.402823e38f, t_max = (R)3.402823e38f;
+                    int entry = 4, valid = 1;
+                    for (int fi = 0; fi < 4; ++fi) {
+                        const int* f = FN(FACES)[fi];
+                        V3 n = fn[fi];
+                        R num = FN(dot)(n, FN(sub)(v[f[0]], ro));
+                        R den = FN(dot)(n, rd);
+                        if (RABS(den) < (R)1e-20f) { if (num > 0) valid = 0; continue; }
+                        R tt = num / den;
+                        if (den > 0) { if (tt > t_min) { t_min = tt; entry = fi; } }
+                        else { if (tt < t_max) t_max = tt; }
+                    }
+                    if (!(valid && t_min < t_max)) { ss[4 * pl + 0]++; continue; }
+                    R base_offset = FN(dot)(g, FN(sub)(ro, v[0]));
+                    V3 bc = FN(mk)(col.x + base_offset, col.y + base_offset, col.z + base_offset);
+                    R dc_dt = FN(dot)(g, rd);
+                    R a0 = dc_dt * t_min, a1 = dc_dt * t_max;
+                    V3 cs = FN(mk)(FN(rmax)(bc.x + a0, 0), FN(rmax)(bc.y + a0, 0), FN(rmax)(bc.z + a0, 0));
+                    V3 ce = FN(mk)(FN(rmax)(bc.x + a1, 0), FN(rmax)(bc.y + a1, 0), FN(rmax)(bc.z + a1, 0));
+                    R dist = t_max - t_min;
+                    R od = FN(clampr)(density * dist, (R)1e-8f, (R)88.0);
+                    R alpha_t = FN(safe_exp)(-od);
+                    R phi_v = FN(phi)(od);
+                    R w0 = phi_v - alpha_t, w1 = (R)1.0 - phi_v;
+                    V3 cp = FN(safe_clip3)(FN(mk)(ce.x * w0 + cs.x * w1, ce.y * w0 + cs.y * w1, ce.z * w0 + cs.z * w1),
+                                           (R)0.0, MAX_VAL_R);
+                    R alpha = (R)1.0 - alpha_t;
+                    R Tj = FN(safe_exp)(s[3]);
+                    if (od < (R)0.01) ss[4 * pl + 1]++;
+                    else if (Tj < (R)1e-6f) ss[4 * pl + 2]++;
+                    else ss[4 * pl + 3]++;
+                    if (Tj >= (R)1e-6f) {
+                        s[0] = s[0] + cp.x * Tj; s[1] = s[1] + cp.y * Tj; s[2] = s[2] + cp.z * Tj;
+                        s[3] = s[3] - od;
+                        /* aux: normals + depth, entropy (rasterize_compute.wgsl:434-456) */
+                        V3 en = FN(mk)(0, 0, 0);
+                        if (entry < 4) en = FN(normalize)(FN(neg)(fn[entry]));
+                        R depth_premul = w0 * t_max + w1 * t_min;
+                        R ta = Tj * alpha;
+                        s[4] = s[4] + ta * en.x; s[5] = s[5] + ta * en.y; s[6] = s[6] + ta * en.z;
+                        s[7] = s[7] + Tj * depth_premul;
+                        R wt = alpha * Tj;
+                        R c_mid = (t_min + t_max) * (R)0.5;
+                        R wc = wt * c_mid;
+                        s[8] = s[8] + wt;
+                        s[9] = s[9] + (wt > (R)1e-20f ? wt * RLOG(wt) : (R)0.0);
+                        s[10] = s[10] + wt * c_mid;
+                        s[11] = s[11] + (wc > (R)1e-20f ? wc * RLOG(wc) : (R)0.0);
+                    }
+                }
+            }
+        }
+        for (uint32_t i = 0; i < TS * TS; ++i) {
+            uint32_t row = i / TS, colu = i % TS;
+            uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+            if (px < w && py < h) {
+                size_t pix = (size_t)py * w + px;
+                R* s = st + 12 * i;
+                rgba[4 * pix + 0] = s[0]; rgba[4 * pix + 1] = s[1]; rgba[4 * pix + 2] = s[2];
+                rgba[4 * pix + 3] = (R)1.0 - FN(safe_exp)(s[3]);
+                if (aux) for (int k = 0; k < 8; ++k) aux[8 * pix + k] = s[4 + k];
+                if (stats) for (int k = 0; k < 4; ++k) stats[4 * pix + k] = ss[4 * i + k];
+            }
+        }
+    }
+    free(st);
+    free(ss);
+}
+
+/* ------------------------------------------------------------------ */
+/* K14: per-pixel loss gradient (+ loss value, an extension: the         */
+/* reference never writes its loss buffer).                             */
+/* crates/rmesh-train/src/wgsl/loss_compute.wgsl:24-72                  */
+/* ------------------------------------------------------------------ */
+double FN(orc_loss)(const R* rgba, const R* gt, uint32_t w, uint32_t h, uint32_t loss_type, R* dl) {
+    R n_pixels = (R)(w * h);
+    double loss = 0.0;
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            R d = rgba[4 * i + k] - gt[3 * i + k];
+            R g;
+            if (loss_type == 0) {
+                R sg = d > 0 ? (R)1.0 : (d < 0 ? (R)-1.0 : (R)0.0);
+                g = sg / n_pixels;
+                loss += fabs((double)d);
+            } else {
+                g = ((R)2.0 * d) / n_pixels;
+                loss += (double)d * (double)d;
+            }
+            dl[4 * i + k] = g;
+        }
+        dl[4 * i + 3] = 0;
+    }
+    return loss / (double)n_pixels;
+}
+
+/* ------------------------------------------------------------------ */
+/* K13: reverse-order tiled adjoint                                      */
+/* crates/rmesh-backward/src/wgsl/backward_tiled_compute.wgsl:154-587   */
+/* Gradients accumulate (+=) into the output arrays.                    */
+/* ------------------------------------------------------------------ */
+void FN(orc_backward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                      const uint32_t* sorted_vals, const uint32_t* ranges,
+                      const float* cam_pos, const float* vp, const float* c2w_intr,
+                      uint32_t w, uint32_t h, uint32_t ts,
+                      const R* rgba, const R* dl, uint32_t tile_begin, uint32_t tile_end,
+                      R* d_verts, R* d_dens, R* d_grads, R* d_colors) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    uint32_t tiles_x = (w + ts - 1) / ts;
+    uint32_t TS = ts;
+    R* st = (R*)calloc((size_t)TS * TS * 5, sizeof(R)); /* color3, log_t, d_log_t */
+    V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
+    for (uint32_t tile = tile_begin; tile < tile_end; ++tile) {
+        uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
+        R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
+        for (uint32_t i = 0; i < TS * TS; ++i) {
+            uint32_t row = i / TS, colu = i % TS;
+            uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+            R* s = st + 5 * i;
+            if (px < w && py < h) {
+                size_t pix = (size_t)py * w + px;
+                R a = rgba[4 * pix + 3];
+                s[0] = rgba[4 * pix]; s[1] = rgba[4 * pix + 1]; s[2] = rgba[4 * pix + 2];
+                s[3] = RLOG(FN(rmax)((R)1.0 - a, (R)1e-20f));
+                s[4] = -dl[4 * pix + 3] * ((R)1.0 - a);
+            } else {
+                s[0] = s[1] = s[2] = s[3] = s[4] = 0;
+            }
+        }
+        uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+        for (uint32_t cursor = rs; cursor < re; ++cursor) {
+            uint32_t t = sorted_vals[cursor];
+            uint32_t vi[4];
+            V3 v[4];
+            for (int i = 0; i < 4; ++i) { vi[i] = idx[4 * t + i]; v[i] = FN(load3)(verts, vi[i]); }
+            R proj[4][2];
+            int any_behind = FN(tile_proj)(&c, v, tile_ox, tile_oy, proj);
+            R density = dens[t];
+            V3 col = FN(load3)(colors, t), g = FN(load3)(grads, t);
+            R acc_d = 0;
+            V3 acc_v[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            V3 acc_g = FN(mk)(0, 0, 0), acc_c = FN(mk)(0, 0, 0);
+            for (uint32_t row = 0; row < TS; ++row) {
+                int32_t xl_i = 0, xr_i = -1;
+                if (!any_behind) FN(row_span)(proj, row, TS, 0, &xl_i, &xr_i);
+                for (int32_t col_i = xl_i; col_i <= xr_i; ++col_i) {
+                    uint32_t colu = (uint32_t)col_i;
+                    uint32_t pl = row * TS + colu;
+                    uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+                    if (px >= w || py >= h) continue;
+                    size_t pix = (size_t)py * w + px;
+                    V3 rd = FN(pixel_ray)(&c, px, py);
+                    R t_min = (R)-3.402823e38f, t_max = (R)3.402823e38f;
+                    int valid = 1, min_face = 0, max_face = 0;
+                    for (int fi = 0; fi < 4; ++fi) {
+                        const int* f = FN(FACES)[fi];
+                        V3 n = FN(cross)(FN(sub)(v[f[2]], v[f[0]]), FN(sub)(v[f[1]], v[f[0]]));
+                        if (FN(dot)(n, FN(sub)(v[f[3]], v[f[0]])) < 0) n = FN(neg)(n);
+                        R num = FN(dot)(n, FN(sub)(v[f[0]], cam));
+                        R den = FN(dot)(n, rd);
+                        if (RABS(den) < (R)1e-20f) { if (num > 0) valid = 0; continue; }
+                        R tt = num / den;
+                        if (den > 0) { if (tt > t_min) { t_min = tt; min_face = fi; } }
+                        else { if (tt < t_max) { t_max = tt; max_face = fi; } }
+                    }
+                    if (!valid || t_min >= t_max) continue;
+                    R base_offset = FN(dot)(g, FN(sub)(cam, v[0]));
+                    V3 bc = FN(mk)(col.x + base_offset, col.y + base_offset, col.z + base_offset);
+                    R dc_dt = FN(dot)(g, rd);
+                    R a0 = dc_dt * t_min, a1 = dc_dt * t_max;
+                    V3 csr = FN(mk)(bc.x + a0, bc.y + a0, bc.z + a0);
+                    V3 cer = FN(mk)(bc.x + a1, bc.y + a1, bc.z + a1);
+                    V3 cs = FN(mk)(FN(rmax)(csr.x, 0), FN(rmax)(csr.y, 0), FN(rmax)(csr.z, 0));
+                    V3 ce = FN(mk)(FN(rmax)(cer.x, 0), FN(rmax)(cer.y, 0), FN(rmax)(cer.z, 0));
+                    R dist = t_max - t_min;
+                    R od = FN(clampr)(density * dist, (R)1e-8f, (R)88.0);
+                    R alpha_t = FN(safe_exp)(-od);
+                    R phi_v = FN(phi)(od);
+                    R w0 = phi_v - alpha_t, w1 = (R)1.0 - phi_v;
+                    V3 cp = FN(safe_clip3)(FN(mk)(ce.x * w0 + cs.x * w1, ce.y * w0 + cs.y * w1, ce.z * w0 + cs.z * w1),
+                                           MIN_VAL_R, MAX_VAL_R);
+                    R* s = st + 5 * pl;
+                    R log_t = s[3], d_log_t = s[4];
+                    R prev_log_t = FN(rmin)(log_t + od, (R)0.0);
+                    R t_prev = FN(safe_exp)(prev_log_t);
+                    s[0] = s[0] - cp.x * t_prev; s[1] = s[1] - cp.y * t_prev; s[2] = s[2] - cp.z * t_prev;
+                    s[3] = prev_log_t;
+                    if (t_prev < (R)1e-6f) continue;
+                    V3 dcol = FN(mk)(dl[4 * pix], dl[4 * pix + 1], dl[4 * pix + 2]);
+                    V3 dcp = FN(safe_clip3)(FN(scl)(dcol, t_prev), MIN_VAL_R, MAX_VAL_R);
+                    R d_od_state = -d_log_t;
+                    s[4] = FN(safe_clip)(d_log_t + FN(dot)(dcol, cp) * t_prev, MIN_VAL_R, MAX_VAL_R);
+                    R dphi = FN(dphi_dx)(od);
+                    R dw0 = dphi + alpha_t, dw1 = -dphi;
+                    V3 dce_i = FN(scl)(dcp, w0), dcs_i = FN(scl)(dcp, w1);
+                    R d_od_int = FN(dot)(dcp, FN(mk)(ce.x * dw0 + cs.x * dw1, ce.y * dw0 + cs.y * dw1, ce.z * dw0 + cs.z * dw1));
+                    R d_od = FN(safe_clip)(d_od_state + d_od_int, MIN_VAL_R, MAX_VAL_R);
+                    R od_raw = density * dist;
+                    int active = (od_raw >= (R)1e-8f) && (od_raw <= (R)88.0);
+                    R d_od_raw = active ? d_od : (R)0.0;
+                    R d_dens_l = FN(safe_clip)(d_od_raw * dist, MIN_VAL_R, MAX_VAL_R);
+                    R d_dist = FN(safe_clip)(d_od_raw * density, MIN_VAL_R, MAX_VAL_R);
+                    R d_tmin = -d_dist, d_tmax = d_dist;
+                    V3 dcs = FN(mk)(csr.x > 0 ? dcs_i.x : 0, csr.y > 0 ? dcs_i.y : 0, csr.z > 0 ? dcs_i.z : 0);
+                    V3 dce = FN(mk)(cer.x > 0 ? dce_i.x : 0, cer.y > 0 ? dce_i.y : 0, cer.z > 0 ? dce_i.z : 0);
+                    V3 dbc = FN(add)(dcs, dce);
+                    R sum_s = (dcs.x + dcs.y) + dcs.z, sum_e = (dce.x + dce.y) + dce.z;
+                    R d_dcdt = sum_s * t_min + sum_e * t_max;
+                    d_tmin = d_tmin + sum_s * dc_dt;
+                    d_tmax = d_tmax + sum_e * dc_dt;
+                    R dbo = (dbc.x + dbc.y) + dbc.z;
+                    V3 dg_l = FN(scl)(FN(sub)(cam, v[0]), dbo);
+                    V3 dv0b = FN(scl)(FN(neg)(g), dbo);
+                    dg_l = FN(add)(dg_l, FN(scl)(rd, d_dcdt));
+                    acc_c = FN(add)(acc_c, FN(safe_clip3)(dbc, MIN_VAL_R, MAX_VAL_R));
+                    V3 dvl[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                    for (int side = 0; side < 2; ++side) {
+                        const int* f = FN(FACES)[side == 0 ? min_face : max_face];
+                        R tt = side == 0 ? t_min : t_max;
+                        R dt = side == 0 ? d_tmin : d_tmax;
+                        V3 va = v[f[0]], vb = v[f[1]], vc = v[f[2]];
+                        V3 n = FN(cross)(FN(sub)(vc, va), FN(sub)(vb, va));
+                        R den = FN(dot)(n, rd);
+                        R inv_den = FN(safe_div)((R)1.0, den);
+                        V3 hit = FN(add)(cam, FN(scl)(rd, tt));
+                        V3 ah = FN(sub)(va, hit);
+                        V3 dta = FN(scl)(FN(add)(FN(cross)(ah, FN(sub)(vb, vc)), n), inv_den);
+                        V3 dtb = FN(scl)(FN(cross)(ah, FN(sub)(vc, va)), inv_den);
+                        V3 dtc = FN(scl)(FN(cross)(ah, FN(sub)(va, vb)), inv_den);
+                        dvl[f[0]] = FN(add)(dvl[f[0]], FN(safe_clip3)(FN(scl)(dta, dt), MIN_VAL_R, MAX_VAL_R));
+                        dvl[f[1]] = FN(add)(dvl[f[1]], FN(safe_clip3)(FN(scl)(dtb, dt), MIN_VAL_R, MAX_VAL_R));
+                        dvl[f[2]] = FN(add)(dvl[f[2]], FN(safe_clip3)(FN(scl)(dtc, dt), MIN_VAL_R, MAX_VAL_R));
+                    }
+                    dvl[0] = FN(add)(dvl[0], dv0b);
+                    acc_d = acc_d + FN(safe_clip)(d_dens_l, MIN_VAL_R, MAX_VAL_R);
+                    for (int k = 0; k < 4; ++k) acc_v[k] = FN(add)(acc_v[k], FN(safe_clip3)(dvl[k], MIN_VAL_R, MAX_VAL_R));
+                    acc_g = FN(add)(acc_g, FN(safe_clip3)(dg_l, MIN_VAL_R, MAX_VAL_R));
+                }
+            }
+            d_dens[t] += acc_d;
+            for (int k = 0; k < 4; ++k) {
+                d_verts[3 * vi[k]] += acc_v[k].x; d_verts[3 * vi[k] + 1] += acc_v[k].y; d_verts[3 * vi[k] + 2] += acc_v[k].z;
+            }
+            d_grads[3 * t] += acc_g.x; d_grads[3 * t + 1] += acc_g.y; d_grads[3 * t + 2] += acc_g.z;
+            d_colors[3 * t] += acc_c.x; d_colors[3 * t + 1] += acc_c.y; d_colors[3 * t + 2] += acc_c.z;
+        }
+    }
+    free(st);
+}
+
+/* ------------------------------------------------------------------ */
+/* SH colour evaluation — crates/rmesh-util/src/wgsl/sh_eval_compute.wgsl:22-117 */
+/* sh: [M][3][nc] fp32, nc = (deg+1)^2.  Colours = softplus(Σ Y_k sh + 0.5 + g·(v0−centroid)). */
+/* ------------------------------------------------------------------ */
+#define SH_C0 ((R)0.28209479177387814)
+#define SH_C1 ((R)0.4886025119029199)
+#define SH_C2_0 ((R)1.0925484305920792)
+#define SH_C2_1 ((R)-1.0925484305920792)
+#define SH_C2_2 ((R)0.31539156525252005)
+#define SH_C2_3 ((R)-1.0925484305920792)
+#define SH_C2_4 ((R)0.5462742152960396)
+#define SH_C3_0 ((R)-0.5900435899266435)
+#define SH_C3_1 ((R)2.890611442640554)
+#define SH_C3_2 ((R)-0.4570457994644658)
+#define SH_C3_3 ((R)0.3731763325901154)
+#define SH_C3_4 ((R)-0.4570457994644658)
+#define SH_C3_5 ((R)1.445305721320277)
+#define SH_C3_6 ((R)-0.5900435899266435)
+
+/* basis values Y_k(dir) with the signs folded in exactly as the shader applies them */
+static void FN(sh_basis)(V3 d, uint32_t deg, R* Y) {
+    R x = d.x, y = d.y, z = d.z;
+    Y[0] = SH_C0;
+    if (deg >= 1) { Y[1] = -(SH_C1 * y); Y[2] = SH_C1 * z; Y[3] = -(SH_C1 * x); }
+    if (deg >= 2) {
+        R xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+        Y[4] = SH_C2_0 * xy; Y[5] = SH_C2_1 * yz; Y[6] = SH_C2_2 * (((R)2.0 * zz - xx) - yy);
+        Y[7] = SH_C2_3 * xz; Y[8] = SH_C2_4 * (xx - yy);
+    }
+    if (deg >= 3) {
+        R xx = x * x, yy = y * y, zz = z * z;
+        Y[9] = (SH_C3_0 * y) * ((R)3.0 * xx - yy);
+        Y[10] = ((SH_C3_1 * x) * y) * z;
+        Y[11] = (SH_C3_2 * y) * (((R)4.0 * zz - xx) - yy);
+        Y[12] = (SH_C3_3 * z) * (((R)2.0 * zz - (R)3.0 * xx) - (R)3.0 * yy);
+        Y[13] = (SH_C3_4 * x) * (((R)4.0 * zz - xx) - yy);
+        Y[14] = (SH_C3_5 * z) * (xx - yy);
+        Y[15] = (SH_C3_6 * x) * (xx - (R)3.0 * yy);
+    }
+}
+
+/* d Y_k / d dir (3 components each) */
+static void FN(sh_basis_grad)(V3 d, uint32_t deg, R dY[16][3]) {
+    R x = d.x, y = d.y, z = d.z;
+    for (int k = 0; k < 16; ++k) dY[k][0] = dY[k][1] = dY[k][2] = 0;
+    if (deg >= 1) { dY[1][1] = -SH_C1; dY[2][2] = SH_C1; dY[3][0] = -SH_C1; }
+    if (deg >= 2) {
+        dY[4][0] = SH_C2_0 * y; dY[4][1] = SH_C2_0 * x;
+        dY[5][1] = SH_C2_1 * z; dY[5][2] = SH_C2_1 * y;
+        dY[6][0] = SH_C2_2 * (-2 * x); dY[6][1] = SH_C2_2 * (-2 * y); dY[6][2] = SH_C2_2 * (4 * z);
+        dY[7][0] = SH_C2_3 * z; dY[7][2] = SH_C2_3 * x;
+        dY[8][0] = SH_C2_4 * (2 * x); dY[8][1] = SH_C2_4 * (-2 * y);
+    }
+    if (deg >= 3) {
+        R xx = x * x, yy = y * y, zz = z * z;
+        dY[9][0] = SH_C3_0 * (6 * x * y); dY[9][1] = SH_C3_0 * (3 * xx - 3 * yy);
+        dY[10][0] = SH_C3_1 * y * z; dY[10][1] = SH_C3_1 * x * z; dY[10][2] = SH_C3_1 * x * y;
+        dY[11][0] = SH_C3_2 * (-2 * x * y); dY[11][1] = SH_C3_2 * (4 * zz - xx - 3 * yy); dY[11][2] = SH_C3_2 * (8 * y * z);
+        dY[12][0] = SH_C3_3 * (-6 * x * z); dY[12][1] = SH_C3_3 * (-6 * y * z); dY[12][2] = SH_C3_3 * (6 * zz - 3 * xx - 3 * yy);
+        dY[13][0] = SH_C3_4 * (4 * zz - 3 * xx - yy); dY[13][1] = SH_C3_4 * (-2 * x * y); dY[13][2] = SH_C3_4 * (8 * x * z);
+        dY[14][0] = SH_C3_5 * (2 * x * z); dY[14][1] = SH_C3_5 * (-2 * y * z); dY[14][2] = SH_C3_5 * (xx - yy);
+        dY[15][0] = SH_C3_6 * (3 * xx - 3 * yy); dY[15][1] = SH_C3_6 * (-6 * x * y);
+    }
+}
+
+static void FN(sh_dir)(const R* verts, const uint32_t* idx, uint32_t t, const float* cam_pos,
+                       V3 v[4], V3* centroid, V3* raw, R* len, V3* dir) {
+    for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+    *centroid = FN(scl)(FN(add)(FN(add)(FN(add)(v[0], v[1]), v[2]), v[3]), (R)0.25);
+    *raw = FN(sub)(*centroid, FN(mk)((R)cam_pos[0], (R)cam_pos[1], (R)cam_pos[2]));
+    *len = RSQRT(FN(dot)(*raw, *raw));
+    *dir = FN(mk)(0, 0, 0);
+    if (*len > 0) *dir = FN(mk)(raw->x / *len, raw->y / *len, raw->z / *len);
+}
+
+void FN(orc_sh_colors)(const R* verts, const uint32_t* idx, const R* sh, const R* grads, uint32_t M,
+                       uint32_t deg, const float* cam_pos, R* colors) {
+    uint32_t nc = (deg + 1) * (deg + 1);
+    for (uint32_t t = 0; t < M; ++t) {
+        V3 v[4], centroid, raw, dir;
+        R len;
+        FN(sh_dir)(verts, idx, t, cam_pos, v, &centroid, &raw, &len, &dir);
+        R Y[16];
+        FN(sh_basis)(dir, deg, Y);
+        V3 g = FN(load3)(grads, t);
+        R offset = FN(dot)(g, FN(sub)(v[0], centroid));
+        for (int ch = 0; ch < 3; ++ch) {
+            const R* s = sh + ((size_t)t * 3 + ch) * nc;
+            R val = Y[0] * s[0];
+            for (uint32_t k = 1; k < nc; ++k) val = val + Y[k] * s[k];
+            colors[3 * t + ch] = FN(softplus)((val + (R)0.5) + offset);
+        }
+    }
+}
+
+/* Adjoint of orc_sh_colors (new: the reference has no SH adjoint).
+ * Accumulates into d_sh [M*3*nc], d_grads [M*3], d_verts [N*3]. */
+void FN(orc_sh_backward)(const R* verts, const uint32_t* idx, const R* sh, const R* grads, uint32_t M,
+                         uint32_t deg, const float* cam_pos, const R* d_colors,
+                         R* d_sh, R* d_grads, R* d_verts) {
+    uint32_t nc = (deg + 1) * (deg + 1);
+    for (uint32_t t = 0; t < M; ++t) {
+        V3 v[4], centroid, raw, dir;
+        R len;
+        FN(sh_dir)(verts, idx, t, cam_pos, v, &centroid, &raw, &len, &dir);
+        R Y[16], dY[16][3];
+        FN(sh_basis)(dir, deg, Y);
+        FN(sh_basis_grad)(dir, deg, dY);
+        V3 g = FN(load3)(grads, t);
+        R offset = FN(dot)(g, FN(sub)(v[0], centroid));
+        R s_sum = 0;
+        V3 d_dir = FN(mk)(0, 0, 0);
+        for (int ch = 0; ch < 3; ++ch) {
+            const R* s = sh + ((size_t)t * 3 + ch) * nc;
+            R val = Y[0] * s[0];
+            for (uint32_t k = 1; k < nc; ++k) val = val + Y[k] * s[k];
+            R pre = (val + (R)0.5) + offset;
+            R d_pre = d_colors[3 * t + ch] * FN(dsoftplus)(pre);
+            s_sum += d_pre;
+            R* ds = d_sh + ((size_t)t * 3 + ch) * nc;
+            for (uint32_t k = 0; k < nc; ++k) {
+                ds[k] += d_pre * Y[k];
+                d_dir.x += d_pre * s[k] * dY[k][0];
+                d_dir.y += d_pre * s[k] * dY[k][1];
+                d_dir.z += d_pre * s[k] * dY[k][2];
+            }
+        }
+        V3 off = FN(sub)(v[0], centroid);
+        d_grads[3 * t] += s_sum * off.x; d_grads[3 * t + 1] += s_sum * off.y; d_grads[3 * t + 2] += s_sum * off.z;
+        /* offset = g·(v0 − centroid): d v0 += 0.75 s g, d v_{1,2,3} −= 0.25 s g */
+        V3 dcen = FN(scl)(g, -s_sum);
+        /* dir = raw/len: d raw = (d_dir − dir (dir·d_dir)) / len */
+        if (len > 0) {
+            R dd = FN(dot)(dir, d_dir);
+            V3 draw = FN(scl)(FN(sub)(d_dir, FN(scl)(dir, dd)), (R)1.0 / len);
+            dcen = FN(add)(dcen, draw);
+        }
+        for (int i = 0; i < 4; ++i) {
+            uint32_t vi = idx[4 * t + i];
+            V3 d = FN(scl)(dcen, (R)0.25);
+            if (i == 0) d = FN(add)(d, FN(scl)(g, s_sum));
+            d_verts[3 * vi] += d.x; d_verts[3 * vi + 1] += d.y; d_verts[3 * vi + 2] += d.z;
+        }
+    }
+}
+
+#undef V3
+#undef CAM
+#undef FN
+#undef CAT
+#undef CAT2

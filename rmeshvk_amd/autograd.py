@@ -1,0 +1,68 @@
+"""PyTorch autograd integration — drop-in for rmesh_wgpu.autograd
+(crates/rmesh-python/python/rmesh_wgpu/autograd.py:15-151).
+
+``RMeshForward`` / ``RMeshModule`` keep the reference's signatures: forward
+uploads the parameters and renders one view through the tiled pipeline;
+backward hands dL/d(image) to the renderer's hand-written adjoint and returns
+gradients for (vertices, base_colors, densities, color_grads).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from .renderer import RMeshRenderer
+
+
+def _np(t):
+    return np.ascontiguousarray(t.detach().cpu().numpy().ravel().astype(np.float32))
+
+
+class RMeshForward(torch.autograd.Function):
+    """Custom autograd function delegating rendering to the HIP renderer."""
+
+    @staticmethod
+    def forward(ctx, renderer, cam_pos, vp, c2w_intrinsics, vertices, base_colors, densities, color_grads,
+                render_mode="tiled"):
+        if render_mode != "tiled":
+            raise NotImplementedError(f"render_mode={render_mode!r}: only 'tiled' is served")
+        renderer.update_params(_np(vertices), _np(base_colors), _np(densities), _np(color_grads))
+        image_np = renderer.forward_tiled(_np(cam_pos), _np(vp), _np(c2w_intrinsics))
+        ctx.render_mode = render_mode
+        ctx.renderer = renderer
+        ctx.save_for_backward(vertices, base_colors, densities, color_grads)
+        return torch.from_numpy(image_np.copy()).to(vertices.device)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        dl = np.ascontiguousarray(grad_output.detach().cpu().numpy().astype(np.float32))
+        grads = ctx.renderer.backward(dl)
+        device = ctx.saved_tensors[0].device
+        t = lambda k: torch.from_numpy(grads[k].copy()).to(device)  # noqa: E731
+        return (None, None, None, None, t("d_vertices"), t("d_base_colors"), t("d_densities"),
+                t("d_color_grads"), None)
+
+
+class RMeshModule(nn.Module):
+    """nn.Module holding learnable parameters + the renderer (autograd.py:79-151)."""
+
+    def __init__(self, vertices, indices, base_colors, densities, color_grads, circumdata, width, height, **kw):
+        super().__init__()
+        self.vertices = nn.Parameter(torch.tensor(vertices.ravel(), dtype=torch.float32))
+        self.base_colors = nn.Parameter(torch.tensor(base_colors.ravel(), dtype=torch.float32))
+        self.densities = nn.Parameter(torch.tensor(densities.ravel(), dtype=torch.float32))
+        self.color_grads = nn.Parameter(torch.tensor(color_grads.ravel(), dtype=torch.float32))
+        self._indices = indices.ravel().astype(np.uint32)
+        self._circumdata = circumdata.ravel().astype(np.float32)
+        self._width = width
+        self._height = height
+        self.renderer = RMeshRenderer(
+            vertices.ravel().astype(np.float32), self._indices, base_colors.ravel().astype(np.float32),
+            densities.ravel().astype(np.float32), color_grads.ravel().astype(np.float32), self._circumdata,
+            width, height, **kw)
+
+    def forward(self, cam_pos, vp, c2w_intrinsics, render_mode="tiled"):
+        return RMeshForward.apply(self.renderer, cam_pos, vp, c2w_intrinsics, self.vertices, self.base_colors,
+                                  self.densities, self.color_grads, render_mode)

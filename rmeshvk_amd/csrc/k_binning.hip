@@ -1,0 +1,368 @@
+// Projection, tile counting, pair generation and per-tile ranges.
+//
+// Reference kernels restated here (crates/rmesh-render, crates/rmesh-tile):
+//   project_compute.wgsl:88-262        -> k_project  (cull + conservative tile count)
+//   prepare_dispatch.wgsl + rts.wgsl   -> block sums in k_project + k_spine_scan
+//   tile_fill_compute.wgsl             -> sentinel padding inside k_pairs
+//   tile_gen_scan_compute.wgsl:61-203  -> k_pairs
+//   tile_ranges_compute.wgsl:10-45     -> k_ranges
+//
+// MI355X design: one thread per tet looping over every view of the batch,
+// so a tet record (64 B, 4 x float4, coalesced across the wave) is fetched
+// once per step instead of once per view; pair offsets come from per-block
+// sums written by k_project, one spine scan, and a wave64 block scan redone
+// inside k_pairs — no offsets array and no atomics on the ordering path, so
+// the pre-sort pair order is deterministic (tet-id order within a view).
+#include "rmesh_common.hpp"
+#include "rmesh_kernels.hpp"
+
+namespace rmesh {
+
+// Tet record: r0 = (v0, density), r1 = (v1, g.x), r2 = (v2, g.y), r3 = (v3, g.z)
+__global__ __launch_bounds__(256) void k_prep_records(const float* __restrict__ verts, const uint32_t* __restrict__ idx,
+                                                      const float* __restrict__ dens, const float* __restrict__ grads,
+                                                      uint32_t M, float4* __restrict__ rec) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    uint4 q = reinterpret_cast<const uint4*>(idx)[t];
+    uint32_t vi[4] = {q.x, q.y, q.z, q.w};
+    float extra[4] = {dens[t], grads[3ull * t], grads[3ull * t + 1], grads[3ull * t + 2]};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float* p = verts + 3ull * vi[i];
+        rec[4ull * t + i] = make_float4(p[0], p[1], p[2], extra[i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy_colors(const float* __restrict__ base, uint32_t M, float4* __restrict__ col) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    col[t] = make_float4(base[3ull * t], base[3ull * t + 1], base[3ull * t + 2], 0.0f);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, uint64_t n) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
+}
+
+__device__ __forceinline__ void load_rec(const float4* __restrict__ rec, uint32_t t, V3 v[4], float& dens, V3& g) {
+    float4 r0 = rec[4ull * t], r1 = rec[4ull * t + 1], r2 = rec[4ull * t + 2], r3 = rec[4ull * t + 3];
+    v[0] = mk3(r0.x, r0.y, r0.z);
+    v[1] = mk3(r1.x, r1.y, r1.z);
+    v[2] = mk3(r2.x, r2.y, r2.z);
+    v[3] = mk3(r3.x, r3.y, r3.z);
+    dens = r0.w;
+    g = mk3(r1.w, r2.w, r3.w);
+}
+
+// Scanline tile-row extent at tile granularity (project_compute.wgsl:219-258,
+// tile_gen_scan_compute.wgsl:151-182).
+__device__ __forceinline__ bool tile_row_extent(const float proj[4][2], uint32_t ty, float T, float& xl, float& xr) {
+    xl = 1e10f;
+    xr = -1e10f;
+    const int EI[6] = {0, 0, 0, 1, 1, 2};
+    const int EJ[6] = {1, 2, 3, 2, 3, 3};
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        float yi = proj[EI[e]][1], yj = proj[EJ[e]][1];
+        float xi = proj[EI[e]][0], xj = proj[EJ[e]][0];
+#pragma unroll
+        for (uint32_t b = 0; b < 2; ++b) {
+            float ytest = (float)(ty + b) * T;
+            if ((yi <= ytest && yj > ytest) || (yj <= ytest && yi > ytest)) {
+                float tt = (ytest - yi) / (yj - yi);
+                float x = xi + tt * (xj - xi);
+                xl = fminf(xl, x);
+                xr = fmaxf(xr, x);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float vy = proj[k][1];
+        if (vy >= (float)ty * T && vy <= (float)(ty + 1u) * T) {
+            xl = fminf(xl, proj[k][0]);
+            xr = fmaxf(xr, proj[k][0]);
+        }
+    }
+    return xl <= xr;
+}
+
+struct TileBox {
+    uint32_t x0, x1, y0, y1;
+};
+
+// AABB in tiles (project_compute.wgsl:202-216)
+__device__ __forceinline__ bool tile_box(const float proj[4][2], float W, float H, float T, TileBox& b) {
+    float pix_min_x = fmaxf(fminf(fminf(proj[0][0], proj[1][0]), fminf(proj[2][0], proj[3][0])), 0.0f);
+    float pix_max_x = fminf(fmaxf(fmaxf(proj[0][0], proj[1][0]), fmaxf(proj[2][0], proj[3][0])), W - 1.0f);
+    float pix_min_y = fmaxf(fminf(fminf(proj[0][1], proj[1][1]), fminf(proj[2][1], proj[3][1])), 0.0f);
+    float pix_max_y = fminf(fmaxf(fmaxf(proj[0][1], proj[1][1]), fmaxf(proj[2][1], proj[3][1])), H - 1.0f);
+    if (pix_min_x > pix_max_x || pix_min_y > pix_max_y) return false;
+    uint32_t tiles_x_total = sat_u32(ceilf(W / T));
+    b.x0 = sat_u32(pix_min_x / T);
+    b.x1 = min(sat_u32(pix_max_x / T), tiles_x_total - 1u);
+    b.y0 = sat_u32(pix_min_y / T);
+    b.y1 = min(sat_u32(pix_max_y / T), sat_u32(ceilf(H / T)) - 1u);
+    return true;
+}
+
+__device__ __forceinline__ void row_tiles(float xl, float xr, float T, const TileBox& b, uint32_t& tx0, uint32_t& tx1) {
+    tx0 = uclamp(sat_u32(fmaxf(floorf(xl / T), 0.0f)), b.x0, b.x1);
+    tx1 = uclamp(sat_u32(fmaxf(floorf(xr / T), 0.0f)), b.x0, b.x1);
+}
+
+// K1 for one (tet, view): tile count; `visible` is the reference's vis_idx
+// flag (project_compute.wgsl:111-262).
+__device__ __forceinline__ uint32_t project_count(const V3 v[4], float dens, const ViewParams& vw, bool& visible) {
+    float p[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float clip[4];
+        mul_vp(vw.vp, v[i], clip);
+        float inv_w = 1.0f / (clip[3] + 1e-6f);
+        p[i][0] = clip[0] * inv_w;
+        p[i][1] = clip[1] * inv_w;
+        p[i][2] = clip[2] * inv_w;
+        p[i][3] = clip[3];
+    }
+    bool any_behind = (p[0][3] <= 0.0f) || (p[1][3] <= 0.0f) || (p[2][3] <= 0.0f) || (p[3][3] <= 0.0f);
+    float W = vw.Wf, H = vw.Hf;
+    bool vis = true;
+    if (!any_behind) {
+        float min_x = fminf(fminf(p[0][0], p[1][0]), fminf(p[2][0], p[3][0]));
+        float max_x = fmaxf(fmaxf(p[0][0], p[1][0]), fmaxf(p[2][0], p[3][0]));
+        float min_y = fminf(fminf(p[0][1], p[1][1]), fminf(p[2][1], p[3][1]));
+        float max_y = fmaxf(fmaxf(p[0][1], p[1][1]), fmaxf(p[2][1], p[3][1]));
+        float min_z = fminf(fminf(p[0][2], p[1][2]), fminf(p[2][2], p[3][2]));
+        if (max_x < -1.0f || min_x > 1.0f || max_y < -1.0f || min_y > 1.0f || min_z > 1.0f) vis = false;
+        float ext_x = (max_x - min_x) * W;
+        float ext_y = (max_y - min_y) * H;
+        if (ext_x * ext_y < 1.0f) vis = false;
+    }
+    if (!vis) {
+        visible = false;
+        return 0;
+    }
+    {
+        V3 e01 = sub3(v[1], v[0]), e02 = sub3(v[2], v[0]), e03 = sub3(v[3], v[0]);
+        V3 e12 = sub3(v[2], v[1]), e13 = sub3(v[3], v[1]), e23 = sub3(v[3], v[2]);
+        float m = fmaxf(fmaxf(dot3(e01, e01), fmaxf(dot3(e02, e02), dot3(e03, e03))),
+                        fmaxf(dot3(e12, e12), fmaxf(dot3(e13, e13), dot3(e23, e23))));
+        float max_od = dens * sqrtf(m);
+        if (max_od < 0.01f) {
+            visible = false;
+            return 0;
+        }
+    }
+    visible = true;
+    if (any_behind) return 0;
+    float proj[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        proj[i][0] = ((p[i][0] + 1.0f) * 0.5f) * W;
+        proj[i][1] = ((1.0f - p[i][1]) * 0.5f) * H;
+    }
+    float T = (float)vw.tile_size;
+    TileBox b;
+    if (!tile_box(proj, W, H, T, b)) return 0;
+    uint32_t count = 0;
+    for (uint32_t ty = b.y0; ty <= b.y1; ++ty) {
+        float xl, xr;
+        if (tile_row_extent(proj, ty, T, xl, xr)) {
+            uint32_t tx0, tx1;
+            row_tiles(xl, xr, T, b, tx0, tx1);
+            count += tx1 - tx0 + 1u;
+        }
+    }
+    return count;
+}
+
+// K1, batched: thread per tet, loop over views.  counts[t*V + v]; per-block
+// pair totals for the scan; per-view visible counts (reference's
+// indirect_args.instance_count, project_compute.wgsl:181).
+__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec, uint32_t M,
+                                                 const ViewParams* __restrict__ views, uint32_t V,
+                                                 uint32_t* __restrict__ counts, uint32_t* __restrict__ block_sums,
+                                                 uint32_t* __restrict__ vis_count) {
+    __shared__ uint32_t red[256 / kWave];
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = t < M;
+    V3 v[4];
+    float dens = 0.0f;
+    V3 g;
+    if (live) load_rec(rec, t, v, dens, g);
+    uint32_t sum = 0;
+    for (uint32_t vv = 0; vv < V; ++vv) {
+        bool vis = false;
+        uint32_t c = 0;
+        if (live) {
+            c = project_count(v, dens, views[vv], vis);
+            counts[(uint64_t)t * V + vv] = c;
+        }
+        sum += c;
+        uint64_t bal = __ballot(vis);
+        if (__lane_id() == 0 && bal) atomicAdd(&vis_count[vv], (uint32_t)__popcll(bal));
+    }
+    uint32_t total;
+    (void)block_excl_scan(sum, red, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of block sums in place (single workgroup of 1024 threads);
+// writes the grand total (number of pairs P) to *total.
+__global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums, uint32_t n, uint32_t* __restrict__ total) {
+    __shared__ uint32_t red[1024 / kWave];
+    uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    uint32_t lo = threadIdx.x * per;
+    uint32_t hi = min(lo + per, n);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += sums[i];
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(s, red, tot);
+    for (uint32_t i = lo; i < hi; ++i) {
+        uint32_t x = sums[i];
+        sums[i] = ex;
+        ex += x;
+    }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+// K7 (+ K6 sentinel padding): thread per tet, loop over views, pairs written
+// at the scanned offset.  Keys: (global_tile << 32) | ~bits(depth).
+__global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, const float4* __restrict__ circ, uint32_t M,
+                                               const ViewParams* __restrict__ views, uint32_t V,
+                                               const uint32_t* __restrict__ counts, const uint32_t* __restrict__ block_off,
+                                               uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t cap,
+                                               uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t red[256 / kWave];
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = t < M;
+    uint32_t sum = 0;
+    if (live)
+        for (uint32_t vv = 0; vv < V; ++vv) sum += counts[(uint64_t)t * V + vv];
+    uint32_t tot;
+    uint32_t base = block_excl_scan(sum, red, tot) + block_off[blockIdx.x];
+    if (!live || sum == 0) return;
+    V3 v[4];
+    float dens;
+    V3 g;
+    load_rec(rec, t, v, dens, g);
+    float4 cd = circ[t];
+    for (uint32_t vv = 0; vv < V; ++vv) {
+        uint32_t max_write = counts[(uint64_t)t * V + vv];
+        uint32_t start = base;
+        base += max_write;
+        if (max_write == 0) continue;
+        const ViewParams& vw = views[vv];
+        float W = vw.Wf, H = vw.Hf, T = (float)vw.tile_size;
+        float proj[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float c[4];
+            mul_vp(vw.vp, v[i], c);
+            float nx = c[0] / (c[3] + 1e-6f);
+            float ny = c[1] / (c[3] + 1e-6f);
+            proj[i][0] = ((nx + 1.0f) * 0.5f) * W;
+            proj[i][1] = ((1.0f - ny) * 0.5f) * H;
+        }
+        uint32_t local = 0;
+        TileBox b;
+        if (tile_box(proj, W, H, T, b)) {
+            V3 diff = sub3(mk3(cd.x, cd.y, cd.z), mk3(vw.cam[0], vw.cam[1], vw.cam[2]));
+            float depth = dot3(diff, diff) - cd.w;
+            uint32_t inv_depth = ~__float_as_uint(depth);
+            for (uint32_t ty = b.y0; ty <= b.y1 && local < max_write; ++ty) {
+                float xl, xr;
+                if (!tile_row_extent(proj, ty, T, xl, xr)) continue;
+                uint32_t tx0, tx1;
+                row_tiles(xl, xr, T, b, tx0, tx1);
+                for (uint32_t tx = tx0; tx <= tx1 && local < max_write; ++tx) {
+                    uint32_t gt = vw.tile_base + ty * vw.tiles_x + tx;
+                    uint32_t wi = start + local;
+                    if (wi < cap) {
+                        keys[wi] = ((uint64_t)gt << 32) | inv_depth;
+                        vals[wi] = t;
+                    } else {
+                        *overflow = 1u;
+                    }
+                    local++;
+                }
+            }
+        }
+        // Slots counted by K1 but not produced by the division-based
+        // re-projection keep the tile_fill sentinel (tile_gen_scan_compute.wgsl:202).
+        for (; local < max_write; ++local) {
+            uint32_t wi = start + local;
+            if (wi < cap) {
+                keys[wi] = ~0ull;
+                vals[wi] = 0u;
+            } else {
+                *overflow = 1u;
+            }
+        }
+    }
+}
+
+// K11: per-tile [start, end) in the sorted pair list, including the
+// reference's behaviour for trailing sentinels (tile_ranges_compute.wgsl:10-45).
+__global__ void k_ranges(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr, uint32_t cap,
+                         uint32_t* __restrict__ ranges) {
+    uint32_t n = min(*n_ptr, cap);
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t tile = (uint32_t)(keys[i] >> 32);
+        if (tile == 0xFFFFFFFFu) continue;
+        if (i == 0) {
+            ranges[2ull * tile] = i;
+        } else {
+            uint32_t prev = (uint32_t)(keys[i - 1] >> 32);
+            if (tile != prev) {
+                ranges[2ull * tile] = i;
+                if (prev != 0xFFFFFFFFu) ranges[2ull * prev + 1] = i;
+            }
+        }
+        if (i == n - 1) ranges[2ull * tile + 1] = i + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+void launch_prep_records(const float* verts, const uint32_t* idx, const float* dens, const float* grads, uint32_t M,
+                         float4* rec, hipStream_t s) {
+    if (!M) return;
+    hipLaunchKernelGGL(k_prep_records, dim3((M + 255) / 256), dim3(256), 0, s, verts, idx, dens, grads, M, rec);
+}
+void launch_copy_colors(const float* base, uint32_t M, float4* col, hipStream_t s) {
+    if (!M) return;
+    hipLaunchKernelGGL(k_copy_colors, dim3((M + 255) / 256), dim3(256), 0, s, base, M, col);
+}
+void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, n);
+}
+uint32_t project_blocks(uint32_t M) { return (M + 255) / 256; }
+void launch_project(const float4* rec, uint32_t M, const ViewParams* views, uint32_t V, uint32_t* counts,
+                    uint32_t* block_sums, uint32_t* vis_count, hipStream_t s) {
+    hipLaunchKernelGGL(k_project, dim3(project_blocks(M)), dim3(256), 0, s, rec, M, views, V, counts, block_sums, vis_count);
+}
+void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, hipStream_t s) {
+    hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total);
+}
+void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewParams* views, uint32_t V,
+                  const uint32_t* counts, const uint32_t* block_off, uint64_t* keys, uint32_t* vals, uint32_t cap,
+                  uint32_t* overflow, hipStream_t s) {
+    hipLaunchKernelGGL(k_pairs, dim3(project_blocks(M)), dim3(256), 0, s, rec, circ, M, views, V, counts, block_off,
+                       keys, vals, cap, overflow);
+}
+void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges, hipStream_t s) {
+    uint32_t blocks = (cap + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, s, keys, n_ptr, cap, ranges);
+}
+
+}  // namespace rmesh

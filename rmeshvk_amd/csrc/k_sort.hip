@@ -1,0 +1,192 @@
+// Stable LSD radix sort of (u64 key, u32 value) pairs, 8-bit digits over the
+// low `sorting_bits` bits of the key — the semantics of rmesh-sort's
+// DeviceRadixSort (crates/rmesh-sort/src/drs.rs:23-26,180-279): bits above
+// sorting_bits are ignored and equal masked keys keep their input order.
+//
+// MI355X design (not the reference's upsweep/scan/downsweep port):
+//  * fixed grid of G blocks, each owning one contiguous chunk of the input
+//    whose length is derived on the device from the pair count — the launch
+//    shape never depends on a host readback, so a whole step can be captured
+//    in a hipGraph, and no block ever waits on another (no look-back spins);
+//  * stable in-tile ranking with wave64 ballots: each wave matches the 8 digit
+//    bits of its 64 keys (8 ballots), ranks by popcount, and keeps a private
+//    256-entry running count in LDS — no barriers inside the ranking rounds;
+//  * the tile (4096 keys) is re-ordered by digit in LDS before the scatter,
+//    so the global writes are runs per digit instead of 8-byte scatters.
+#include "rmesh_common.hpp"
+#include "rmesh_kernels.hpp"
+
+namespace rmesh {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kRounds = 16;                         // 64-key rounds per wave per tile
+constexpr int kWaveKeys = kRounds * kWave;          // 1024
+constexpr int kTileKeys = kWaveKeys * kSortWaves;   // 4096
+
+__device__ __forceinline__ void chunk_of(uint32_t n, uint32_t G, uint32_t b, uint32_t& lo, uint32_t& hi) {
+    uint32_t per = (n + G - 1) / G;
+    per = (per + kTileKeys - 1) / kTileKeys * kTileKeys;
+    uint64_t l = (uint64_t)per * b;
+    lo = (uint32_t)min<uint64_t>(l, n);
+    hi = (uint32_t)min<uint64_t>(l + per, n);
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_upsweep(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr,
+                                                             uint32_t cap, uint32_t shift, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kSortWaves][256];
+    uint32_t G = gridDim.x, b = blockIdx.x;
+    uint32_t n = min(*n_ptr, cap);
+    uint32_t lo, hi;
+    chunk_of(n, G, b, lo, hi);
+    for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads) (&h[0][0])[i] = 0;
+    __syncthreads();
+    uint32_t w = threadIdx.x / kWave;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
+        uint32_t d = (uint32_t)(keys[i] >> shift) & 255u;
+        atomicAdd(&h[w][d], 1u);
+    }
+    __syncthreads();
+    uint32_t d = threadIdx.x;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kSortWaves; ++k) s += h[k][d];
+    hist[(uint64_t)d * G + b] = s;
+}
+
+// Exclusive scan over hist[256][G] in digit-major order (one workgroup).
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t total_n) {
+    __shared__ uint32_t red[1024 / kWave];
+    uint32_t per = (total_n + blockDim.x - 1) / blockDim.x;
+    uint32_t lo = threadIdx.x * per;
+    uint32_t hi = min(lo + per, total_n);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += hist[i];
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(s, red, tot);
+    for (uint32_t i = lo; i < hi; ++i) {
+        uint32_t x = hist[i];
+        hist[i] = ex;
+        ex += x;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                               const uint32_t* __restrict__ n_ptr, uint32_t cap, uint32_t shift,
+                                                               const uint32_t* __restrict__ hist) {
+    __shared__ uint64_t s_keys[kTileKeys];
+    __shared__ uint32_t s_vals[kTileKeys];
+    __shared__ uint32_t s_wcnt[kSortWaves][256];
+    __shared__ uint32_t s_loc[256];
+    __shared__ uint32_t s_glob[256];
+    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_red[kSortWaves];
+
+    uint32_t G = gridDim.x, b = blockIdx.x;
+    uint32_t n = min(*n_ptr, cap);
+    uint32_t lo, hi;
+    chunk_of(n, G, b, lo, hi);
+    const uint32_t tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
+    s_run[tid] = hist[(uint64_t)tid * G + b];
+    const uint64_t lt = lanemask_lt();
+
+    for (uint32_t base = lo; base < hi; base += kTileKeys) {
+        for (int i = tid; i < kSortWaves * 256; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+        __syncthreads();
+        uint64_t key[kRounds];
+        uint32_t val[kRounds];
+        uint32_t dr[kRounds];  // digit | rank << 9 ; digit 256 marks an empty slot
+        // Phase 1: per-wave stable ranking, 64 keys per round.
+#pragma unroll
+        for (int j = 0; j < kRounds; ++j) {
+            uint32_t i = base + w * kWaveKeys + j * kWave + lane;
+            bool valid = i < hi;
+            key[j] = valid ? kin[i] : 0ull;
+            val[j] = valid ? vin[i] : 0u;
+            uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                uint64_t bb = __ballot((d >> bit) & 1u);
+                peers &= ((d >> bit) & 1u) ? bb : ~bb;
+            }
+            uint32_t r = 0;
+            if (valid) r = s_wcnt[w][d] + (uint32_t)__popcll(peers & lt);
+            __builtin_amdgcn_wave_barrier();
+            bool leader = valid && ((peers & lt) == 0ull);
+            if (leader) s_wcnt[w][d] += (uint32_t)__popcll(peers);
+            __builtin_amdgcn_wave_barrier();
+            dr[j] = valid ? (d | (r << 9)) : 256u;
+        }
+        __syncthreads();
+        // Phase 2: per digit — wave prefixes, tile counts, local starts, global bases.
+        {
+            uint32_t d = tid;
+            uint32_t c[kSortWaves];
+            uint32_t tot = 0;
+#pragma unroll
+            for (int k = 0; k < kSortWaves; ++k) {
+                c[k] = s_wcnt[k][d];
+                s_wcnt[k][d] = tot;
+                tot += c[k];
+            }
+            uint32_t all;
+            uint32_t loc = block_excl_scan(tot, s_red, all);
+            s_loc[d] = loc;
+            s_glob[d] = s_run[d];
+            s_run[d] += tot;
+        }
+        __syncthreads();
+        // Phase 3: stage in digit order.
+#pragma unroll
+        for (int j = 0; j < kRounds; ++j) {
+            uint32_t d = dr[j] & 511u;
+            if (d < 256u) {
+                uint32_t lp = s_loc[d] + s_wcnt[w][d] + (dr[j] >> 9);
+                s_keys[lp] = key[j];
+                s_vals[lp] = val[j];
+            }
+        }
+        __syncthreads();
+        // Phase 4: coalesced scatter, runs per digit.
+        uint32_t tile_n = min((uint32_t)kTileKeys, hi - base);
+        for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
+            uint64_t k = s_keys[i];
+            uint32_t d = (uint32_t)(k >> shift) & 255u;
+            uint32_t gp = s_glob[d] + (i - s_loc[d]);
+            kout[gp] = k;
+            vout[gp] = s_vals[i];
+        }
+        __syncthreads();
+    }
+}
+
+uint32_t sort_grid(uint32_t cap) {
+    uint32_t g = (cap + kTileKeys - 1) / kTileKeys;
+    if (g > 2048) g = 2048;
+    if (g < 1) g = 1;
+    return g;
+}
+
+// Sorts the pairs in (ka, va); returns true when the result ended in (kb, vb).
+bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
+                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s) {
+    uint32_t G = sort_grid(cap);
+    bool in_b = false;
+    for (uint32_t shift = 0; shift < sorting_bits; shift += 8) {
+        const uint64_t* kin = in_b ? kb : ka;
+        const uint32_t* vin = in_b ? vb : va;
+        uint64_t* kout = in_b ? ka : kb;
+        uint32_t* vout = in_b ? va : vb;
+        hipLaunchKernelGGL(k_rs_upsweep, dim3(G), dim3(kSortThreads), 0, s, kin, n_ptr, cap, shift, hist);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, s, hist, 256u * G);
+        hipLaunchKernelGGL(k_rs_downsweep, dim3(G), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_ptr, cap, shift, hist);
+        in_b = !in_b;
+    }
+    return in_b;
+}
+
+size_t radix_hist_words(uint32_t cap) { return 256ull * sort_grid(cap); }
+
+}  // namespace rmesh

@@ -1,0 +1,876 @@
+// Host orchestration and the C ABI (include/rmesh_hip.h).
+//
+// Mirrors the call sequence of rmeshvk's RMeshRenderer (crates/rmesh-python/
+// src/lib.rs): forward_tiled = project -> scan -> pair generation -> sort ->
+// tile ranges -> compositing (lib.rs:1040-1204); backward reuses the sorted
+// lists of the last forward (the reference regenerates and re-sorts the same
+// pairs, lib.rs:1938-1958); train_step = forward + loss + backward + Adam with
+// the step counter incremented once per call (lib.rs:2405-2636).
+//
+// MI355X specifics: one HIP stream per handle (or the caller's, e.g. torch's
+// current stream); every per-step buffer is allocated once at a capacity
+// derived from the first run, so a batched step is a fixed launch sequence
+// that can be captured into a hipGraph and replayed; all clears inside that
+// sequence are fill kernels (no memset nodes); the pair count, the step
+// counter and the capacity-overflow flag stay in device memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rmesh_hip.h"
+#include "rmesh_common.hpp"
+#include "rmesh_kernels.hpp"
+
+using namespace rmesh;
+
+namespace {
+
+thread_local std::string g_err;
+
+rmesh_status fail(rmesh_status code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t _e = (expr);                                                                           \
+        if (_e != hipSuccess)                                                                             \
+            return fail(RMESH_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));             \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+}  // namespace
+
+struct rmesh_renderer {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t N = 0, M = 0, W = 0, H = 0, TS = 12, tiles_x = 0, tiles_y = 0, T = 0;
+    uint32_t max_views = 1;
+    bool has_sh = false;
+    uint32_t sh_deg = 0, nc = 1;
+    float min_t = 0.0f;
+    uint32_t step = 0;
+    float lr_sh = 0.0f, lr_base = 0.0f;
+
+    // parameters
+    float *verts = nullptr, *base = nullptr, *dens = nullptr, *grads = nullptr, *sh = nullptr, *circ = nullptr;
+    uint32_t* idx = nullptr;
+    // optimiser state (allocated on first train step)
+    float *m_v = nullptr, *v_v = nullptr, *m_d = nullptr, *v_d = nullptr, *m_g = nullptr, *v_g = nullptr;
+    float *m_sh = nullptr, *v_sh = nullptr, *m_b = nullptr, *v_b = nullptr;
+    // gradients
+    float *d_verts = nullptr, *d_dens = nullptr, *d_grads = nullptr, *d_colors = nullptr, *d_sh = nullptr;
+    // derived per step
+    float4* rec = nullptr;
+    float4* colors = nullptr;  // [max_views][M] (SH) or [M]
+    // binning
+    ViewParams* d_views = nullptr;
+    uint32_t *counts = nullptr, *block_sums = nullptr, *vis_count = nullptr, *n_pairs = nullptr, *overflow = nullptr;
+    uint32_t* d_step = nullptr;
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;
+    uint32_t *vals_a = nullptr, *vals_b = nullptr;
+    uint32_t cap = 0;
+    uint32_t* hist = nullptr;
+    uint32_t* ranges = nullptr;
+    bool sorted_in_b = false;
+    // images
+    float *rgba = nullptr, *aux = nullptr, *dl = nullptr, *gt = nullptr, *loss_partial = nullptr, *loss_out = nullptr;
+    bool aux_alloc = false;
+    // last forward
+    uint32_t last_views = 0;
+    bool have_forward = false;
+    // graph
+    bool graph_mode = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_views = 0, graph_loss = 0;
+    float graph_lrs[4] = {0, 0, 0, 0};
+    std::vector<ViewParams> hviews;
+
+    uint32_t gtiles(uint32_t V) const { return V * T; }
+    uint32_t sorting_bits(uint32_t V) const { return rmesh_sorting_bits(gtiles(V)); }
+};
+
+extern "C" uint32_t rmesh_sorting_bits(uint32_t num_tiles) {
+    uint32_t tile_bits = 1;
+    if (num_tiles > 1) {
+        uint32_t x = num_tiles - 1;
+        tile_bits = 0;
+        while (x) {
+            tile_bits++;
+            x >>= 1;
+        }
+    }
+    return (32u + tile_bits + 7u) & ~7u;
+}
+
+extern "C" const char* rmesh_last_error(void) { return g_err.c_str(); }
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        hipGetDevice(&prev);
+        if (prev != dev) hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+    }
+};
+
+void make_view(const rmesh_renderer* h, uint32_t v, const float* cam, const float* vp, const float* ci, ViewParams& o) {
+    std::memset(&o, 0, sizeof(o));
+    for (int i = 0; i < 16; ++i) o.vp[i] = vp[i];
+    for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) o.c2w[j * 3 + k] = ci[j * 4 + k];
+    for (int i = 0; i < 4; ++i) o.intr[i] = ci[12 + i];
+    for (int i = 0; i < 3; ++i) o.cam[i] = cam[i];
+    o.min_t = h->min_t;
+    o.Wf = (float)h->W;
+    o.Hf = (float)h->H;
+    o.width = h->W;
+    o.height = h->H;
+    o.tiles_x = h->tiles_x;
+    o.tiles_y = h->tiles_y;
+    o.tile_base = v * h->T;
+    o.tile_size = h->TS;
+}
+
+rmesh_status upload_views(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps, const float* c2ws) {
+    h->hviews.resize(V);
+    for (uint32_t v = 0; v < V; ++v) make_view(h, v, cams + 3 * v, vps + 16 * v, c2ws + 16 * v, h->hviews[v]);
+    HIP_TRY(hipMemcpyAsync(h->d_views, h->hviews.data(), sizeof(ViewParams) * V, hipMemcpyHostToDevice, h->stream));
+    return RMESH_OK;
+}
+
+rmesh_status ensure_capacity(rmesh_renderer* h, uint64_t need) {
+    if (need <= h->cap && h->keys_a && h->hist) return RMESH_OK;
+    if (need > 0xFFFFFFF0ull) return fail(RMESH_ERR_CAPACITY, "tile-tet pair count exceeds 2^32");
+    uint64_t c = std::max<uint64_t>(need + need / 4 + (1u << 16), 1u << 20);
+    if (c > 0xFFFFFFF0ull) c = 0xFFFFFFF0ull;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    hipFree(h->keys_a);
+    hipFree(h->keys_b);
+    hipFree(h->vals_a);
+    hipFree(h->vals_b);
+    hipFree(h->hist);
+    h->keys_a = h->keys_b = nullptr;
+    h->vals_a = h->vals_b = nullptr;
+    h->hist = nullptr;
+    h->cap = 0;
+    HIP_TRY(dalloc(&h->keys_a, c));
+    HIP_TRY(dalloc(&h->keys_b, c));
+    HIP_TRY(dalloc(&h->vals_a, c));
+    HIP_TRY(dalloc(&h->vals_b, c));
+    HIP_TRY(dalloc(&h->hist, radix_hist_words((uint32_t)c)));
+    h->cap = (uint32_t)c;
+    if (h->graph_exec) {
+        hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    if (h->graph) {
+        hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return RMESH_OK;
+}
+
+// Colours + projection + scan; in eager mode also reads the pair count back
+// to size the pair buffers (skipped under graph capture).
+rmesh_status stage_binning_front(rmesh_renderer* h, uint32_t V, bool capture) {
+    hipStream_t s = h->stream;
+    launch_prep_records(h->verts, h->idx, h->dens, h->grads, h->M, h->rec, s);
+    if (h->has_sh) launch_sh_colors(h->rec, h->sh, h->sh_deg, h->M, h->d_views, V, h->colors, s);
+    else launch_copy_colors(h->base, h->M, h->colors, s);
+    launch_fill_u32(h->vis_count, 0u, h->max_views, s);
+    launch_fill_u32(h->overflow, 0u, 1, s);
+    launch_project(h->rec, h->M, h->d_views, V, h->counts, h->block_sums, h->vis_count, s);
+    launch_spine_scan(h->block_sums, project_blocks(h->M), h->n_pairs, s);
+    if (!capture) {
+        uint32_t P = 0;
+        HIP_TRY(hipMemcpyAsync(&P, h->n_pairs, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        rmesh_status st = ensure_capacity(h, P);
+        if (st != RMESH_OK) return st;
+    }
+    return RMESH_OK;
+}
+
+rmesh_status stage_binning_back(rmesh_renderer* h, uint32_t V) {
+    hipStream_t s = h->stream;
+    launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, V, h->counts, h->block_sums,
+                 h->keys_a, h->vals_a, h->cap, h->overflow, s);
+    h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
+                                       h->sorting_bits(V), h->hist, s);
+    launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
+    launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
+    return RMESH_OK;
+}
+
+const uint32_t* sorted_vals(const rmesh_renderer* h) { return h->sorted_in_b ? h->vals_b : h->vals_a; }
+
+uint64_t color_stride(const rmesh_renderer* h) { return h->has_sh ? h->M : 0; }
+
+rmesh_status stage_forward(rmesh_renderer* h, uint32_t V, bool want_aux) {
+    FwdArgs a;
+    a.rec = h->rec;
+    a.colors = h->colors;
+    a.color_view_stride = color_stride(h);
+    a.sorted_vals = sorted_vals(h);
+    a.ranges = h->ranges;
+    a.views = h->d_views;
+    a.tiles_per_view = h->T;
+    a.num_gtiles = h->gtiles(V);
+    a.rgba = h->rgba;
+    a.aux = want_aux ? h->aux : nullptr;
+    if (!launch_forward(a, h->TS, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
+    return RMESH_OK;
+}
+
+void zero_grads(rmesh_renderer* h, uint32_t V) {
+    hipStream_t s = h->stream;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_verts), 0u, 3ull * h->N, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_dens), 0u, h->M, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_grads), 0u, 3ull * h->M, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_colors), 0u, 3ull * h->M * (h->has_sh ? V : 1), s);
+}
+
+rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, const float* gt, uint32_t loss_type) {
+    BwdArgs a;
+    a.rec = h->rec;
+    a.idx = h->idx;
+    a.colors = h->colors;
+    a.color_view_stride = color_stride(h);
+    a.sorted_vals = sorted_vals(h);
+    a.ranges = h->ranges;
+    a.views = h->d_views;
+    a.tiles_per_view = h->T;
+    a.num_gtiles = h->gtiles(V);
+    a.rgba = h->rgba;
+    a.dl_img = dl;
+    a.gt = gt;
+    a.loss_type = loss_type;
+    a.loss_partial = gt ? h->loss_partial : nullptr;
+    a.d_verts = h->d_verts;
+    a.d_dens = h->d_dens;
+    a.d_grads = h->d_grads;
+    a.d_colors = h->d_colors;
+    a.dcolor_view_stride = h->has_sh ? h->M : 0;
+    if (!launch_backward(a, h->TS, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
+    return RMESH_OK;
+}
+
+rmesh_status ensure_optimizer(rmesh_renderer* h) {
+    if (h->m_v) return RMESH_OK;
+    size_t nv = 3ull * h->N, nd = h->M, ng = 3ull * h->M;
+    HIP_TRY(dalloc(&h->m_v, nv));
+    HIP_TRY(dalloc(&h->v_v, nv));
+    HIP_TRY(dalloc(&h->m_d, nd));
+    HIP_TRY(dalloc(&h->v_d, nd));
+    HIP_TRY(dalloc(&h->m_g, ng));
+    HIP_TRY(dalloc(&h->v_g, ng));
+    HIP_TRY(dalloc(&h->m_b, ng));
+    HIP_TRY(dalloc(&h->v_b, ng));
+    HIP_TRY(hipMemset(h->m_v, 0, nv * 4));
+    HIP_TRY(hipMemset(h->v_v, 0, nv * 4));
+    HIP_TRY(hipMemset(h->m_d, 0, nd * 4));
+    HIP_TRY(hipMemset(h->v_d, 0, nd * 4));
+    HIP_TRY(hipMemset(h->m_g, 0, ng * 4));
+    HIP_TRY(hipMemset(h->v_g, 0, ng * 4));
+    HIP_TRY(hipMemset(h->m_b, 0, ng * 4));
+    HIP_TRY(hipMemset(h->v_b, 0, ng * 4));
+    if (h->has_sh) {
+        size_t ns = 3ull * h->M * h->nc;
+        HIP_TRY(dalloc(&h->m_sh, ns));
+        HIP_TRY(dalloc(&h->v_sh, ns));
+        HIP_TRY(hipMemset(h->m_sh, 0, ns * 4));
+        HIP_TRY(hipMemset(h->v_sh, 0, ns * 4));
+    }
+    return RMESH_OK;
+}
+
+// Adam stage (adam_compute.wgsl; groups and order of lib.rs:2590-2626):
+// vertices, densities, color_grads (+ base_colors / SH when enabled).
+rmesh_status stage_adam(rmesh_renderer* h, uint32_t V, float lr_v, float lr_d, float lr_g, float lr_sh) {
+    hipStream_t s = h->stream;
+    launch_step_inc(h->d_step, h->overflow, s);
+    AdamStep st{h->d_step, h->overflow, 0.9f, 0.999f, 1e-8f};
+    if (h->has_sh) {
+        ShBwdArgs a;
+        a.rec = h->rec;
+        a.idx = h->idx;
+        a.sh = h->sh;
+        a.deg = h->sh_deg;
+        a.M = h->M;
+        a.views = h->d_views;
+        a.V = V;
+        a.d_colors = h->d_colors;
+        a.d_sh = h->d_sh;
+        a.d_grads = h->d_grads;
+        a.d_verts = h->d_verts;
+        a.adam = lr_sh != 0.0f;
+        a.m_sh = h->m_sh;
+        a.v_sh = h->v_sh;
+        a.lr = lr_sh;
+        a.st = st;
+        launch_sh_backward(a, s);
+    }
+    AdamArgs aa;
+    std::memset(&aa, 0, sizeof(aa));
+    aa.st = st;
+    int n = 0;
+    aa.grp[n++] = AdamGroup{h->verts, h->d_verts, h->m_v, h->v_v, 3ull * h->N, lr_v};
+    aa.grp[n++] = AdamGroup{h->dens, h->d_dens, h->m_d, h->v_d, h->M, lr_d};
+    aa.grp[n++] = AdamGroup{h->grads, h->d_grads, h->m_g, h->v_g, 3ull * h->M, lr_g};
+    if (!h->has_sh && h->lr_base != 0.0f) aa.grp[n++] = AdamGroup{h->base, h->d_colors, h->m_b, h->v_b, 3ull * h->M, h->lr_base};
+    aa.ngroups = n;
+    launch_adam(aa, s);
+    launch_reduce_loss(h->loss_partial, h->T, V, h->loss_out, 1.0f / (float)(h->W * h->H), s);
+    return RMESH_OK;
+}
+
+rmesh_status record_train(rmesh_renderer* h, uint32_t V, const float* gt, uint32_t loss_type, float lr_v, float lr_d,
+                          float lr_g, float lr_sh, bool capture) {
+    rmesh_status st = stage_binning_front(h, V, capture);
+    if (st != RMESH_OK) return st;
+    zero_grads(h, V);
+    st = stage_binning_back(h, V);
+    if (st != RMESH_OK) return st;
+    st = stage_forward(h, V, false);
+    if (st != RMESH_OK) return st;
+    st = stage_backward(h, V, nullptr, gt, loss_type);
+    if (st != RMESH_OK) return st;
+    return stage_adam(h, V, lr_v, lr_d, lr_g, lr_sh);
+}
+
+void free_all(rmesh_renderer* h) {
+    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    void* ptrs[] = {h->verts, h->base, h->dens, h->grads, h->sh, h->circ, h->idx, h->m_v, h->v_v, h->m_d, h->v_d,
+                    h->m_g, h->v_g, h->m_sh, h->v_sh, h->m_b, h->v_b, h->d_verts, h->d_dens, h->d_grads, h->d_colors,
+                    h->d_sh, h->rec, h->colors, h->d_views, h->counts, h->block_sums, h->vis_count, h->n_pairs,
+                    h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a, h->vals_b, h->hist, h->ranges, h->rgba,
+                    h->aux, h->dl, h->gt, h->loss_partial, h->loss_out};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (h->own_stream) hipStreamDestroy(h->own_stream);
+}
+
+rmesh_status check_views(rmesh_renderer* h, uint32_t V) {
+    if (V == 0 || V > h->max_views)
+        return fail(RMESH_ERR_INVALID, "n_views must be in [1, max_views=" + std::to_string(h->max_views) + "]");
+    return RMESH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer** out) {
+    if (!d || !out) return fail(RMESH_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (!d->vertices || !d->indices || !d->base_colors || !d->densities || !d->color_grads || !d->circumdata)
+        return fail(RMESH_ERR_INVALID, "scene arrays must be non-null");
+    if (d->aux_dim != 0) return fail(RMESH_ERR_UNSUPPORTED, "custom aux channels (aux_dim > 0) are not supported");
+    if (d->width == 0 || d->height == 0) return fail(RMESH_ERR_INVALID, "width and height must be positive");
+    if (d->n_tets > 0xFFFFFFFFull || d->n_vertices > 0x55555555ull) return fail(RMESH_ERR_INVALID, "scene too large");
+    uint32_t ts = d->tile_size ? d->tile_size : 12;
+    if (ts != 8 && ts != 12 && ts != 16) return fail(RMESH_ERR_INVALID, "tile_size must be 8, 12 or 16");
+    if (d->sh_coeffs && d->sh_degree > 3) return fail(RMESH_ERR_INVALID, "sh_degree must be 0..3");
+    for (uint64_t i = 0; i < 4 * d->n_tets; ++i)
+        if (d->indices[i] >= d->n_vertices) return fail(RMESH_ERR_INVALID, "tet index out of range");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RMESH_ERR_DEVICE, "no HIP device available");
+    int dev = d->device_id;
+    if (dev < 0) hipGetDevice(&dev);
+    if (dev >= ndev) return fail(RMESH_ERR_INVALID, "device_id out of range");
+
+    rmesh_renderer* h = new rmesh_renderer();
+    h->device = dev;
+    DeviceGuard g(dev);
+    h->N = (uint32_t)d->n_vertices;
+    h->M = (uint32_t)d->n_tets;
+    h->W = d->width;
+    h->H = d->height;
+    h->TS = ts;
+    h->tiles_x = (h->W + ts - 1) / ts;
+    h->tiles_y = (h->H + ts - 1) / ts;
+    h->T = h->tiles_x * h->tiles_y;
+    h->max_views = d->max_views ? d->max_views : 1;
+    h->has_sh = d->sh_coeffs != nullptr;
+    h->sh_deg = h->has_sh ? d->sh_degree : 0;
+    h->nc = (h->sh_deg + 1) * (h->sh_deg + 1);
+    const uint32_t V = h->max_views;
+    auto bail = [&](rmesh_status s) {
+        free_all(h);
+        delete h;
+        return s;
+    };
+#define CREATE_TRY(expr)                                                                                 \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return bail(fail(RMESH_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e))); \
+    } while (0)
+    CREATE_TRY(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+    h->stream = h->own_stream;
+    size_t N3 = 3ull * h->N, M = h->M, M3 = 3ull * h->M;
+    CREATE_TRY(dalloc(&h->verts, N3));
+    CREATE_TRY(dalloc(&h->base, M3));
+    CREATE_TRY(dalloc(&h->dens, M));
+    CREATE_TRY(dalloc(&h->grads, M3));
+    CREATE_TRY(dalloc(&h->circ, 4 * M));
+    CREATE_TRY(dalloc(&h->idx, 4 * M));
+    CREATE_TRY(hipMemcpy(h->verts, d->vertices, N3 * 4, hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(h->base, d->base_colors, M3 * 4, hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(h->dens, d->densities, M * 4, hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(h->grads, d->color_grads, M3 * 4, hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(h->circ, d->circumdata, 4 * M * 4, hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(h->idx, d->indices, 4 * M * 4, hipMemcpyHostToDevice));
+    if (h->has_sh) {
+        size_t ns = M3 * h->nc;
+        CREATE_TRY(dalloc(&h->sh, ns));
+        CREATE_TRY(dalloc(&h->d_sh, ns));
+        CREATE_TRY(hipMemcpy(h->sh, d->sh_coeffs, ns * 4, hipMemcpyHostToDevice));
+    }
+    CREATE_TRY(dalloc(&h->d_verts, N3));
+    CREATE_TRY(dalloc(&h->d_dens, M));
+    CREATE_TRY(dalloc(&h->d_grads, M3));
+    CREATE_TRY(dalloc(&h->d_colors, M3 * (h->has_sh ? V : 1)));
+    CREATE_TRY(dalloc(&h->rec, 4 * M));
+    CREATE_TRY(dalloc(&h->colors, M * (h->has_sh ? V : 1)));
+    CREATE_TRY(dalloc(&h->d_views, V));
+    CREATE_TRY(dalloc(&h->counts, M * V));
+    CREATE_TRY(dalloc(&h->block_sums, project_blocks(h->M) + 1));
+    CREATE_TRY(dalloc(&h->vis_count, V));
+    CREATE_TRY(dalloc(&h->n_pairs, 1));
+    CREATE_TRY(dalloc(&h->overflow, 1));
+    CREATE_TRY(dalloc(&h->d_step, 1));
+    CREATE_TRY(hipMemset(h->d_step, 0, 4));
+    CREATE_TRY(dalloc(&h->ranges, 2ull * h->T * V));
+    size_t npx = (size_t)h->W * h->H;
+    CREATE_TRY(dalloc(&h->rgba, 4 * npx * V));
+    CREATE_TRY(dalloc(&h->dl, 4 * npx));
+    CREATE_TRY(dalloc(&h->gt, 3 * npx * V));
+    CREATE_TRY(dalloc(&h->loss_partial, (size_t)h->T * V));
+    CREATE_TRY(dalloc(&h->loss_out, V));
+#undef CREATE_TRY
+    *out = h;
+    return RMESH_OK;
+}
+
+extern "C" void rmesh_destroy(rmesh_renderer* h) {
+    if (!h) return;
+    DeviceGuard g(h->device);
+    hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+}
+
+extern "C" rmesh_status rmesh_set_min_t(rmesh_renderer* h, float min_t) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    h->min_t = min_t;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_set_stream(rmesh_renderer* h, void* stream) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    hipStream_t s = stream ? (hipStream_t)stream : h->own_stream;
+    if (s != h->stream) {
+        DeviceGuard g(h->device);
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->stream = s;
+    }
+    return RMESH_OK;
+}
+
+static rmesh_status forward_views(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps, const float* c2ws,
+                                  bool want_aux) {
+    rmesh_status st = upload_views(h, V, cams, vps, c2ws);
+    if (st != RMESH_OK) return st;
+    st = stage_binning_front(h, V, false);
+    if (st != RMESH_OK) return st;
+    st = stage_binning_back(h, V);
+    if (st != RMESH_OK) return st;
+    if (want_aux && !h->aux_alloc) {
+        HIP_TRY(dalloc(&h->aux, 8ull * h->W * h->H * h->max_views));
+        h->aux_alloc = true;
+    }
+    st = stage_forward(h, V, want_aux);
+    if (st != RMESH_OK) return st;
+    h->last_views = V;
+    h->have_forward = true;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_forward_tiled(rmesh_renderer* h, const float* cam, const float* vp, const float* ci,
+                                            float* out_rgba, float* out_aux) {
+    if (!h || !cam || !vp || !ci || !out_rgba) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    rmesh_status st = forward_views(h, 1, cam, vp, ci, out_aux != nullptr);
+    if (st != RMESH_OK) return st;
+    size_t npx = (size_t)h->W * h->H;
+    HIP_TRY(hipMemcpyAsync(out_rgba, h->rgba, npx * 16, hipMemcpyDeviceToHost, h->stream));
+    if (out_aux) HIP_TRY(hipMemcpyAsync(out_aux, h->aux, npx * 32, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpy(&ovf, h->overflow, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(RMESH_ERR_CAPACITY, "pair buffer overflow");
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_forward(rmesh_renderer* h, const float* cam, const float* vp, const float* ci,
+                                      float* out_rgba) {
+    return rmesh_forward_tiled(h, cam, vp, ci, out_rgba, nullptr);
+}
+
+extern "C" rmesh_status rmesh_read_instance_count(rmesh_renderer* h, uint32_t* out) {
+    if (!h || !out) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(out, h->vis_count, 4, hipMemcpyDeviceToHost));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_backward(rmesh_renderer* h, const float* dl, float* dV, float* dD, float* dB, float* dG) {
+    if (!h || !dl || !dV || !dD || !dB || !dG) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->have_forward || h->last_views != 1) return fail(RMESH_ERR_INVALID, "backward requires a preceding forward_tiled");
+    DeviceGuard g(h->device);
+    hipStream_t s = h->stream;
+    size_t npx = (size_t)h->W * h->H;
+    HIP_TRY(hipMemcpyAsync(h->dl, dl, npx * 16, hipMemcpyHostToDevice, s));
+    zero_grads(h, 1);
+    rmesh_status st = stage_backward(h, 1, h->dl, nullptr, 0);
+    if (st != RMESH_OK) return st;
+    // colour gradient returned as d_base_colors (the per-tet colours of this view)
+    HIP_TRY(hipMemcpyAsync(dB, h->d_colors, 3ull * h->M * 4, hipMemcpyDeviceToHost, s));
+    if (h->has_sh) {
+        ShBwdArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.rec = h->rec;
+        a.idx = h->idx;
+        a.sh = h->sh;
+        a.deg = h->sh_deg;
+        a.M = h->M;
+        a.views = h->d_views;
+        a.V = 1;
+        a.d_colors = h->d_colors;
+        a.d_sh = h->d_sh;
+        a.d_grads = h->d_grads;
+        a.d_verts = h->d_verts;
+        a.adam = false;
+        launch_sh_backward(a, s);
+    }
+    HIP_TRY(hipMemcpyAsync(dV, h->d_verts, 3ull * h->N * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(dD, h->d_dens, 1ull * h->M * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(dG, h->d_grads, 3ull * h->M * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_get_sh_grad(rmesh_renderer* h, float* d_sh) {
+    if (!h || !d_sh) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(d_sh, h->d_sh, 3ull * h->M * h->nc * 4, hipMemcpyDeviceToHost));
+    return RMESH_OK;
+}
+
+static rmesh_status finish_train(rmesh_renderer* h, uint32_t V, float* out_loss) {
+    std::vector<float> lv(V);
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpyAsync(lv.data(), h->loss_out, 4ull * V, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(&ovf, h->overflow, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (ovf) return fail(RMESH_ERR_CAPACITY, "pair buffer overflow");
+    h->step += 1;
+    h->last_views = V;
+    h->have_forward = true;
+    if (out_loss) {
+        double s = 0.0;
+        for (float x : lv) s += x;
+        *out_loss = (float)(s / V);
+    }
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_train_step(rmesh_renderer* h, const float* cam, const float* vp, const float* ci,
+                                         const float* gt, float lr_v, float lr_d, float lr_g, uint32_t loss_type,
+                                         float* out_loss) {
+    if (!h || !cam || !vp || !ci || !gt) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    rmesh_status st = ensure_optimizer(h);
+    if (st != RMESH_OK) return st;
+    st = upload_views(h, 1, cam, vp, ci);
+    if (st != RMESH_OK) return st;
+    HIP_TRY(hipMemcpyAsync(h->gt, gt, 12ull * h->W * h->H, hipMemcpyHostToDevice, h->stream));
+    st = record_train(h, 1, h->gt, loss_type, lr_v, lr_d, lr_g, h->lr_sh, false);
+    if (st != RMESH_OK) return st;
+    return finish_train(h, 1, out_loss);
+}
+
+extern "C" rmesh_status rmesh_set_extra_lrs(rmesh_renderer* h, float lr_sh, float lr_base) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    h->lr_sh = lr_sh;
+    h->lr_base = lr_base;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_update_params(rmesh_renderer* h, const float* v, const float* b, const float* d,
+                                            const float* gr) {
+    if (!h || !v || !b || !d || !gr) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(h->verts, v, 12ull * h->N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->base, b, 12ull * h->M, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->dens, d, 4ull * h->M, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->grads, gr, 12ull * h->M, hipMemcpyHostToDevice));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_get_params(rmesh_renderer* h, float* v, float* b, float* d, float* gr) {
+    if (!h || !v || !b || !d || !gr) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(v, h->verts, 12ull * h->N, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b, h->base, 12ull * h->M, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(d, h->dens, 4ull * h->M, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(gr, h->grads, 12ull * h->M, hipMemcpyDeviceToHost));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_update_sh(rmesh_renderer* h, const float* sh) {
+    if (!h || !sh) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(h->sh, sh, 12ull * h->M * h->nc, hipMemcpyHostToDevice));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_get_sh(rmesh_renderer* h, float* sh) {
+    if (!h || !sh) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(sh, h->sh, 12ull * h->M * h->nc, hipMemcpyDeviceToHost));
+    return RMESH_OK;
+}
+
+extern "C" uint32_t rmesh_get_step(const rmesh_renderer* h) { return h ? h->step : 0; }
+extern "C" uint32_t rmesh_get_width(const rmesh_renderer* h) { return h ? h->W : 0; }
+extern "C" uint32_t rmesh_get_height(const rmesh_renderer* h) { return h ? h->H : 0; }
+
+extern "C" rmesh_status rmesh_forward_batched(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps,
+                                              const float* c2ws, float* out_rgba) {
+    if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    rmesh_status st = check_views(h, V);
+    if (st != RMESH_OK) return st;
+    DeviceGuard g(h->device);
+    st = forward_views(h, V, cams, vps, c2ws, false);
+    if (st != RMESH_OK) return st;
+    if (out_rgba)
+        HIP_TRY(hipMemcpyAsync(out_rgba, h->rgba, 16ull * h->W * h->H * V, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_set_graph_mode(rmesh_renderer* h, int enable) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    h->graph_mode = enable != 0;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps,
+                                                 const float* c2ws, const float* gt_dev, uint32_t loss_type, float lr_v,
+                                                 float lr_d, float lr_g, float lr_sh, float* out_loss) {
+    if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    rmesh_status st = check_views(h, V);
+    if (st != RMESH_OK) return st;
+    DeviceGuard g(h->device);
+    st = ensure_optimizer(h);
+    if (st != RMESH_OK) return st;
+    st = upload_views(h, V, cams, vps, c2ws);
+    if (st != RMESH_OK) return st;
+    if (gt_dev && gt_dev != h->gt)
+        HIP_TRY(hipMemcpyAsync(h->gt, gt_dev, 12ull * h->W * h->H * V, hipMemcpyDeviceToDevice, h->stream));
+    float lrs[4] = {lr_v, lr_d, lr_g, lr_sh};
+    bool graph_ok = h->graph_exec && h->graph_views == V && h->graph_loss == loss_type &&
+                    std::memcmp(lrs, h->graph_lrs, sizeof(lrs)) == 0;
+    if (h->graph_mode && h->cap > 0 && !graph_ok) {
+        if (h->graph_exec) {
+            hipGraphExecDestroy(h->graph_exec);
+            h->graph_exec = nullptr;
+        }
+        if (h->graph) {
+            hipGraphDestroy(h->graph);
+            h->graph = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        rmesh_status cst = record_train(h, V, h->gt, loss_type, lr_v, lr_d, lr_g, lr_sh, true);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(h->stream, &graph);
+        if (cst != RMESH_OK) {
+            if (graph) hipGraphDestroy(graph);
+            return cst;
+        }
+        if (e != hipSuccess) return fail(RMESH_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        h->graph = graph;
+        HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+        h->graph_views = V;
+        h->graph_loss = loss_type;
+        std::memcpy(h->graph_lrs, lrs, sizeof(lrs));
+        graph_ok = true;
+    }
+    if (h->graph_mode && graph_ok) {
+        HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+        st = finish_train(h, V, out_loss);
+        if (st == RMESH_ERR_CAPACITY) {
+            // Parameters were left untouched on overflow: grow and redo eagerly.
+            uint32_t P = 0;
+            HIP_TRY(hipMemcpy(&P, h->n_pairs, 4, hipMemcpyDeviceToHost));
+            st = ensure_capacity(h, P);
+            if (st != RMESH_OK) return st;
+            st = record_train(h, V, h->gt, loss_type, lr_v, lr_d, lr_g, lr_sh, false);
+            if (st != RMESH_OK) return st;
+            return finish_train(h, V, out_loss);
+        }
+        return st;
+    }
+    st = record_train(h, V, h->gt, loss_type, lr_v, lr_d, lr_g, lr_sh, false);
+    if (st != RMESH_OK) return st;
+    return finish_train(h, V, out_loss);
+}
+
+extern "C" void* rmesh_device_buffer(rmesh_renderer* h, int which) {
+    if (!h) return nullptr;
+    switch (which) {
+        case RMESH_BUF_RGBA: return h->rgba;
+        case RMESH_BUF_GT: return h->gt;
+        case RMESH_BUF_VERTICES: return h->verts;
+        case RMESH_BUF_DENSITIES: return h->dens;
+        case RMESH_BUF_COLOR_GRADS: return h->grads;
+        case RMESH_BUF_BASE_COLORS: return h->base;
+        case RMESH_BUF_SH: return h->sh;
+        case RMESH_BUF_D_VERTICES: return h->d_verts;
+        case RMESH_BUF_D_DENSITIES: return h->d_dens;
+        case RMESH_BUF_D_COLOR_GRADS: return h->d_grads;
+        default: return nullptr;
+    }
+}
+
+extern "C" rmesh_status rmesh_synchronize(rmesh_renderer* h) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_debug_tiles_touched(rmesh_renderer* h, uint32_t view, uint32_t* out) {
+    if (!h || !out) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->have_forward || view >= h->last_views) return fail(RMESH_ERR_INVALID, "no such view in the last forward");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<uint32_t> all((size_t)h->M * h->last_views);
+    HIP_TRY(hipMemcpy(all.data(), h->counts, all.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < h->M; ++t) out[t] = all[(size_t)t * h->last_views + view];
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_debug_pairs(rmesh_renderer* h, const float* cam, const float* vp, const float* ci,
+                                          uint64_t* keys, uint32_t* vals, uint64_t cap, uint64_t* n_pairs) {
+    if (!h || !cam || !vp || !ci || !n_pairs) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    rmesh_status st = upload_views(h, 1, cam, vp, ci);
+    if (st != RMESH_OK) return st;
+    st = stage_binning_front(h, 1, false);
+    if (st != RMESH_OK) return st;
+    launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, 1, h->counts, h->block_sums,
+                 h->keys_a, h->vals_a, h->cap, h->overflow, h->stream);
+    uint32_t P = 0;
+    HIP_TRY(hipMemcpyAsync(&P, h->n_pairs, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *n_pairs = P;
+    uint64_t n = std::min<uint64_t>(P, cap);
+    if (keys && n) HIP_TRY(hipMemcpy(keys, h->keys_a, n * 8, hipMemcpyDeviceToHost));
+    if (vals && n) HIP_TRY(hipMemcpy(vals, h->vals_a, n * 4, hipMemcpyDeviceToHost));
+    h->have_forward = false;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_debug_sorted(rmesh_renderer* h, uint64_t* keys, uint32_t* vals, uint64_t cap,
+                                           uint64_t* n_pairs, uint32_t* ranges, uint64_t ranges_cap) {
+    if (!h || !n_pairs) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->have_forward) return fail(RMESH_ERR_INVALID, "no forward has run");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    uint32_t P = 0;
+    HIP_TRY(hipMemcpy(&P, h->n_pairs, 4, hipMemcpyDeviceToHost));
+    *n_pairs = P;
+    uint64_t n = std::min<uint64_t>(P, cap);
+    if (keys && n) HIP_TRY(hipMemcpy(keys, h->sorted_in_b ? h->keys_b : h->keys_a, n * 8, hipMemcpyDeviceToHost));
+    if (vals && n) HIP_TRY(hipMemcpy(vals, sorted_vals(h), n * 4, hipMemcpyDeviceToHost));
+    uint64_t nr = std::min<uint64_t>(ranges_cap, 2ull * h->gtiles(h->last_views));
+    if (ranges && nr) HIP_TRY(hipMemcpy(ranges, h->ranges, nr * 4, hipMemcpyDeviceToHost));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bits, int32_t device_id) {
+    if ((!keys || !vals) && n) return fail(RMESH_ERR_INVALID, "null argument");
+    if (bits % 8 != 0 || bits > 64) return fail(RMESH_ERR_INVALID, "sorting_bits must be a multiple of 8 up to 64");
+    if (n > 0xFFFFFFF0ull) return fail(RMESH_ERR_INVALID, "too many pairs");
+    if (n == 0) return RMESH_OK;
+    int dev = device_id;
+    if (dev < 0) hipGetDevice(&dev);
+    DeviceGuard g(dev);
+    uint32_t cap = (uint32_t)n;
+    uint64_t *ka = nullptr, *kb = nullptr;
+    uint32_t *va = nullptr, *vb = nullptr, *hist = nullptr, *np = nullptr;
+    hipStream_t s = nullptr;
+    rmesh_status st = RMESH_OK;
+    do {
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+            dalloc(&ka, cap) != hipSuccess || dalloc(&kb, cap) != hipSuccess || dalloc(&va, cap) != hipSuccess ||
+            dalloc(&vb, cap) != hipSuccess || dalloc(&hist, radix_hist_words(cap)) != hipSuccess ||
+            dalloc(&np, 1) != hipSuccess) {
+            st = fail(RMESH_ERR_DEVICE, "allocation failed");
+            break;
+        }
+        hipMemcpy(ka, keys, n * 8, hipMemcpyHostToDevice);
+        hipMemcpy(va, vals, n * 4, hipMemcpyHostToDevice);
+        hipMemcpy(np, &cap, 4, hipMemcpyHostToDevice);
+        bool in_b = launch_radix_sort(ka, va, kb, vb, np, cap, bits, hist, s);
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            st = fail(RMESH_ERR_DEVICE, std::string("sort: ") + hipGetErrorString(e));
+            break;
+        }
+        hipMemcpy(keys, in_b ? kb : ka, n * 8, hipMemcpyDeviceToHost);
+        hipMemcpy(vals, in_b ? vb : va, n * 4, hipMemcpyDeviceToHost);
+    } while (0);
+    hipFree(ka);
+    hipFree(kb);
+    hipFree(va);
+    hipFree(vb);
+    hipFree(hist);
+    hipFree(np);
+    if (s) hipStreamDestroy(s);
+    return st;
+}

@@ -1,0 +1,119 @@
+// Host-side launchers of the rmesh HIP kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace rmesh {
+
+struct ViewParams;
+
+// binning (k_binning.hip)
+void launch_prep_records(const float* verts, const uint32_t* idx, const float* dens, const float* grads, uint32_t M,
+                         float4* rec, hipStream_t s);
+void launch_copy_colors(const float* base, uint32_t M, float4* col, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+uint32_t project_blocks(uint32_t M);
+void launch_project(const float4* rec, uint32_t M, const ViewParams* views, uint32_t V, uint32_t* counts,
+                    uint32_t* block_sums, uint32_t* vis_count, hipStream_t s);
+void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, hipStream_t s);
+void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewParams* views, uint32_t V,
+                  const uint32_t* counts, const uint32_t* block_off, uint64_t* keys, uint32_t* vals, uint32_t cap,
+                  uint32_t* overflow, hipStream_t s);
+void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges, hipStream_t s);
+
+// sort (k_sort.hip)
+bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
+                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s);
+size_t radix_hist_words(uint32_t cap);
+
+// compositing (k_composite.hip)
+struct FwdArgs {
+    const float4* rec;
+    const float4* colors;
+    uint64_t color_view_stride;  // in float4 units; 0 = view-independent colours
+    const uint32_t* sorted_vals;
+    const uint32_t* ranges;
+    const ViewParams* views;
+    uint32_t tiles_per_view;
+    uint32_t num_gtiles;
+    float* rgba;
+    float* aux;  // nullable: [V][H][W][8]
+};
+bool launch_forward(const FwdArgs& a, uint32_t tile_size, hipStream_t s);
+
+struct BwdArgs {
+    const float4* rec;
+    const uint32_t* idx;
+    const float4* colors;
+    uint64_t color_view_stride;
+    const uint32_t* sorted_vals;
+    const uint32_t* ranges;
+    const ViewParams* views;
+    uint32_t tiles_per_view;
+    uint32_t num_gtiles;
+    const float* rgba;
+    const float* dl_img;      // used when gt == nullptr
+    const float* gt;          // [V][H][W][3] or nullptr
+    uint32_t loss_type;
+    float* loss_partial;      // [num_gtiles] or nullptr
+    float* d_verts;
+    float* d_dens;
+    float* d_grads;
+    float* d_colors;          // [V][M*3] (SH) or [M*3] (stride 0)
+    uint64_t dcolor_view_stride;  // in tets
+};
+bool launch_backward(const BwdArgs& a, uint32_t tile_size, hipStream_t s);
+
+// colours, SH and optimiser (k_optim.hip)
+void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t M, const ViewParams* views, uint32_t V,
+                      float4* colors, hipStream_t s);
+
+// The step counter and the overflow flag live in device memory so a captured
+// graph reads the current values at replay time.
+struct AdamStep {
+    const uint32_t* step;   // device: Adam step after this update (>= 1)
+    const uint32_t* skip;   // device: non-zero -> leave parameters untouched
+    float beta1, beta2, eps;
+};
+struct ShBwdArgs {
+    const float4* rec;
+    const uint32_t* idx;
+    float* sh;               // parameters (updated in place when adam)
+    uint32_t deg;
+    uint32_t M;
+    const ViewParams* views;
+    uint32_t V;
+    const float* d_colors;   // [V][M*3]
+    float* d_sh;             // output gradient (when !adam)
+    float* d_grads;          // += (per tet, plain read-modify-write)
+    float* d_verts;          // += (atomic)
+    bool adam;
+    float* m_sh;
+    float* v_sh;
+    float lr;
+    AdamStep st;
+};
+void launch_sh_backward(const ShBwdArgs& a, hipStream_t s);
+
+struct AdamGroup {
+    float* p;
+    const float* g;
+    float* m;
+    float* v;
+    uint64_t n;
+    float lr;
+};
+constexpr int kMaxAdamGroups = 6;
+struct AdamArgs {
+    AdamGroup grp[kMaxAdamGroups];
+    int ngroups;
+    AdamStep st;
+};
+void launch_adam(const AdamArgs& a, hipStream_t s);
+void launch_step_inc(uint32_t* step, const uint32_t* skip, hipStream_t s);
+void launch_reduce_loss(const float* partial, uint32_t tiles_per_view, uint32_t V, float* out, float inv_npx,
+                        hipStream_t s);
+
+}  // namespace rmesh

@@ -1,0 +1,295 @@
+"""``RMeshRenderer`` — MI355X drop-in for rmeshvk's PyO3 class.
+
+Mirrors ``rmesh_wgpu._native.RMeshRenderer`` (crates/rmesh-python/src/lib.rs:
+138-2705): same constructor arguments, method names, argument meaning, return
+types and error classes.  Every call goes through the C ABI of
+librmesh_hip.so (include/rmesh_hip.h); nothing here computes on the CPU.
+
+Extensions (keyword-only or new methods, positional signatures unchanged):
+``tile_size``, ``device_id``, ``sh_coeffs``/``sh_degree`` (SH colour model),
+``max_views`` with ``forward_batched`` / ``train_step_batched`` (many views
+per launch, optional hipGraph replay), ``lr_sh``/``lr_base`` in
+``train_step``, and read-only inspection of the binning state.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+_OUT_OF_SCOPE = (
+    "forward_interval_tiled", "backward_interval_tiled", "forward_raytrace", "backward_raytrace",
+    "backward_hw", "forward_error", "locate_tets", "read_aux", "set_vertex_normals",
+    "get_vertex_normals", "set_aux_data", "get_aux_data",
+)
+
+
+def _f32(name, a, n=None):
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"{name}: expected a numpy.ndarray of float32")
+    if a.dtype != np.float32:
+        raise TypeError(f"{name}: expected dtype float32, got {a.dtype}")
+    if not a.flags.c_contiguous:
+        raise ValueError(f"{name}: array is not contiguous")
+    if n is not None and a.size != n:
+        raise ValueError(f"{name}: expected {n} elements, got {a.size}")
+    return a
+
+
+def _u32(name, a, n=None):
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"{name}: expected a numpy.ndarray of uint32")
+    if a.dtype != np.uint32:
+        raise TypeError(f"{name}: expected dtype uint32, got {a.dtype}")
+    if not a.flags.c_contiguous:
+        raise ValueError(f"{name}: array is not contiguous")
+    if n is not None and a.size != n:
+        raise ValueError(f"{name}: expected {n} elements, got {a.size}")
+    return a
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class RMeshRenderer:
+    """Differentiable radiance-mesh renderer on one MI355X (see module doc)."""
+
+    def __init__(self, vertices, indices, base_colors, densities, color_grads, circumdata, width, height,
+                 aux_dim=0, *, tile_size=12, device_id=-1, sh_coeffs=None, sh_degree=0, max_views=1):
+        lib = _lib.load()
+        _f32("vertices", vertices)
+        _u32("indices", indices)
+        if vertices.size % 3 or indices.size % 4:
+            raise ValueError("vertices must be [N*3] and indices [M*4]")
+        n, m = vertices.size // 3, indices.size // 4
+        _f32("base_colors", base_colors, 3 * m)
+        _f32("densities", densities, m)
+        _f32("color_grads", color_grads, 3 * m)
+        _f32("circumdata", circumdata, 4 * m)
+        if sh_coeffs is not None:
+            _f32("sh_coeffs", sh_coeffs, 3 * m * (sh_degree + 1) ** 2)
+        self._keep = (vertices, indices, base_colors, densities, color_grads, circumdata, sh_coeffs)
+        d = _lib.SceneDesc(
+            vertices=vertices.ctypes.data, n_vertices=n, indices=indices.ctypes.data, n_tets=m,
+            base_colors=base_colors.ctypes.data, densities=densities.ctypes.data,
+            color_grads=color_grads.ctypes.data, circumdata=circumdata.ctypes.data,
+            width=int(width), height=int(height), aux_dim=int(aux_dim), tile_size=int(tile_size),
+            device_id=int(device_id), sh_coeffs=None if sh_coeffs is None else sh_coeffs.ctypes.data,
+            sh_degree=int(sh_degree), max_views=int(max_views))
+        h = ctypes.c_void_p()
+        _lib.check(lib.rmesh_create(ctypes.byref(d), ctypes.byref(h)))
+        self._keep = None
+        self._h = h
+        self._lib = lib
+        self._n, self._m = n, m
+        self._w, self._hgt = int(width), int(height)
+        self._sh_degree = int(sh_degree) if sh_coeffs is not None else None
+        self._max_views = int(max_views)
+        self.tile_size = int(tile_size)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.rmesh_destroy(h)
+            self._h = None
+
+    # ---- getters (lib.rs:2689-2704) ----
+    @property
+    def step(self) -> int:
+        return int(self._lib.rmesh_get_step(self._h))
+
+    @property
+    def width(self) -> int:
+        return int(self._lib.rmesh_get_width(self._h))
+
+    @property
+    def height(self) -> int:
+        return int(self._lib.rmesh_get_height(self._h))
+
+    @property
+    def num_tets(self) -> int:
+        return self._m
+
+    @property
+    def num_vertices(self) -> int:
+        return self._n
+
+    def set_min_t(self, min_t):
+        _lib.check(self._lib.rmesh_set_min_t(self._h, float(min_t)))
+
+    def set_stream(self, stream_handle):
+        """Run on an external HIP stream (an integer handle, e.g. torch's cuda_stream)."""
+        _lib.check(self._lib.rmesh_set_stream(self._h, ctypes.c_void_p(stream_handle or 0)))
+
+    # ---- forward (lib.rs:932-1204) ----
+    def _cam(self, cam_pos, vp, c2w_intrinsics):
+        return _f32("cam_pos", cam_pos, 3), _f32("vp", vp, 16), _f32("c2w_intrinsics", c2w_intrinsics, 16)
+
+    def forward_tiled(self, cam_pos, vp, c2w_intrinsics, render_aux=False):
+        c, v, ci = self._cam(cam_pos, vp, c2w_intrinsics)
+        rgba = np.empty((self._hgt, self._w, 4), dtype=np.float32)
+        aux = np.empty((self._hgt, self._w, 8), dtype=np.float32) if render_aux else None
+        _lib.check(self._lib.rmesh_forward_tiled(self._h, _p(c), _p(v), _p(ci), _p(rgba), _p(aux)))
+        return (rgba, aux) if render_aux else rgba
+
+    def forward(self, cam_pos, vp, c2w_intrinsics):
+        c, v, ci = self._cam(cam_pos, vp, c2w_intrinsics)
+        rgba = np.empty((self._hgt, self._w, 4), dtype=np.float32)
+        _lib.check(self._lib.rmesh_forward(self._h, _p(c), _p(v), _p(ci), _p(rgba)))
+        return rgba
+
+    def read_instance_count(self) -> int:
+        out = ctypes.c_uint32()
+        _lib.check(self._lib.rmesh_read_instance_count(self._h, ctypes.byref(out)))
+        return int(out.value)
+
+    # ---- backward (lib.rs:1894-2031) ----
+    def backward(self, dl_d_image):
+        _f32("dl_d_image", dl_d_image, self._hgt * self._w * 4)
+        d_v = np.empty(3 * self._n, dtype=np.float32)
+        d_d = np.empty(self._m, dtype=np.float32)
+        d_b = np.empty(3 * self._m, dtype=np.float32)
+        d_g = np.empty(3 * self._m, dtype=np.float32)
+        _lib.check(self._lib.rmesh_backward(self._h, _p(dl_d_image), _p(d_v), _p(d_d), _p(d_b), _p(d_g)))
+        out = {"d_vertices": d_v, "d_densities": d_d, "d_base_colors": d_b, "d_color_grads": d_g}
+        if self._sh_degree is not None:
+            d_sh = np.empty(3 * self._m * (self._sh_degree + 1) ** 2, dtype=np.float32)
+            _lib.check(self._lib.rmesh_get_sh_grad(self._h, _p(d_sh)))
+            out["d_sh"] = d_sh
+        return out
+
+    # ---- training (lib.rs:2405-2636) ----
+    def train_step(self, cam_pos, vp, c2w_intrinsics, gt_image, lr_verts, lr_dens, lr_grads, loss_type,
+                   *, lr_sh=0.0, lr_base=0.0):
+        c, v, ci = self._cam(cam_pos, vp, c2w_intrinsics)
+        _f32("gt_image", gt_image, self._hgt * self._w * 3)
+        _lib.check(self._lib.rmesh_set_extra_lrs(self._h, float(lr_sh), float(lr_base)))
+        loss = ctypes.c_float()
+        _lib.check(self._lib.rmesh_train_step(self._h, _p(c), _p(v), _p(ci), _p(gt_image), float(lr_verts),
+                                              float(lr_dens), float(lr_grads), int(loss_type), ctypes.byref(loss)))
+        return float(loss.value)
+
+    # ---- parameters (lib.rs:2641-2686) ----
+    def update_params(self, vertices, base_colors, densities, color_grads):
+        _f32("vertices", vertices, 3 * self._n)
+        _f32("base_colors", base_colors, 3 * self._m)
+        _f32("densities", densities, self._m)
+        _f32("color_grads", color_grads, 3 * self._m)
+        _lib.check(self._lib.rmesh_update_params(self._h, _p(vertices), _p(base_colors), _p(densities),
+                                                 _p(color_grads)))
+
+    def get_params(self):
+        v = np.empty(3 * self._n, dtype=np.float32)
+        b = np.empty(3 * self._m, dtype=np.float32)
+        d = np.empty(self._m, dtype=np.float32)
+        g = np.empty(3 * self._m, dtype=np.float32)
+        _lib.check(self._lib.rmesh_get_params(self._h, _p(v), _p(b), _p(d), _p(g)))
+        out = {"vertices": v, "base_colors": b, "densities": d, "color_grads": g}
+        if self._sh_degree is not None:
+            out["sh_coeffs"] = self.get_sh()
+        return out
+
+    def update_sh(self, sh_coeffs):
+        _f32("sh_coeffs", sh_coeffs, 3 * self._m * (self._sh_degree + 1) ** 2 if self._sh_degree is not None else None)
+        _lib.check(self._lib.rmesh_update_sh(self._h, _p(sh_coeffs)))
+
+    def get_sh(self):
+        if self._sh_degree is None:
+            raise ValueError("renderer has no SH coefficients")
+        out = np.empty(3 * self._m * (self._sh_degree + 1) ** 2, dtype=np.float32)
+        _lib.check(self._lib.rmesh_get_sh(self._h, _p(out)))
+        return out
+
+    # ---- batched extension ----
+    @staticmethod
+    def _stack_cams(cams):
+        cp = np.ascontiguousarray(np.stack([c[0] for c in cams]).astype(np.float32))
+        vp = np.ascontiguousarray(np.stack([c[1] for c in cams]).astype(np.float32))
+        ci = np.ascontiguousarray(np.stack([c[2] for c in cams]).astype(np.float32))
+        return cp, vp, ci
+
+    def forward_batched(self, cams, copy_out=True):
+        """Render a list of ``(cam_pos, vp, c2w_intrinsics)``; returns [V, H, W, 4] (or None)."""
+        cp, vp, ci = self._stack_cams(cams)
+        out = np.empty((len(cams), self._hgt, self._w, 4), dtype=np.float32) if copy_out else None
+        _lib.check(self._lib.rmesh_forward_batched(self._h, len(cams), _p(cp), _p(vp), _p(ci), _p(out)))
+        return out
+
+    def train_step_batched(self, cams, loss_type, lr_verts, lr_dens, lr_grads, lr_sh=0.0, gt_device_ptr=None):
+        """One step over all views.  Targets come from ``gt_device_ptr`` (device
+        [V][H][W][3]) or, when None, from the renderer's own target buffer
+        (``device_buffer(BUF_GT)`` / ``set_targets``)."""
+        cp, vp, ci = self._stack_cams(cams)
+        loss = ctypes.c_float()
+        _lib.check(self._lib.rmesh_train_step_batched(
+            self._h, len(cams), _p(cp), _p(vp), _p(ci), ctypes.c_void_p(gt_device_ptr or 0), int(loss_type),
+            float(lr_verts), float(lr_dens), float(lr_grads), float(lr_sh), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def set_graph_mode(self, enable: bool):
+        _lib.check(self._lib.rmesh_set_graph_mode(self._h, 1 if enable else 0))
+
+    def device_buffer(self, which: int) -> int:
+        return int(self._lib.rmesh_device_buffer(self._h, which) or 0)
+
+    def synchronize(self):
+        _lib.check(self._lib.rmesh_synchronize(self._h))
+
+    # ---- inspection of the binning state ----
+    def debug_tiles_touched(self, view=0):
+        out = np.empty(self._m, dtype=np.uint32)
+        _lib.check(self._lib.rmesh_debug_tiles_touched(self._h, int(view), _p(out)))
+        return out
+
+    def debug_pairs(self, cam_pos, vp, c2w_intrinsics, cap=None):
+        c, v, ci = self._cam(cam_pos, vp, c2w_intrinsics)
+        n = ctypes.c_uint64()
+        _lib.check(self._lib.rmesh_debug_pairs(self._h, _p(c), _p(v), _p(ci), None, None, 0, ctypes.byref(n)))
+        cap = int(n.value) if cap is None else cap
+        keys = np.empty(max(cap, 1), dtype=np.uint64)
+        vals = np.empty(max(cap, 1), dtype=np.uint32)
+        _lib.check(self._lib.rmesh_debug_pairs(self._h, _p(c), _p(v), _p(ci), _p(keys), _p(vals), cap,
+                                               ctypes.byref(n)))
+        k = min(cap, int(n.value))
+        return keys[:k], vals[:k], int(n.value)
+
+    def debug_sorted(self, views=1):
+        n = ctypes.c_uint64()
+        _lib.check(self._lib.rmesh_debug_sorted(self._h, None, None, 0, ctypes.byref(n), None, 0))
+        P = int(n.value)
+        keys = np.empty(max(P, 1), dtype=np.uint64)
+        vals = np.empty(max(P, 1), dtype=np.uint32)
+        tx = (self._w + self.tile_size - 1) // self.tile_size
+        ty = (self._hgt + self.tile_size - 1) // self.tile_size
+        ranges = np.zeros(2 * tx * ty * views, dtype=np.uint32)
+        _lib.check(self._lib.rmesh_debug_sorted(self._h, _p(keys), _p(vals), P, ctypes.byref(n), _p(ranges),
+                                                ranges.size))
+        return keys[:P], vals[:P], ranges
+
+
+def _not_implemented(name):
+    def f(self, *a, **k):
+        raise NotImplementedError(f"{name} is outside the tiled train-step path served by this renderer")
+    f.__name__ = name
+    return f
+
+
+for _n in _OUT_OF_SCOPE:
+    setattr(RMeshRenderer, _n, _not_implemented(_n))
+
+
+def sort_pairs(keys: np.ndarray, vals: np.ndarray, sorting_bits: int, device_id: int = -1):
+    """GPU radix sort of (u64 key, u32 value) pairs (the pipeline's sort)."""
+    lib = _lib.load()
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    v = np.array(vals, dtype=np.uint32, copy=True)
+    _lib.check(lib.rmesh_sort_pairs(_p(k), _p(v), k.size, int(sorting_bits), int(device_id)))
+    return k, v
+
+
+def sorting_bits(num_tiles: int) -> int:
+    return int(_lib.load().rmesh_sorting_bits(int(num_tiles)))
