@@ -142,6 +142,17 @@ rmesh_status rmesh_set_graph_mode(rmesh_renderer* h, int enable);
 #define RMESH_BUF_D_COLOR_GRADS 9
 /* Device pointer of an internal buffer (zero-copy for torch callers). */
 void* rmesh_device_buffer(rmesh_renderer* h, int which);
+/* Upload n_views target images (host [n][H][W][3]) into RMESH_BUF_GT. */
+rmesh_status rmesh_set_targets(rmesh_renderer* h, uint32_t n_views, const float* gt);
+/* Per-stage timing of training steps: when enabled, HIP events are recorded
+ * between the stages of every train step (also inside a captured graph);
+ * rmesh_stage_times returns the 7 stage durations of the last step in ms:
+ * colours+projection+scan, pair generation, sort, ranges, forward, adjoint,
+ * SH adjoint + Adam + loss. */
+rmesh_status rmesh_set_timing(rmesh_renderer* h, int enable);
+rmesh_status rmesh_stage_times(rmesh_renderer* h, float* ms7);
+/* Number of tile-tet pairs of the last step (all views). */
+rmesh_status rmesh_pair_count(rmesh_renderer* h, uint64_t* out);
 /* Synchronise the renderer's stream. */
 rmesh_status rmesh_synchronize(rmesh_renderer* h);
 

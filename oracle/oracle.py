@@ -74,12 +74,12 @@ def lib():
             f.argtypes = [rp, _u32p, rp, _u32, _f32p, _f32p, _f32p, _u32, _u32, _u32, _u32p, _u64p, _u32p, _u64]
             f.restype = _u64
             getattr(L, "orc_forward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p, _f32p,
-                                                         _u32, _u32, _u32, _f32, _u32, _u32, rp, _vp, _vp]
+                                                         _u32, _u32, _u32, _f32, _u32, _u32, _u32, rp, _vp, _vp]
             f = getattr(L, "orc_loss" + sfx)
             f.argtypes = [rp, rp, _u32, _u32, _u32, rp]
             f.restype = ctypes.c_double
             getattr(L, "orc_backward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p, _f32p,
-                                                          _u32, _u32, _u32, rp, rp, _u32, _u32, rp, rp, rp, rp]
+                                                          _u32, _u32, _u32, rp, rp, _u32, _u32, _u32, rp, rp, rp, rp]
             getattr(L, "orc_sh_colors" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp]
             getattr(L, "orc_sh_backward" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp, rp, rp, rp]
         L.orc_sorting_bits.argtypes = [_u32]
@@ -151,7 +151,7 @@ def ranges(sorted_keys, num_tiles):
 
 
 def forward(scene, cam, w, h, sorted_vals, rng_, ts=12, min_t=0.0, colors=None, dtype=np.float32,
-            tiles=None, want_aux=False, want_stats=False):
+            tiles=None, want_aux=False, want_stats=False, tile_step=1):
     cp, vp, ci = cam
     _, _, T = tile_grid(w, h, ts)
     t0, t1 = (0, T) if tiles is None else tiles
@@ -163,7 +163,7 @@ def forward(scene, cam, w, h, sorted_vals, rng_, ts=12, min_t=0.0, colors=None, 
         _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
         _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
         _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
-        w, h, ts, min_t, t0, t1, rgba,
+        w, h, ts, min_t, t0, t1, tile_step, rgba,
         None if aux is None else aux.ctypes.data, None if stats is None else stats.ctypes.data)
     out = {"rgba": rgba.reshape(h, w, 4)}
     if want_aux:
@@ -181,7 +181,8 @@ def loss_grad(rgba, gt, loss_type, dtype=np.float32):
     return dl.reshape(h, w, 4), float(loss)
 
 
-def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, dtype=np.float32, tiles=None):
+def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, dtype=np.float32, tiles=None,
+             tile_step=1):
     cp, vp, ci = cam
     _, _, T = tile_grid(w, h, ts)
     t0, t1 = (0, T) if tiles is None else tiles
@@ -196,7 +197,7 @@ def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, 
         _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
         _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
         _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
-        w, h, ts, _a(rgba, dtype).ravel(), _a(dl, dtype).ravel(), t0, t1, dV, dD, dG, dC)
+        w, h, ts, _a(rgba, dtype).ravel(), _a(dl, dtype).ravel(), t0, t1, tile_step, dV, dD, dG, dC)
     return {"d_vertices": dV, "d_densities": dD, "d_color_grads": dG, "d_base_colors": dC}
 
 

@@ -389,7 +389,7 @@ void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const
                      const uint32_t* sorted_vals, const uint32_t* ranges,
                      const float* cam_pos, const float* vp, const float* c2w_intr,
                      uint32_t w, uint32_t h, uint32_t ts, float min_t,
-                     uint32_t tile_begin, uint32_t tile_end, R* rgba, R* aux, uint32_t* stats) {
+                     uint32_t tile_begin, uint32_t tile_end, uint32_t tile_step, R* rgba, R* aux, uint32_t* stats) {
     CAM c;
     FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, min_t);
     uint32_t tiles_x = (w + ts - 1) / ts;
@@ -397,7 +397,7 @@ void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const
     R* st = (R*)calloc((size_t)TS * TS * 12, sizeof(R)); /* color3, log_t, nd4, ent4 */
     uint32_t* ss = (uint32_t*)calloc((size_t)TS * TS * 4, sizeof(uint32_t));
     V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
-    for (uint32_t tile = tile_begin; tile < tile_end; ++tile) {
+    for (uint32_t tile = tile_begin; tile < tile_end; tile += (tile_step ? tile_step : 1)) {
         uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
         R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
         memset(st, 0, sizeof(R) * TS * TS * 12);
@@ -537,7 +537,7 @@ void FN(orc_backward)(const R* verts, const uint32_t* idx, const R* colors, cons
                       const uint32_t* sorted_vals, const uint32_t* ranges,
                       const float* cam_pos, const float* vp, const float* c2w_intr,
                       uint32_t w, uint32_t h, uint32_t ts,
-                      const R* rgba, const R* dl, uint32_t tile_begin, uint32_t tile_end,
+                      const R* rgba, const R* dl, uint32_t tile_begin, uint32_t tile_end, uint32_t tile_step,
                       R* d_verts, R* d_dens, R* d_grads, R* d_colors) {
     CAM c;
     FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
@@ -545,7 +545,7 @@ void FN(orc_backward)(const R* verts, const uint32_t* idx, const R* colors, cons
     uint32_t TS = ts;
     R* st = (R*)calloc((size_t)TS * TS * 5, sizeof(R)); /* color3, log_t, d_log_t */
     V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
-    for (uint32_t tile = tile_begin; tile < tile_end; ++tile) {
+    for (uint32_t tile = tile_begin; tile < tile_end; tile += (tile_step ? tile_step : 1)) {
         uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
         R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
         for (uint32_t i = 0; i < TS * TS; ++i) {

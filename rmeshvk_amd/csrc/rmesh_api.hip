@@ -99,6 +99,9 @@ struct rmesh_renderer {
     uint32_t graph_views = 0, graph_loss = 0;
     float graph_lrs[4] = {0, 0, 0, 0};
     std::vector<ViewParams> hviews;
+    // optional per-stage timing (HIP events recorded between stages)
+    bool timing = false;
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
     uint32_t gtiles(uint32_t V) const { return V * T; }
     uint32_t sorting_bits(uint32_t V) const { return rmesh_sorting_bits(gtiles(V)); }
@@ -189,6 +192,10 @@ rmesh_status ensure_capacity(rmesh_renderer* h, uint64_t need) {
         h->graph = nullptr;
     }
     return RMESH_OK;
+}
+
+void mark(rmesh_renderer* h, int i) {
+    if (h->timing) (void)hipEventRecord(h->ev[i], h->stream);
 }
 
 // Colours + projection + scan; in eager mode also reads the pair count back
@@ -345,18 +352,34 @@ rmesh_status stage_adam(rmesh_renderer* h, uint32_t V, float lr_v, float lr_d, f
     return RMESH_OK;
 }
 
+// Stage boundaries for timing: 0 start | 1 colours+projection+scan | 2 pair
+// generation | 3 sort | 4 ranges | 5 forward | 6 adjoint | 7 SH adjoint + Adam.
 rmesh_status record_train(rmesh_renderer* h, uint32_t V, const float* gt, uint32_t loss_type, float lr_v, float lr_d,
                           float lr_g, float lr_sh, bool capture) {
+    mark(h, 0);
     rmesh_status st = stage_binning_front(h, V, capture);
     if (st != RMESH_OK) return st;
     zero_grads(h, V);
-    st = stage_binning_back(h, V);
-    if (st != RMESH_OK) return st;
+    mark(h, 1);
+    hipStream_t s = h->stream;
+    launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, V, h->counts, h->block_sums,
+                 h->keys_a, h->vals_a, h->cap, h->overflow, s);
+    mark(h, 2);
+    h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
+                                       h->sorting_bits(V), h->hist, s);
+    mark(h, 3);
+    launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
+    launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
+    mark(h, 4);
     st = stage_forward(h, V, false);
     if (st != RMESH_OK) return st;
+    mark(h, 5);
     st = stage_backward(h, V, nullptr, gt, loss_type);
     if (st != RMESH_OK) return st;
-    return stage_adam(h, V, lr_v, lr_d, lr_g, lr_sh);
+    mark(h, 6);
+    st = stage_adam(h, V, lr_v, lr_d, lr_g, lr_sh);
+    mark(h, 7);
+    return st;
 }
 
 void free_all(rmesh_renderer* h) {
@@ -369,6 +392,8 @@ void free_all(rmesh_renderer* h) {
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    for (hipEvent_t e : h->ev)
+        if (e) (void)hipEventDestroy(e);
     if (h->own_stream) hipStreamDestroy(h->own_stream);
 }
 
@@ -873,4 +898,52 @@ extern "C" rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_
     hipFree(np);
     if (s) hipStreamDestroy(s);
     return st;
+}
+
+extern "C" rmesh_status rmesh_set_targets(rmesh_renderer* h, uint32_t n_views, const float* gt) {
+    if (!h || !gt) return fail(RMESH_ERR_INVALID, "null argument");
+    rmesh_status st = check_views(h, n_views);
+    if (st != RMESH_OK) return st;
+    DeviceGuard g(h->device);
+    HIP_TRY(hipMemcpyAsync(h->gt, gt, 12ull * h->W * h->H * n_views, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_set_timing(rmesh_renderer* h, int enable) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    DeviceGuard g(h->device);
+    if (enable && !h->ev[0])
+        for (hipEvent_t& e : h->ev) HIP_TRY(hipEventCreate(&e));
+    if ((enable != 0) != h->timing) {
+        h->timing = enable != 0;
+        if (h->graph_exec) {
+            (void)hipGraphExecDestroy(h->graph_exec);
+            h->graph_exec = nullptr;
+        }
+        if (h->graph) {
+            (void)hipGraphDestroy(h->graph);
+            h->graph = nullptr;
+        }
+    }
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_stage_times(rmesh_renderer* h, float* ms) {
+    if (!h || !ms) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->timing) return fail(RMESH_ERR_INVALID, "timing is not enabled");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < 7; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_pair_count(rmesh_renderer* h, uint64_t* out) {
+    if (!h || !out) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    uint32_t P = 0;
+    HIP_TRY(hipMemcpy(&P, h->n_pairs, 4, hipMemcpyDeviceToHost));
+    *out = P;
+    return RMESH_OK;
 }

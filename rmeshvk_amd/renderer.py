@@ -239,6 +239,29 @@ class RMeshRenderer:
     def synchronize(self):
         _lib.check(self._lib.rmesh_synchronize(self._h))
 
+    def set_targets(self, gt):
+        """Upload target images [V, H, W, 3] (float32) into the renderer's target buffer."""
+        _f32("gt", gt)
+        if gt.size % (self._hgt * self._w * 3):
+            raise ValueError("targets must be [V, H, W, 3]")
+        _lib.check(self._lib.rmesh_set_targets(self._h, gt.size // (self._hgt * self._w * 3), _p(gt)))
+
+    STAGES = ("binning", "pairs", "sort", "ranges", "forward", "backward", "adam")
+
+    def set_timing(self, enable: bool):
+        _lib.check(self._lib.rmesh_set_timing(self._h, 1 if enable else 0))
+
+    def stage_times(self):
+        """Durations (ms) of the stages of the last train step (see STAGES)."""
+        out = np.empty(7, dtype=np.float32)
+        _lib.check(self._lib.rmesh_stage_times(self._h, _p(out)))
+        return dict(zip(self.STAGES, out.tolist()))
+
+    def pair_count(self) -> int:
+        n = ctypes.c_uint64()
+        _lib.check(self._lib.rmesh_pair_count(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     # ---- inspection of the binning state ----
     def debug_tiles_touched(self, view=0):
         out = np.empty(self._m, dtype=np.uint32)
