@@ -153,6 +153,8 @@ rmesh_status rmesh_set_timing(rmesh_renderer* h, int enable);
 rmesh_status rmesh_stage_times(rmesh_renderer* h, float* ms7);
 /* Number of tile-tet pairs of the last step (all views). */
 rmesh_status rmesh_pair_count(rmesh_renderer* h, uint64_t* out);
+/* Copy nbytes of an internal buffer (RMESH_BUF_*) to host memory. */
+rmesh_status rmesh_read_buffer(rmesh_renderer* h, int which, void* host, uint64_t nbytes);
 /* Synchronise the renderer's stream. */
 rmesh_status rmesh_synchronize(rmesh_renderer* h);
 

@@ -71,6 +71,7 @@ SIGNATURES = {
     "rmesh_set_timing": (_st, [_vp, ctypes.c_int]),
     "rmesh_stage_times": (_st, [_vp, _vp]),
     "rmesh_pair_count": (_st, [_vp, ctypes.POINTER(_u64)]),
+    "rmesh_read_buffer": (_st, [_vp, ctypes.c_int, _vp, _u64]),
     "rmesh_debug_tiles_touched": (_st, [_vp, _u32, _vp]),
     "rmesh_debug_pairs": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]),
     "rmesh_debug_sorted": (_st, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp, _u64]),

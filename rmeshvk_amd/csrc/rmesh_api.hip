@@ -947,3 +947,25 @@ extern "C" rmesh_status rmesh_pair_count(rmesh_renderer* h, uint64_t* out) {
     *out = P;
     return RMESH_OK;
 }
+
+extern "C" rmesh_status rmesh_read_buffer(rmesh_renderer* h, int which, void* host, uint64_t nbytes) {
+    if (!h || !host) return fail(RMESH_ERR_INVALID, "null argument");
+    uint64_t cap = 0;
+    size_t npx = (size_t)h->W * h->H;
+    switch (which) {
+        case RMESH_BUF_RGBA: cap = 16ull * npx * h->max_views; break;
+        case RMESH_BUF_GT: cap = 12ull * npx * h->max_views; break;
+        case RMESH_BUF_VERTICES: case RMESH_BUF_D_VERTICES: cap = 12ull * h->N; break;
+        case RMESH_BUF_DENSITIES: case RMESH_BUF_D_DENSITIES: cap = 4ull * h->M; break;
+        case RMESH_BUF_COLOR_GRADS: case RMESH_BUF_BASE_COLORS: case RMESH_BUF_D_COLOR_GRADS: cap = 12ull * h->M; break;
+        case RMESH_BUF_SH: cap = h->has_sh ? 12ull * h->M * h->nc : 0; break;
+        default: return fail(RMESH_ERR_INVALID, "unknown buffer");
+    }
+    if (nbytes > cap) return fail(RMESH_ERR_INVALID, "read beyond the buffer");
+    void* src = rmesh_device_buffer(h, which);
+    if (!src) return fail(RMESH_ERR_INVALID, "buffer not allocated");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(host, src, nbytes, hipMemcpyDeviceToHost));
+    return RMESH_OK;
+}

@@ -257,6 +257,18 @@ class RMeshRenderer:
         _lib.check(self._lib.rmesh_stage_times(self._h, _p(out)))
         return dict(zip(self.STAGES, out.tolist()))
 
+    def read_buffer(self, which: int, count: int, dtype=np.float32):
+        """Host copy of the first `count` elements of an internal buffer (_lib.BUF_*)."""
+        out = np.empty(count, dtype=dtype)
+        _lib.check(self._lib.rmesh_read_buffer(self._h, int(which), _p(out), out.nbytes))
+        return out
+
+    def gradients(self):
+        """Gradients left by the last backward / train step (device buffers)."""
+        return {"d_vertices": self.read_buffer(_lib.BUF_D_VERTICES, 3 * self._n),
+                "d_densities": self.read_buffer(_lib.BUF_D_DENSITIES, self._m),
+                "d_color_grads": self.read_buffer(_lib.BUF_D_COLOR_GRADS, 3 * self._m)}
+
     def pair_count(self) -> int:
         n = ctypes.c_uint64()
         _lib.check(self._lib.rmesh_pair_count(self._h, ctypes.byref(n)))
