@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -201,6 +203,20 @@ def main():
         cpu = cpu_baseline(scene, cams[0], Wd, args.cpu_stride, args.tile_size)
         log(f"cpu baseline: {cpu['value']:.4f} views/s ({time.perf_counter() - tc:.1f} s)")
 
+    copy_probe = None
+    if args.copy_probe_mb:
+        import ctypes
+
+        from rmeshvk_amd import _lib
+
+        ms = ctypes.c_float()
+        nbytes = args.copy_probe_mb << 20
+        _lib.check(_lib.load().rmesh_copy_probe(0, nbytes, 3, ctypes.byref(ms)))
+        copy_probe = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms.value,
+                      "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+        log(f"copy probe (k_copy_probe): {args.copy_probe_mb} MiB read + written, {ms.value:.3f} ms, "
+            f"{copy_probe['GBps']:.0f} GB/s")
+
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -225,6 +241,8 @@ def main():
         "cpu_baseline": cpu,
         "plain_per_view": plain,
     }
+    if copy_probe:
+        out["copy_probe"] = copy_probe
     print(json.dumps(out), flush=True)
 
 

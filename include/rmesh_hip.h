@@ -155,6 +155,10 @@ rmesh_status rmesh_stage_times(rmesh_renderer* h, float* ms7);
 rmesh_status rmesh_pair_count(rmesh_renderer* h, uint64_t* out);
 /* Copy nbytes of an internal buffer (RMESH_BUF_*) to host memory. */
 rmesh_status rmesh_read_buffer(rmesh_renderer* h, int which, void* host, uint64_t nbytes);
+/* Streaming copy of nbytes (read nbytes + write nbytes per copy), iters
+ * times; reports the average ms per copy.  Used to calibrate HBM byte
+ * counters against a kernel of known traffic. */
+rmesh_status rmesh_copy_probe(int32_t device_id, uint64_t nbytes, uint32_t iters, float* ms_per_copy);
 /* Synchronise the renderer's stream. */
 rmesh_status rmesh_synchronize(rmesh_renderer* h);
 

@@ -72,6 +72,7 @@ SIGNATURES = {
     "rmesh_stage_times": (_st, [_vp, _vp]),
     "rmesh_pair_count": (_st, [_vp, ctypes.POINTER(_u64)]),
     "rmesh_read_buffer": (_st, [_vp, ctypes.c_int, _vp, _u64]),
+    "rmesh_copy_probe": (_st, [_i32, _u64, _u32, ctypes.POINTER(_f32)]),
     "rmesh_debug_tiles_touched": (_st, [_vp, _u32, _vp]),
     "rmesh_debug_pairs": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]),
     "rmesh_debug_sorted": (_st, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp, _u64]),

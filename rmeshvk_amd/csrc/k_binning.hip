@@ -45,6 +45,13 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
 }
 
+// Plain streaming copy used to calibrate HBM byte counters (FETCH_SIZE /
+// WRITE_SIZE) against a kernel of known traffic.
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __device__ __forceinline__ void load_rec(const float4* __restrict__ rec, uint32_t t, V3 v[4], float& dens, V3& g) {
     float4 r0 = rec[4ull * t], r1 = rec[4ull * t + 1], r2 = rec[4ull * t + 2], r3 = rec[4ull * t + 3];
     v[0] = mk3(r0.x, r0.y, r0.z);
@@ -343,6 +350,9 @@ void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, n);
+}
+void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, src, dst, n16);
 }
 uint32_t project_blocks(uint32_t M) { return (M + 255) / 256; }
 void launch_project(const float4* rec, uint32_t M, const ViewParams* views, uint32_t V, uint32_t* counts,

@@ -110,47 +110,72 @@ struct BatchLDS {
     uint32_t behind[B];
     uint32_t span[B * SP];  // [tet][row] packed span
     uint32_t rpref[B * SP]; // [tet][row] exclusive prefix of row widths; [tet][TS] = tet total
-    uint32_t tpref[B + 1];  // exclusive prefix of tet totals
+    uint32_t tpref[B + 1];  // exclusive prefix of covered pixels per tet (record slots)
+    uint32_t cpref[B + 1];  // exclusive prefix of work chunks per tet
     uint32_t nbp;           // tets processed this batch (capacity cut)
-    // adjoint only
-    float v[BWD ? 4 : 1][3][B];
+    // adjoint only: per face (va, vb, vc, raw normal, 1/|n|^2) in face order, v0, vertex ids, tet id
+    // per tet 4 faces x 16 floats, padded to 68 floats so that lanes reading
+    // different tets start 4 banks apart (16-byte reads stay conflict-free)
+    static constexpr int FS = 4 * 16 + 4;
+    alignas(16) float face[BWD ? B * FS : 1];
+    float v0[BWD ? 3 : 1][B];
     uint32_t vi[BWD ? 4 : 1][B];
     uint32_t tid[BWD ? B : 1];
-    uint32_t flip[BWD ? B : 1];
 };
 
+// Global loads of one tet (issued one batch ahead, consumed by store_tet).
+struct TetFetch {
+    float4 r[4];
+    float4 col;
+    uint4 q;
+    uint32_t t;
+};
+
+template <bool BWD>
+__device__ __forceinline__ void fetch_tet(TetFetch& f, uint32_t t, const float4* __restrict__ rec,
+                                          const uint32_t* __restrict__ idx, const float4* __restrict__ colors) {
+    f.t = t;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.r[i] = rec[4ull * t + i];
+    f.col = colors[t];
+    if (BWD) f.q = reinterpret_cast<const uint4*>(idx)[t];
+}
+
 template <int TS, int B, bool BWD>
-__device__ __forceinline__ void load_tet(BatchLDS<TS, B, BWD>& L, int k, uint32_t t, const float4* __restrict__ rec,
-                                         const uint32_t* __restrict__ idx, const float4* __restrict__ colors,
-                                         const ViewParams& vw, const TileCtx& tc) {
-    float4 r0 = rec[4ull * t], r1 = rec[4ull * t + 1], r2 = rec[4ull * t + 2], r3 = rec[4ull * t + 3];
-    V3 v[4] = {mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), mk3(r2.x, r2.y, r2.z), mk3(r3.x, r3.y, r3.z)};
-    V3 g = mk3(r1.w, r2.w, r3.w);
+__device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const TetFetch& f, const ViewParams& vw,
+                                          const TileCtx& tc) {
+    V3 v[4] = {mk3(f.r[0].x, f.r[0].y, f.r[0].z), mk3(f.r[1].x, f.r[1].y, f.r[1].z), mk3(f.r[2].x, f.r[2].y, f.r[2].z),
+               mk3(f.r[3].x, f.r[3].y, f.r[3].z)};
+    V3 g = mk3(f.r[1].w, f.r[2].w, f.r[3].w);
     V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
-    float4 col = colors[t];
-    uint32_t flips = 0;
 #pragma unroll
     for (int fi = 0; fi < 4; ++fi) {
         int a, b, c, o;
         face_idx(fi, a, b, c, o);
-        V3 nn = cross3(sub3(v[c], v[a]), sub3(v[b], v[a]));
-        if (dot3(nn, sub3(v[o], v[a])) < 0.0f) {
-            nn = neg3(nn);
-            flips |= 1u << fi;
-        }
+        V3 raw = cross3(sub3(v[c], v[a]), sub3(v[b], v[a]));
+        V3 nn = raw;
+        if (dot3(nn, sub3(v[o], v[a])) < 0.0f) nn = neg3(nn);
         L.n[fi][0][k] = nn.x;
         L.n[fi][1][k] = nn.y;
         L.n[fi][2][k] = nn.z;
         L.num[fi][k] = dot3(nn, sub3(v[a], cam));
+        if (BWD) {
+            float* F = &L.face[k * L.FS + fi * 16];
+            F[0] = v[a].x; F[1] = v[a].y; F[2] = v[a].z;
+            F[3] = v[b].x; F[4] = v[b].y; F[5] = v[b].z;
+            F[6] = v[c].x; F[7] = v[c].y; F[8] = v[c].z;
+            F[9] = raw.x; F[10] = raw.y; F[11] = raw.z;
+            F[12] = 1.0f / dot3(raw, raw);
+        }
     }
     float bo = dot3(g, sub3(cam, v[0]));
-    L.bc[0][k] = col.x + bo;
-    L.bc[1][k] = col.y + bo;
-    L.bc[2][k] = col.z + bo;
+    L.bc[0][k] = f.col.x + bo;
+    L.bc[1][k] = f.col.y + bo;
+    L.bc[2][k] = f.col.z + bo;
     L.g[0][k] = g.x;
     L.g[1][k] = g.y;
     L.g[2][k] = g.z;
-    L.dens[k] = r0.w;
+    L.dens[k] = f.r[0].w;
     float clip[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) mul_vp(vw.vp, v[i], clip[i]);
@@ -164,23 +189,22 @@ __device__ __forceinline__ void load_tet(BatchLDS<TS, B, BWD>& L, int k, uint32_
         }
     }
     if (BWD) {
-        uint4 q = reinterpret_cast<const uint4*>(idx)[t];
-        uint32_t vv[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            L.v[i][0][k] = v[i].x;
-            L.v[i][1][k] = v[i].y;
-            L.v[i][2][k] = v[i].z;
-            L.vi[i][k] = vv[i];
-        }
-        L.tid[k] = t;
-        L.flip[k] = flips;
+        L.v0[0][k] = v[0].x;
+        L.v0[1][k] = v[0].y;
+        L.v0[2][k] = v[0].z;
+        L.vi[0][k] = f.q.x;
+        L.vi[1][k] = f.q.y;
+        L.vi[2][k] = f.q.z;
+        L.vi[3][k] = f.q.w;
+        L.tid[k] = f.t;
     }
 }
 
-// Spans of the batch, then (wave 0) per-tet row prefixes, the tet prefix and
-// the capacity cut.  Ends with the batch's pair count in L.tpref[L.nbp].
-template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int NT>
+// Spans of the batch, then (wave 0) per-tet row prefixes, the pixel and chunk
+// prefixes over tets and the capacity cut (records <= CAP).  Rows outside the
+// tet's projected y-range are empty without evaluating the scanline (no edge
+// crossing and no vertex can fall in them).
+template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int R, int NT>
 __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb) {
     constexpr int SP = TS + 1;
     for (int it = threadIdx.x; it < nb * TS; it += NT) {
@@ -189,7 +213,10 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb) {
         if (!L.behind[k]) {
             float px_[4] = {L.pxy[0][k], L.pxy[1][k], L.pxy[2][k], L.pxy[3][k]};
             float py_[4] = {L.pxy[4][k], L.pxy[5][k], L.pxy[6][k], L.pxy[7][k]};
-            sp = row_span<FORWARD_RULE>(px_, py_, (uint32_t)row, TS);
+            float ymin = fminf(fminf(py_[0], py_[1]), fminf(py_[2], py_[3]));
+            float ymax = fmaxf(fmaxf(py_[0], py_[1]), fmaxf(py_[2], py_[3]));
+            float yc = (float)row + 0.5f;
+            if (!(yc + 0.5f < ymin || yc - 0.5f > ymax)) sp = row_span<FORWARD_RULE>(px_, py_, (uint32_t)row, TS);
         }
         L.span[k * SP + row] = sp;
     }
@@ -207,42 +234,99 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb) {
             L.rpref[k * SP + TS] = tot;
         }
         uint32_t inc = wave_incl_scan(tot);
-        if (k < nb) L.tpref[k + 1] = inc;
-        if (k == 0) L.tpref[0] = 0;
+        uint32_t cinc = wave_incl_scan((tot + R - 1) / R);
+        if (k < nb) {
+            L.tpref[k + 1] = inc;
+            L.cpref[k + 1] = cinc;
+        }
+        if (k == 0) {
+            L.tpref[0] = 0;
+            L.cpref[0] = 0;
+        }
         uint64_t fits = __ballot(k < nb && inc <= (uint32_t)CAP);
         if (k == 0) L.nbp = max((uint32_t)__popcll(fits), 1u);
     }
     __syncthreads();
 }
 
-// (tet, row, col) of pair `idx` of the batch.
-template <int TS, int B, bool BWD>
-__device__ __forceinline__ void pair_locate(const BatchLDS<TS, B, BWD>& L, uint32_t idx, uint32_t nbp, int& k, int& row,
-                                            int& col) {
-    constexpr int SP = TS + 1;
+// Work chunk j of the batch: tet k and its covered pixels [l0, l0 + cnt).
+template <int TS, int B, bool BWD, int R>
+__device__ __forceinline__ void chunk_locate(const BatchLDS<TS, B, BWD>& L, uint32_t j, uint32_t nbp, int& k,
+                                             uint32_t& l0, uint32_t& cnt) {
     int lo = 0, hi = (int)nbp;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
-        if (L.tpref[mid] <= idx) lo = mid;
+        if (L.cpref[mid] <= j) lo = mid;
         else hi = mid;
     }
     k = lo;
-    uint32_t local = idx - L.tpref[k];
-    int rl = 0, rh = TS;
-    while (rh - rl > 1) {
-        int mid = (rl + rh) >> 1;
-        if (L.rpref[k * SP + mid] <= local) rl = mid;
-        else rh = mid;
-    }
-    row = rl;
-    col = span_xl(L.span[k * SP + row]) + (int)(local - L.rpref[k * SP + row]);
+    l0 = (j - L.cpref[k]) * R;
+    cnt = min((uint32_t)R, L.tpref[k + 1] - L.tpref[k] - l0);
 }
 
-// Ray–tet slab intersection against the batch's inward planes
-// (rasterize_compute.wgsl:354-390).
+// Walks the covered pixels of one tet in (row, col) order starting at local index l.
+template <int TS>
+struct PixWalk {
+    int row, col, xr;
+    template <int B, bool BWD>
+    __device__ __forceinline__ void init(const BatchLDS<TS, B, BWD>& L, int k, uint32_t l) {
+        constexpr int SP = TS + 1;
+        int rl = 0, rh = TS;
+        while (rh - rl > 1) {
+            int mid = (rl + rh) >> 1;
+            if (L.rpref[k * SP + mid] <= l) rl = mid;
+            else rh = mid;
+        }
+        row = rl;
+        uint32_t sp = L.span[k * SP + row];
+        col = span_xl(sp) + (int)(l - L.rpref[k * SP + row]);
+        xr = span_xr(sp);
+    }
+    template <int B, bool BWD>
+    __device__ __forceinline__ void next(const BatchLDS<TS, B, BWD>& L, int k) {
+        constexpr int SP = TS + 1;
+        if (++col > xr) {
+            for (int r = row + 1; r < TS; ++r) {
+                uint32_t sp = L.span[k * SP + r];
+                if (span_xr(sp) >= span_xl(sp)) {
+                    row = r;
+                    col = span_xl(sp);
+                    xr = span_xr(sp);
+                    return;
+                }
+            }
+            row = TS;
+        }
+    }
+};
+
+// Per-tet data held in registers while a lane walks its chunk.
+struct TetRegs {
+    V3 n[4];
+    float num[4];
+    float bc[3];
+    V3 g;
+    float dens;
+};
+
 template <int TS, int B, bool BWD>
-__device__ __forceinline__ bool intersect(const BatchLDS<TS, B, BWD>& L, int k, V3 rd, float min_t, float& t_min,
-                                          float& t_max, int& fmin, int& fmax) {
+__device__ __forceinline__ void tet_regs(const BatchLDS<TS, B, BWD>& L, int k, TetRegs& T) {
+#pragma unroll
+    for (int fi = 0; fi < 4; ++fi) {
+        T.n[fi] = mk3(L.n[fi][0][k], L.n[fi][1][k], L.n[fi][2][k]);
+        T.num[fi] = L.num[fi][k];
+    }
+    T.bc[0] = L.bc[0][k];
+    T.bc[1] = L.bc[1][k];
+    T.bc[2] = L.bc[2][k];
+    T.g = mk3(L.g[0][k], L.g[1][k], L.g[2][k]);
+    T.dens = L.dens[k];
+}
+
+// Ray–tet slab intersection against inward planes (rasterize_compute.wgsl:354-390).
+__device__ __forceinline__ bool intersect(const TetRegs& T, V3 rd, float min_t, float& t_min, float& t_max, int& fmin,
+                                          int& fmax) {
+#pragma clang fp contract(fast)
     t_min = -3.402823e38f;
     t_max = 3.402823e38f;
     fmin = 4;
@@ -250,15 +334,14 @@ __device__ __forceinline__ bool intersect(const BatchLDS<TS, B, BWD>& L, int k, 
     bool valid = true;
 #pragma unroll
     for (int fi = 0; fi < 4; ++fi) {
-        V3 nn = mk3(L.n[fi][0][k], L.n[fi][1][k], L.n[fi][2][k]);
-        float den = dot3(nn, rd);
-        float num = L.num[fi][k];
+        float den = dot3(T.n[fi], rd);
+        float num = T.num[fi];
         if (min_t != 0.0f) num = num - min_t * den;
         if (fabsf(den) < 1e-20f) {
             if (num > 0.0f) valid = false;
             continue;
         }
-        float t = num / den;
+        float t = fast_div(num, den);
         if (den > 0.0f) {
             if (t > t_min) {
                 t_min = t;
@@ -272,6 +355,36 @@ __device__ __forceinline__ bool intersect(const BatchLDS<TS, B, BWD>& L, int k, 
     return valid && t_min < t_max;
 }
 
+// Volume integral of the linear colour field over [t_min, t_max]
+// (rasterize_compute.wgsl:394-410): optical depth, premultiplied colour.
+struct Integral {
+    float od, alpha_t, w0, w1;
+    float cs[3], ce[3], csr[3], cer[3];
+    float cp[3];
+};
+__device__ __forceinline__ void integral(const TetRegs& T, V3 rd, float t_min, float t_max, float bo_extra, float clip_lo,
+                                         Integral& I) {
+#pragma clang fp contract(fast)
+    float dc_dt = dot3(T.g, rd);
+    float a0 = dc_dt * t_min, a1 = dc_dt * t_max;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float b = T.bc[c] + bo_extra;
+        I.csr[c] = b + a0;
+        I.cer[c] = b + a1;
+        I.cs[c] = relu(I.csr[c]);
+        I.ce[c] = relu(I.cer[c]);
+    }
+    float dist = t_max - t_min;
+    I.od = med_clip(T.dens * dist, 1e-8f, 88.0f);
+    I.alpha_t = fast_safe_exp(-I.od);
+    float ph = fast_phi(I.od, I.alpha_t);
+    I.w0 = ph - I.alpha_t;
+    I.w1 = 1.0f - ph;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) I.cp[c] = med_clip(I.ce[c] * I.w0 + I.cs[c] * I.w1, clip_lo, kMaxVal);
+}
+
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
@@ -280,6 +393,7 @@ struct FwdCfg {
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
     static constexpr int B = 64;
     static constexpr int CAP = 1024;
+    static constexpr int R = 2;
 };
 
 template <int TS, bool AUX>
@@ -288,14 +402,15 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
     const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges, const ViewParams* __restrict__ views,
     uint32_t tiles_per_view, float* __restrict__ rgba, float* __restrict__ aux) {
     using C = FwdCfg<TS>;
-    constexpr int NT = C::NT, B = C::B, CAP = C::CAP, SP = TS + 1;
+    constexpr int NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1;
     constexpr int NA = AUX ? CAP : 1;
     __shared__ BatchLDS<TS, B, false> L;
     __shared__ float s_od[CAP];
     __shared__ float s_cp[3][CAP];
-    __shared__ float s_aux[3][NA];     // t_min, t_max, w0 (w1 = 1 - phi recomputed from w0 + alpha_t)
+    __shared__ float s_aux[3][NA];     // t_min, t_max, w0
     __shared__ uint32_t s_ent[NA];
     __shared__ float s_ray[3][TS * TS];
+    __shared__ unsigned long long s_mask[TS * TS];  // covered tets of the batch per pixel
 
     const uint32_t gtile = blockIdx.x;
     const uint32_t view = gtile / tiles_per_view;
@@ -307,7 +422,6 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
     const uint32_t px = tc.tile_x * TS + colp, py = tc.tile_y * TS + row;
     const bool active = tid < TS * TS && px < vw.width && py < vw.height;
     const float min_t = vw.min_t;
-    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
 
     if (tid < TS * TS) {
         V3 rd = mk3(0.f, 0.f, 1.f);
@@ -322,74 +436,71 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
     const uint32_t rs = ranges[2ull * gtile], re = ranges[2ull * gtile + 1];
     const uint32_t count = re > rs ? re - rs : 0u;
     uint32_t done = 0;
+    int nb = (int)min<uint32_t>(B, count);
+    TetFetch f;
+    if (tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, col_v);
     while (done < count) {
         if (__syncthreads_and(!active || log_t < -13.8f)) break;
-        const int nb = (int)min<uint32_t>(B, count - done);
-        if (tid < nb) load_tet<TS, B, false>(L, tid, sorted_vals[re - 1 - done - tid], rec, nullptr, col_v, vw, tc);
+        if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
+        if (tid < TS * TS) s_mask[tid] = 0ull;
         __syncthreads();
-        batch_layout<TS, B, false, true, CAP, NT>(L, nb);
+        batch_layout<TS, B, false, true, CAP, R, NT>(L, nb);
         const uint32_t nbp = L.nbp;
-        const uint32_t npairs = L.tpref[nbp];
-        // pair-parallel: intersection + volume integral
-        for (uint32_t i = tid; i < npairs; i += NT) {
-            int k, r, c;
-            pair_locate<TS, B, false>(L, i, nbp, k, r, c);
-            const uint32_t gx = tc.tile_x * TS + c, gy = tc.tile_y * TS + r;
-            float od_rec = -1.0f;
-            float cp0 = 0.f, cp1 = 0.f, cp2 = 0.f;
-            float tmn = 0.f, tmx = 0.f, w0 = 0.f;
-            int ent = 4;
-            if (gx < vw.width && gy < vw.height) {
-                const int p = r * TS + c;
-                V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
-                float t_min, t_max;
-                int fmin, fmax;
-                if (intersect<TS, B, false>(L, k, rd, min_t, t_min, t_max, fmin, fmax)) {
-                    V3 g = mk3(L.g[0][k], L.g[1][k], L.g[2][k]);
-                    float dc_dt = dot3(g, rd);
-                    float bo = min_t != 0.0f ? min_t * dc_dt : 0.0f;
-                    float b0 = L.bc[0][k] + bo, b1 = L.bc[1][k] + bo, b2 = L.bc[2][k] + bo;
-                    float a0 = dc_dt * t_min, a1 = dc_dt * t_max;
-                    float cs0 = fmaxf(b0 + a0, 0.f), cs1 = fmaxf(b1 + a0, 0.f), cs2 = fmaxf(b2 + a0, 0.f);
-                    float ce0 = fmaxf(b0 + a1, 0.f), ce1 = fmaxf(b1 + a1, 0.f), ce2 = fmaxf(b2 + a1, 0.f);
-                    float dist = t_max - t_min;
-                    float od = clampf(L.dens[k] * dist, 1e-8f, 88.0f);
-                    float alpha_t = safe_exp(-od);
-                    float ph = phi_f(od);
-                    w0 = ph - alpha_t;
-                    float w1 = 1.0f - ph;
-                    cp0 = safe_clip(ce0 * w0 + cs0 * w1, 0.0f, kMaxVal);
-                    cp1 = safe_clip(ce1 * w0 + cs1 * w1, 0.0f, kMaxVal);
-                    cp2 = safe_clip(ce2 * w0 + cs2 * w1, 0.0f, kMaxVal);
-                    od_rec = od;
-                    tmn = t_min;
-                    tmx = t_max;
-                    ent = fmin;
+        const uint32_t nchunks = L.cpref[nbp];
+        // prefetch the next batch's tet records while this one is processed
+        const uint32_t next = done + nbp;
+        const int nb_next = (int)min<uint32_t>(B, count - next);
+        if (tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
+        // chunk-parallel: intersection + volume integral of R pixels of one tet
+        for (uint32_t j = tid; j < nchunks; j += NT) {
+            int k;
+            uint32_t l0, cnt;
+            chunk_locate<TS, B, false, R>(L, j, nbp, k, l0, cnt);
+            TetRegs T;
+            tet_regs(L, k, T);
+            PixWalk<TS> w;
+            w.init(L, k, l0);
+            const uint32_t slot0 = L.tpref[k] + l0;
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const uint32_t slot = slot0 + q;
+                const uint32_t gx = tc.tile_x * TS + w.col, gy = tc.tile_y * TS + w.row;
+                if (gx < vw.width && gy < vw.height) {
+                    const int p = w.row * TS + w.col;
+                    V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
+                    float t_min, t_max;
+                    int fmin, fmax;
+                    if (intersect(T, rd, min_t, t_min, t_max, fmin, fmax)) {
+                        Integral I;
+                        integral(T, rd, t_min, t_max, min_t != 0.0f ? min_t * dot3(T.g, rd) : 0.0f, 0.0f, I);
+                        s_od[slot] = I.od;
+                        s_cp[0][slot] = I.cp[0];
+                        s_cp[1][slot] = I.cp[1];
+                        s_cp[2][slot] = I.cp[2];
+                        if (AUX) {
+                            s_aux[0][slot] = t_min;
+                            s_aux[1][slot] = t_max;
+                            s_aux[2][slot] = I.w0;
+                            s_ent[slot] = (uint32_t)fmin;
+                        }
+                        atomicOr(&s_mask[p], 1ull << k);
+                    }
                 }
-            }
-            s_od[i] = od_rec;
-            s_cp[0][i] = cp0;
-            s_cp[1][i] = cp1;
-            s_cp[2][i] = cp2;
-            if (AUX) {
-                s_aux[0][i] = tmn;
-                s_aux[1][i] = tmx;
-                s_aux[2][i] = w0;
-                s_ent[i] = (uint32_t)ent;
+                w.next(L, k);
             }
         }
         __syncthreads();
-        // pixel-sequential compositing in depth order (rasterize_compute.wgsl:336-463)
+        // pixel-sequential compositing in depth order over this pixel's covered
+        // tets (rasterize_compute.wgsl:336-463)
         if (active) {
-            for (uint32_t k = 0; k < nbp; ++k) {
+            unsigned long long m = s_mask[tid];
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                if (log_t < -13.8f) break;
                 const uint32_t sp = L.span[k * SP + row];
-                const int xl = span_xl(sp);
-                if (colp < xl || colp > span_xr(sp)) continue;
-                if (log_t < -13.8f) continue;
-                const uint32_t slot = L.tpref[k] + L.rpref[k * SP + row] + (uint32_t)(colp - xl);
+                const uint32_t slot = L.tpref[k] + L.rpref[k * SP + row] + (uint32_t)(colp - span_xl(sp));
                 const float od = s_od[slot];
-                if (od < 0.0f) continue;
-                const float Tj = safe_exp(log_t);
+                const float Tj = fast_safe_exp(log_t);
                 if (Tj >= 1e-6f) {
                     cr = cr + s_cp[0][slot] * Tj;
                     cg = cg + s_cp[1][slot] * Tj;
@@ -397,7 +508,7 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
                     log_t = log_t - od;
                     if (AUX) {
                         // rasterize_compute.wgsl:434-456
-                        float alpha_t = safe_exp(-od);
+                        float alpha_t = fast_safe_exp(-od);
                         float alpha = 1.0f - alpha_t;
                         float w0 = s_aux[2][slot];
                         float w1 = 1.0f - (w0 + alpha_t);
@@ -422,7 +533,8 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
                 }
             }
         }
-        done += nbp;
+        done = next;
+        nb = nb_next;
     }
     if (active) {
         uint64_t pix = ((uint64_t)view * vw.height + py) * vw.width + px;
@@ -444,6 +556,7 @@ struct BwdCfg {
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
     static constexpr int B = 32;
     static constexpr int CAP = 768;
+    static constexpr int R = 4;
 };
 
 // Segmented inclusive sum across the wave; lanes with equal `key` form
@@ -471,6 +584,100 @@ __device__ __forceinline__ float seg_scan(float v, const SegMasks& s) {
     return v;
 }
 
+// Vertex slot of each face role (a, b, c) for face fi; the fourth slot is the
+// opposite vertex.  FACES = {0,2,1}, {1,2,3}, {0,3,2}, {3,0,1}.
+__device__ __forceinline__ int face_slot(int fi, int role) {
+    // 4 bits per (face, role): slots packed as a | b << 4 | c << 8, faces 12 bits apart
+    constexpr uint64_t T = (0x120ull) | (0x321ull << 12) | (0x230ull << 24) | (0x103ull << 36);
+    return (int)((T >> (fi * 12 + role * 4)) & 0xFull);
+}
+
+// Gradient terms of one covered (pixel, tet) pair, accumulated into acc
+// (backward_tiled_compute.wgsl:446-543, per-pixel clipping as there).  The
+// intersection derivatives use dt/dva = ((vc - hit) x (vb - hit)) / den (and
+// cyclic), algebraically identical to the reference's
+// (cross(va - hit, vb - vc) + n) / den; since the three crosses are parallel
+// to the face normal n they equal lambda_j * n with the hit's barycentric
+// weights lambda_j, which are scattered to the face's vertex slots.
+template <int TS, int B>
+__device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k, const TetRegs& T, V3 v0, V3 cam,
+                                          V3 rd, float dc0, float dc1, float dc2, float t_prev, float d_log_t_b,
+                                          float t_min, float t_max, int min_face, int max_face, float* acc) {
+#pragma clang fp contract(fast)
+    Integral I;
+    integral(T, rd, t_min, t_max, 0.0f, kMinVal, I);
+    const float dist = t_max - t_min;
+    const float dens = T.dens;
+    const float dc_dt = dot3(T.g, rd);
+    float dcp0 = med_clip(dc0 * t_prev, kMinVal, kMaxVal);
+    float dcp1 = med_clip(dc1 * t_prev, kMinVal, kMaxVal);
+    float dcp2 = med_clip(dc2 * t_prev, kMinVal, kMaxVal);
+    float d_od_state = -d_log_t_b;
+    float dphi = fast_dphi(I.od, I.alpha_t);
+    float dw0 = dphi + I.alpha_t, dw1 = -dphi;
+    float d_od_int = (dcp0 * (I.ce[0] * dw0 + I.cs[0] * dw1) + dcp1 * (I.ce[1] * dw0 + I.cs[1] * dw1)) +
+                     dcp2 * (I.ce[2] * dw0 + I.cs[2] * dw1);
+    float d_od = med_clip(d_od_state + d_od_int, kMinVal, kMaxVal);
+    float od_raw = dens * dist;
+    bool act = (od_raw >= 1e-8f) && (od_raw <= 88.0f);
+    float d_od_raw = act ? d_od : 0.0f;
+    float d_dens_l = med_clip(d_od_raw * dist, kMinVal, kMaxVal);
+    float d_dist = med_clip(d_od_raw * dens, kMinVal, kMaxVal);
+    float d_tmin = -d_dist, d_tmax = d_dist;
+    float dcs0 = I.csr[0] > 0.f ? dcp0 * I.w1 : 0.f, dcs1 = I.csr[1] > 0.f ? dcp1 * I.w1 : 0.f;
+    float dcs2 = I.csr[2] > 0.f ? dcp2 * I.w1 : 0.f;
+    float dce0 = I.cer[0] > 0.f ? dcp0 * I.w0 : 0.f, dce1 = I.cer[1] > 0.f ? dcp1 * I.w0 : 0.f;
+    float dce2 = I.cer[2] > 0.f ? dcp2 * I.w0 : 0.f;
+    float dbc0 = dcs0 + dce0, dbc1 = dcs1 + dce1, dbc2 = dcs2 + dce2;
+    float sum_s = (dcs0 + dcs1) + dcs2, sum_e = (dce0 + dce1) + dce2;
+    float d_dcdt = sum_s * t_min + sum_e * t_max;
+    d_tmin = d_tmin + sum_s * dc_dt;
+    d_tmax = d_tmax + sum_e * dc_dt;
+    float dbo = (dbc0 + dbc1) + dbc2;
+    V3 dg = add3(scl3(sub3(cam, v0), dbo), scl3(rd, d_dcdt));
+    float wv[2][4];   // per side: barycentric weight of each vertex slot
+    V3 G[2];          // per side: n * (dt / den)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int fi = side == 0 ? min_face : max_face;
+        const float tt = side == 0 ? t_min : t_max;
+        const float dt = side == 0 ? d_tmin : d_tmax;
+        const float4* F = reinterpret_cast<const float4*>(&L.face[k * L.FS + fi * 16]);
+        const float4 f0 = F[0], f1 = F[1], f2 = F[2], f3 = F[3];
+        const V3 va = mk3(f0.x, f0.y, f0.z), vb = mk3(f0.w, f1.x, f1.y), vc = mk3(f1.z, f1.w, f2.x);
+        const V3 nn = mk3(f2.y, f2.z, f2.w);
+        const float inv_nn = f3.x;
+        const float inv_den = fast_safe_div(1.0f, dot3(nn, rd));
+        const V3 hit = add3(cam, scl3(rd, tt));
+        const V3 a = sub3(va, hit), b = sub3(vb, hit), c = sub3(vc, hit);
+        const float la = dot3(cross3(c, b), nn) * inv_nn;
+        const float lb = dot3(cross3(a, c), nn) * inv_nn;
+        const float lc = dot3(cross3(b, a), nn) * inv_nn;
+        G[side] = scl3(nn, dt * inv_den);
+        const int sa = face_slot(fi, 0), sb = face_slot(fi, 1), sc = face_slot(fi, 2);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) wv[side][s] = s == sa ? la : (s == sb ? lb : (s == sc ? lc : 0.0f));
+    }
+    acc[0] += med_clip(d_dens_l, kMinVal, kMaxVal);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        V3 d = add3(med_clip3(scl3(G[0], wv[0][s]), kMinVal, kMaxVal), med_clip3(scl3(G[1], wv[1][s]), kMinVal, kMaxVal));
+        if (s == 0) d = add3(d, scl3(neg3(T.g), dbo));
+        d = med_clip3(d, kMinVal, kMaxVal);
+        acc[1 + 3 * s] += d.x;
+        acc[2 + 3 * s] += d.y;
+        acc[3 + 3 * s] += d.z;
+    }
+    V3 dgc = med_clip3(dg, kMinVal, kMaxVal);
+    acc[13] += dgc.x;
+    acc[14] += dgc.y;
+    acc[15] += dgc.z;
+    V3 dbcc = med_clip3(mk3(dbc0, dbc1, dbc2), kMinVal, kMaxVal);
+    acc[16] += dbcc.x;
+    acc[17] += dbcc.y;
+    acc[18] += dbcc.z;
+}
+
 template <int TS>
 __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
     const float4* __restrict__ rec, const uint32_t* __restrict__ idx, const float4* __restrict__ colors,
@@ -480,7 +687,7 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
     float* __restrict__ loss_partial, float* __restrict__ d_verts, float* __restrict__ d_dens,
     float* __restrict__ d_grads, float* __restrict__ d_colors, uint64_t dcolor_view_stride) {
     using C = BwdCfg<TS>;
-    constexpr int NT = C::NT, B = C::B, CAP = C::CAP, SP = TS + 1, NW = NT / 64;
+    constexpr int NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1, NW = NT / 64;
     __shared__ BatchLDS<TS, B, true> L;
     // per-pair record: od -> t_prev, cp0 -> d_log_t before, cp1, cp2, t_min, t_max, meta
     __shared__ float s_r[6][CAP];
@@ -488,7 +695,9 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
     __shared__ float s_ray[3][TS * TS];
     __shared__ float s_dc[3][TS * TS];
     __shared__ float s_acc[B][kTerms];
+    __shared__ unsigned int s_mask[TS * TS];  // covered tets of the batch per pixel (B <= 32)
     __shared__ float s_loss[NW];
+    static_assert(B <= 32, "pixel masks hold 32 tets");
 
     const uint32_t gtile = blockIdx.x;
     const uint32_t view = gtile / tiles_per_view;
@@ -557,188 +766,115 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
     const uint32_t rs = ranges[2ull * gtile], re = ranges[2ull * gtile + 1];
     const uint32_t count = re > rs ? re - rs : 0u;
     uint32_t done = 0;
+    int nb = (int)min<uint32_t>(B, count);
+    TetFetch f;
+    if (tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
     while (done < count) {
-        const int nb = (int)min<uint32_t>(B, count - done);
         __syncthreads();
-        if (tid < nb) load_tet<TS, B, true>(L, tid, sorted_vals[rs + done + tid], rec, idx, col_v, vw, tc);
+        if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
+        if (tid < TS * TS) s_mask[tid] = 0u;
         __syncthreads();
-        batch_layout<TS, B, true, false, CAP, NT>(L, nb);
+        batch_layout<TS, B, true, false, CAP, R, NT>(L, nb);
         const uint32_t nbp = L.nbp;
-        const uint32_t npairs = L.tpref[nbp];
-        // phase A (pair-parallel): intersection + integral
-        for (uint32_t i = tid; i < npairs; i += NT) {
-            int k, r, c;
-            pair_locate<TS, B, true>(L, i, nbp, k, r, c);
-            const uint32_t gx = tc.tile_x * TS + c, gy = tc.tile_y * TS + r;
-            uint32_t meta = 0;
-            float od = 0.f, cp0 = 0.f, cp1 = 0.f, cp2 = 0.f, t_min = 0.f, t_max = 0.f;
-            if (gx < vw.width && gy < vw.height) {
-                const int p = r * TS + c;
-                V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
-                int fmin, fmax;
-                if (intersect<TS, B, true>(L, k, rd, 0.0f, t_min, t_max, fmin, fmax)) {
-                    // the reference's adjoint defaults both faces to 0 (backward_tiled_compute.wgsl:365-366)
-                    if (fmin == 4) fmin = 0;
-                    V3 g = mk3(L.g[0][k], L.g[1][k], L.g[2][k]);
-                    float dc_dt = dot3(g, rd);
-                    float b0 = L.bc[0][k], b1 = L.bc[1][k], b2 = L.bc[2][k];
-                    float a0 = dc_dt * t_min, a1 = dc_dt * t_max;
-                    float cs0 = fmaxf(b0 + a0, 0.f), cs1 = fmaxf(b1 + a0, 0.f), cs2 = fmaxf(b2 + a0, 0.f);
-                    float ce0 = fmaxf(b0 + a1, 0.f), ce1 = fmaxf(b1 + a1, 0.f), ce2 = fmaxf(b2 + a1, 0.f);
-                    float dist = t_max - t_min;
-                    od = clampf(L.dens[k] * dist, 1e-8f, 88.0f);
-                    float alpha_t = safe_exp(-od);
-                    float ph = phi_f(od);
-                    float w0 = ph - alpha_t, w1 = 1.0f - ph;
-                    cp0 = safe_clip(ce0 * w0 + cs0 * w1, kMinVal, kMaxVal);
-                    cp1 = safe_clip(ce1 * w0 + cs1 * w1, kMinVal, kMaxVal);
-                    cp2 = safe_clip(ce2 * w0 + cs2 * w1, kMinVal, kMaxVal);
-                    meta = (uint32_t)fmin | ((uint32_t)fmax << 2) | (1u << 4);
+        const uint32_t nchunks = L.cpref[nbp];
+        const uint32_t next = done + nbp;
+        const int nb_next = (int)min<uint32_t>(B, count - next);
+        if (tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
+        // phase A (chunk-parallel): intersection + integral
+        for (uint32_t j = tid; j < nchunks; j += NT) {
+            int k;
+            uint32_t l0, cnt;
+            chunk_locate<TS, B, true, R>(L, j, nbp, k, l0, cnt);
+            TetRegs T;
+            tet_regs(L, k, T);
+            PixWalk<TS> w;
+            w.init(L, k, l0);
+            const uint32_t slot0 = L.tpref[k] + l0;
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const uint32_t slot = slot0 + q;
+                const uint32_t gx = tc.tile_x * TS + w.col, gy = tc.tile_y * TS + w.row;
+                uint32_t meta = 0;
+                if (gx < vw.width && gy < vw.height) {
+                    const int p = w.row * TS + w.col;
+                    V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
+                    float t_min, t_max;
+                    int fmin, fmax;
+                    if (intersect(T, rd, 0.0f, t_min, t_max, fmin, fmax)) {
+                        // the reference's adjoint defaults both faces to 0 (backward_tiled_compute.wgsl:365-366)
+                        if (fmin == 4) fmin = 0;
+                        Integral I;
+                        integral(T, rd, t_min, t_max, 0.0f, kMinVal, I);
+                        s_r[0][slot] = I.od;
+                        s_r[1][slot] = I.cp[0];
+                        s_r[2][slot] = I.cp[1];
+                        s_r[3][slot] = I.cp[2];
+                        s_r[4][slot] = t_min;
+                        s_r[5][slot] = t_max;
+                        meta = (uint32_t)fmin | ((uint32_t)fmax << 2) | (1u << 4);
+                        atomicOr(&s_mask[p], 1u << k);
+                    }
                 }
+                s_meta[slot] = meta;
+                w.next(L, k);
             }
-            s_r[0][i] = od;
-            s_r[1][i] = cp0;
-            s_r[2][i] = cp1;
-            s_r[3][i] = cp2;
-            s_r[4][i] = t_min;
-            s_r[5][i] = t_max;
-            s_meta[i] = meta;
         }
         __syncthreads();
-        // phase B (pixel-sequential, back to front): undo and the d(log T) chain
-        // (backward_tiled_compute.wgsl:419-451)
+        // phase B (pixel-sequential, back to front over this pixel's covered
+        // tets): undo and the d(log T) chain (backward_tiled_compute.wgsl:419-451)
         if (active) {
             const float dc0 = s_dc[0][tid], dc1 = s_dc[1][tid], dc2 = s_dc[2][tid];
-            for (uint32_t k = 0; k < nbp; ++k) {
+            unsigned int m = s_mask[tid];
+            while (m) {
+                const uint32_t k = (uint32_t)__ffs((int)m) - 1u;
+                m &= m - 1u;
                 const uint32_t sp = L.span[k * SP + row];
-                const int xl = span_xl(sp);
-                if (colp < xl || colp > span_xr(sp)) continue;
-                const uint32_t slot = L.tpref[k] + L.rpref[k * SP + row] + (uint32_t)(colp - xl);
-                const uint32_t meta = s_meta[slot];
-                if (!(meta & 16u)) continue;
+                const uint32_t slot = L.tpref[k] + L.rpref[k * SP + row] + (uint32_t)(colp - span_xl(sp));
                 const float od = s_r[0][slot];
                 const float prev_log_t = fminf(log_t + od, 0.0f);
-                const float t_prev = safe_exp(prev_log_t);
+                const float t_prev = fast_safe_exp(prev_log_t);
                 log_t = prev_log_t;
                 if (t_prev < 1e-6f) continue;
                 const float cp0 = s_r[1][slot], cp1 = s_r[2][slot], cp2 = s_r[3][slot];
                 s_r[0][slot] = t_prev;
                 s_r[1][slot] = d_log_t;
-                s_meta[slot] = meta | 32u;
+                s_meta[slot] |= 32u;
                 d_log_t = safe_clip(d_log_t + ((dc0 * cp0 + dc1 * cp1) + dc2 * cp2) * t_prev, kMinVal, kMaxVal);
             }
         }
         __syncthreads();
-        // phase C (pair-parallel): gradient terms, segmented reduction by tet
-        // (backward_tiled_compute.wgsl:446-543)
-        for (uint32_t base = (uint32_t)wid * 64; base < npairs; base += NT) {
-            const uint32_t i = base + lane;
+        // phase C (chunk-parallel): gradient terms of R pixels of one tet per
+        // lane, then a segmented wave sum keyed by tet
+        for (uint32_t base = (uint32_t)wid * 64; base < nchunks; base += NT) {
+            const uint32_t j = base + lane;
             float acc[kTerms];
 #pragma unroll
             for (int q = 0; q < kTerms; ++q) acc[q] = 0.0f;
             int key = 0x7fffffff;
-            if (i < npairs) {
-                int k, r, c;
-                pair_locate<TS, B, true>(L, i, nbp, k, r, c);
+            if (j < nchunks) {
+                int k;
+                uint32_t l0, cnt;
+                chunk_locate<TS, B, true, R>(L, j, nbp, k, l0, cnt);
                 key = k;
-                const uint32_t meta = s_meta[i];
-                if (meta & 32u) {
-                    const int p = r * TS + c;
-                    const V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
-                    const float dc0 = s_dc[0][p], dc1 = s_dc[1][p], dc2 = s_dc[2][p];
-                    const float t_prev = s_r[0][i], d_log_t_b = s_r[1][i];
-                    const float t_min = s_r[4][i], t_max = s_r[5][i];
-                    const int min_face = (int)(meta & 3u), max_face = (int)((meta >> 2) & 3u);
-                    const float dens = L.dens[k];
-                    V3 g = mk3(L.g[0][k], L.g[1][k], L.g[2][k]);
-                    float dc_dt = dot3(g, rd);
-                    float b0 = L.bc[0][k], b1 = L.bc[1][k], b2 = L.bc[2][k];
-                    float a0 = dc_dt * t_min, a1 = dc_dt * t_max;
-                    float csr0 = b0 + a0, csr1 = b1 + a0, csr2 = b2 + a0;
-                    float cer0 = b0 + a1, cer1 = b1 + a1, cer2 = b2 + a1;
-                    float cs0 = fmaxf(csr0, 0.f), cs1 = fmaxf(csr1, 0.f), cs2 = fmaxf(csr2, 0.f);
-                    float ce0 = fmaxf(cer0, 0.f), ce1 = fmaxf(cer1, 0.f), ce2 = fmaxf(cer2, 0.f);
-                    float dist = t_max - t_min;
-                    float od = clampf(dens * dist, 1e-8f, 88.0f);
-                    float alpha_t = safe_exp(-od);
-                    float ph = phi_f(od);
-                    float w0 = ph - alpha_t, w1 = 1.0f - ph;
-                    float dcp0 = safe_clip(dc0 * t_prev, kMinVal, kMaxVal);
-                    float dcp1 = safe_clip(dc1 * t_prev, kMinVal, kMaxVal);
-                    float dcp2 = safe_clip(dc2 * t_prev, kMinVal, kMaxVal);
-                    float d_od_state = -d_log_t_b;
-                    float dphi = dphi_dx_f(od);
-                    float dw0 = dphi + alpha_t, dw1 = -dphi;
-                    float d_od_int = (dcp0 * (ce0 * dw0 + cs0 * dw1) + dcp1 * (ce1 * dw0 + cs1 * dw1)) +
-                                     dcp2 * (ce2 * dw0 + cs2 * dw1);
-                    float d_od = safe_clip(d_od_state + d_od_int, kMinVal, kMaxVal);
-                    float od_raw = dens * dist;
-                    bool act = (od_raw >= 1e-8f) && (od_raw <= 88.0f);
-                    float d_od_raw = act ? d_od : 0.0f;
-                    float d_dens_l = safe_clip(d_od_raw * dist, kMinVal, kMaxVal);
-                    float d_dist = safe_clip(d_od_raw * dens, kMinVal, kMaxVal);
-                    float d_tmin = -d_dist, d_tmax = d_dist;
-                    float dcs0 = csr0 > 0.f ? dcp0 * w1 : 0.f, dcs1 = csr1 > 0.f ? dcp1 * w1 : 0.f;
-                    float dcs2 = csr2 > 0.f ? dcp2 * w1 : 0.f;
-                    float dce0 = cer0 > 0.f ? dcp0 * w0 : 0.f, dce1 = cer1 > 0.f ? dcp1 * w0 : 0.f;
-                    float dce2 = cer2 > 0.f ? dcp2 * w0 : 0.f;
-                    float dbc0 = dcs0 + dce0, dbc1 = dcs1 + dce1, dbc2 = dcs2 + dce2;
-                    float sum_s = (dcs0 + dcs1) + dcs2, sum_e = (dce0 + dce1) + dce2;
-                    float d_dcdt = sum_s * t_min + sum_e * t_max;
-                    d_tmin = d_tmin + sum_s * dc_dt;
-                    d_tmax = d_tmax + sum_e * dc_dt;
-                    float dbo = (dbc0 + dbc1) + dbc2;
-                    V3 v0 = mk3(L.v[0][0][k], L.v[0][1][k], L.v[0][2][k]);
-                    V3 dg = add3(scl3(sub3(cam, v0), dbo), scl3(rd, d_dcdt));
-                    V3 dv[4] = {mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
-                    const uint32_t flips = L.flip[k];
-#pragma unroll
-                    for (int side = 0; side < 2; ++side) {
-                        int fi = side == 0 ? min_face : max_face;
-                        float tt = side == 0 ? t_min : t_max;
-                        float dt = side == 0 ? d_tmin : d_tmax;
-                        int a, b, cc, o;
-                        face_idx(fi, a, b, cc, o);
-                        V3 va = mk3(L.v[a][0][k], L.v[a][1][k], L.v[a][2][k]);
-                        V3 vb = mk3(L.v[b][0][k], L.v[b][1][k], L.v[b][2][k]);
-                        V3 vc = mk3(L.v[cc][0][k], L.v[cc][1][k], L.v[cc][2][k]);
-                        V3 nn = mk3(L.n[fi][0][k], L.n[fi][1][k], L.n[fi][2][k]);
-                        if ((flips >> fi) & 1u) nn = neg3(nn);  // raw (unflipped) normal
-                        float den = dot3(nn, rd);
-                        float inv_den = safe_div(1.0f, den);
-                        V3 hit = add3(cam, scl3(rd, tt));
-                        V3 ah = sub3(va, hit);
-                        V3 dta = scl3(add3(cross3(ah, sub3(vb, vc)), nn), inv_den);
-                        V3 dtb = scl3(cross3(ah, sub3(vc, va)), inv_den);
-                        V3 dtc = scl3(cross3(ah, sub3(va, vb)), inv_den);
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) {
-                            if (s == a) dv[s] = add3(dv[s], safe_clip3(scl3(dta, dt), kMinVal, kMaxVal));
-                            if (s == b) dv[s] = add3(dv[s], safe_clip3(scl3(dtb, dt), kMinVal, kMaxVal));
-                            if (s == cc) dv[s] = add3(dv[s], safe_clip3(scl3(dtc, dt), kMinVal, kMaxVal));
-                        }
+                TetRegs T;
+                tet_regs(L, k, T);
+                const V3 v0 = mk3(L.v0[0][k], L.v0[1][k], L.v0[2][k]);
+                PixWalk<TS> w;
+                w.init(L, k, l0);
+                const uint32_t slot0 = L.tpref[k] + l0;
+                for (uint32_t q = 0; q < cnt; ++q) {
+                    const uint32_t slot = slot0 + q;
+                    const uint32_t meta = s_meta[slot];
+                    if (meta & 32u) {
+                        const int p = w.row * TS + w.col;
+                        const V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
+                        pair_grad<TS, B>(L, k, T, v0, cam, rd, s_dc[0][p], s_dc[1][p], s_dc[2][p], s_r[0][slot],
+                                         s_r[1][slot], s_r[4][slot], s_r[5][slot], (int)(meta & 3u),
+                                         (int)((meta >> 2) & 3u), acc);
                     }
-                    dv[0] = add3(dv[0], scl3(neg3(g), dbo));
-                    acc[0] = safe_clip(d_dens_l, kMinVal, kMaxVal);
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        V3 c3 = safe_clip3(dv[s], kMinVal, kMaxVal);
-                        acc[1 + 3 * s] = c3.x;
-                        acc[2 + 3 * s] = c3.y;
-                        acc[3 + 3 * s] = c3.z;
-                    }
-                    V3 dgc = safe_clip3(dg, kMinVal, kMaxVal);
-                    acc[13] = dgc.x;
-                    acc[14] = dgc.y;
-                    acc[15] = dgc.z;
-                    V3 dbcc = safe_clip3(mk3(dbc0, dbc1, dbc2), kMinVal, kMaxVal);
-                    acc[16] = dbcc.x;
-                    acc[17] = dbcc.y;
-                    acc[18] = dbcc.z;
+                    w.next(L, k);
                 }
             }
-            // segmented sum over consecutive pairs of the same tet
             const SegMasks sm = seg_masks(key);
             const int next_key = __shfl_down(key, 1, 64);
             const bool tail = key != 0x7fffffff && (lane == 63 || next_key != key);
@@ -762,7 +898,8 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
             else dst = dcol_v + 3ull * t + (q - 16);
             unsafeAtomicAdd(dst, s);
         }
-        done += nbp;
+        done = next;
+        nb = nb_next;
     }
 }
 

@@ -54,27 +54,30 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_upsweep(const uint64_t* __r
     hist[(uint64_t)d * G + b] = s;
 }
 
-// Exclusive scan over hist[256][G] in digit-major order (one workgroup).
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t total_n) {
-    __shared__ uint32_t red[1024 / kWave];
-    uint32_t per = (total_n + blockDim.x - 1) / blockDim.x;
+// Exclusive scan of hist[d][0..G) for one digit d per workgroup (256
+// workgroups in parallel); the digit total goes to totals[d].
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t G, uint32_t* __restrict__ totals) {
+    __shared__ uint32_t red[256 / kWave];
+    uint32_t* h = hist + (uint64_t)blockIdx.x * G;
+    uint32_t per = (G + blockDim.x - 1) / blockDim.x;
     uint32_t lo = threadIdx.x * per;
-    uint32_t hi = min(lo + per, total_n);
+    uint32_t hi = min(lo + per, G);
     uint32_t s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += hist[i];
+    for (uint32_t i = lo; i < hi; ++i) s += h[i];
     uint32_t tot;
     uint32_t ex = block_excl_scan(s, red, tot);
     for (uint32_t i = lo; i < hi; ++i) {
-        uint32_t x = hist[i];
-        hist[i] = ex;
+        uint32_t x = h[i];
+        h[i] = ex;
         ex += x;
     }
+    if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                                uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                                const uint32_t* __restrict__ n_ptr, uint32_t cap, uint32_t shift,
-                                                               const uint32_t* __restrict__ hist) {
+                                                               const uint32_t* __restrict__ hist, const uint32_t* __restrict__ totals) {
     __shared__ uint64_t s_keys[kTileKeys];
     __shared__ uint32_t s_vals[kTileKeys];
     __shared__ uint32_t s_wcnt[kSortWaves][256];
@@ -88,7 +91,12 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
     uint32_t lo, hi;
     chunk_of(n, G, b, lo, hi);
     const uint32_t tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
-    s_run[tid] = hist[(uint64_t)tid * G + b];
+    {
+        // global start of digit `tid` = exclusive scan of the digit totals
+        uint32_t all;
+        uint32_t dstart = block_excl_scan(totals[tid], s_red, all);
+        s_run[tid] = dstart + hist[(uint64_t)tid * G + b];
+    }
     const uint64_t lt = lanemask_lt();
 
     for (uint32_t base = lo; base < hi; base += kTileKeys) {
@@ -180,13 +188,14 @@ bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, c
         uint64_t* kout = in_b ? ka : kb;
         uint32_t* vout = in_b ? va : vb;
         hipLaunchKernelGGL(k_rs_upsweep, dim3(G), dim3(kSortThreads), 0, s, kin, n_ptr, cap, shift, hist);
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, s, hist, 256u * G);
-        hipLaunchKernelGGL(k_rs_downsweep, dim3(G), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_ptr, cap, shift, hist);
+        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, s, hist, G, hist + 256ull * G);
+        hipLaunchKernelGGL(k_rs_downsweep, dim3(G), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_ptr, cap, shift, hist,
+                           hist + 256ull * G);
         in_b = !in_b;
     }
     return in_b;
 }
 
-size_t radix_hist_words(uint32_t cap) { return 256ull * sort_grid(cap); }
+size_t radix_hist_words(uint32_t cap) { return 256ull * sort_grid(cap) + 256; }
 
 }  // namespace rmesh

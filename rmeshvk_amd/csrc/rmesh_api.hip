@@ -969,3 +969,38 @@ extern "C" rmesh_status rmesh_read_buffer(rmesh_renderer* h, int which, void* ho
     HIP_TRY(hipMemcpy(host, src, nbytes, hipMemcpyDeviceToHost));
     return RMESH_OK;
 }
+
+extern "C" rmesh_status rmesh_copy_probe(int32_t device_id, uint64_t nbytes, uint32_t iters, float* ms_per_copy) {
+    if (nbytes < 16 || nbytes % 16) return fail(RMESH_ERR_INVALID, "nbytes must be a positive multiple of 16");
+    int dev = device_id;
+    if (dev < 0) (void)hipGetDevice(&dev);
+    DeviceGuard g(dev);
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rmesh_status st = RMESH_OK;
+    if (dalloc(&a, nbytes / 16) != hipSuccess || dalloc(&b, nbytes / 16) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        st = fail(RMESH_ERR_DEVICE, "copy probe allocation failed");
+    } else {
+        launch_fill_u32(reinterpret_cast<uint32_t*>(a), 0x3f800000u, nbytes / 4, s);
+        launch_copy_probe(a, b, nbytes / 16, s);
+        (void)hipEventRecord(e0, s);
+        for (uint32_t i = 0; i < iters; ++i) launch_copy_probe(a, b, nbytes / 16, s);
+        (void)hipEventRecord(e1, s);
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(RMESH_ERR_DEVICE, std::string("copy probe: ") + hipGetErrorString(e));
+        else if (ms_per_copy) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *ms_per_copy = iters ? ms / iters : 0.f;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return st;
+}

@@ -89,6 +89,34 @@ __device__ __forceinline__ float dsoftplus_f(float x) {
     return safe_div(e, 1.0f + e);
 }
 
+// Fast variants for the per-pair compositing math, where the result is
+// compared within tolerance (images 1e-4, gradients 1e-3): hardware
+// reciprocal (v_rcp_f32, 1 ulp) and base-2 exponential (v_exp_f32) instead of
+// the IEEE division / accurate expf sequences.  Coverage decisions (scanline
+// spans) never use these.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+__device__ __forceinline__ float fast_safe_div(float a, float b) {
+    if (fabsf(b) < kTiny) return safe_clip(a * (1.0f / kTiny), kMinVal, kMaxVal);
+    return safe_clip(fast_div(a, b), kMinVal, kMaxVal);
+}
+// Clamps for the per-pair math as one v_med3_f32 (no NaN canonicalisation).
+__device__ __forceinline__ float med_clip(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
+__device__ __forceinline__ V3 med_clip3(V3 v, float lo, float hi) {
+    return V3{med_clip(v.x, lo, hi), med_clip(v.y, lo, hi), med_clip(v.z, lo, hi)};
+}
+__device__ __forceinline__ float relu(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, __builtin_huge_valf()); }
+__device__ __forceinline__ float fast_safe_exp(float v) { return fast_exp(clampf(v, -88.0f, kLogMaxVal)); }
+__device__ __forceinline__ float fast_phi(float x, float ex) {  // ex = exp(-x)
+    if (fabsf(x) < 0.02f) return 1.0f + x * (-0.5f + x * ((float)(1.0 / 6.0) + x * ((float)(-1.0 / 24.0))));
+    return fast_safe_div(1.0f - ex, x);
+}
+__device__ __forceinline__ float fast_dphi(float x, float ex) {
+    if (fabsf(x) < 0.02f) return -0.5f + x * ((float)(1.0 / 3.0) + x * ((float)(-1.0 / 8.0) + x * ((float)(1.0 / 30.0))));
+    return fast_safe_div(ex * (x + 1.0f) - 1.0f, x * x);
+}
+
 // WGSL-style saturating conversions (explicit: no reliance on UB rules).
 __device__ __forceinline__ uint32_t sat_u32(float x) {
     if (!(x > 0.0f)) return 0u;

@@ -1,0 +1,192 @@
+"""CPU: the oracle's forward path (projection, pairs, sort, ranges, compositing)
+against an independent float64 quadrature renderer (tests/npref.py), closed
+forms of the regular tet, and brute-force tile coverage.
+
+Scenes here keep every chord's colour positive, since the reference clamps
+endpoint colours at 0 (a kink the quadrature does not model); the clamp
+itself is exercised by the GPU-vs-oracle parity tests.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import npref
+from oracle import oracle as orc
+from rmeshvk_amd.camera import fibonacci_cameras, make_camera
+from rmeshvk_amd.scenes import make_scene, regular_tet_scene, two_tet_scene, uniform_points
+
+IMG_TOL = 5e-5  # float32 oracle vs float64 quadrature
+
+
+def _positive_colour_scene(n_points, seed):
+    rng = np.random.default_rng(seed)
+    s = make_scene(uniform_points(n_points, rng), rng)
+    M = len(s["densities"])
+    s["base_colors"] = rng.uniform(0.3, 1.0, 3 * M).astype(np.float32)
+    s["color_grads"] = rng.uniform(-0.08, 0.08, 3 * M).astype(np.float32)
+    return s
+
+
+def test_regular_tet_closed_form():
+    """g = 0: RGB = c * alpha with alpha = 1 - exp(-sigma * chord)."""
+    s = regular_tet_scene(3.0)
+    s["base_colors"] = np.array([0.9, 0.5, 0.2], np.float32)
+    cam = make_camera((0.0, -2.0, 0.5), (0.0, 0.0, 0.0), math.pi / 2, 64, 64)
+    o, d = npref.pixel_rays(cam, 64, 64)
+    t0, t1, hit = npref.tet_chords(np.asarray(s["vertices"], np.float64).reshape(1, 4, 3), o, d)
+    alpha = np.where(hit[:, 0], 1 - np.exp(-3.0 * (t1[:, 0] - t0[:, 0])), 0.0).reshape(64, 64)
+    img = orc.render(s, cam, 64, 64)["rgba"]
+    assert alpha.max() > 0.5
+    np.testing.assert_allclose(img[..., 3], alpha, atol=IMG_TOL)
+    for c in range(3):
+        np.testing.assert_allclose(img[..., c], s["base_colors"][c] * alpha, atol=IMG_TOL)
+
+
+def test_regular_tet_linear_colour_quadrature():
+    s = regular_tet_scene(2.0)
+    s["base_colors"] = np.array([1.0, 0.8, 0.9], np.float32)
+    s["color_grads"] = np.array([0.3, -0.2, 0.25], np.float32)
+    cam = make_camera((0.7, -2.0, 0.9), (0.0, 0.1, 0.0), math.radians(70), 48, 40)
+    assert npref.endpoint_colours_positive(s, cam, 48, 40)
+    ref = npref.render(s, cam, 48, 40)
+    img = orc.render(s, cam, 48, 40)["rgba"]
+    assert np.abs(img - ref).max() < IMG_TOL
+
+
+def test_two_tet_quadrature():
+    s = two_tet_scene(np.random.default_rng(1))
+    cam = make_camera((0.5, -2.0, 1.5), (0.5, 0.3, 0.0), math.radians(60), 64, 64)
+    assert npref.endpoint_colours_positive(s, cam, 64, 64)
+    ref = npref.render(s, cam, 64, 64)
+    img = orc.render(s, cam, 64, 64)["rgba"]
+    assert (ref[..., 3] > 0.01).sum() > 100
+    assert np.abs(img - ref).max() < IMG_TOL
+
+
+@pytest.mark.parametrize("view", [0, 1, 2])
+def test_delaunay_quadrature(view):
+    """~280 tets: depth order, tile binning, spans and the closed-form integral."""
+    s = _positive_colour_scene(60, 7)
+    cam = fibonacci_cameras(3, 3.0, 50.0, 64, 64)[view]
+    assert npref.endpoint_colours_positive(s, cam, 64, 64)
+    ref = npref.render(s, cam, 64, 64)
+    img = orc.render(s, cam, 64, 64)["rgba"]
+    assert (ref[..., 3] > 0.1).sum() > 500
+    assert np.abs(img - ref).max() < IMG_TOL
+
+
+def test_min_t_reparametrises_without_clipping():
+    """min_t moves the ray origin to cam + min_t * dir (rasterize_compute.wgsl:351-352)
+    but the slab test still takes the whole line, so the image is the min_t = 0
+    image (up to rounding)."""
+    s = _positive_colour_scene(60, 7)
+    cam = fibonacci_cameras(3, 3.0, 50.0, 64, 64)[1]
+    ref = npref.render(s, cam, 64, 64)
+    img = orc.render(s, cam, 64, 64, min_t=2.5)["rgba"]
+    assert np.abs(img - ref).max() < IMG_TOL
+
+
+@pytest.mark.parametrize("ts", [8, 12, 16])
+def test_tile_size_invariance(ts):
+    s = _positive_colour_scene(60, 11)
+    cam = fibonacci_cameras(2, 3.0, 50.0, 70, 52)[0]
+    a = orc.render(s, cam, 70, 52, ts)["rgba"]
+    b = orc.render(s, cam, 70, 52, 12)["rgba"]
+    assert np.abs(a - b).max() < 1e-5
+
+
+def test_aux_normals_depth_single_tet():
+    """Single tet: normals = alpha * outward unit normal of the entry face;
+    depth = the transmittance-weighted mean ray parameter, by quadrature."""
+    s = regular_tet_scene(2.0)
+    cam = make_camera((0.6, -2.0, 0.8), (0.0, 0.0, 0.0), math.radians(70), 40, 40)
+    out = orc.render(s, cam, 40, 40, want_aux=True)
+    aux, img = out["aux"], out["rgba"]
+    o, d = npref.pixel_rays(cam, 40, 40)
+    tv = np.asarray(s["vertices"], np.float64).reshape(4, 3)
+    t0, t1, hit = npref.tet_chords(tv[None], o, d)
+    t0, t1, hit = t0[:, 0], t1[:, 0], hit[:, 0]
+    # entry face: the face plane the ray crosses at t0, found by distance to each plane
+    centroid = tv.mean(0)
+    p_in = o[None, :] + d * t0[:, None]
+    normals = np.zeros((len(d), 3))
+    best = np.full(len(d), np.inf)
+    for opp in range(4):
+        tri = [i for i in range(4) if i != opp]
+        n = np.cross(tv[tri[1]] - tv[tri[0]], tv[tri[2]] - tv[tri[0]])
+        n /= np.linalg.norm(n)
+        if np.dot(n, centroid - tv[tri[0]]) > 0:
+            n = -n  # outward
+        dist = np.abs((p_in - tv[tri[0]]) @ n)
+        upd = dist < best
+        best = np.where(upd, dist, best)
+        normals = np.where(upd[:, None], n[None, :], normals)
+    alpha = np.where(hit, 1 - np.exp(-2.0 * (t1 - t0)), 0.0)
+    L = np.where(hit, t1 - t0, 0.0)
+    x, wq = np.polynomial.legendre.leggauss(24)
+    depth = np.zeros(len(d))
+    for xi, wi in zip(x, wq):
+        ss = 0.5 * L * (xi + 1)
+        depth += 0.5 * L * wi * 2.0 * np.exp(-2.0 * ss) * (t0 + ss)
+    depth = np.where(hit, depth, 0.0)
+    sel = (alpha > 1e-3).reshape(40, 40)
+    assert sel.sum() > 100
+    np.testing.assert_allclose(aux[..., :3][sel], (alpha[:, None] * normals).reshape(40, 40, 3)[sel], atol=1e-4)
+    np.testing.assert_allclose(aux[..., 3][sel], depth.reshape(40, 40)[sel], atol=1e-4)
+    np.testing.assert_allclose(img[..., 3], alpha.reshape(40, 40), atol=IMG_TOL)
+
+
+# --------------------------------------------------------------------------
+# binning
+# --------------------------------------------------------------------------
+def test_binning_covers_every_hit_pixel():
+    """Every pixel whose ray has a non-grazing chord through a tet lies in a tile
+    the tet was binned to (the scanline is conservative)."""
+    s = _positive_colour_scene(80, 3)
+    w, h, ts = 72, 60, 12
+    for cam in fibonacci_cameras(3, 3.0, 50.0, w, h):
+        tt, _ = orc.project(s, cam, w, h, ts)
+        keys, vals, P = orc.pairs(s, cam, w, h, tt, ts)
+        assert P == int(tt.sum())
+        tx = (w + ts - 1) // ts
+        binned = set(zip((keys >> np.uint64(32)).astype(np.int64).tolist(), vals.astype(np.int64).tolist()))
+        o, d = npref.pixel_rays(cam, w, h)
+        V = np.asarray(s["vertices"], np.float64).reshape(-1, 3)
+        tv = V[np.asarray(s["indices"], np.int64).reshape(-1, 4)]
+        t0, t1, hit = npref.tet_chords(tv, o, d)
+        solid = hit & (t1 - t0 > 1e-3) & (t0 > 0)
+        pix, tet = np.nonzero(solid)
+        tiles = (pix // w // ts) * tx + (pix % w) // ts
+        missing = [(a, b) for a, b in zip(tiles.tolist(), tet.tolist()) if (a, b) not in binned]
+        assert not missing, missing[:10]
+
+
+def test_keys_decode_to_tile_and_depth():
+    s = _positive_colour_scene(80, 5)
+    w = h = 64
+    cam = fibonacci_cameras(2, 3.0, 50.0, w, h)[1]
+    tt, _ = orc.project(s, cam, w, h, 12)
+    keys, vals, P = orc.pairs(s, cam, w, h, tt, 12)
+    cd = np.asarray(s["circumdata"], np.float32).reshape(-1, 4)[vals]
+    c = np.asarray(cam[0], np.float32)
+    dx, dy, dz = cd[:, 0] - c[0], cd[:, 1] - c[1], cd[:, 2] - c[2]
+    depth32 = ((dx * dx + dy * dy) + dz * dz) - cd[:, 3]
+    low = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    np.testing.assert_array_equal(low, ~depth32.astype(np.float32).view(np.uint32))
+    d2 = ((cd[:, :3].astype(np.float64) - c) ** 2).sum(1)
+    depth64 = d2 - cd[:, 3]
+    err = np.abs((~low).view(np.float32).astype(np.float64) - depth64)
+    assert np.all(err <= 1e-6 * (d2 + cd[:, 3]))  # float32 rounding of the operands
+    assert np.all((keys >> np.uint64(32)) < np.uint64(((w + 11) // 12) ** 2))
+
+
+def test_f32_f64_oracle_agree():
+    s = _positive_colour_scene(60, 2)
+    cam = fibonacci_cameras(2, 3.0, 50.0, 48, 48)[0]
+    a = orc.render(s, cam, 48, 48)["rgba"]
+    b = orc.render(s, cam, 48, 48, dtype=np.float64)["rgba"]
+    assert np.abs(a - b).max() < IMG_TOL
