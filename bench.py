@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--tile-size", type=int, default=12)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep --tile-size and the default kernel variants")
+    ap.add_argument("--tuning", default="", help="TS,F,B: tile size and kernel variants, no autotune")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
@@ -148,6 +150,16 @@ def main():
     r.set_targets(np.ascontiguousarray(targets))
     log(f"targets rendered; setup {time.perf_counter() - t_setup:.1f} s")
 
+    tuning = None
+    if args.tuning:
+        ts_, fv_, bv_ = (int(x) for x in args.tuning.split(","))
+        r.set_tuning(ts_, fv_, bv_)
+    elif not args.no_autotune:
+        ta = time.perf_counter()
+        tuning = r.autotune(cams, loss_type=0, reps=3)
+        log(f"autotune: tile/fwd/bwd = {tuning['tuning']}, {tuning['ms']} ({time.perf_counter() - ta:.1f} s)")
+    tile_size = r.get_tuning()[0]
+
     graph = not args.no_graph
     r.set_graph_mode(graph)
     r.set_timing(True)
@@ -171,7 +183,7 @@ def main():
     stages = {k: v / args.steps for k, v in stage_acc.items()}
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value:.1f} views/s, loss {loss:.6f}; stages {stages}")
 
-    T = ((Wd + args.tile_size - 1) // args.tile_size) ** 2
+    T = ((Wd + tile_size - 1) // tile_size) ** 2
     dom = max(("forward", "backward"), key=lambda k: stages[k])
     alg = algorithmic_bytes(dom, P, V, Wd, Wd, T, N, M)
     achieved = alg / (stages[dom] * 1e-3) / 1e9
@@ -200,7 +212,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         tc = time.perf_counter()
-        cpu = cpu_baseline(scene, cams[0], Wd, args.cpu_stride, args.tile_size)
+        cpu = cpu_baseline(scene, cams[0], Wd, args.cpu_stride, tile_size)
         log(f"cpu baseline: {cpu['value']:.4f} views/s ({time.perf_counter() - tc:.1f} s)")
 
     copy_probe = None
@@ -233,7 +245,9 @@ def main():
         "config": {
             "workload": (f"{args.points}-point 8-cluster Delaunay mesh ({M} tets), SH degree 3, {V} views at "
                          f"{Wd}x{Wd}, forward+backward+Adam per step, L1 vs perturbed-parameter targets"),
-            "tets": M, "vertices": N, "views_per_step": V, "tile_size": args.tile_size, "hipgraph": graph,
+            "tets": M, "vertices": N, "views_per_step": V, "tile_size": tile_size, "hipgraph": graph,
+            "kernel_variants": {"forward": r.get_tuning()[1], "backward": r.get_tuning()[2]},
+            "autotune_ms": tuning["ms"] if tuning else None,
             "pairs_per_step": P, "loss_first_warmup": loss0, "loss_last": loss,
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},

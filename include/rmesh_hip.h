@@ -130,6 +130,26 @@ rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t n_views, const
                                       float lr_dens, float lr_grads, float lr_sh, float* out_loss);
 rmesh_status rmesh_set_graph_mode(rmesh_renderer* h, int enable);
 
+/* ---- tuning (no counterpart in the reference, which fixes the tile size at
+ * 12 in its Python path, lib.rs:442, and has one kernel shape) ---- */
+
+/* Tile size (8, 12, 16) and batch-geometry variants of the forward and
+ * adjoint compositing kernels (0 .. rmesh_num_variants(0|1) - 1).  Changing
+ * them drops a captured graph and the last forward's state. */
+rmesh_status rmesh_set_tuning(rmesh_renderer* h, uint32_t tile_size, int32_t fwd_variant, int32_t bwd_variant);
+/* out3 = {tile_size, fwd_variant, bwd_variant}. */
+rmesh_status rmesh_get_tuning(const rmesh_renderer* h, int32_t* out3);
+/* Number of kernel variants: which = 0 forward, 1 adjoint. */
+int32_t rmesh_num_variants(int32_t which);
+/* Autotuner: times binning + sort, every forward variant and every adjoint
+ * variant for each tile size on these views (best of `reps` runs, HIP
+ * events) and keeps the fastest combination.  Parameters and optimiser state
+ * are untouched; gradient buffers are cleared.  gt_dev as in
+ * rmesh_train_step_batched.  out_ms (nullable, 4 floats): best total,
+ * binning + sort, forward, adjoint. */
+rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t n_views, const float* cams, const float* vps,
+                            const float* c2ws, const float* gt_dev, uint32_t loss_type, uint32_t reps, float* out_ms);
+
 #define RMESH_BUF_RGBA 0        /* float [max_views][H][W][4] */
 #define RMESH_BUF_GT 1          /* float [max_views][H][W][3] */
 #define RMESH_BUF_VERTICES 2    /* float [N*3] */

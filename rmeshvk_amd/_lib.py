@@ -65,6 +65,10 @@ SIGNATURES = {
     "rmesh_train_step_batched": (_st, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32, _f32,
                                        ctypes.POINTER(_f32)]),
     "rmesh_set_graph_mode": (_st, [_vp, ctypes.c_int]),
+    "rmesh_set_tuning": (_st, [_vp, _u32, _i32, _i32]),
+    "rmesh_get_tuning": (_st, [_vp, _vp]),
+    "rmesh_num_variants": (_i32, [_i32]),
+    "rmesh_autotune": (_st, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "rmesh_device_buffer": (_vp, [_vp, ctypes.c_int]),
     "rmesh_synchronize": (_st, [_vp]),
     "rmesh_set_targets": (_st, [_vp, _u32, _vp]),

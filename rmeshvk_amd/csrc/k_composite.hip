@@ -109,7 +109,7 @@ struct BatchLDS {
     float pxy[8][B];        // tile-local projected vertices (x0..x3, y0..y3)
     uint32_t behind[B];
     uint32_t span[B * SP];  // [tet][row] packed span
-    uint32_t rpref[B * SP]; // [tet][row] exclusive prefix of row widths; [tet][TS] = tet total
+    uint32_t rpref[B * SP]; // [tet][row] prefix of row widths ([tet][TS] = tet total), then record base - first column
     uint32_t tpref[B + 1];  // exclusive prefix of covered pixels per tet (record slots)
     uint32_t cpref[B + 1];  // exclusive prefix of work chunks per tet
     uint32_t nbp;           // tets processed this batch (capacity cut)
@@ -160,12 +160,22 @@ __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const 
         L.n[fi][2][k] = nn.z;
         L.num[fi][k] = dot3(nn, sub3(v[a], cam));
         if (BWD) {
+            // barycentric weights of a point h of the face plane, affine in h:
+            // lambda_a(h) = lambda_a(v0) + grad_a . (h - v0), grad_a = ((c - b) x raw) / |raw|^2
+            // (and cyclic: (a - c) for b, (b - a) for c)
+            const float inv = 1.0f / dot3(raw, raw);
+            const V3 ga = scl3(cross3(sub3(v[c], v[b]), raw), inv);
+            const V3 gb = scl3(cross3(sub3(v[a], v[c]), raw), inv);
+            const V3 gc = scl3(cross3(sub3(v[b], v[a]), raw), inv);
+            const V3 pa = sub3(v[a], v[0]), pb = sub3(v[b], v[0]), pc = sub3(v[c], v[0]);
             float* F = &L.face[k * L.FS + fi * 16];
-            F[0] = v[a].x; F[1] = v[a].y; F[2] = v[a].z;
-            F[3] = v[b].x; F[4] = v[b].y; F[5] = v[b].z;
-            F[6] = v[c].x; F[7] = v[c].y; F[8] = v[c].z;
+            F[0] = ga.x; F[1] = ga.y; F[2] = ga.z;
+            F[3] = gb.x; F[4] = gb.y; F[5] = gb.z;
+            F[6] = gc.x; F[7] = gc.y; F[8] = gc.z;
             F[9] = raw.x; F[10] = raw.y; F[11] = raw.z;
-            F[12] = 1.0f / dot3(raw, raw);
+            F[12] = dot3(cross3(pc, pb), raw) * inv;  // lambda_a(v0)
+            F[13] = dot3(cross3(pa, pc), raw) * inv;  // lambda_b(v0)
+            F[14] = dot3(cross3(pb, pa), raw) * inv;  // lambda_c(v0)
         }
     }
     float bo = dot3(g, sub3(cam, v[0]));
@@ -249,19 +259,40 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb) {
     __syncthreads();
 }
 
-// Work chunk j of the batch: tet k and its covered pixels [l0, l0 + cnt).
-template <int TS, int B, bool BWD, int R>
-__device__ __forceinline__ void chunk_locate(const BatchLDS<TS, B, BWD>& L, uint32_t j, uint32_t nbp, int& k,
-                                             uint32_t& l0, uint32_t& cnt) {
-    int lo = 0, hi = (int)nbp;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (L.cpref[mid] <= j) lo = mid;
-        else hi = mid;
+// Chunk table of a batch.  Chunk j covers up to R consecutive covered pixels
+// of one tet (in row, col order); its entry packs
+//   tet k (6 bits) | row (5) | first column (5) | first record slot (11) | count (5).
+// Built in parallel over (tet, row): each row emits the chunks that start in it.
+// The row's prefix entry is then replaced by its record base minus its first
+// column, so the pixel-sequential phase finds a pair's record as base + column.
+constexpr int chunk_capacity(int CAP, int R, int B) { return CAP / R + B; }
+
+template <int TS, int B, bool BWD, int R, int NT>
+__device__ __forceinline__ void build_chunks(BatchLDS<TS, B, BWD>& L, uint32_t nbp, uint32_t* __restrict__ tab) {
+    constexpr int SP = TS + 1;
+    for (int it = threadIdx.x; it < (int)nbp * TS; it += NT) {
+        const int k = it / TS, r = it % TS;
+        const uint32_t sp = L.span[k * SP + r];
+        const int xl = span_xl(sp), w = span_xr(sp) - xl + 1;
+        if (w <= 0) continue;
+        const uint32_t base = L.rpref[k * SP + r], tot = L.rpref[k * SP + TS];
+        const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
+        for (uint32_t c = c0; c <= c1; ++c) {
+            const uint32_t l = c * R;
+            tab[L.cpref[k] + c] = (uint32_t)k | ((uint32_t)r << 6) | ((uint32_t)(xl + (int)(l - base)) << 11) |
+                                  ((L.tpref[k] + l) << 16) | (min((uint32_t)R, tot - l) << 27);
+        }
+        L.rpref[k * SP + r] = L.tpref[k] + base - (uint32_t)xl;  // read only by this thread above
     }
-    k = lo;
-    l0 = (j - L.cpref[k]) * R;
-    cnt = min((uint32_t)R, L.tpref[k + 1] - L.tpref[k] - l0);
+}
+
+struct ChunkRef {
+    int k, row, col;
+    uint32_t slot0, cnt;
+};
+__device__ __forceinline__ ChunkRef chunk_at(const uint32_t* tab, uint32_t j) {
+    const uint32_t e = tab[j];
+    return ChunkRef{(int)(e & 63u), (int)((e >> 6) & 31u), (int)((e >> 11) & 31u), (e >> 16) & 2047u, e >> 27};
 }
 
 // Walks the covered pixels of one tet in (row, col) order starting at local index l.
@@ -269,18 +300,10 @@ template <int TS>
 struct PixWalk {
     int row, col, xr;
     template <int B, bool BWD>
-    __device__ __forceinline__ void init(const BatchLDS<TS, B, BWD>& L, int k, uint32_t l) {
-        constexpr int SP = TS + 1;
-        int rl = 0, rh = TS;
-        while (rh - rl > 1) {
-            int mid = (rl + rh) >> 1;
-            if (L.rpref[k * SP + mid] <= l) rl = mid;
-            else rh = mid;
-        }
-        row = rl;
-        uint32_t sp = L.span[k * SP + row];
-        col = span_xl(sp) + (int)(l - L.rpref[k * SP + row]);
-        xr = span_xr(sp);
+    __device__ __forceinline__ void init(const BatchLDS<TS, B, BWD>& L, const ChunkRef& c) {
+        row = c.row;
+        col = c.col;
+        xr = span_xr(L.span[c.k * (TS + 1) + c.row]);
     }
     template <int B, bool BWD>
     __device__ __forceinline__ void next(const BatchLDS<TS, B, BWD>& L, int k) {
@@ -388,22 +411,28 @@ __device__ __forceinline__ void integral(const TetRegs& T, V3 rd, float t_min, f
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
-template <int TS>
+// Batch geometry of the forward: B tets per LDS batch (<= 64, one mask bit
+// each), CAP (pixel, tet) records per batch, R pixels per lane in the pair
+// phase, PF = fetch the next batch's records while this batch runs.
+template <int TS_, int B_, int CAP_, int R_, bool PF_>
 struct FwdCfg {
+    static constexpr int TS = TS_;
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
-    static constexpr int B = 64;
-    static constexpr int CAP = 1024;
-    static constexpr int R = 2;
+    static constexpr int B = B_;
+    static constexpr int CAP = CAP_;
+    static constexpr int R = R_;
+    static constexpr bool PF = PF_;
 };
 
-template <int TS, bool AUX>
-__global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
+template <class C, bool AUX>
+__global__ __launch_bounds__(C::NT) void k_forward(
     const float4* __restrict__ rec, const float4* __restrict__ colors, uint64_t color_view_stride,
     const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges, const ViewParams* __restrict__ views,
     uint32_t tiles_per_view, float* __restrict__ rgba, float* __restrict__ aux) {
-    using C = FwdCfg<TS>;
-    constexpr int NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1;
+    constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1;
     constexpr int NA = AUX ? CAP : 1;
+    static_assert(B <= 64, "pixel masks hold 64 tets; the layout scan is one wave");
+    static_assert(CAP >= TS * TS, "one tet's pixels of a tile must fit a batch");
     __shared__ BatchLDS<TS, B, false> L;
     __shared__ float s_od[CAP];
     __shared__ float s_cp[3][CAP];
@@ -411,6 +440,8 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
     __shared__ uint32_t s_ent[NA];
     __shared__ float s_ray[3][TS * TS];
     __shared__ unsigned long long s_mask[TS * TS];  // covered tets of the batch per pixel
+    __shared__ uint32_t s_chunk[chunk_capacity(CAP, R, B)];
+    static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
 
     const uint32_t gtile = blockIdx.x;
     const uint32_t view = gtile / tiles_per_view;
@@ -438,29 +469,31 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
     TetFetch f;
-    if (tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, col_v);
+    if (C::PF && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, col_v);
     while (done < count) {
         if (__syncthreads_and(!active || log_t < -13.8f)) break;
+        if (!C::PF && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - done - tid], rec, nullptr, col_v);
         if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
         if (tid < TS * TS) s_mask[tid] = 0ull;
         __syncthreads();
         batch_layout<TS, B, false, true, CAP, R, NT>(L, nb);
         const uint32_t nbp = L.nbp;
         const uint32_t nchunks = L.cpref[nbp];
+        build_chunks<TS, B, false, R, NT>(L, nbp, s_chunk);
+        __syncthreads();
         // prefetch the next batch's tet records while this one is processed
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
-        if (tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
+        if (C::PF && tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
         // chunk-parallel: intersection + volume integral of R pixels of one tet
         for (uint32_t j = tid; j < nchunks; j += NT) {
-            int k;
-            uint32_t l0, cnt;
-            chunk_locate<TS, B, false, R>(L, j, nbp, k, l0, cnt);
+            const ChunkRef c = chunk_at(s_chunk, j);
+            const int k = c.k;
+            const uint32_t cnt = c.cnt, slot0 = c.slot0;
             TetRegs T;
             tet_regs(L, k, T);
             PixWalk<TS> w;
-            w.init(L, k, l0);
-            const uint32_t slot0 = L.tpref[k] + l0;
+            w.init(L, c);
             for (uint32_t q = 0; q < cnt; ++q) {
                 const uint32_t slot = slot0 + q;
                 const uint32_t gx = tc.tile_x * TS + w.col, gy = tc.tile_y * TS + w.row;
@@ -497,8 +530,7 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
                 const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
                 m &= m - 1ull;
                 if (log_t < -13.8f) break;
-                const uint32_t sp = L.span[k * SP + row];
-                const uint32_t slot = L.tpref[k] + L.rpref[k * SP + row] + (uint32_t)(colp - span_xl(sp));
+                const uint32_t slot = L.rpref[k * SP + row] + (uint32_t)colp;
                 const float od = s_od[slot];
                 const float Tj = fast_safe_exp(log_t);
                 if (Tj >= 1e-6f) {
@@ -551,12 +583,18 @@ __global__ __launch_bounds__(FwdCfg<TS>::NT) void k_forward(
 // ---------------------------------------------------------------------------
 constexpr int kTerms = 19;  // d_density, d_v[4][3], d_grad[3], d_colour[3]
 
-template <int TS>
+// Batch geometry of the adjoint (B <= 32, one mask bit each); SEG = sum each
+// tet's terms across the wave with a segmented scan before the LDS adds
+// (otherwise every lane adds its own terms).
+template <int TS_, int B_, int CAP_, int R_, bool PF_, bool SEG_>
 struct BwdCfg {
+    static constexpr int TS = TS_;
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
-    static constexpr int B = 32;
-    static constexpr int CAP = 768;
-    static constexpr int R = 4;
+    static constexpr int B = B_;
+    static constexpr int CAP = CAP_;
+    static constexpr int R = R_;
+    static constexpr bool PF = PF_;
+    static constexpr bool SEG = SEG_;
 };
 
 // Segmented inclusive sum across the wave; lanes with equal `key` form
@@ -598,7 +636,10 @@ __device__ __forceinline__ int face_slot(int fi, int role) {
 // cyclic), algebraically identical to the reference's
 // (cross(va - hit, vb - vc) + n) / den; since the three crosses are parallel
 // to the face normal n they equal lambda_j * n with the hit's barycentric
-// weights lambda_j, which are scattered to the face's vertex slots.
+// weights lambda_j.  The weights are affine in the hit point, so each face
+// carries their gradients and their values at v0 (store_tet) and a pair
+// evaluates them with three dot products; they are scattered to the face's
+// vertex slots.
 template <int TS, int B>
 __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k, const TetRegs& T, V3 v0, V3 cam,
                                           V3 rd, float dc0, float dc1, float dc2, float t_prev, float d_log_t_b,
@@ -634,7 +675,8 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
     d_tmin = d_tmin + sum_s * dc_dt;
     d_tmax = d_tmax + sum_e * dc_dt;
     float dbo = (dbc0 + dbc1) + dbc2;
-    V3 dg = add3(scl3(sub3(cam, v0), dbo), scl3(rd, d_dcdt));
+    const V3 cam_v0 = sub3(cam, v0);
+    V3 dg = add3(scl3(cam_v0, dbo), scl3(rd, d_dcdt));
     float wv[2][4];   // per side: barycentric weight of each vertex slot
     V3 G[2];          // per side: n * (dt / den)
 #pragma unroll
@@ -644,15 +686,13 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
         const float dt = side == 0 ? d_tmin : d_tmax;
         const float4* F = reinterpret_cast<const float4*>(&L.face[k * L.FS + fi * 16]);
         const float4 f0 = F[0], f1 = F[1], f2 = F[2], f3 = F[3];
-        const V3 va = mk3(f0.x, f0.y, f0.z), vb = mk3(f0.w, f1.x, f1.y), vc = mk3(f1.z, f1.w, f2.x);
+        const V3 ga = mk3(f0.x, f0.y, f0.z), gb = mk3(f0.w, f1.x, f1.y), gc = mk3(f1.z, f1.w, f2.x);
         const V3 nn = mk3(f2.y, f2.z, f2.w);
-        const float inv_nn = f3.x;
         const float inv_den = fast_safe_div(1.0f, dot3(nn, rd));
-        const V3 hit = add3(cam, scl3(rd, tt));
-        const V3 a = sub3(va, hit), b = sub3(vb, hit), c = sub3(vc, hit);
-        const float la = dot3(cross3(c, b), nn) * inv_nn;
-        const float lb = dot3(cross3(a, c), nn) * inv_nn;
-        const float lc = dot3(cross3(b, a), nn) * inv_nn;
+        const V3 h = add3(cam_v0, scl3(rd, tt));  // hit - v0
+        const float la = f3.x + dot3(ga, h);
+        const float lb = f3.y + dot3(gb, h);
+        const float lc = f3.z + dot3(gc, h);
         G[side] = scl3(nn, dt * inv_den);
         const int sa = face_slot(fi, 0), sb = face_slot(fi, 1), sc = face_slot(fi, 2);
 #pragma unroll
@@ -678,16 +718,15 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
     acc[18] += dbcc.z;
 }
 
-template <int TS>
-__global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_backward(
     const float4* __restrict__ rec, const uint32_t* __restrict__ idx, const float4* __restrict__ colors,
     uint64_t color_view_stride, const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges,
     const ViewParams* __restrict__ views, uint32_t tiles_per_view, const float* __restrict__ rgba,
     const float* __restrict__ dl_img, const float* __restrict__ gt, uint32_t loss_type,
     float* __restrict__ loss_partial, float* __restrict__ d_verts, float* __restrict__ d_dens,
     float* __restrict__ d_grads, float* __restrict__ d_colors, uint64_t dcolor_view_stride) {
-    using C = BwdCfg<TS>;
-    constexpr int NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1, NW = NT / 64;
+    constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1, NW = NT / 64;
     __shared__ BatchLDS<TS, B, true> L;
     // per-pair record: od -> t_prev, cp0 -> d_log_t before, cp1, cp2, t_min, t_max, meta
     __shared__ float s_r[6][CAP];
@@ -696,8 +735,11 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
     __shared__ float s_dc[3][TS * TS];
     __shared__ float s_acc[B][kTerms];
     __shared__ unsigned int s_mask[TS * TS];  // covered tets of the batch per pixel (B <= 32)
+    __shared__ uint32_t s_chunk[chunk_capacity(CAP, R, B)];
+    static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
     __shared__ float s_loss[NW];
     static_assert(B <= 32, "pixel masks hold 32 tets");
+    static_assert(CAP >= TS * TS, "one tet's pixels of a tile must fit a batch");
 
     const uint32_t gtile = blockIdx.x;
     const uint32_t view = gtile / tiles_per_view;
@@ -768,9 +810,10 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
     TetFetch f;
-    if (tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
+    if (C::PF && tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
     while (done < count) {
         __syncthreads();
+        if (!C::PF && tid < nb) fetch_tet<true>(f, sorted_vals[rs + done + tid], rec, idx, col_v);
         if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
@@ -778,19 +821,20 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
         batch_layout<TS, B, true, false, CAP, R, NT>(L, nb);
         const uint32_t nbp = L.nbp;
         const uint32_t nchunks = L.cpref[nbp];
+        build_chunks<TS, B, true, R, NT>(L, nbp, s_chunk);
+        __syncthreads();
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
-        if (tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
+        if (C::PF && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
         // phase A (chunk-parallel): intersection + integral
         for (uint32_t j = tid; j < nchunks; j += NT) {
-            int k;
-            uint32_t l0, cnt;
-            chunk_locate<TS, B, true, R>(L, j, nbp, k, l0, cnt);
+            const ChunkRef c = chunk_at(s_chunk, j);
+            const int k = c.k;
+            const uint32_t cnt = c.cnt, slot0 = c.slot0;
             TetRegs T;
             tet_regs(L, k, T);
             PixWalk<TS> w;
-            w.init(L, k, l0);
-            const uint32_t slot0 = L.tpref[k] + l0;
+            w.init(L, c);
             for (uint32_t q = 0; q < cnt; ++q) {
                 const uint32_t slot = slot0 + q;
                 const uint32_t gx = tc.tile_x * TS + w.col, gy = tc.tile_y * TS + w.row;
@@ -828,8 +872,7 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
             while (m) {
                 const uint32_t k = (uint32_t)__ffs((int)m) - 1u;
                 m &= m - 1u;
-                const uint32_t sp = L.span[k * SP + row];
-                const uint32_t slot = L.tpref[k] + L.rpref[k * SP + row] + (uint32_t)(colp - span_xl(sp));
+                const uint32_t slot = L.rpref[k * SP + row] + (uint32_t)colp;
                 const float od = s_r[0][slot];
                 const float prev_log_t = fminf(log_t + od, 0.0f);
                 const float t_prev = fast_safe_exp(prev_log_t);
@@ -852,16 +895,15 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
             for (int q = 0; q < kTerms; ++q) acc[q] = 0.0f;
             int key = 0x7fffffff;
             if (j < nchunks) {
-                int k;
-                uint32_t l0, cnt;
-                chunk_locate<TS, B, true, R>(L, j, nbp, k, l0, cnt);
+                const ChunkRef c = chunk_at(s_chunk, j);
+                const int k = c.k;
+                const uint32_t cnt = c.cnt, slot0 = c.slot0;
                 key = k;
                 TetRegs T;
                 tet_regs(L, k, T);
                 const V3 v0 = mk3(L.v0[0][k], L.v0[1][k], L.v0[2][k]);
                 PixWalk<TS> w;
-                w.init(L, k, l0);
-                const uint32_t slot0 = L.tpref[k] + l0;
+                w.init(L, c);
                 for (uint32_t q = 0; q < cnt; ++q) {
                     const uint32_t slot = slot0 + q;
                     const uint32_t meta = s_meta[slot];
@@ -875,13 +917,21 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
                     w.next(L, k);
                 }
             }
-            const SegMasks sm = seg_masks(key);
-            const int next_key = __shfl_down(key, 1, 64);
-            const bool tail = key != 0x7fffffff && (lane == 63 || next_key != key);
+            if (C::SEG) {
+                // lanes of one tet are contiguous: a segmented wave scan leaves
+                // each tet's sum in its last lane, which adds it to LDS
+                const SegMasks sm = seg_masks(key);
+                const int next_key = __shfl_down(key, 1, 64);
+                const bool tail = key != 0x7fffffff && (lane == 63 || next_key != key);
 #pragma unroll
-            for (int q = 0; q < kTerms; ++q) {
-                float s = seg_scan(acc[q], sm);
-                if (tail && s != 0.0f) atomicAdd(&s_acc[key][q], s);
+                for (int q = 0; q < kTerms; ++q) {
+                    float s = seg_scan(acc[q], sm);
+                    if (tail && s != 0.0f) atomicAdd(&s_acc[key][q], s);
+                }
+            } else if (key != 0x7fffffff) {
+#pragma unroll
+                for (int q = 0; q < kTerms; ++q)
+                    if (acc[q] != 0.0f) atomicAdd(&s_acc[key][q], acc[q]);
             }
         }
         __syncthreads();
@@ -904,41 +954,60 @@ __global__ __launch_bounds__(BwdCfg<TS>::NT) void k_backward(
 }
 
 // ---------------------------------------------------------------------------
-// host launchers
+// host launchers: kFwdVariants x kBwdVariants batch geometries per tile size,
+// picked per renderer by rmesh_autotune (variant 0 is the measured default)
 // ---------------------------------------------------------------------------
-template <int TS>
+template <class C>
 static void fwd_launch(const FwdArgs& a, hipStream_t s) {
-    constexpr int NT = FwdCfg<TS>::NT;
     if (a.aux)
-        hipLaunchKernelGGL((k_forward<TS, true>), dim3(a.num_gtiles), dim3(NT), 0, s, a.rec, a.colors, a.color_view_stride,
-                           a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.aux);
+        hipLaunchKernelGGL((k_forward<C, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
+                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.aux);
     else
-        hipLaunchKernelGGL((k_forward<TS, false>), dim3(a.num_gtiles), dim3(NT), 0, s, a.rec, a.colors, a.color_view_stride,
-                           a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.aux);
+        hipLaunchKernelGGL((k_forward<C, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
+                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.aux);
 }
 
-bool launch_forward(const FwdArgs& a, uint32_t tile_size, hipStream_t s) {
-    switch (tile_size) {
-        case 8: fwd_launch<8>(a, s); return true;
-        case 12: fwd_launch<12>(a, s); return true;
-        case 16: fwd_launch<16>(a, s); return true;
+template <int TS>
+static bool fwd_variant(const FwdArgs& a, int v, hipStream_t s) {
+    switch (v) {
+        case 0: fwd_launch<FwdCfg<TS, 32, 768, 1, true>>(a, s); return true;
+        case 1: fwd_launch<FwdCfg<TS, 48, 768, 1, true>>(a, s); return true;
+        case 2: fwd_launch<FwdCfg<TS, 64, 768, 1, true>>(a, s); return true;
         default: return false;
     }
 }
 
-template <int TS>
-static void bwd_launch(const BwdArgs& a, hipStream_t s) {
-    constexpr int NT = BwdCfg<TS>::NT;
-    hipLaunchKernelGGL((k_backward<TS>), dim3(a.num_gtiles), dim3(NT), 0, s, a.rec, a.idx, a.colors, a.color_view_stride,
-                       a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.dl_img, a.gt, a.loss_type,
-                       a.loss_partial, a.d_verts, a.d_dens, a.d_grads, a.d_colors, a.dcolor_view_stride);
+bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream_t s) {
+    switch (tile_size) {
+        case 8: return fwd_variant<8>(a, variant, s);
+        case 12: return fwd_variant<12>(a, variant, s);
+        case 16: return fwd_variant<16>(a, variant, s);
+        default: return false;
+    }
 }
 
-bool launch_backward(const BwdArgs& a, uint32_t tile_size, hipStream_t s) {
+template <class C>
+static void bwd_launch(const BwdArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_backward<C>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
+                       a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.dl_img, a.gt,
+                       a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads, a.d_colors, a.dcolor_view_stride);
+}
+
+template <int TS>
+static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
+    switch (v) {
+        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, false, true>>(a, s); return true;
+        case 1: bwd_launch<BwdCfg<TS, 32, 512, 2, false, true>>(a, s); return true;
+        case 2: bwd_launch<BwdCfg<TS, 32, 384, 2, false, true>>(a, s); return true;
+        default: return false;
+    }
+}
+
+bool launch_backward(const BwdArgs& a, uint32_t tile_size, int variant, hipStream_t s) {
     switch (tile_size) {
-        case 8: bwd_launch<8>(a, s); return true;
-        case 12: bwd_launch<12>(a, s); return true;
-        case 16: bwd_launch<16>(a, s); return true;
+        case 8: return bwd_variant<8>(a, variant, s);
+        case 12: return bwd_variant<12>(a, variant, s);
+        case 16: return bwd_variant<16>(a, variant, s);
         default: return false;
     }
 }

@@ -174,90 +174,105 @@ __device__ __forceinline__ float adam_update(float& p, float gr, float& m, float
     return p;
 }
 
+// SH adjoint (+ optional fused Adam on the SH coefficients): one lane per
+// (tet, colour channel), 64 tets per 192-lane workgroup.  A lane streams its
+// channel's (deg+1)^2 coefficients, m and v as one contiguous row (adjacent
+// lanes read adjacent rows: coalesced), accumulates over the views, and the
+// three channel partials of a tet's direction/offset adjoint are summed in a
+// fixed order through LDS before one lane scatters them to the vertices.
+constexpr int kShTets = 64;
+
 template <int DEG, bool ADAM>
-__global__ __launch_bounds__(256) void k_sh_backward(ShBwdArgs a) {
+__global__ __launch_bounds__(3 * kShTets) void k_sh_backward(ShBwdArgs a) {
     constexpr int NC = (DEG + 1) * (DEG + 1);
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.M) return;
-    V3 v[4], g, cen;
-    tet_geom(a.rec, t, v, g, cen);
-    float c[3 * NC], dsh[3 * NC];
-    float* s = a.sh + (uint64_t)t * 3 * NC;
+    __shared__ float red[kShTets][3][4];  // per channel: d(centroid).xyz, d(offset)
+    const uint32_t lt = threadIdx.x / 3, ch = threadIdx.x % 3;
+    const uint32_t t = blockIdx.x * kShTets + lt;
+    const bool live = t < a.M;
+    V3 v[4], g, cen, off;
+    float c[NC], dsh[NC];
+    float* s = a.sh + ((uint64_t)t * 3 + ch) * NC;
+    if (live) {
+        tet_geom(a.rec, t, v, g, cen);
+        off = sub3(v[0], cen);
+        const float offset = dot3(g, off);
 #pragma unroll
-    for (int i = 0; i < 3 * NC; ++i) {
-        c[i] = s[i];
-        dsh[i] = 0.0f;
-    }
-    V3 off = sub3(v[0], cen);
-    float offset = dot3(g, off);
-    float s_all = 0.0f;
-    V3 dcen = mk3(0.f, 0.f, 0.f);
-    for (uint32_t vv = 0; vv < a.V; ++vv) {
-        const float* dcv = a.d_colors + ((uint64_t)vv * a.M + t) * 3;
-        float dcol[3] = {dcv[0], dcv[1], dcv[2]};
-        if (dcol[0] == 0.0f && dcol[1] == 0.0f && dcol[2] == 0.0f) continue;
-        float len;
-        V3 d = view_dir(cen, a.views[vv], len);
-        float Y[NC];
-        sh_basis<DEG>(d, Y);
-        float w[NC];
+        for (int k = 0; k < NC; ++k) {
+            c[k] = s[k];
+            dsh[k] = 0.0f;
+        }
+        float s_all = 0.0f;
+        V3 dcen = mk3(0.f, 0.f, 0.f);
+        for (uint32_t vv = 0; vv < a.V; ++vv) {
+            const float dc = a.d_colors[((uint64_t)vv * a.M + t) * 3 + ch];
+            if (dc == 0.0f) continue;
+            float len;
+            const V3 d = view_dir(cen, a.views[vv], len);
+            float Y[NC];
+            sh_basis<DEG>(d, Y);
+            float val = Y[0] * c[0];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) w[k] = 0.0f;
-        float s_sum = 0.0f;
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-            float val = Y[0] * c[ch * NC];
-#pragma unroll
-            for (int k = 1; k < NC; ++k) val = val + Y[k] * c[ch * NC + k];
-            float d_pre = dcol[ch] * dsoftplus_f((val + 0.5f) + offset);
-            s_sum += d_pre;
+            for (int k = 1; k < NC; ++k) val = val + Y[k] * c[k];
+            const float d_pre = dc * dsoftplus_f((val + 0.5f) + offset);
+            s_all += d_pre;
+            float w[NC];
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                dsh[ch * NC + k] += d_pre * Y[k];
-                w[k] += d_pre * c[ch * NC + k];
+                dsh[k] += d_pre * Y[k];
+                w[k] = d_pre * c[k];
+            }
+            if (len > 0.0f) {
+                const V3 d_dir = sh_basis_vjp<DEG>(d, w);
+                const float dd = dot3(d, d_dir);
+                dcen = add3(dcen, scl3(sub3(d_dir, scl3(d, dd)), 1.0f / len));
             }
         }
-        s_all += s_sum;
-        if (len > 0.0f) {
-            V3 d_dir = sh_basis_vjp<DEG>(d, w);
-            float dd = dot3(d, d_dir);
-            dcen = add3(dcen, scl3(sub3(d_dir, scl3(d, dd)), 1.0f / len));
+        red[lt][ch][0] = dcen.x;
+        red[lt][ch][1] = dcen.y;
+        red[lt][ch][2] = dcen.z;
+        red[lt][ch][3] = s_all;
+    }
+    __syncthreads();
+    if (live && ch == 0) {
+        const float s_all = (red[lt][0][3] + red[lt][1][3]) + red[lt][2][3];
+        V3 dcen = mk3((red[lt][0][0] + red[lt][1][0]) + red[lt][2][0], (red[lt][0][1] + red[lt][1][1]) + red[lt][2][1],
+                      (red[lt][0][2] + red[lt][1][2]) + red[lt][2][2]);
+        // offset = g . (v0 - centroid)
+        a.d_grads[3ull * t] += s_all * off.x;
+        a.d_grads[3ull * t + 1] += s_all * off.y;
+        a.d_grads[3ull * t + 2] += s_all * off.z;
+        dcen = add3(dcen, scl3(g, -s_all));
+        const uint4 q = reinterpret_cast<const uint4*>(a.idx)[t];
+        const uint32_t vi[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            V3 dvv = scl3(dcen, 0.25f);
+            if (i == 0) dvv = add3(dvv, scl3(g, s_all));
+            if (dvv.x != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i], dvv.x);
+            if (dvv.y != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 1, dvv.y);
+            if (dvv.z != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 2, dvv.z);
         }
     }
-    // offset = g . (v0 - centroid)
-    a.d_grads[3ull * t] += s_all * off.x;
-    a.d_grads[3ull * t + 1] += s_all * off.y;
-    a.d_grads[3ull * t + 2] += s_all * off.z;
-    dcen = add3(dcen, scl3(g, -s_all));
-    uint4 q = reinterpret_cast<const uint4*>(a.idx)[t];
-    uint32_t vi[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        V3 dvv = scl3(dcen, 0.25f);
-        if (i == 0) dvv = add3(dvv, scl3(g, s_all));
-        if (dvv.x != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i], dvv.x);
-        if (dvv.y != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 1, dvv.y);
-        if (dvv.z != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 2, dvv.z);
-    }
+    if (!live) return;
     if (ADAM) {
         if (*a.st.skip) return;
-        float stepf = (float)*a.st.step;
-        float b1p = expf(stepf * logf(a.st.beta1));
-        float b2p = expf(stepf * logf(a.st.beta2));
-        float* mm = a.m_sh + (uint64_t)t * 3 * NC;
-        float* vv2 = a.v_sh + (uint64_t)t * 3 * NC;
+        const float stepf = (float)*a.st.step;
+        const float b1p = expf(stepf * logf(a.st.beta1));
+        const float b2p = expf(stepf * logf(a.st.beta2));
+        float* mm = a.m_sh + ((uint64_t)t * 3 + ch) * NC;
+        float* vv2 = a.v_sh + ((uint64_t)t * 3 + ch) * NC;
 #pragma unroll
-        for (int i = 0; i < 3 * NC; ++i) {
-            float p = c[i], m = mm[i], v2 = vv2[i];
-            adam_update(p, dsh[i], m, v2, a.lr, a.st.beta1, a.st.beta2, a.st.eps, b1p, b2p);
-            s[i] = p;
-            mm[i] = m;
-            vv2[i] = v2;
+        for (int k = 0; k < NC; ++k) {
+            float p = c[k], m = mm[k], v2 = vv2[k];
+            adam_update(p, dsh[k], m, v2, a.lr, a.st.beta1, a.st.beta2, a.st.eps, b1p, b2p);
+            s[k] = p;
+            mm[k] = m;
+            vv2[k] = v2;
         }
     } else {
-        float* o = a.d_sh + (uint64_t)t * 3 * NC;
+        float* o = a.d_sh + ((uint64_t)t * 3 + ch) * NC;
 #pragma unroll
-        for (int i = 0; i < 3 * NC; ++i) o[i] = dsh[i];
+        for (int k = 0; k < NC; ++k) o[k] = dsh[k];
     }
 }
 
@@ -316,7 +331,7 @@ void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t
 
 template <int DEG>
 static void shb(const ShBwdArgs& a, hipStream_t s) {
-    dim3 gr((a.M + 255) / 256), bl(256);
+    dim3 gr((a.M + kShTets - 1) / kShTets), bl(3 * kShTets);
     if (a.adam) hipLaunchKernelGGL((k_sh_backward<DEG, true>), gr, bl, 0, s, a);
     else hipLaunchKernelGGL((k_sh_backward<DEG, false>), gr, bl, 0, s, a);
 }

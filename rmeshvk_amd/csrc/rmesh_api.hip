@@ -58,6 +58,8 @@ struct rmesh_renderer {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     uint32_t N = 0, M = 0, W = 0, H = 0, TS = 12, tiles_x = 0, tiles_y = 0, T = 0;
+    uint32_t T_alloc = 0;          // per-tile buffers are sized for the smallest tile (8)
+    int fwd_var = 0, bwd_var = 0;  // batch-geometry variants of the compositing kernels
     uint32_t max_views = 1;
     bool has_sh = false;
     uint32_t sh_deg = 0, nc = 1;
@@ -162,6 +164,27 @@ rmesh_status upload_views(rmesh_renderer* h, uint32_t V, const float* cams, cons
     return RMESH_OK;
 }
 
+void drop_graph(rmesh_renderer* h) {
+    if (h->graph_exec) {
+        hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    if (h->graph) {
+        hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+}
+
+// Tile grid of tile size ts; invalidates everything derived from the old grid.
+void set_tiles(rmesh_renderer* h, uint32_t ts) {
+    h->TS = ts;
+    h->tiles_x = (h->W + ts - 1) / ts;
+    h->tiles_y = (h->H + ts - 1) / ts;
+    h->T = h->tiles_x * h->tiles_y;
+    h->have_forward = false;
+    drop_graph(h);
+}
+
 rmesh_status ensure_capacity(rmesh_renderer* h, uint64_t need) {
     if (need <= h->cap && h->keys_a && h->hist) return RMESH_OK;
     if (need > 0xFFFFFFF0ull) return fail(RMESH_ERR_CAPACITY, "tile-tet pair count exceeds 2^32");
@@ -183,14 +206,7 @@ rmesh_status ensure_capacity(rmesh_renderer* h, uint64_t need) {
     HIP_TRY(dalloc(&h->vals_b, c));
     HIP_TRY(dalloc(&h->hist, radix_hist_words((uint32_t)c)));
     h->cap = (uint32_t)c;
-    if (h->graph_exec) {
-        hipGraphExecDestroy(h->graph_exec);
-        h->graph_exec = nullptr;
-    }
-    if (h->graph) {
-        hipGraphDestroy(h->graph);
-        h->graph = nullptr;
-    }
+    drop_graph(h);
     return RMESH_OK;
 }
 
@@ -246,7 +262,7 @@ rmesh_status stage_forward(rmesh_renderer* h, uint32_t V, bool want_aux) {
     a.num_gtiles = h->gtiles(V);
     a.rgba = h->rgba;
     a.aux = want_aux ? h->aux : nullptr;
-    if (!launch_forward(a, h->TS, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
+    if (!launch_forward(a, h->TS, h->fwd_var, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
     return RMESH_OK;
 }
 
@@ -279,7 +295,7 @@ rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, cons
     a.d_grads = h->d_grads;
     a.d_colors = h->d_colors;
     a.dcolor_view_stride = h->has_sh ? h->M : 0;
-    if (!launch_backward(a, h->TS, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
+    if (!launch_backward(a, h->TS, h->bwd_var, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
     return RMESH_OK;
 }
 
@@ -434,10 +450,8 @@ extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer**
     h->M = (uint32_t)d->n_tets;
     h->W = d->width;
     h->H = d->height;
-    h->TS = ts;
-    h->tiles_x = (h->W + ts - 1) / ts;
-    h->tiles_y = (h->H + ts - 1) / ts;
-    h->T = h->tiles_x * h->tiles_y;
+    set_tiles(h, ts);
+    h->T_alloc = ((h->W + 7) / 8) * ((h->H + 7) / 8);
     h->max_views = d->max_views ? d->max_views : 1;
     h->has_sh = d->sh_coeffs != nullptr;
     h->sh_deg = h->has_sh ? d->sh_degree : 0;
@@ -488,12 +502,12 @@ extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer**
     CREATE_TRY(dalloc(&h->overflow, 1));
     CREATE_TRY(dalloc(&h->d_step, 1));
     CREATE_TRY(hipMemset(h->d_step, 0, 4));
-    CREATE_TRY(dalloc(&h->ranges, 2ull * h->T * V));
+    CREATE_TRY(dalloc(&h->ranges, 2ull * h->T_alloc * V));
     size_t npx = (size_t)h->W * h->H;
     CREATE_TRY(dalloc(&h->rgba, 4 * npx * V));
     CREATE_TRY(dalloc(&h->dl, 4 * npx));
     CREATE_TRY(dalloc(&h->gt, 3 * npx * V));
-    CREATE_TRY(dalloc(&h->loss_partial, (size_t)h->T * V));
+    CREATE_TRY(dalloc(&h->loss_partial, (size_t)h->T_alloc * V));
     CREATE_TRY(dalloc(&h->loss_out, V));
 #undef CREATE_TRY
     *out = h;
@@ -720,6 +734,127 @@ extern "C" rmesh_status rmesh_forward_batched(rmesh_renderer* h, uint32_t V, con
 extern "C" rmesh_status rmesh_set_graph_mode(rmesh_renderer* h, int enable) {
     if (!h) return fail(RMESH_ERR_INVALID, "null handle");
     h->graph_mode = enable != 0;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_set_tuning(rmesh_renderer* h, uint32_t tile_size, int32_t fwd_variant,
+                                         int32_t bwd_variant) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    if (tile_size != 8 && tile_size != 12 && tile_size != 16) return fail(RMESH_ERR_INVALID, "tile_size must be 8, 12 or 16");
+    if (fwd_variant < 0 || fwd_variant >= kFwdVariants || bwd_variant < 0 || bwd_variant >= kBwdVariants)
+        return fail(RMESH_ERR_INVALID, "kernel variant out of range");
+    if (tile_size != h->TS) set_tiles(h, tile_size);
+    if (fwd_variant != h->fwd_var || bwd_variant != h->bwd_var) drop_graph(h);
+    h->fwd_var = fwd_variant;
+    h->bwd_var = bwd_variant;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_get_tuning(const rmesh_renderer* h, int32_t* out3) {
+    if (!h || !out3) return fail(RMESH_ERR_INVALID, "null argument");
+    out3[0] = (int32_t)h->TS;
+    out3[1] = h->fwd_var;
+    out3[2] = h->bwd_var;
+    return RMESH_OK;
+}
+
+extern "C" int32_t rmesh_num_variants(int32_t which) { return which == 0 ? kFwdVariants : kBwdVariants; }
+
+// Times the binning + sort, every forward variant and every adjoint variant
+// for each tile size on the given views (eager, HIP events, best of `reps`),
+// then keeps the fastest combination.  Parameters and optimiser state are
+// untouched; the gradient buffers are cleared afterwards.
+extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps,
+                                       const float* c2ws, const float* gt_dev, uint32_t loss_type, uint32_t reps,
+                                       float* out_ms) {
+    if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    rmesh_status st = check_views(h, V);
+    if (st != RMESH_OK) return st;
+    DeviceGuard g(h->device);
+    if (gt_dev && gt_dev != h->gt)
+        HIP_TRY(hipMemcpyAsync(h->gt, gt_dev, 12ull * h->W * h->H * V, hipMemcpyDeviceToDevice, h->stream));
+    if (reps == 0) reps = 3;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    auto timed = [&](auto&& body, float& best) -> rmesh_status {
+        best = 3.0e38f;
+        for (uint32_t r = 0; r < reps; ++r) {
+            HIP_TRY(hipEventRecord(e0, h->stream));
+            rmesh_status s2 = body();
+            if (s2 != RMESH_OK) return s2;
+            HIP_TRY(hipEventRecord(e1, h->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        return RMESH_OK;
+    };
+    const uint32_t sizes[3] = {8, 12, 16};
+    float best_total = 3.0e38f, best_parts[3] = {0, 0, 0};
+    uint32_t best_ts = h->TS;
+    int best_f = 0, best_b = 0;
+    rmesh_status result = RMESH_OK;
+    for (uint32_t ts : sizes) {
+        set_tiles(h, ts);
+        if ((result = upload_views(h, V, cams, vps, c2ws)) != RMESH_OK) break;
+        if ((result = stage_binning_front(h, V, false)) != RMESH_OK) break;  // sizes the pair buffers
+        float t_bin = 0.0f, t_fwd = 3.0e38f, t_bwd = 3.0e38f;
+        int bf = 0, bb = 0;
+        result = timed([&]() {
+            rmesh_status s2 = stage_binning_front(h, V, true);
+            if (s2 != RMESH_OK) return s2;
+            return stage_binning_back(h, V);
+        }, t_bin);
+        if (result != RMESH_OK) break;
+        for (int f = 0; f < kFwdVariants && result == RMESH_OK; ++f) {
+            h->fwd_var = f;
+            float t = 0.0f;
+            result = timed([&]() { return stage_forward(h, V, false); }, t);
+            if (t < t_fwd) {
+                t_fwd = t;
+                bf = f;
+            }
+        }
+        if (result != RMESH_OK) break;
+        h->fwd_var = bf;
+        if ((result = stage_forward(h, V, false)) != RMESH_OK) break;  // image of the chosen variant
+        for (int b = 0; b < kBwdVariants && result == RMESH_OK; ++b) {
+            h->bwd_var = b;
+            float t = 0.0f;
+            result = timed([&]() { return stage_backward(h, V, nullptr, h->gt, loss_type); }, t);
+            if (t < t_bwd) {
+                t_bwd = t;
+                bb = b;
+            }
+        }
+        if (result != RMESH_OK) break;
+        const float total = t_bin + t_fwd + t_bwd;
+        if (total < best_total) {
+            best_total = total;
+            best_ts = ts;
+            best_f = bf;
+            best_b = bb;
+            best_parts[0] = t_bin;
+            best_parts[1] = t_fwd;
+            best_parts[2] = t_bwd;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_tiles(h, best_ts);
+    h->fwd_var = best_f;
+    h->bwd_var = best_b;
+    zero_grads(h, V);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (result != RMESH_OK) return result;
+    if (out_ms) {
+        out_ms[0] = best_total;
+        out_ms[1] = best_parts[0];
+        out_ms[2] = best_parts[1];
+        out_ms[3] = best_parts[2];
+    }
     return RMESH_OK;
 }
 

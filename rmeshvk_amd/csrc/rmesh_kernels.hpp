@@ -42,7 +42,10 @@ struct FwdArgs {
     float* rgba;
     float* aux;  // nullable: [V][H][W][8]
 };
-bool launch_forward(const FwdArgs& a, uint32_t tile_size, hipStream_t s);
+// Batch-geometry variants of the compositing kernels (rmesh_autotune picks one).
+constexpr int kFwdVariants = 3;
+constexpr int kBwdVariants = 3;
+bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
 
 struct BwdArgs {
     const float4* rec;
@@ -65,7 +68,7 @@ struct BwdArgs {
     float* d_colors;          // [V][M*3] (SH) or [M*3] (stride 0)
     uint64_t dcolor_view_stride;  // in tets
 };
-bool launch_backward(const BwdArgs& a, uint32_t tile_size, hipStream_t s);
+bool launch_backward(const BwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
 
 // colours, SH and optimiser (k_optim.hip)
 void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t M, const ViewParams* views, uint32_t V,

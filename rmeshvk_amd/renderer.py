@@ -233,6 +233,33 @@ class RMeshRenderer:
     def set_graph_mode(self, enable: bool):
         _lib.check(self._lib.rmesh_set_graph_mode(self._h, 1 if enable else 0))
 
+    # ---- tuning extension ----
+    def set_tuning(self, tile_size: int, fwd_variant: int = 0, bwd_variant: int = 0):
+        """Tile size (8/12/16) and batch-geometry variants of the compositing kernels."""
+        _lib.check(self._lib.rmesh_set_tuning(self._h, int(tile_size), int(fwd_variant), int(bwd_variant)))
+
+    def get_tuning(self):
+        """(tile_size, fwd_variant, bwd_variant) in use."""
+        out = np.zeros(3, dtype=np.int32)
+        _lib.check(self._lib.rmesh_get_tuning(self._h, _p(out)))
+        return tuple(int(x) for x in out)
+
+    @staticmethod
+    def num_variants():
+        """(forward variants, adjoint variants)."""
+        lib = _lib.load()
+        return int(lib.rmesh_num_variants(0)), int(lib.rmesh_num_variants(1))
+
+    def autotune(self, cams, loss_type=0, reps=3, gt_device_ptr=None):
+        """Pick tile size and kernel variants by timing them on these views (targets
+        as in ``train_step_batched``); returns {"tuning": (ts, fwd, bwd), "ms": ...}."""
+        cp, vp, ci = self._stack_cams(cams)
+        ms = np.zeros(4, dtype=np.float32)
+        _lib.check(self._lib.rmesh_autotune(self._h, len(cams), _p(cp), _p(vp), _p(ci),
+                                            ctypes.c_void_p(gt_device_ptr or 0), int(loss_type), int(reps), _p(ms)))
+        return {"tuning": self.get_tuning(),
+                "ms": dict(zip(("total", "binning_sort", "forward", "backward"), ms.tolist()))}
+
     def device_buffer(self, which: int) -> int:
         return int(self._lib.rmesh_device_buffer(self._h, which) or 0)
 

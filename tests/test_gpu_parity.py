@@ -163,3 +163,49 @@ def test_empty_view():
     assert np.all(img == 0.0)
     g = r.backward(np.ones((h, w, 4), dtype=np.float32))
     assert all(np.all(v == 0.0) for v in g.values())
+
+
+def _variant_grid():
+    from rmeshvk_amd import RMeshRenderer
+
+    nf, nb = RMeshRenderer.num_variants()
+    return [(ts, v, v % nb) for ts in (8, 12, 16) for v in range(max(nf, nb)) if v < nf]
+
+
+@pytest.mark.parametrize("ts,fv,bv", _variant_grid())
+def test_kernel_variants_match_oracle(ts, fv, bv, config_a):
+    """Every batch geometry the autotuner can pick renders and differentiates
+    like the oracle (the geometry changes the schedule, not the arithmetic)."""
+    scene, cam, w, h = config_a
+    r = _renderer(scene, w, h)
+    r.set_tuning(ts, fv, bv)
+    assert r.get_tuning() == (ts, fv, bv)
+    ref = orc.render(scene, cam, w, h, ts=ts)
+    img = r.forward_tiled(*cam)
+    assert np.abs(img - ref["rgba"]).max() < IMG_TOL
+    rng = np.random.default_rng(ts + fv)
+    dl = np.ascontiguousarray(rng.normal(0, 1e-3, (h, w, 4)).astype(np.float32))
+    g = r.backward(dl)
+    gr = orc.backward(scene, cam, w, h, ref["sorted_vals"], ref["ranges"], ref["rgba"], dl, ts)
+    for k in ("d_vertices", "d_densities", "d_color_grads", "d_base_colors"):
+        a, b = g[k].astype(np.float64), gr[k].astype(np.float64)
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rel < GRAD_TOL, (ts, fv, bv, k, rel)
+
+
+def test_autotune_keeps_parameters(config_a):
+    scene, cam, w, h = config_a
+    r = _renderer(scene, w, h, max_views=2)
+    cams = [cam, make_camera((0.0, 3.0, 0.5), (0, 0, 0), math.radians(50.0), w, h)]
+    r.set_targets(np.zeros((2, h, w, 3), np.float32))
+    before = r.get_params()
+    res = r.autotune(cams, loss_type=0, reps=1)
+    ts, fv, bv = res["tuning"]
+    assert ts in (8, 12, 16) and res["ms"]["total"] > 0
+    after = r.get_params()
+    for k in before:
+        np.testing.assert_array_equal(before[k], after[k])
+    assert r.step == 0
+    # the chosen configuration still matches the oracle
+    ref = orc.render(scene, cam, w, h, ts=ts)
+    assert np.abs(r.forward_tiled(*cam) - ref["rgba"]).max() < IMG_TOL
