@@ -71,6 +71,28 @@ def algorithmic_bytes(stage, P, V, W, H, T, N, M):
     raise ValueError(stage)
 
 
+def profiled_traffic(kernel, tile_size):
+    """HBM bytes per launch of `kernel` at this tile size from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/*_hbm_traffic.md, counters
+    already scaled by the copy-kernel calibration); None when not profiled."""
+    import glob
+    import re
+
+    best = None
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.md"))):
+        for line in open(path):
+            m = re.match(r"\| `(.+?)` \| (\d+) \| [\d.]+ \| [\d.]+ \| ([\d.]+) \| ([\d.]+) \|", line)
+            if not m or kernel not in m.group(1) or f"<{tile_size}," not in m.group(1).replace(" ", ""):
+                continue
+            n = int(m.group(2))
+            if best is None or n > best[0]:
+                best = (n, m.group(1), float(m.group(3)) * 1e9, float(m.group(4)) * 1e9, os.path.basename(path))
+    if best is None:
+        return None
+    return {"bytes": best[2] + best[3], "read": best[2], "written": best[3], "kernel": best[1],
+            "source": "profiles/" + best[4]}
+
+
 def cpu_baseline(scene, cam, width, stride, tile_size):
     """Oracle (C, one thread) on one view: full binning, SH colours/adjoint and
     Adam over every parameter, forward/backward on every `stride`-th tile."""
@@ -127,7 +149,7 @@ def main():
     ap.add_argument("--tuning", default="", help="TS,F,B: tile size and kernel variants, no autotune")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -187,10 +209,16 @@ def main():
     dom = max(("forward", "backward"), key=lambda k: stages[k])
     alg = algorithmic_bytes(dom, P, V, Wd, Wd, T, N, M)
     achieved = alg / (stages[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_backward" if dom == "backward" else "k_forward",
+    kname = "k_backward" if dom == "backward" else "k_forward"
+    prof = profiled_traffic(kname, tile_size)
+    roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(prof["bytes"]) if prof else None,
                 "algorithmic_bytes": int(alg), "kernel_ms": round(stages[dom], 4), "pairs": P}
+    if prof:
+        roofline["traffic_detail"] = {k: prof[k] for k in ("read", "written", "kernel", "source")}
+        roofline["traffic_note"] = ("FETCH_SIZE x2 + WRITE_SIZE x1 per launch (copy-kernel calibration); "
+                                    "writes are dominated by the per-(tile, tet) gradient atomics")
 
     plain = None
     if not args.no_plain:

@@ -101,11 +101,11 @@ __device__ __forceinline__ int32_t span_xr(uint32_t sp) { return (int32_t)(int16
 template <int TS, int B, bool BWD>
 struct BatchLDS {
     static constexpr int SP = TS + 1;
-    float n[4][3][B];       // inward face normals
-    float num[4][B];        // n . (va - cam)
-    float bc[3][B];         // colour + g . (cam - v0)
-    float g[3][B];
-    float dens[B];
+    // per tet, 7 float4 (112 B: lanes reading different tets spread over the
+    // banks): 4 x (inward face normal, n . (va - cam)), (colour + g . (cam - v0),
+    // density), (g, 0), (v0, 0)
+    static constexpr int TR = 7;
+    float4 tr[B * TR];
     float pxy[8][B];        // tile-local projected vertices (x0..x3, y0..y3)
     uint32_t behind[B];
     uint32_t span[B * SP];  // [tet][row] packed span
@@ -113,12 +113,12 @@ struct BatchLDS {
     uint32_t tpref[B + 1];  // exclusive prefix of covered pixels per tet (record slots)
     uint32_t cpref[B + 1];  // exclusive prefix of work chunks per tet
     uint32_t nbp;           // tets processed this batch (capacity cut)
-    // adjoint only: per face (va, vb, vc, raw normal, 1/|n|^2) in face order, v0, vertex ids, tet id
-    // per tet 4 faces x 16 floats, padded to 68 floats so that lanes reading
-    // different tets start 4 banks apart (16-byte reads stay conflict-free)
+    // adjoint only: per face (grad lambda_a, grad lambda_b, grad lambda_c, raw
+    // normal, lambda_{a,b,c}(v0), pad) in face order, 4 faces x 16 floats padded
+    // to 68 floats so that lanes reading different tets start 4 banks apart;
+    // vertex ids; tet id
     static constexpr int FS = 4 * 16 + 4;
     alignas(16) float face[BWD ? B * FS : 1];
-    float v0[BWD ? 3 : 1][B];
     uint32_t vi[BWD ? 4 : 1][B];
     uint32_t tid[BWD ? B : 1];
 };
@@ -155,10 +155,7 @@ __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const 
         V3 raw = cross3(sub3(v[c], v[a]), sub3(v[b], v[a]));
         V3 nn = raw;
         if (dot3(nn, sub3(v[o], v[a])) < 0.0f) nn = neg3(nn);
-        L.n[fi][0][k] = nn.x;
-        L.n[fi][1][k] = nn.y;
-        L.n[fi][2][k] = nn.z;
-        L.num[fi][k] = dot3(nn, sub3(v[a], cam));
+        L.tr[k * L.TR + fi] = make_float4(nn.x, nn.y, nn.z, dot3(nn, sub3(v[a], cam)));
         if (BWD) {
             // barycentric weights of a point h of the face plane, affine in h:
             // lambda_a(h) = lambda_a(v0) + grad_a . (h - v0), grad_a = ((c - b) x raw) / |raw|^2
@@ -179,13 +176,9 @@ __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const 
         }
     }
     float bo = dot3(g, sub3(cam, v[0]));
-    L.bc[0][k] = f.col.x + bo;
-    L.bc[1][k] = f.col.y + bo;
-    L.bc[2][k] = f.col.z + bo;
-    L.g[0][k] = g.x;
-    L.g[1][k] = g.y;
-    L.g[2][k] = g.z;
-    L.dens[k] = f.r[0].w;
+    L.tr[k * L.TR + 4] = make_float4(f.col.x + bo, f.col.y + bo, f.col.z + bo, f.r[0].w);
+    L.tr[k * L.TR + 5] = make_float4(g.x, g.y, g.z, 0.0f);
+    L.tr[k * L.TR + 6] = make_float4(v[0].x, v[0].y, v[0].z, 0.0f);
     float clip[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) mul_vp(vw.vp, v[i], clip[i]);
@@ -199,9 +192,6 @@ __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const 
         }
     }
     if (BWD) {
-        L.v0[0][k] = v[0].x;
-        L.v0[1][k] = v[0].y;
-        L.v0[2][k] = v[0].z;
         L.vi[0][k] = f.q.x;
         L.vi[1][k] = f.q.y;
         L.vi[2][k] = f.q.z;
@@ -336,14 +326,16 @@ template <int TS, int B, bool BWD>
 __device__ __forceinline__ void tet_regs(const BatchLDS<TS, B, BWD>& L, int k, TetRegs& T) {
 #pragma unroll
     for (int fi = 0; fi < 4; ++fi) {
-        T.n[fi] = mk3(L.n[fi][0][k], L.n[fi][1][k], L.n[fi][2][k]);
-        T.num[fi] = L.num[fi][k];
+        const float4 q = L.tr[k * L.TR + fi];
+        T.n[fi] = mk3(q.x, q.y, q.z);
+        T.num[fi] = q.w;
     }
-    T.bc[0] = L.bc[0][k];
-    T.bc[1] = L.bc[1][k];
-    T.bc[2] = L.bc[2][k];
-    T.g = mk3(L.g[0][k], L.g[1][k], L.g[2][k]);
-    T.dens = L.dens[k];
+    const float4 c = L.tr[k * L.TR + 4], gg = L.tr[k * L.TR + 5];
+    T.bc[0] = c.x;
+    T.bc[1] = c.y;
+    T.bc[2] = c.z;
+    T.dens = c.w;
+    T.g = mk3(gg.x, gg.y, gg.z);
 }
 
 // Ray–tet slab intersection against inward planes (rasterize_compute.wgsl:354-390).
@@ -547,7 +539,10 @@ __global__ __launch_bounds__(C::NT) void k_forward(
                         float t_min = s_aux[0][slot], t_max = s_aux[1][slot];
                         int entry = (int)s_ent[slot];
                         V3 en = mk3(0.f, 0.f, 0.f);
-                        if (entry < 4) en = normalize3(neg3(mk3(L.n[entry][0][k], L.n[entry][1][k], L.n[entry][2][k])));
+                        if (entry < 4) {
+                            const float4 q = L.tr[k * L.TR + entry];
+                            en = normalize3(neg3(mk3(q.x, q.y, q.z)));
+                        }
                         float depth_premul = w0 * t_max + w1 * t_min;
                         float ta = Tj * alpha;
                         nx = nx + ta * en.x;
@@ -583,10 +578,10 @@ __global__ __launch_bounds__(C::NT) void k_forward(
 // ---------------------------------------------------------------------------
 constexpr int kTerms = 19;  // d_density, d_v[4][3], d_grad[3], d_colour[3]
 
-// Batch geometry of the adjoint (B <= 32, one mask bit each); SEG = sum each
-// tet's terms across the wave with a segmented scan before the LDS adds
-// (otherwise every lane adds its own terms).
-template <int TS_, int B_, int CAP_, int R_, bool PF_, bool SEG_>
+// Batch geometry of the adjoint (B <= 32, one mask bit each); SCAN = how a
+// wave sums each tet's terms before the LDS adds: 2 segmented scan on DPP,
+// 1 segmented scan on ds_bpermute, 0 none (every lane adds its own terms).
+template <int TS_, int B_, int CAP_, int R_, bool PF_, int SCAN_>
 struct BwdCfg {
     static constexpr int TS = TS_;
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
@@ -594,7 +589,7 @@ struct BwdCfg {
     static constexpr int CAP = CAP_;
     static constexpr int R = R_;
     static constexpr bool PF = PF_;
-    static constexpr bool SEG = SEG_;
+    static constexpr int SCAN = SCAN_;
 };
 
 // Segmented inclusive sum across the wave; lanes with equal `key` form
@@ -620,6 +615,50 @@ __device__ __forceinline__ float seg_scan(float v, const SegMasks& s) {
         if (s.m[j]) v += up;
     }
     return v;
+}
+
+// The same segmented inclusive sum on DPP: four row_shr steps inside each
+// 16-lane row, then row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3)
+// carry the previous rows' partial sums.  Lanes whose DPP source lies outside
+// the row or the row mask read 0.  Masks are 0/1 floats (the summands are
+// finite: every gradient term is clamped to +-1e20 before it is summed).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v, int old) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float v) {
+    // full row mask: bound_ctrl (out-of-row lanes read 0) lets the shift fold into the FMA
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, ROWS == 0xF));
+}
+struct SegMasksDpp {
+    float m[6];
+};
+__device__ __forceinline__ SegMasksDpp seg_masks_dpp(int key) {
+    SegMasksDpp s;
+    s.m[0] = dpp_i<0x111, 0xF>(key, -1) == key ? 1.0f : 0.0f;  // row_shr:1
+    s.m[1] = dpp_i<0x112, 0xF>(key, -1) == key ? 1.0f : 0.0f;  // row_shr:2
+    s.m[2] = dpp_i<0x114, 0xF>(key, -1) == key ? 1.0f : 0.0f;  // row_shr:4
+    s.m[3] = dpp_i<0x118, 0xF>(key, -1) == key ? 1.0f : 0.0f;  // row_shr:8
+    s.m[4] = dpp_i<0x142, 0xA>(key, -1) == key ? 1.0f : 0.0f;  // row_bcast:15
+    s.m[5] = dpp_i<0x143, 0xC>(key, -1) == key ? 1.0f : 0.0f;  // row_bcast:31
+    return s;
+}
+// All N sums level by level (independent DPP reads back to back, no hazard stalls).
+template <int N>
+__device__ __forceinline__ void seg_scan_dpp(float* v, const SegMasksDpp& s) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = fmaf(dpp_f<0x111, 0xF>(v[q]), s.m[0], v[q]);
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = fmaf(dpp_f<0x112, 0xF>(v[q]), s.m[1], v[q]);
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = fmaf(dpp_f<0x114, 0xF>(v[q]), s.m[2], v[q]);
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = fmaf(dpp_f<0x118, 0xF>(v[q]), s.m[3], v[q]);
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = fmaf(dpp_f<0x142, 0xA>(v[q]), s.m[4], v[q]);
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = fmaf(dpp_f<0x143, 0xC>(v[q]), s.m[5], v[q]);
 }
 
 // Vertex slot of each face role (a, b, c) for face fi; the fourth slot is the
@@ -901,7 +940,8 @@ __global__ __launch_bounds__(C::NT) void k_backward(
                 key = k;
                 TetRegs T;
                 tet_regs(L, k, T);
-                const V3 v0 = mk3(L.v0[0][k], L.v0[1][k], L.v0[2][k]);
+                const float4 v0q = L.tr[k * L.TR + 6];
+                const V3 v0 = mk3(v0q.x, v0q.y, v0q.z);
                 PixWalk<TS> w;
                 w.init(L, c);
                 for (uint32_t q = 0; q < cnt; ++q) {
@@ -917,9 +957,17 @@ __global__ __launch_bounds__(C::NT) void k_backward(
                     w.next(L, k);
                 }
             }
-            if (C::SEG) {
+            if (C::SCAN == 2) {
                 // lanes of one tet are contiguous: a segmented wave scan leaves
                 // each tet's sum in its last lane, which adds it to LDS
+                const SegMasksDpp sm = seg_masks_dpp(key);
+                const int next_key = dpp_i<0x130, 0xF>(key, -1);  // wave_shl:1 (lane 63 reads -1)
+                const bool tail = key != 0x7fffffff && next_key != key;
+                seg_scan_dpp<kTerms>(acc, sm);
+#pragma unroll
+                for (int q = 0; q < kTerms; ++q)
+                    if (tail && acc[q] != 0.0f) atomicAdd(&s_acc[key][q], acc[q]);
+            } else if (C::SCAN == 1) {
                 const SegMasks sm = seg_masks(key);
                 const int next_key = __shfl_down(key, 1, 64);
                 const bool tail = key != 0x7fffffff && (lane == 63 || next_key != key);
@@ -996,9 +1044,9 @@ static void bwd_launch(const BwdArgs& a, hipStream_t s) {
 template <int TS>
 static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
     switch (v) {
-        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, false, true>>(a, s); return true;
-        case 1: bwd_launch<BwdCfg<TS, 32, 512, 2, false, true>>(a, s); return true;
-        case 2: bwd_launch<BwdCfg<TS, 32, 384, 2, false, true>>(a, s); return true;
+        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, false, 2>>(a, s); return true;
+        case 1: bwd_launch<BwdCfg<TS, 32, 512, 2, false, 2>>(a, s); return true;
+        case 2: bwd_launch<BwdCfg<TS, 32, 384, 2, false, 2>>(a, s); return true;
         default: return false;
     }
 }

@@ -96,8 +96,9 @@ struct rmesh_renderer {
     bool have_forward = false;
     // graph
     bool graph_mode = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph[8] = {};          // one graph, or one per stage when timing
+    hipGraphExec_t graph_exec[8] = {};
+    int graph_parts = 0;
     uint32_t graph_views = 0, graph_loss = 0;
     float graph_lrs[4] = {0, 0, 0, 0};
     std::vector<ViewParams> hviews;
@@ -165,14 +166,13 @@ rmesh_status upload_views(rmesh_renderer* h, uint32_t V, const float* cams, cons
 }
 
 void drop_graph(rmesh_renderer* h) {
-    if (h->graph_exec) {
-        hipGraphExecDestroy(h->graph_exec);
-        h->graph_exec = nullptr;
+    for (int p = 0; p < 8; ++p) {
+        if (h->graph_exec[p]) (void)hipGraphExecDestroy(h->graph_exec[p]);
+        if (h->graph[p]) (void)hipGraphDestroy(h->graph[p]);
+        h->graph_exec[p] = nullptr;
+        h->graph[p] = nullptr;
     }
-    if (h->graph) {
-        hipGraphDestroy(h->graph);
-        h->graph = nullptr;
-    }
+    h->graph_parts = 0;
 }
 
 // Tile grid of tile size ts; invalidates everything derived from the old grid.
@@ -368,39 +368,53 @@ rmesh_status stage_adam(rmesh_renderer* h, uint32_t V, float lr_v, float lr_d, f
     return RMESH_OK;
 }
 
-// Stage boundaries for timing: 0 start | 1 colours+projection+scan | 2 pair
-// generation | 3 sort | 4 ranges | 5 forward | 6 adjoint | 7 SH adjoint + Adam.
+// The train step as 7 stages; timing marks sit between them:
+// 0 colours + projection + scan | 1 pair generation | 2 sort | 3 ranges |
+// 4 forward | 5 adjoint | 6 SH adjoint + Adam + loss.
+constexpr int kStages = 7;
+
+rmesh_status record_stage(rmesh_renderer* h, int stage, uint32_t V, const float* gt, uint32_t loss_type, float lr_v,
+                          float lr_d, float lr_g, float lr_sh, bool capture) {
+    hipStream_t s = h->stream;
+    switch (stage) {
+        case 0: {
+            rmesh_status st = stage_binning_front(h, V, capture);
+            if (st != RMESH_OK) return st;
+            zero_grads(h, V);
+            return RMESH_OK;
+        }
+        case 1:
+            launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, V, h->counts,
+                         h->block_sums, h->keys_a, h->vals_a, h->cap, h->overflow, s);
+            return RMESH_OK;
+        case 2:
+            h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
+                                               h->sorting_bits(V), h->hist, s);
+            return RMESH_OK;
+        case 3:
+            launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
+            launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
+            return RMESH_OK;
+        case 4: return stage_forward(h, V, false);
+        case 5: return stage_backward(h, V, nullptr, gt, loss_type);
+        default: return stage_adam(h, V, lr_v, lr_d, lr_g, lr_sh);
+    }
+}
+
+// Eager step (or one capture of the whole step), with timing marks.
 rmesh_status record_train(rmesh_renderer* h, uint32_t V, const float* gt, uint32_t loss_type, float lr_v, float lr_d,
                           float lr_g, float lr_sh, bool capture) {
-    mark(h, 0);
-    rmesh_status st = stage_binning_front(h, V, capture);
-    if (st != RMESH_OK) return st;
-    zero_grads(h, V);
-    mark(h, 1);
-    hipStream_t s = h->stream;
-    launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, V, h->counts, h->block_sums,
-                 h->keys_a, h->vals_a, h->cap, h->overflow, s);
-    mark(h, 2);
-    h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
-                                       h->sorting_bits(V), h->hist, s);
-    mark(h, 3);
-    launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
-    launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
-    mark(h, 4);
-    st = stage_forward(h, V, false);
-    if (st != RMESH_OK) return st;
-    mark(h, 5);
-    st = stage_backward(h, V, nullptr, gt, loss_type);
-    if (st != RMESH_OK) return st;
-    mark(h, 6);
-    st = stage_adam(h, V, lr_v, lr_d, lr_g, lr_sh);
-    mark(h, 7);
-    return st;
+    for (int i = 0; i < kStages; ++i) {
+        mark(h, i);
+        rmesh_status st = record_stage(h, i, V, gt, loss_type, lr_v, lr_d, lr_g, lr_sh, capture);
+        if (st != RMESH_OK) return st;
+    }
+    mark(h, kStages);
+    return RMESH_OK;
 }
 
 void free_all(rmesh_renderer* h) {
-    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
-    if (h->graph) hipGraphDestroy(h->graph);
+    drop_graph(h);
     void* ptrs[] = {h->verts, h->base, h->dens, h->grads, h->sh, h->circ, h->idx, h->m_v, h->v_v, h->m_d, h->v_d,
                     h->m_g, h->v_g, h->m_sh, h->v_sh, h->m_b, h->v_b, h->d_verts, h->d_dens, h->d_grads, h->d_colors,
                     h->d_sh, h->rec, h->colors, h->d_views, h->counts, h->block_sums, h->vis_count, h->n_pairs,
@@ -872,35 +886,45 @@ extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, 
     if (gt_dev && gt_dev != h->gt)
         HIP_TRY(hipMemcpyAsync(h->gt, gt_dev, 12ull * h->W * h->H * V, hipMemcpyDeviceToDevice, h->stream));
     float lrs[4] = {lr_v, lr_d, lr_g, lr_sh};
-    bool graph_ok = h->graph_exec && h->graph_views == V && h->graph_loss == loss_type &&
+    // With timing on, every stage is its own graph and the timing events are
+    // recorded between graph launches (never inside a capture).
+    const int parts = h->timing ? kStages : 1;
+    bool graph_ok = h->graph_parts == parts && h->graph_views == V && h->graph_loss == loss_type &&
                     std::memcmp(lrs, h->graph_lrs, sizeof(lrs)) == 0;
     if (h->graph_mode && h->cap > 0 && !graph_ok) {
-        if (h->graph_exec) {
-            hipGraphExecDestroy(h->graph_exec);
-            h->graph_exec = nullptr;
+        drop_graph(h);
+        for (int p = 0; p < parts; ++p) {
+            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            rmesh_status cst = RMESH_OK;
+            for (int i = (parts == 1 ? 0 : p); i < (parts == 1 ? kStages : p + 1) && cst == RMESH_OK; ++i)
+                cst = record_stage(h, i, V, h->gt, loss_type, lr_v, lr_d, lr_g, lr_sh, true);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(h->stream, &graph);
+            if (cst != RMESH_OK || e != hipSuccess) {
+                if (graph) hipGraphDestroy(graph);
+                drop_graph(h);
+                if (cst != RMESH_OK) return cst;
+                return fail(RMESH_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            }
+            h->graph[p] = graph;
+            hipError_t ie = hipGraphInstantiate(&h->graph_exec[p], graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                drop_graph(h);
+                return fail(RMESH_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            }
         }
-        if (h->graph) {
-            hipGraphDestroy(h->graph);
-            h->graph = nullptr;
-        }
-        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        rmesh_status cst = record_train(h, V, h->gt, loss_type, lr_v, lr_d, lr_g, lr_sh, true);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(h->stream, &graph);
-        if (cst != RMESH_OK) {
-            if (graph) hipGraphDestroy(graph);
-            return cst;
-        }
-        if (e != hipSuccess) return fail(RMESH_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        h->graph = graph;
-        HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+        h->graph_parts = parts;
         h->graph_views = V;
         h->graph_loss = loss_type;
         std::memcpy(h->graph_lrs, lrs, sizeof(lrs));
         graph_ok = true;
     }
     if (h->graph_mode && graph_ok) {
-        HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+        for (int p = 0; p < h->graph_parts; ++p) {
+            if (h->graph_parts > 1) mark(h, p);
+            HIP_TRY(hipGraphLaunch(h->graph_exec[p], h->stream));
+        }
+        if (h->graph_parts > 1) mark(h, kStages);
         st = finish_train(h, V, out_loss);
         if (st == RMESH_ERR_CAPACITY) {
             // Parameters were left untouched on overflow: grow and redo eagerly.
@@ -1052,14 +1076,7 @@ extern "C" rmesh_status rmesh_set_timing(rmesh_renderer* h, int enable) {
         for (hipEvent_t& e : h->ev) HIP_TRY(hipEventCreate(&e));
     if ((enable != 0) != h->timing) {
         h->timing = enable != 0;
-        if (h->graph_exec) {
-            (void)hipGraphExecDestroy(h->graph_exec);
-            h->graph_exec = nullptr;
-        }
-        if (h->graph) {
-            (void)hipGraphDestroy(h->graph);
-            h->graph = nullptr;
-        }
+        drop_graph(h);
     }
     return RMESH_OK;
 }

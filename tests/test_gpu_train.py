@@ -199,3 +199,27 @@ def test_autograd_finite_diff():
             a = float(an[i])
             rel = abs(a - num) / max(abs(a), abs(num), 1e-7)
             assert rel < 0.1 or abs(a - num) < 5e-5, (k, i, a, num)
+
+
+def test_stage_timing_graph_matches_untimed(small_sh):
+    """Timing mode replays one graph per stage with events between the launches;
+    results equal the single-graph replay and every stage reports a duration."""
+    scene, cams, w, h = small_sh
+    V = len(cams)
+    gts = np.ascontiguousarray(np.random.default_rng(5).random((V, h, w, 3)).astype(np.float32))
+    runs = []
+    for timing in (False, True):
+        r = _renderer(scene, w, h, max_views=V)
+        r.set_targets(gts)
+        r.set_graph_mode(True)
+        r.set_timing(timing)
+        losses = [r.train_step_batched(cams, 0, 1e-4, 1e-2, 1e-3, 2.5e-3) for _ in range(3)]
+        if timing:
+            st = r.stage_times()
+            assert set(st) == set(r.STAGES)
+            assert all(v >= 0.0 for v in st.values()) and st["forward"] > 0.0 and st["backward"] > 0.0
+        runs.append((losses, r.get_params()))
+    (l0, p0), (l1, p1) = runs
+    np.testing.assert_allclose(l0, l1, rtol=1e-4)
+    for k in ("vertices", "densities", "color_grads", "sh_coeffs"):
+        assert _rel(p1[k], p0[k]) < 1e-5, k
