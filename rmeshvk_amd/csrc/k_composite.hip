@@ -141,6 +141,41 @@ __device__ __forceinline__ void fetch_tet(TetFetch& f, uint32_t t, const float4*
     if (BWD) f.q = reinterpret_cast<const uint4*>(idx)[t];
 }
 
+// LDS staging of a batch's tet data, filled by direct global -> LDS loads
+// (global_load_lds_dwordx4: lane l's 16 bytes land at base + 16 l), so the
+// next batch's gather is in flight during this batch without holding VGPRs.
+template <int B>
+struct Staging {
+    float4 rec[4][B];
+    float4 col[B];
+    uint4 q[B];
+    uint32_t t[B];
+};
+
+// Wave 0 only (lane == tet slot): start loading tet t for the lanes with `on`.
+template <int B>
+__device__ __forceinline__ void stage_async(Staging<B>& S, uint32_t t, bool on, const float4* __restrict__ rec,
+                                            const uint32_t* __restrict__ idx, const float4* __restrict__ colors) {
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    if (on) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            __builtin_amdgcn_global_load_lds((const void*)(rec + 4ull * t + i), (lds_ptr)&S.rec[i][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(colors + t), (lds_ptr)&S.col[0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(idx + 4ull * t), (lds_ptr)&S.q[0], 16, 0, 0);
+        S.t[__lane_id()] = t;
+    }
+}
+
+template <int B>
+__device__ __forceinline__ void staged_tet(const Staging<B>& S, int k, TetFetch& f) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.r[i] = S.rec[i][k];
+    f.col = S.col[k];
+    f.q = S.q[k];
+    f.t = S.t[k];
+}
+
 template <int TS, int B, bool BWD>
 __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const TetFetch& f, const ViewParams& vw,
                                           const TileCtx& tc) {
@@ -581,14 +616,14 @@ constexpr int kTerms = 19;  // d_density, d_v[4][3], d_grad[3], d_colour[3]
 // Batch geometry of the adjoint (B <= 32, one mask bit each); SCAN = how a
 // wave sums each tet's terms before the LDS adds: 2 segmented scan on DPP,
 // 1 segmented scan on ds_bpermute, 0 none (every lane adds its own terms).
-template <int TS_, int B_, int CAP_, int R_, bool PF_, int SCAN_>
+template <int TS_, int B_, int CAP_, int R_, int PF_, int SCAN_>
 struct BwdCfg {
     static constexpr int TS = TS_;
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
     static constexpr int B = B_;
     static constexpr int CAP = CAP_;
     static constexpr int R = R_;
-    static constexpr bool PF = PF_;
+    static constexpr int PF = PF_;  // next batch's records: 0 at batch start, 1 in registers, 2 LDS DMA
     static constexpr int SCAN = SCAN_;
 };
 
@@ -767,8 +802,10 @@ __global__ __launch_bounds__(C::NT) void k_backward(
     float* __restrict__ d_grads, float* __restrict__ d_colors, uint64_t dcolor_view_stride) {
     constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1, NW = NT / 64;
     __shared__ BatchLDS<TS, B, true> L;
-    // per-pair record: od -> t_prev, cp0 -> d_log_t before, cp1, cp2, t_min, t_max, meta
-    __shared__ float s_r[6][CAP];
+    // per-pair record: od -> t_prev, dc . cp -> d_log_t before, t_min, t_max
+    // (dc . cp, the pixel's image gradient dotted with the pair's premultiplied
+    // colour, is all the d(log T) chain needs of the colour)
+    __shared__ float s_r[4][CAP];
     __shared__ uint32_t s_meta[CAP];   // fmin | fmax << 2 | valid << 4 | did << 5
     __shared__ float s_ray[3][TS * TS];
     __shared__ float s_dc[3][TS * TS];
@@ -849,10 +886,20 @@ __global__ __launch_bounds__(C::NT) void k_backward(
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
     TetFetch f;
-    if (C::PF && tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
+    static_assert(C::PF != 2 || B <= 32, "LDS-DMA staging: one wave, next batch inside a 64-entry index window");
+    __shared__ Staging<C::PF == 2 ? B : 1> S;
+    if (C::PF == 1 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
+    if (C::PF == 2 && wid == 0) {
+        const uint32_t t = lane < nb ? sorted_vals[rs + lane] : 0u;
+        stage_async(S, t, lane < nb, rec, idx, col_v);
+    }
     while (done < count) {
+        if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // staged records of this batch have landed
         __syncthreads();
-        if (!C::PF && tid < nb) fetch_tet<true>(f, sorted_vals[rs + done + tid], rec, idx, col_v);
+        uint32_t win = 0;  // wave 0: tet ids of this and the next batch
+        if (C::PF == 2 && wid == 0) win = done + lane < count ? sorted_vals[rs + done + lane] : 0u;
+        if (C::PF == 0 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + done + tid], rec, idx, col_v);
+        if (C::PF == 2 && tid < nb) staged_tet(S, tid, f);
         if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
@@ -864,7 +911,11 @@ __global__ __launch_bounds__(C::NT) void k_backward(
         __syncthreads();
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
-        if (C::PF && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
+        if (C::PF == 1 && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
+        if (C::PF == 2 && wid == 0) {
+            const uint32_t tn = (uint32_t)__shfl((int)win, (int)nbp + lane, 64);
+            stage_async(S, tn, lane < nb_next, rec, idx, col_v);
+        }
         // phase A (chunk-parallel): intersection + integral
         for (uint32_t j = tid; j < nchunks; j += NT) {
             const ChunkRef c = chunk_at(s_chunk, j);
@@ -889,11 +940,9 @@ __global__ __launch_bounds__(C::NT) void k_backward(
                         Integral I;
                         integral(T, rd, t_min, t_max, 0.0f, kMinVal, I);
                         s_r[0][slot] = I.od;
-                        s_r[1][slot] = I.cp[0];
-                        s_r[2][slot] = I.cp[1];
-                        s_r[3][slot] = I.cp[2];
-                        s_r[4][slot] = t_min;
-                        s_r[5][slot] = t_max;
+                        s_r[1][slot] = (s_dc[0][p] * I.cp[0] + s_dc[1][p] * I.cp[1]) + s_dc[2][p] * I.cp[2];
+                        s_r[2][slot] = t_min;
+                        s_r[3][slot] = t_max;
                         meta = (uint32_t)fmin | ((uint32_t)fmax << 2) | (1u << 4);
                         atomicOr(&s_mask[p], 1u << k);
                     }
@@ -906,7 +955,6 @@ __global__ __launch_bounds__(C::NT) void k_backward(
         // phase B (pixel-sequential, back to front over this pixel's covered
         // tets): undo and the d(log T) chain (backward_tiled_compute.wgsl:419-451)
         if (active) {
-            const float dc0 = s_dc[0][tid], dc1 = s_dc[1][tid], dc2 = s_dc[2][tid];
             unsigned int m = s_mask[tid];
             while (m) {
                 const uint32_t k = (uint32_t)__ffs((int)m) - 1u;
@@ -917,11 +965,11 @@ __global__ __launch_bounds__(C::NT) void k_backward(
                 const float t_prev = fast_safe_exp(prev_log_t);
                 log_t = prev_log_t;
                 if (t_prev < 1e-6f) continue;
-                const float cp0 = s_r[1][slot], cp1 = s_r[2][slot], cp2 = s_r[3][slot];
+                const float dcp = s_r[1][slot];
                 s_r[0][slot] = t_prev;
                 s_r[1][slot] = d_log_t;
                 s_meta[slot] |= 32u;
-                d_log_t = safe_clip(d_log_t + ((dc0 * cp0 + dc1 * cp1) + dc2 * cp2) * t_prev, kMinVal, kMaxVal);
+                d_log_t = safe_clip(d_log_t + dcp * t_prev, kMinVal, kMaxVal);
             }
         }
         __syncthreads();
@@ -951,7 +999,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
                         const int p = w.row * TS + w.col;
                         const V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
                         pair_grad<TS, B>(L, k, T, v0, cam, rd, s_dc[0][p], s_dc[1][p], s_dc[2][p], s_r[0][slot],
-                                         s_r[1][slot], s_r[4][slot], s_r[5][slot], (int)(meta & 3u),
+                                         s_r[1][slot], s_r[2][slot], s_r[3][slot], (int)(meta & 3u),
                                          (int)((meta >> 2) & 3u), acc);
                     }
                     w.next(L, k);
@@ -1044,9 +1092,9 @@ static void bwd_launch(const BwdArgs& a, hipStream_t s) {
 template <int TS>
 static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
     switch (v) {
-        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, false, 2>>(a, s); return true;
-        case 1: bwd_launch<BwdCfg<TS, 32, 512, 2, false, 2>>(a, s); return true;
-        case 2: bwd_launch<BwdCfg<TS, 32, 384, 2, false, 2>>(a, s); return true;
+        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2>>(a, s); return true;
+        case 1: bwd_launch<BwdCfg<TS, 32, 512, 3, 0, 2>>(a, s); return true;
+        case 2: bwd_launch<BwdCfg<TS, 32, 576, 3, 2, 2>>(a, s); return true;
         default: return false;
     }
 }
