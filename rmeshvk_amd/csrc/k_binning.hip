@@ -185,54 +185,64 @@ __device__ __forceinline__ uint32_t project_count(const V3 v[4], float dens, con
     return count;
 }
 
-// K1, batched: thread per tet, loop over views.  counts[t*V + v]; per-block
-// pair totals for the scan; per-view visible counts (reference's
-// indirect_args.instance_count, project_compute.wgsl:181).
+// K1, batched: one thread per (tet, view) item, item = tet * V + view (the
+// pair order of the reference within every view: tet-major).  counts[item];
+// per-block pair totals for the scan; visible tets of view 0 (the
+// reference's indirect_args.instance_count, project_compute.wgsl:181).
 __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec, uint32_t M,
                                                  const ViewParams* __restrict__ views, uint32_t V,
                                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ block_sums,
                                                  uint32_t* __restrict__ vis_count) {
     __shared__ uint32_t red[256 / kWave];
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = t < M;
-    V3 v[4];
-    float dens = 0.0f;
-    V3 g;
-    if (live) load_rec(rec, t, v, dens, g);
-    uint32_t sum = 0;
-    for (uint32_t vv = 0; vv < V; ++vv) {
+    __shared__ uint32_t s_vis;
+    if (threadIdx.x == 0) s_vis = 0;
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = item < (uint64_t)M * V;
+    uint32_t c = 0;
+    bool vis0 = false;
+    if (live) {
+        const uint32_t t = (uint32_t)(item / V), vv = (uint32_t)(item % V);
+        V3 v[4];
+        float dens;
+        V3 g;
+        load_rec(rec, t, v, dens, g);
         bool vis = false;
-        uint32_t c = 0;
-        if (live) {
-            c = project_count(v, dens, views[vv], vis);
-            counts[(uint64_t)t * V + vv] = c;
-        }
-        sum += c;
-        uint64_t bal = __ballot(vis);
-        if (__lane_id() == 0 && bal) atomicAdd(&vis_count[vv], (uint32_t)__popcll(bal));
+        c = project_count(v, dens, views[vv], vis);
+        counts[item] = c;
+        vis0 = vis && vv == 0;
     }
+    const uint64_t bal = __ballot(vis0);
+    __syncthreads();
+    if (__lane_id() == 0 && bal) atomicAdd(&s_vis, (uint32_t)__popcll(bal));
     uint32_t total;
-    (void)block_excl_scan(sum, red, total);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+    (void)block_excl_scan(c, red, total);  // ends with a barrier: s_vis is complete
+    if (threadIdx.x == 0) {
+        block_sums[blockIdx.x] = total;
+        if (s_vis) atomicAdd(&vis_count[0], s_vis);
+    }
 }
 
-// Exclusive scan of block sums in place (single workgroup of 1024 threads);
-// writes the grand total (number of pairs P) to *total.
+// Exclusive scan of block sums in place (single workgroup of 1024 threads,
+// 4096 sums per round, each thread 4 consecutive ones: coalesced); writes the
+// grand total (number of pairs P) to *total.
 __global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums, uint32_t n, uint32_t* __restrict__ total) {
     __shared__ uint32_t red[1024 / kWave];
-    uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-    uint32_t lo = threadIdx.x * per;
-    uint32_t hi = min(lo + per, n);
-    uint32_t s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += sums[i];
-    uint32_t tot;
-    uint32_t ex = block_excl_scan(s, red, tot);
-    for (uint32_t i = lo; i < hi; ++i) {
-        uint32_t x = sums[i];
-        sums[i] = ex;
-        ex += x;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += 4096) {
+        const uint32_t i0 = base + 4u * threadIdx.x;
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = i0 + k < n ? sums[i0 + k] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan((x[0] + x[1]) + (x[2] + x[3]), red, tot) + carry;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < n) sums[i0 + k] = ex;
+            ex += x[k];
+        }
+        carry += tot;
     }
-    if (threadIdx.x == 0) *total = tot;
+    if (threadIdx.x == 0) *total = carry;
 }
 
 // K7 (+ K6 sentinel padding): thread per tet, loop over views, pairs written
@@ -243,70 +253,63 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, c
                                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t cap,
                                                uint32_t* __restrict__ overflow) {
     __shared__ uint32_t red[256 / kWave];
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = t < M;
-    uint32_t sum = 0;
-    if (live)
-        for (uint32_t vv = 0; vv < V; ++vv) sum += counts[(uint64_t)t * V + vv];
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = item < (uint64_t)M * V;
+    const uint32_t max_write = live ? counts[item] : 0u;
     uint32_t tot;
-    uint32_t base = block_excl_scan(sum, red, tot) + block_off[blockIdx.x];
-    if (!live || sum == 0) return;
+    const uint32_t start = block_excl_scan(max_write, red, tot) + block_off[blockIdx.x];
+    if (max_write == 0) return;
+    const uint32_t t = (uint32_t)(item / V), vv = (uint32_t)(item % V);
     V3 v[4];
     float dens;
     V3 g;
     load_rec(rec, t, v, dens, g);
-    float4 cd = circ[t];
-    for (uint32_t vv = 0; vv < V; ++vv) {
-        uint32_t max_write = counts[(uint64_t)t * V + vv];
-        uint32_t start = base;
-        base += max_write;
-        if (max_write == 0) continue;
-        const ViewParams& vw = views[vv];
-        float W = vw.Wf, H = vw.Hf, T = (float)vw.tile_size;
-        float proj[4][2];
+    const float4 cd = circ[t];
+    const ViewParams& vw = views[vv];
+    const float W = vw.Wf, H = vw.Hf, T = (float)vw.tile_size;
+    float proj[4][2];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float c[4];
-            mul_vp(vw.vp, v[i], c);
-            float nx = c[0] / (c[3] + 1e-6f);
-            float ny = c[1] / (c[3] + 1e-6f);
-            proj[i][0] = ((nx + 1.0f) * 0.5f) * W;
-            proj[i][1] = ((1.0f - ny) * 0.5f) * H;
-        }
-        uint32_t local = 0;
-        TileBox b;
-        if (tile_box(proj, W, H, T, b)) {
-            V3 diff = sub3(mk3(cd.x, cd.y, cd.z), mk3(vw.cam[0], vw.cam[1], vw.cam[2]));
-            float depth = dot3(diff, diff) - cd.w;
-            uint32_t inv_depth = ~__float_as_uint(depth);
-            for (uint32_t ty = b.y0; ty <= b.y1 && local < max_write; ++ty) {
-                float xl, xr;
-                if (!tile_row_extent(proj, ty, T, xl, xr)) continue;
-                uint32_t tx0, tx1;
-                row_tiles(xl, xr, T, b, tx0, tx1);
-                for (uint32_t tx = tx0; tx <= tx1 && local < max_write; ++tx) {
-                    uint32_t gt = vw.tile_base + ty * vw.tiles_x + tx;
-                    uint32_t wi = start + local;
-                    if (wi < cap) {
-                        keys[wi] = ((uint64_t)gt << 32) | inv_depth;
-                        vals[wi] = t;
-                    } else {
-                        *overflow = 1u;
-                    }
-                    local++;
+    for (int i = 0; i < 4; ++i) {
+        float c[4];
+        mul_vp(vw.vp, v[i], c);
+        float nx = c[0] / (c[3] + 1e-6f);
+        float ny = c[1] / (c[3] + 1e-6f);
+        proj[i][0] = ((nx + 1.0f) * 0.5f) * W;
+        proj[i][1] = ((1.0f - ny) * 0.5f) * H;
+    }
+    uint32_t local = 0;
+    TileBox b;
+    if (tile_box(proj, W, H, T, b)) {
+        V3 diff = sub3(mk3(cd.x, cd.y, cd.z), mk3(vw.cam[0], vw.cam[1], vw.cam[2]));
+        float depth = dot3(diff, diff) - cd.w;
+        uint32_t inv_depth = ~__float_as_uint(depth);
+        for (uint32_t ty = b.y0; ty <= b.y1 && local < max_write; ++ty) {
+            float xl, xr;
+            if (!tile_row_extent(proj, ty, T, xl, xr)) continue;
+            uint32_t tx0, tx1;
+            row_tiles(xl, xr, T, b, tx0, tx1);
+            for (uint32_t tx = tx0; tx <= tx1 && local < max_write; ++tx) {
+                uint32_t gt = vw.tile_base + ty * vw.tiles_x + tx;
+                uint32_t wi = start + local;
+                if (wi < cap) {
+                    keys[wi] = ((uint64_t)gt << 32) | inv_depth;
+                    vals[wi] = t;
+                } else {
+                    *overflow = 1u;
                 }
+                local++;
             }
         }
-        // Slots counted by K1 but not produced by the division-based
-        // re-projection keep the tile_fill sentinel (tile_gen_scan_compute.wgsl:202).
-        for (; local < max_write; ++local) {
-            uint32_t wi = start + local;
-            if (wi < cap) {
-                keys[wi] = ~0ull;
-                vals[wi] = 0u;
-            } else {
-                *overflow = 1u;
-            }
+    }
+    // Slots counted by K1 but not produced by the division-based
+    // re-projection keep the tile_fill sentinel (tile_gen_scan_compute.wgsl:202).
+    for (; local < max_write; ++local) {
+        uint32_t wi = start + local;
+        if (wi < cap) {
+            keys[wi] = ~0ull;
+            vals[wi] = 0u;
+        } else {
+            *overflow = 1u;
         }
     }
 }
@@ -354,10 +357,12 @@ void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
 void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, src, dst, n16);
 }
-uint32_t project_blocks(uint32_t M) { return (M + 255) / 256; }
+uint32_t project_blocks(uint32_t M, uint32_t V) { return (uint32_t)(((uint64_t)M * V + 255) / 256); }
 void launch_project(const float4* rec, uint32_t M, const ViewParams* views, uint32_t V, uint32_t* counts,
                     uint32_t* block_sums, uint32_t* vis_count, hipStream_t s) {
-    hipLaunchKernelGGL(k_project, dim3(project_blocks(M)), dim3(256), 0, s, rec, M, views, V, counts, block_sums, vis_count);
+    if (!M || !V) return;
+    hipLaunchKernelGGL(k_project, dim3(project_blocks(M, V)), dim3(256), 0, s, rec, M, views, V, counts, block_sums,
+                       vis_count);
 }
 void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, hipStream_t s) {
     hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total);
@@ -365,7 +370,8 @@ void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, hipStream_t 
 void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewParams* views, uint32_t V,
                   const uint32_t* counts, const uint32_t* block_off, uint64_t* keys, uint32_t* vals, uint32_t cap,
                   uint32_t* overflow, hipStream_t s) {
-    hipLaunchKernelGGL(k_pairs, dim3(project_blocks(M)), dim3(256), 0, s, rec, circ, M, views, V, counts, block_off,
+    if (!M || !V) return;
+    hipLaunchKernelGGL(k_pairs, dim3(project_blocks(M, V)), dim3(256), 0, s, rec, circ, M, views, V, counts, block_off,
                        keys, vals, cap, overflow);
 }
 void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges, hipStream_t s) {

@@ -153,7 +153,7 @@ struct Staging {
 };
 
 // Wave 0 only (lane == tet slot): start loading tet t for the lanes with `on`.
-template <int B>
+template <bool BWD, int B>
 __device__ __forceinline__ void stage_async(Staging<B>& S, uint32_t t, bool on, const float4* __restrict__ rec,
                                             const uint32_t* __restrict__ idx, const float4* __restrict__ colors) {
     typedef __attribute__((address_space(3))) void* lds_ptr;
@@ -162,54 +162,55 @@ __device__ __forceinline__ void stage_async(Staging<B>& S, uint32_t t, bool on, 
         for (int i = 0; i < 4; ++i)
             __builtin_amdgcn_global_load_lds((const void*)(rec + 4ull * t + i), (lds_ptr)&S.rec[i][0], 16, 0, 0);
         __builtin_amdgcn_global_load_lds((const void*)(colors + t), (lds_ptr)&S.col[0], 16, 0, 0);
-        __builtin_amdgcn_global_load_lds((const void*)(idx + 4ull * t), (lds_ptr)&S.q[0], 16, 0, 0);
+        if (BWD) __builtin_amdgcn_global_load_lds((const void*)(idx + 4ull * t), (lds_ptr)&S.q[0], 16, 0, 0);
         S.t[__lane_id()] = t;
     }
 }
 
-template <int B>
+template <bool BWD, int B>
 __device__ __forceinline__ void staged_tet(const Staging<B>& S, int k, TetFetch& f) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) f.r[i] = S.rec[i][k];
     f.col = S.col[k];
-    f.q = S.q[k];
+    if (BWD) f.q = S.q[k];
     f.t = S.t[k];
 }
 
+// Face fi of tet k: inward plane (and, for the adjoint, the face table entry).
 template <int TS, int B, bool BWD>
-__device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const TetFetch& f, const ViewParams& vw,
-                                          const TileCtx& tc) {
-    V3 v[4] = {mk3(f.r[0].x, f.r[0].y, f.r[0].z), mk3(f.r[1].x, f.r[1].y, f.r[1].z), mk3(f.r[2].x, f.r[2].y, f.r[2].z),
-               mk3(f.r[3].x, f.r[3].y, f.r[3].z)};
-    V3 g = mk3(f.r[1].w, f.r[2].w, f.r[3].w);
-    V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
-#pragma unroll
-    for (int fi = 0; fi < 4; ++fi) {
-        int a, b, c, o;
-        face_idx(fi, a, b, c, o);
-        V3 raw = cross3(sub3(v[c], v[a]), sub3(v[b], v[a]));
-        V3 nn = raw;
-        if (dot3(nn, sub3(v[o], v[a])) < 0.0f) nn = neg3(nn);
-        L.tr[k * L.TR + fi] = make_float4(nn.x, nn.y, nn.z, dot3(nn, sub3(v[a], cam)));
-        if (BWD) {
-            // barycentric weights of a point h of the face plane, affine in h:
-            // lambda_a(h) = lambda_a(v0) + grad_a . (h - v0), grad_a = ((c - b) x raw) / |raw|^2
-            // (and cyclic: (a - c) for b, (b - a) for c)
-            const float inv = 1.0f / dot3(raw, raw);
-            const V3 ga = scl3(cross3(sub3(v[c], v[b]), raw), inv);
-            const V3 gb = scl3(cross3(sub3(v[a], v[c]), raw), inv);
-            const V3 gc = scl3(cross3(sub3(v[b], v[a]), raw), inv);
-            const V3 pa = sub3(v[a], v[0]), pb = sub3(v[b], v[0]), pc = sub3(v[c], v[0]);
-            float* F = &L.face[k * L.FS + fi * 16];
-            F[0] = ga.x; F[1] = ga.y; F[2] = ga.z;
-            F[3] = gb.x; F[4] = gb.y; F[5] = gb.z;
-            F[6] = gc.x; F[7] = gc.y; F[8] = gc.z;
-            F[9] = raw.x; F[10] = raw.y; F[11] = raw.z;
-            F[12] = dot3(cross3(pc, pb), raw) * inv;  // lambda_a(v0)
-            F[13] = dot3(cross3(pa, pc), raw) * inv;  // lambda_b(v0)
-            F[14] = dot3(cross3(pb, pa), raw) * inv;  // lambda_c(v0)
-        }
+__device__ __forceinline__ void store_face(BatchLDS<TS, B, BWD>& L, int k, int fi, V3 va, V3 vb, V3 vc, V3 vo, V3 v0,
+                                           V3 cam) {
+    V3 raw = cross3(sub3(vc, va), sub3(vb, va));
+    V3 nn = raw;
+    if (dot3(nn, sub3(vo, va)) < 0.0f) nn = neg3(nn);
+    L.tr[k * L.TR + fi] = make_float4(nn.x, nn.y, nn.z, dot3(nn, sub3(va, cam)));
+    if (BWD) {
+        // barycentric weights of a point h of the face plane, affine in h:
+        // lambda_a(h) = lambda_a(v0) + grad_a . (h - v0), grad_a = ((c - b) x raw) / |raw|^2
+        // (and cyclic: (a - c) for b, (b - a) for c)
+        const float inv = 1.0f / dot3(raw, raw);
+        const V3 ga = scl3(cross3(sub3(vc, vb), raw), inv);
+        const V3 gb = scl3(cross3(sub3(va, vc), raw), inv);
+        const V3 gc = scl3(cross3(sub3(vb, va), raw), inv);
+        const V3 pa = sub3(va, v0), pb = sub3(vb, v0), pc = sub3(vc, v0);
+        float* F = &L.face[k * L.FS + fi * 16];
+        F[0] = ga.x; F[1] = ga.y; F[2] = ga.z;
+        F[3] = gb.x; F[4] = gb.y; F[5] = gb.z;
+        F[6] = gc.x; F[7] = gc.y; F[8] = gc.z;
+        F[9] = raw.x; F[10] = raw.y; F[11] = raw.z;
+        F[12] = dot3(cross3(pc, pb), raw) * inv;  // lambda_a(v0)
+        F[13] = dot3(cross3(pa, pc), raw) * inv;  // lambda_b(v0)
+        F[14] = dot3(cross3(pb, pa), raw) * inv;  // lambda_c(v0)
     }
+}
+
+// Per-tet data of tet k: base colour at the camera, density, g, v0, the
+// tile-local projected vertices, vertex ids and tet id.
+template <int TS, int B, bool BWD>
+__device__ __forceinline__ void store_rest(BatchLDS<TS, B, BWD>& L, int k, const TetFetch& f, const V3* v,
+                                           const ViewParams& vw, const TileCtx& tc) {
+    const V3 g = mk3(f.r[1].w, f.r[2].w, f.r[3].w);
+    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
     float bo = dot3(g, sub3(cam, v[0]));
     L.tr[k * L.TR + 4] = make_float4(f.col.x + bo, f.col.y + bo, f.col.z + bo, f.r[0].w);
     L.tr[k * L.TR + 5] = make_float4(g.x, g.y, g.z, 0.0f);
@@ -232,6 +233,47 @@ __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const 
         L.vi[2][k] = f.q.z;
         L.vi[3][k] = f.q.w;
         L.tid[k] = f.t;
+    }
+}
+
+// One lane stages a whole tet (records in registers).
+template <int TS, int B, bool BWD>
+__device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const TetFetch& f, const ViewParams& vw,
+                                          const TileCtx& tc) {
+    V3 v[4] = {mk3(f.r[0].x, f.r[0].y, f.r[0].z), mk3(f.r[1].x, f.r[1].y, f.r[1].z), mk3(f.r[2].x, f.r[2].y, f.r[2].z),
+               mk3(f.r[3].x, f.r[3].y, f.r[3].z)};
+    V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
+#pragma unroll
+    for (int fi = 0; fi < 4; ++fi) {
+        int a, b, c, o;
+        face_idx(fi, a, b, c, o);
+        store_face(L, k, fi, v[a], v[b], v[c], v[o], v[0], cam);
+    }
+    store_rest(L, k, f, v, vw, tc);
+}
+
+// Five lanes per tet stage the batch from the LDS staging area: lanes 0-3 a
+// face each, lane 4 the per-tet data.
+template <int TS, int B, bool BWD, int NT>
+__device__ __forceinline__ void store_batch_staged(BatchLDS<TS, B, BWD>& L, const Staging<B>& S, int nb,
+                                                   const ViewParams& vw, const TileCtx& tc) {
+    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
+    for (int it = threadIdx.x; it < 5 * nb; it += NT) {
+        const int k = it / 5, part = it % 5;
+        auto vtx = [&](int i) {
+            const float4 r = S.rec[i][k];
+            return mk3(r.x, r.y, r.z);
+        };
+        if (part < 4) {
+            int a, b, c, o;
+            face_idx(part, a, b, c, o);
+            store_face(L, k, part, vtx(a), vtx(b), vtx(c), vtx(o), vtx(0), cam);
+        } else {
+            TetFetch f;
+            staged_tet<BWD>(S, k, f);
+            const V3 v[4] = {vtx(0), vtx(1), vtx(2), vtx(3)};
+            store_rest(L, k, f, v, vw, tc);
+        }
     }
 }
 
@@ -441,14 +483,14 @@ __device__ __forceinline__ void integral(const TetRegs& T, V3 rd, float t_min, f
 // Batch geometry of the forward: B tets per LDS batch (<= 64, one mask bit
 // each), CAP (pixel, tet) records per batch, R pixels per lane in the pair
 // phase, PF = fetch the next batch's records while this batch runs.
-template <int TS_, int B_, int CAP_, int R_, bool PF_>
+template <int TS_, int B_, int CAP_, int R_, int PF_>
 struct FwdCfg {
     static constexpr int TS = TS_;
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
     static constexpr int B = B_;
     static constexpr int CAP = CAP_;
     static constexpr int R = R_;
-    static constexpr bool PF = PF_;
+    static constexpr int PF = PF_;  // next batch's records: 0 at batch start, 1 in registers, 2 LDS DMA
 };
 
 template <class C, bool AUX>
@@ -466,6 +508,7 @@ __global__ __launch_bounds__(C::NT) void k_forward(
     __shared__ float s_aux[3][NA];     // t_min, t_max, w0
     __shared__ uint32_t s_ent[NA];
     __shared__ float s_ray[3][TS * TS];
+    __shared__ uint32_t s_live[TS * TS];  // pixel inside the image and not yet saturated (log T >= -13.8)
     __shared__ unsigned long long s_mask[TS * TS];  // covered tets of the batch per pixel
     __shared__ uint32_t s_chunk[chunk_capacity(CAP, R, B)];
     static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
@@ -487,6 +530,7 @@ __global__ __launch_bounds__(C::NT) void k_forward(
         s_ray[0][tid] = rd.x;
         s_ray[1][tid] = rd.y;
         s_ray[2][tid] = rd.z;
+        s_live[tid] = active ? 1u : 0u;
     }
     float cr = 0.f, cg = 0.f, cb = 0.f, log_t = 0.f;
     float nx = 0.f, ny = 0.f, nz = 0.f, dep = 0.f, e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
@@ -496,11 +540,23 @@ __global__ __launch_bounds__(C::NT) void k_forward(
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
     TetFetch f;
-    if (C::PF && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, col_v);
+    static_assert(C::PF != 2 || B <= 32, "LDS-DMA staging: one wave, next batch inside a 64-entry index window");
+    __shared__ Staging<C::PF == 2 ? B : 1> S;
+    const int wid = tid / 64, lane = (int)__lane_id();
+    if (C::PF == 1 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, col_v);
+    if (C::PF == 2 && wid == 0) {
+        const uint32_t t = lane < nb ? sorted_vals[re - 1 - lane] : 0u;
+        stage_async<false>(S, t, lane < nb, rec, nullptr, col_v);
+    }
     while (done < count) {
+        // staged records of this batch have landed (and none is in flight at an early exit)
+        if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);
         if (__syncthreads_and(!active || log_t < -13.8f)) break;
-        if (!C::PF && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - done - tid], rec, nullptr, col_v);
-        if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
+        uint32_t win = 0;  // wave 0: tet ids of this and the next batch (back to front of the sorted list)
+        if (C::PF == 2 && wid == 0) win = done + lane < count ? sorted_vals[re - 1 - done - lane] : 0u;
+        if (C::PF == 0 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - done - tid], rec, nullptr, col_v);
+        if constexpr (C::PF == 2) store_batch_staged<TS, B, false, NT>(L, S, nb, vw, tc);
+        else if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
         if (tid < TS * TS) s_mask[tid] = 0ull;
         __syncthreads();
         batch_layout<TS, B, false, true, CAP, R, NT>(L, nb);
@@ -511,7 +567,11 @@ __global__ __launch_bounds__(C::NT) void k_forward(
         // prefetch the next batch's tet records while this one is processed
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
-        if (C::PF && tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
+        if (C::PF == 1 && tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
+        if (C::PF == 2 && wid == 0) {
+            const uint32_t tn = (uint32_t)__shfl((int)win, (int)nbp + lane, 64);
+            stage_async<false>(S, tn, lane < nb_next, rec, nullptr, col_v);
+        }
         // chunk-parallel: intersection + volume integral of R pixels of one tet
         for (uint32_t j = tid; j < nchunks; j += NT) {
             const ChunkRef c = chunk_at(s_chunk, j);
@@ -523,9 +583,9 @@ __global__ __launch_bounds__(C::NT) void k_forward(
             w.init(L, c);
             for (uint32_t q = 0; q < cnt; ++q) {
                 const uint32_t slot = slot0 + q;
-                const uint32_t gx = tc.tile_x * TS + w.col, gy = tc.tile_y * TS + w.row;
-                if (gx < vw.width && gy < vw.height) {
-                    const int p = w.row * TS + w.col;
+                const int p = w.row * TS + w.col;
+                // saturated pixels skip the rest of the tile's tets (their compositing loop would)
+                if (s_live[p]) {
                     V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
                     float t_min, t_max;
                     int fmin, fmax;
@@ -594,10 +654,12 @@ __global__ __launch_bounds__(C::NT) void k_forward(
                     }
                 }
             }
+            if (log_t < -13.8f) s_live[tid] = 0u;
         }
         done = next;
         nb = nb_next;
     }
+    if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // no LDS DMA outlives the workgroup
     if (active) {
         uint64_t pix = ((uint64_t)view * vw.height + py) * vw.width + px;
         reinterpret_cast<float4*>(rgba)[pix] = make_float4(cr, cg, cb, 1.0f - safe_exp(log_t));
@@ -891,7 +953,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
     if (C::PF == 1 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
     if (C::PF == 2 && wid == 0) {
         const uint32_t t = lane < nb ? sorted_vals[rs + lane] : 0u;
-        stage_async(S, t, lane < nb, rec, idx, col_v);
+        stage_async<true>(S, t, lane < nb, rec, idx, col_v);
     }
     while (done < count) {
         if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // staged records of this batch have landed
@@ -899,8 +961,8 @@ __global__ __launch_bounds__(C::NT) void k_backward(
         uint32_t win = 0;  // wave 0: tet ids of this and the next batch
         if (C::PF == 2 && wid == 0) win = done + lane < count ? sorted_vals[rs + done + lane] : 0u;
         if (C::PF == 0 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + done + tid], rec, idx, col_v);
-        if (C::PF == 2 && tid < nb) staged_tet(S, tid, f);
-        if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
+        if constexpr (C::PF == 2) store_batch_staged<TS, B, true, NT>(L, S, nb, vw, tc);
+        else if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
         __syncthreads();
@@ -914,7 +976,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
         if (C::PF == 1 && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
         if (C::PF == 2 && wid == 0) {
             const uint32_t tn = (uint32_t)__shfl((int)win, (int)nbp + lane, 64);
-            stage_async(S, tn, lane < nb_next, rec, idx, col_v);
+            stage_async<true>(S, tn, lane < nb_next, rec, idx, col_v);
         }
         // phase A (chunk-parallel): intersection + integral
         for (uint32_t j = tid; j < nchunks; j += NT) {
@@ -1047,6 +1109,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
         done = next;
         nb = nb_next;
     }
+    if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // no LDS DMA outlives the workgroup
 }
 
 // ---------------------------------------------------------------------------
@@ -1066,9 +1129,9 @@ static void fwd_launch(const FwdArgs& a, hipStream_t s) {
 template <int TS>
 static bool fwd_variant(const FwdArgs& a, int v, hipStream_t s) {
     switch (v) {
-        case 0: fwd_launch<FwdCfg<TS, 32, 768, 1, true>>(a, s); return true;
-        case 1: fwd_launch<FwdCfg<TS, 48, 768, 1, true>>(a, s); return true;
-        case 2: fwd_launch<FwdCfg<TS, 64, 768, 1, true>>(a, s); return true;
+        case 0: fwd_launch<FwdCfg<TS, 32, 768, 1, 2>>(a, s); return true;
+        case 1: fwd_launch<FwdCfg<TS, 48, 768, 1, 1>>(a, s); return true;
+        case 2: fwd_launch<FwdCfg<TS, 32, 640, 1, 2>>(a, s); return true;
         default: return false;
     }
 }

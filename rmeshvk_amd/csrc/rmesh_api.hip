@@ -224,7 +224,7 @@ rmesh_status stage_binning_front(rmesh_renderer* h, uint32_t V, bool capture) {
     launch_fill_u32(h->vis_count, 0u, h->max_views, s);
     launch_fill_u32(h->overflow, 0u, 1, s);
     launch_project(h->rec, h->M, h->d_views, V, h->counts, h->block_sums, h->vis_count, s);
-    launch_spine_scan(h->block_sums, project_blocks(h->M), h->n_pairs, s);
+    launch_spine_scan(h->block_sums, project_blocks(h->M, V), h->n_pairs, s);
     if (!capture) {
         uint32_t P = 0;
         HIP_TRY(hipMemcpyAsync(&P, h->n_pairs, 4, hipMemcpyDeviceToHost, s));
@@ -510,7 +510,7 @@ extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer**
     CREATE_TRY(dalloc(&h->colors, M * (h->has_sh ? V : 1)));
     CREATE_TRY(dalloc(&h->d_views, V));
     CREATE_TRY(dalloc(&h->counts, M * V));
-    CREATE_TRY(dalloc(&h->block_sums, project_blocks(h->M) + 1));
+    CREATE_TRY(dalloc(&h->block_sums, project_blocks(h->M, V) + 1));
     CREATE_TRY(dalloc(&h->vis_count, V));
     CREATE_TRY(dalloc(&h->n_pairs, 1));
     CREATE_TRY(dalloc(&h->overflow, 1));

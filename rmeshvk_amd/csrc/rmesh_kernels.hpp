@@ -15,7 +15,7 @@ void launch_prep_records(const float* verts, const uint32_t* idx, const float* d
 void launch_copy_colors(const float* base, uint32_t M, float4* col, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t s);
-uint32_t project_blocks(uint32_t M);
+uint32_t project_blocks(uint32_t M, uint32_t V);  // workgroups of the (tet, view) item kernels
 void launch_project(const float4* rec, uint32_t M, const ViewParams* views, uint32_t V, uint32_t* counts,
                     uint32_t* block_sums, uint32_t* vis_count, hipStream_t s);
 void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, hipStream_t s);

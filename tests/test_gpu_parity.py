@@ -209,3 +209,36 @@ def test_autotune_keeps_parameters(config_a):
     # the chosen configuration still matches the oracle
     ref = orc.render(scene, cam, w, h, ts=ts)
     assert np.abs(r.forward_tiled(*cam) - ref["rgba"]).max() < IMG_TOL
+
+
+def test_batched_binning_at_scale():
+    """~300K tets x 4 views at 512^2 (> 4096 scan blocks of (tet, view) items):
+    the batched sort holds, view by view, exactly the oracle's sorted pairs
+    (global tile = view * T + tile), and the ranges match their closed form."""
+    from rmeshvk_amd import RMeshRenderer
+
+    rng = np.random.default_rng(21)
+    scene = make_scene(cluster_points(45000, rng, 6, 0.2), rng)
+    w = h = 512
+    cams = fibonacci_cameras(4, 3.0, 50.0, w, h)
+    r = RMeshRenderer(scene["vertices"], scene["indices"], scene["base_colors"], scene["densities"],
+                      scene["color_grads"], scene["circumdata"], w, h, device_id=0, max_views=len(cams))
+    r.forward_batched(cams, copy_out=False)
+    sk, sv, rg = r.debug_sorted(views=len(cams))
+    _, _, T = orc.tile_grid(w, h, 12)
+    M = len(scene["indices"]) // 4
+    assert (M * len(cams) + 255) // 256 > 4096
+    tile = (sk >> np.uint64(32)).astype(np.int64)
+    real = tile != 0xFFFFFFFF
+    for v, cam in enumerate(cams):
+        tt, _ = orc.project(scene, cam, w, h, 12)
+        keys, vals, P = orc.pairs(scene, cam, w, h, tt, 12)
+        ok, ov = orc.sort(keys, vals, orc.sorting_bits(T))
+        sel = real & (tile >= v * T) & (tile < (v + 1) * T)
+        np.testing.assert_array_equal(sk[sel] - (np.uint64(v * T) << np.uint64(32)), ok[(ok >> np.uint64(32)) != 0xFFFFFFFF])
+        np.testing.assert_array_equal(sv[sel], ov[(ok >> np.uint64(32)) != 0xFFFFFFFF])
+    # ranges: closed form over the real pairs
+    t_real = tile[real]
+    for gtile in np.unique(t_real)[::97]:
+        lo = np.searchsorted(t_real, gtile, "left")
+        assert rg[2 * gtile] == lo
