@@ -3,33 +3,42 @@
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
 import this module; the product (rmeshvk_amd, rmesh_wgpu) never does.
 
-What pins each oracle function (checks live in tests/test_oracle_*.py):
+What pins each oracle function (checks in tests/test_oracle_kats.py,
+tests/test_oracle_render.py, tests/test_oracle_grad.py; fixtures in
+tests/golden/reference_kats.json):
 
 ================  ==========================================================
-orc_project       regular-tet known answer (kernel_tests.rs:1618-1671:
-                  visible count 1, tiles > 0); brute-force tile coverage
-                  (every pixel whose centre a tet covers lies in a binned
-                  tile); sum(tiles_touched) == P.
-orc_pairs         same scanline as orc_project (count == generated pairs
-                  except at 1-ulp NDC disagreements); keys decode to
-                  (tile, ~bits(|c-cam|^2 - r^2)) recomputed in numpy f64.
-orc_sort          numpy ``argsort(kind="stable")`` of the masked key; the
-                  reference's 64-key KAT (kernel_tests.rs:96-167).
-orc_ranges        closed form from sorted tile ids (numpy searchsorted);
-                  the reference's sentinel quirk on a planted case.
-orc_forward       closed form for the regular tet with g = 0
-                  (RGB = c * alpha, alpha = 1 - exp(-sigma * chord), chord
-                  from an independent numpy slab test); an independent
-                  per-pixel numpy renderer (tests/test_oracle_render.py).
-orc_loss          KAT of kernel_tests.rs:175-305 (+-1/4096 at 64x64).
-orc_backward      f64 central finite differences of the f64 forward
-                  (micro-scenes and config-A crops); the reference's
-                  tolerances rel 0.10 (kernel_tests.rs:1067-1075).
+orc_project       regular-tet known answers (kernel_tests.rs:1547-1671:
+                  projection round trip, visible count 1, tiles > 0);
+                  brute-force coverage (every pixel whose ray has a
+                  non-grazing chord through a tet lies in a binned tile).
+orc_pairs         sum(tiles_touched) == P; sentinel fill of the padding
+                  (kernel_tests.rs:428-545); keys decode to (tile,
+                  ~bits(float32 |c-cam|^2 - r^2)), the depth agreeing with
+                  float64.
+orc_sort          numpy ``argsort(kind="stable")`` of the masked key (random
+                  bits above sorting_bits, heavy ties); the reference's
+                  64-key check (kernel_tests.rs:96-167).
+orc_sorting_bits  the table of drs.rs:23-26.
+orc_ranges        closed form from sorted tile ids; the sentinel behaviour
+                  of tile_ranges_compute.wgsl:21-36 on a planted case.
+orc_forward       an independent float64 quadrature renderer
+                  (tests/npref.py) on the regular tet, two tets and a
+                  280-tet Delaunay mesh; the regular-tet closed form
+                  RGB = c * alpha; tile-size invariance; min_t; aux normals
+                  and depth of a single tet.
+                  PARITY UNPINNED: the aux entropy channels (aux[4..8]) have
+                  no independent check (they are outside the train step).
+orc_loss          KAT of kernel_tests.rs:176-305 (+-1/4096 at 64x64); the L2
+                  closed form.
+orc_backward      float64 central finite differences (eps 1e-6) of the
+                  float64 forward on four scenes: relative L2 < 1e-6.
 orc_adam          KAT of kernel_tests.rs:313-420 (m = 0.1 g, v = 0.001 g^2,
-                  update ~ -lr sign(g)) plus a closed-form multi-step check.
-orc_sh_colors     degree 0 reduces to softplus(C0 sh0 + 0.5 + g.(v0-c));
-                  basis orthonormality checked by quadrature.
-orc_sh_backward   f64 finite differences (no reference adjoint exists).
+                  update ~ -lr sign(g)); constant-gradient closed form; NaN.
+orc_sh_colors     basis == scipy real spherical harmonics (Condon-Shortley
+                  phase), hence orthonormal; degree-0 closed form.
+orc_sh_backward   float64 finite differences, degrees 0-3.
+orc_circumspheres the regular-tet known answer; equidistance of the vertices.
 ================  ==========================================================
 """
 

@@ -6,6 +6,10 @@
  * binning/sort/ranges and the tolerance checker for images and gradients) and
  * once with R = double (used for finite-difference gradient checks).
  *
+ * What pins each function is listed in oracle.py; one output is parity
+ * unpinned: the aux entropy channels of orc_forward (aux[4..8]) have no
+ * independent check.
+ *
  * Every function names the reference kernel and lines it follows.  Float
  * semantics chosen where WGSL leaves them to the driver (documented in
  * DESIGN.md, "oracle conventions"):

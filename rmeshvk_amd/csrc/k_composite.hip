@@ -108,6 +108,7 @@ struct BatchLDS {
     float4 tr[B * TR];
     float pxy[8][B];        // tile-local projected vertices (x0..x3, y0..y3)
     uint32_t behind[B];
+    uint32_t rows[B];       // tile rows the tet's projected y-range touches: r0 | r1 << 8 (r0 > r1: none)
     uint32_t span[B * SP];  // [tet][row] packed span
     uint32_t rpref[B * SP]; // [tet][row] prefix of row widths ([tet][TS] = tet total), then record base - first column
     uint32_t tpref[B + 1];  // exclusive prefix of covered pixels per tet (record slots)
@@ -220,13 +221,24 @@ __device__ __forceinline__ void store_rest(BatchLDS<TS, B, BWD>& L, int k, const
     for (int i = 0; i < 4; ++i) mul_vp(vw.vp, v[i], clip[i]);
     bool behind = (clip[0][3] <= 0.0f) || (clip[1][3] <= 0.0f) || (clip[2][3] <= 0.0f) || (clip[3][3] <= 0.0f);
     L.behind[k] = behind ? 1u : 0u;
+    uint32_t rr = 0xFFu;
     if (!behind) {
+        float py_[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             L.pxy[i][k] = ((clip[i][0] / clip[i][3] + 1.0f) * 0.5f) * vw.Wf - tc.tile_ox;
-            L.pxy[4 + i][k] = ((1.0f - clip[i][1] / clip[i][3]) * 0.5f) * vw.Hf - tc.tile_oy;
+            py_[i] = ((1.0f - clip[i][1] / clip[i][3]) * 0.5f) * vw.Hf - tc.tile_oy;
+            L.pxy[4 + i][k] = py_[i];
         }
+        // rows whose pixel band [row, row + 1] meets [ymin, ymax]: the only rows
+        // where an edge can cross or a vertex can lie (row + 1 >= ymin, row <= ymax)
+        const float ymin = fminf(fminf(py_[0], py_[1]), fminf(py_[2], py_[3]));
+        const float ymax = fmaxf(fmaxf(py_[0], py_[1]), fmaxf(py_[2], py_[3]));
+        const float r0 = fmaxf(ceilf(ymin) - 1.0f, 0.0f);
+        const float r1 = fminf(floorf(ymax), (float)(TS - 1));
+        if (r0 <= r1) rr = (uint32_t)r0 | ((uint32_t)r1 << 8);
     }
+    L.rows[k] = rr;
     if (BWD) {
         L.vi[0][k] = f.q.x;
         L.vi[1][k] = f.q.y;
@@ -278,32 +290,30 @@ __device__ __forceinline__ void store_batch_staged(BatchLDS<TS, B, BWD>& L, cons
 }
 
 // Spans of the batch, then (wave 0) per-tet row prefixes, the pixel and chunk
-// prefixes over tets and the capacity cut (records <= CAP).  Rows outside the
-// tet's projected y-range are empty without evaluating the scanline (no edge
-// crossing and no vertex can fall in them).
+// prefixes over tets and the capacity cut (records <= CAP).  Only the rows of
+// each tet's projected y-range are evaluated (rows outside it have no edge
+// crossing and no vertex: empty), four lanes per tet.
 template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int R, int NT>
 __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb) {
     constexpr int SP = TS + 1;
-    for (int it = threadIdx.x; it < nb * TS; it += NT) {
-        int k = it / TS, row = it % TS;
-        uint32_t sp = 0xffff0000u;  // xl = 0, xr = -1 (empty)
-        if (!L.behind[k]) {
-            float px_[4] = {L.pxy[0][k], L.pxy[1][k], L.pxy[2][k], L.pxy[3][k]};
-            float py_[4] = {L.pxy[4][k], L.pxy[5][k], L.pxy[6][k], L.pxy[7][k]};
-            float ymin = fminf(fminf(py_[0], py_[1]), fminf(py_[2], py_[3]));
-            float ymax = fmaxf(fmaxf(py_[0], py_[1]), fmaxf(py_[2], py_[3]));
-            float yc = (float)row + 0.5f;
-            if (!(yc + 0.5f < ymin || yc - 0.5f > ymax)) sp = row_span<FORWARD_RULE>(px_, py_, (uint32_t)row, TS);
-        }
-        L.span[k * SP + row] = sp;
+    // four lanes per tet over the tet's row range (rows outside it are empty
+    // and never read)
+    for (int it = threadIdx.x; it < 4 * nb; it += NT) {
+        const int k = it >> 2;
+        const uint32_t rr = L.rows[k], r0 = rr & 0xFFu, r1 = rr >> 8;
+        if (r0 > r1) continue;
+        const float px_[4] = {L.pxy[0][k], L.pxy[1][k], L.pxy[2][k], L.pxy[3][k]};
+        const float py_[4] = {L.pxy[4][k], L.pxy[5][k], L.pxy[6][k], L.pxy[7][k]};
+        for (uint32_t row = r0 + (it & 3); row <= r1; row += 4)
+            L.span[k * SP + row] = row_span<FORWARD_RULE>(px_, py_, row, TS);
     }
     __syncthreads();
     if (threadIdx.x < 64) {
         const int k = threadIdx.x;
         uint32_t tot = 0;
         if (k < nb) {
-#pragma unroll
-            for (int r = 0; r < TS; ++r) {
+            const uint32_t rr = L.rows[k];
+            for (uint32_t r = rr & 0xFFu; r <= (rr >> 8); ++r) {
                 uint32_t sp = L.span[k * SP + r];
                 L.rpref[k * SP + r] = tot;
                 tot += (uint32_t)max(span_xr(sp) - span_xl(sp) + 1, 0);
@@ -337,19 +347,23 @@ constexpr int chunk_capacity(int CAP, int R, int B) { return CAP / R + B; }
 template <int TS, int B, bool BWD, int R, int NT>
 __device__ __forceinline__ void build_chunks(BatchLDS<TS, B, BWD>& L, uint32_t nbp, uint32_t* __restrict__ tab) {
     constexpr int SP = TS + 1;
-    for (int it = threadIdx.x; it < (int)nbp * TS; it += NT) {
-        const int k = it / TS, r = it % TS;
-        const uint32_t sp = L.span[k * SP + r];
-        const int xl = span_xl(sp), w = span_xr(sp) - xl + 1;
-        if (w <= 0) continue;
-        const uint32_t base = L.rpref[k * SP + r], tot = L.rpref[k * SP + TS];
-        const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
-        for (uint32_t c = c0; c <= c1; ++c) {
-            const uint32_t l = c * R;
-            tab[L.cpref[k] + c] = (uint32_t)k | ((uint32_t)r << 6) | ((uint32_t)(xl + (int)(l - base)) << 11) |
-                                  ((L.tpref[k] + l) << 16) | (min((uint32_t)R, tot - l) << 27);
+    for (int it = threadIdx.x; it < 4 * (int)nbp; it += NT) {
+        const int k = it >> 2;
+        const uint32_t rr = L.rows[k];
+        const uint32_t tot = L.rpref[k * SP + TS];
+        for (uint32_t r = (rr & 0xFFu) + (it & 3); r <= (rr >> 8); r += 4) {
+            const uint32_t sp = L.span[k * SP + r];
+            const int xl = span_xl(sp), w = span_xr(sp) - xl + 1;
+            if (w <= 0) continue;
+            const uint32_t base = L.rpref[k * SP + r];
+            const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
+            for (uint32_t c = c0; c <= c1; ++c) {
+                const uint32_t l = c * R;
+                tab[L.cpref[k] + c] = (uint32_t)k | (r << 6) | ((uint32_t)(xl + (int)(l - base)) << 11) |
+                                      ((L.tpref[k] + l) << 16) | (min((uint32_t)R, tot - l) << 27);
+            }
+            L.rpref[k * SP + r] = L.tpref[k] + base - (uint32_t)xl;  // read only by this thread above
         }
-        L.rpref[k * SP + r] = L.tpref[k] + base - (uint32_t)xl;  // read only by this thread above
     }
 }
 
@@ -365,18 +379,19 @@ __device__ __forceinline__ ChunkRef chunk_at(const uint32_t* tab, uint32_t j) {
 // Walks the covered pixels of one tet in (row, col) order starting at local index l.
 template <int TS>
 struct PixWalk {
-    int row, col, xr;
+    int row, col, xr, r1;
     template <int B, bool BWD>
     __device__ __forceinline__ void init(const BatchLDS<TS, B, BWD>& L, const ChunkRef& c) {
         row = c.row;
         col = c.col;
         xr = span_xr(L.span[c.k * (TS + 1) + c.row]);
+        r1 = (int)(L.rows[c.k] >> 8);
     }
     template <int B, bool BWD>
     __device__ __forceinline__ void next(const BatchLDS<TS, B, BWD>& L, int k) {
         constexpr int SP = TS + 1;
         if (++col > xr) {
-            for (int r = row + 1; r < TS; ++r) {
+            for (int r = row + 1; r <= r1; ++r) {
                 uint32_t sp = L.span[k * SP + r];
                 if (span_xr(sp) >= span_xl(sp)) {
                     row = r;

@@ -175,30 +175,42 @@ __device__ __forceinline__ float adam_update(float& p, float gr, float& m, float
 }
 
 // SH adjoint (+ optional fused Adam on the SH coefficients): one lane per
-// (tet, colour channel), 64 tets per 192-lane workgroup.  A lane streams its
-// channel's (deg+1)^2 coefficients, m and v as one contiguous row (adjacent
-// lanes read adjacent rows: coalesced), accumulates over the views, and the
-// three channel partials of a tet's direction/offset adjoint are summed in a
-// fixed order through LDS before one lane scatters them to the vertices.
+// (tet, colour channel), 64 tets per 192-lane workgroup.  The workgroup's
+// coefficients (and Adam's m, v) are one contiguous block in HBM: they are
+// moved between HBM and LDS with coalesced dword accesses, and each lane works
+// on its channel's row in LDS (rows padded to NC + 1 floats: conflict-free).
+// The three channel partials of a tet's direction/offset adjoint are summed in
+// a fixed order through LDS before one lane scatters them to the vertices.
 constexpr int kShTets = 64;
 
 template <int DEG, bool ADAM>
 __global__ __launch_bounds__(3 * kShTets) void k_sh_backward(ShBwdArgs a) {
     constexpr int NC = (DEG + 1) * (DEG + 1);
+    constexpr int SR = NC + 1;                      // LDS row stride
+    constexpr int NIO = ADAM ? 3 : 1;               // sh (+ m, v)
+    __shared__ float s_io[NIO][3 * kShTets * SR];
     __shared__ float red[kShTets][3][4];  // per channel: d(centroid).xyz, d(offset)
     const uint32_t lt = threadIdx.x / 3, ch = threadIdx.x % 3;
-    const uint32_t t = blockIdx.x * kShTets + lt;
+    const uint32_t t0 = blockIdx.x * kShTets;
+    const uint32_t t = t0 + lt;
+    const uint32_t nflt = min((uint32_t)kShTets, a.M - t0) * 3u * NC;  // floats of this workgroup
+    float* io_g[3] = {a.sh + (uint64_t)t0 * 3 * NC, ADAM ? a.m_sh + (uint64_t)t0 * 3 * NC : nullptr,
+                      ADAM ? a.v_sh + (uint64_t)t0 * 3 * NC : nullptr};
+#pragma unroll
+    for (int q = 0; q < NIO; ++q)
+        for (uint32_t i = threadIdx.x; i < nflt; i += 3 * kShTets) s_io[q][(i / NC) * SR + i % NC] = io_g[q][i];
+    __syncthreads();
     const bool live = t < a.M;
+    const int row = (int)(lt * 3 + ch) * SR;
     V3 v[4], g, cen, off;
     float c[NC], dsh[NC];
-    float* s = a.sh + ((uint64_t)t * 3 + ch) * NC;
     if (live) {
         tet_geom(a.rec, t, v, g, cen);
         off = sub3(v[0], cen);
         const float offset = dot3(g, off);
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
-            c[k] = s[k];
+            c[k] = s_io[0][row + k];
             dsh[k] = 0.0f;
         }
         float s_all = 0.0f;
@@ -253,26 +265,33 @@ __global__ __launch_bounds__(3 * kShTets) void k_sh_backward(ShBwdArgs a) {
             if (dvv.z != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 2, dvv.z);
         }
     }
-    if (!live) return;
     if (ADAM) {
-        if (*a.st.skip) return;
-        const float stepf = (float)*a.st.step;
-        const float b1p = expf(stepf * logf(a.st.beta1));
-        const float b2p = expf(stepf * logf(a.st.beta2));
-        float* mm = a.m_sh + ((uint64_t)t * 3 + ch) * NC;
-        float* vv2 = a.v_sh + ((uint64_t)t * 3 + ch) * NC;
+        if (*a.st.skip) return;  // uniform: no barrier follows on this path
+        if (live) {
+            const float stepf = (float)*a.st.step;
+            const float b1p = expf(stepf * logf(a.st.beta1));
+            const float b2p = expf(stepf * logf(a.st.beta2));
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            float p = c[k], m = mm[k], v2 = vv2[k];
-            adam_update(p, dsh[k], m, v2, a.lr, a.st.beta1, a.st.beta2, a.st.eps, b1p, b2p);
-            s[k] = p;
-            mm[k] = m;
-            vv2[k] = v2;
+            for (int k = 0; k < NC; ++k) {
+                float p = c[k], m = s_io[NIO > 1 ? 1 : 0][row + k], v2 = s_io[NIO > 2 ? 2 : 0][row + k];
+                adam_update(p, dsh[k], m, v2, a.lr, a.st.beta1, a.st.beta2, a.st.eps, b1p, b2p);
+                s_io[0][row + k] = p;
+                s_io[NIO > 1 ? 1 : 0][row + k] = m;
+                s_io[NIO > 2 ? 2 : 0][row + k] = v2;
+            }
         }
-    } else {
-        float* o = a.d_sh + ((uint64_t)t * 3 + ch) * NC;
+        __syncthreads();
 #pragma unroll
-        for (int k = 0; k < NC; ++k) o[k] = dsh[k];
+        for (int q = 0; q < NIO; ++q)
+            for (uint32_t i = threadIdx.x; i < nflt; i += 3 * kShTets) io_g[q][i] = s_io[q][(i / NC) * SR + i % NC];
+    } else {
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) s_io[0][row + k] = dsh[k];
+        }
+        __syncthreads();
+        float* o = a.d_sh + (uint64_t)t0 * 3 * NC;
+        for (uint32_t i = threadIdx.x; i < nflt; i += 3 * kShTets) o[i] = s_io[0][(i / NC) * SR + i % NC];
     }
 }
 
