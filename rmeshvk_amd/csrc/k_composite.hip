@@ -50,21 +50,35 @@ __device__ __forceinline__ TileCtx tile_ctx(const ViewParams& vw, uint32_t gtile
     return c;
 }
 
+// Edges of a tet in the order of rasterize_compute.wgsl:176-231.
+constexpr int kEdgeI[6] = {0, 0, 0, 1, 1, 2};
+constexpr int kEdgeJ[6] = {1, 2, 3, 2, 3, 3};
+
+// Reciprocal y extents of the six projected edges (one v_rcp_f32 each).
+__device__ __forceinline__ void edge_rcp(const float* py_, float* ry) {
+#pragma unroll
+    for (int e = 0; e < 6; ++e) ry[e] = __builtin_amdgcn_rcpf(py_[kEdgeJ[e]] - py_[kEdgeI[e]]);
+}
+
 // Row span of the tet in tile-local pixels (rasterize_compute.wgsl:170-275;
 // FORWARD adds the vertex-in-row rule, the adjoint omits it:
 // backward_tiled_compute.wgsl:242-299).  Packed as (xl & 0xffff) | (xr << 16).
+// The edge parameter (yc - yi) / (yj - yi) is formed with the edge's 1-ulp
+// reciprocal ry: the span is widened by 0.001 px on both sides, so the two
+// roundings can only disagree about a pixel centre lying within ~1e-6 px of
+// the widened boundary, 0.001 px outside the silhouette, whose ray misses the
+// tet (the pixel is tested, not composited, by virtue of the span).
 template <bool FORWARD>
-__device__ __forceinline__ uint32_t row_span(const float* px_, const float* py_, uint32_t row, int32_t TS) {
+__device__ __forceinline__ uint32_t row_span(const float* px_, const float* py_, const float* ry, uint32_t row,
+                                             int32_t TS) {
     float xl_f = 1e10f, xr_f = -1e10f;
     float yc = (float)row + 0.5f;
-    const int EI[6] = {0, 0, 0, 1, 1, 2};
-    const int EJ[6] = {1, 2, 3, 2, 3, 3};
 #pragma unroll
     for (int e = 0; e < 6; ++e) {
-        float yi = py_[EI[e]], yj = py_[EJ[e]];
+        float yi = py_[kEdgeI[e]], yj = py_[kEdgeJ[e]];
         if ((yi <= yc && yj > yc) || (yj <= yc && yi > yc)) {
-            float t = (yc - yi) / (yj - yi);
-            float x = px_[EI[e]] + t * (px_[EJ[e]] - px_[EI[e]]);
+            float t = (yc - yi) * ry[e];
+            float x = px_[kEdgeI[e]] + t * (px_[kEdgeJ[e]] - px_[kEdgeI[e]]);
             xl_f = fminf(xl_f, x);
             xr_f = fmaxf(xr_f, x);
         }
@@ -110,10 +124,9 @@ struct BatchLDS {
     uint32_t behind[B];
     uint32_t rows[B];       // tile rows the tet's projected y-range touches: r0 | r1 << 8 (r0 > r1: none)
     uint32_t span[B * SP];  // [tet][row] packed span
-    uint32_t rpref[B * SP]; // [tet][row] prefix of row widths ([tet][TS] = tet total), then record base - first column
-    uint32_t tpref[B + 1];  // exclusive prefix of covered pixels per tet (record slots)
-    uint32_t cpref[B + 1];  // exclusive prefix of work chunks per tet
-    uint32_t nbp;           // tets processed this batch (capacity cut)
+    uint32_t rpref[B * SP]; // [tet][row] record base of the row - its first column
+    uint32_t wsum[4];       // layout scan carry per wave
+    uint32_t nbp;           // chunk count << 8 | tets processed this batch (capacity cut)
     // adjoint only: per face (grad lambda_a, grad lambda_b, grad lambda_c, raw
     // normal, lambda_{a,b,c}(v0), pad) in face order, 4 faces x 16 floats padded
     // to 68 floats so that lanes reading different tets start 4 banks apart;
@@ -289,82 +302,126 @@ __device__ __forceinline__ void store_batch_staged(BatchLDS<TS, B, BWD>& L, cons
     }
 }
 
-// Spans of the batch, then (wave 0) per-tet row prefixes, the pixel and chunk
-// prefixes over tets and the capacity cut (records <= CAP).  Only the rows of
-// each tet's projected y-range are evaluated (rows outside it have no edge
-// crossing and no vertex: empty), four lanes per tet.
-template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int R, int NT>
-__device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb) {
-    constexpr int SP = TS + 1;
-    // four lanes per tet over the tet's row range (rows outside it are empty
-    // and never read)
-    for (int it = threadIdx.x; it < 4 * nb; it += NT) {
-        const int k = it >> 2;
-        const uint32_t rr = L.rows[k], r0 = rr & 0xFFu, r1 = rr >> 8;
-        if (r0 > r1) continue;
-        const float px_[4] = {L.pxy[0][k], L.pxy[1][k], L.pxy[2][k], L.pxy[3][k]};
-        const float py_[4] = {L.pxy[4][k], L.pxy[5][k], L.pxy[6][k], L.pxy[7][k]};
-        for (uint32_t row = r0 + (it & 3); row <= r1; row += 4)
-            L.span[k * SP + row] = row_span<FORWARD_RULE>(px_, py_, row, TS);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int k = threadIdx.x;
-        uint32_t tot = 0;
-        if (k < nb) {
-            const uint32_t rr = L.rows[k];
-            for (uint32_t r = rr & 0xFFu; r <= (rr >> 8); ++r) {
-                uint32_t sp = L.span[k * SP + r];
-                L.rpref[k * SP + r] = tot;
-                tot += (uint32_t)max(span_xr(sp) - span_xl(sp) + 1, 0);
-            }
-            L.rpref[k * SP + TS] = tot;
-        }
-        uint32_t inc = wave_incl_scan(tot);
-        uint32_t cinc = wave_incl_scan((tot + R - 1) / R);
-        if (k < nb) {
-            L.tpref[k + 1] = inc;
-            L.cpref[k + 1] = cinc;
-        }
-        if (k == 0) {
-            L.tpref[0] = 0;
-            L.cpref[0] = 0;
-        }
-        uint64_t fits = __ballot(k < nb && inc <= (uint32_t)CAP);
-        if (k == 0) L.nbp = max((uint32_t)__popcll(fits), 1u);
-    }
-    __syncthreads();
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, ROWS == 0xF);
 }
 
-// Chunk table of a batch.  Chunk j covers up to R consecutive covered pixels
-// of one tet (in row, col order); its entry packs
+// Inclusive sum over the wave: row_shr 1/2/4/8 within 16-lane rows, then
+// row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3).
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
+    v += dpp_u<0x111, 0xF>(v);
+    v += dpp_u<0x112, 0xF>(v);
+    v += dpp_u<0x114, 0xF>(v);
+    v += dpp_u<0x118, 0xF>(v);
+    v += dpp_u<0x142, 0xA>(v);
+    v += dpp_u<0x143, 0xC>(v);
+    return v;
+}
+
+// Value of the last lane of each aligned group of G lanes (quad_perm).
+template <int G>
+__device__ __forceinline__ uint32_t group_last(uint32_t v) {
+    if constexpr (G == 4) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xFF, 0xF, 0xF, false);
+    else if constexpr (G == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xF5, 0xF, 0xF, false);
+    else return v;
+}
+
+// Chunk table entry: up to R consecutive covered pixels of one tet (in row,
+// col order), packed as
 //   tet k (6 bits) | row (5) | first column (5) | first record slot (11) | count (5).
-// Built in parallel over (tet, row): each row emits the chunks that start in it.
-// The row's prefix entry is then replaced by its record base minus its first
-// column, so the pixel-sequential phase finds a pair's record as base + column.
 constexpr int chunk_capacity(int CAP, int R, int B) { return CAP / R + B; }
 
-template <int TS, int B, bool BWD, int R, int NT>
-__device__ __forceinline__ void build_chunks(BatchLDS<TS, B, BWD>& L, uint32_t nbp, uint32_t* __restrict__ tab) {
+// Layout of a batch in one pass: spans, record slots, chunk table and the
+// capacity cut (records <= CAP).  LPT lanes per tet, each over a contiguous
+// block of the rows of the tet's projected y-range (rows outside it are empty
+// and never read).  One wave scan of the lanes' pixel counts, packed with the
+// per-tet chunk counts in the high half-word and carried across waves through
+// L.wsum, gives every lane its record base and its tet's chunk base.  Leaves
+//   span[tet][row], rpref[tet][row] = record base of the row - its first column,
+//   tab[] = chunk table, nbp = chunk count << 8 | tets kept,
+// published by the caller's barrier.  Contains one barrier; every thread of
+// the block calls it.
+template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int R, int NT>
+__device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, uint32_t* __restrict__ tab) {
     constexpr int SP = TS + 1;
-    for (int it = threadIdx.x; it < 4 * (int)nbp; it += NT) {
-        const int k = it >> 2;
-        const uint32_t rr = L.rows[k];
-        const uint32_t tot = L.rpref[k * SP + TS];
-        for (uint32_t r = (rr & 0xFFu) + (it & 3); r <= (rr >> 8); r += 4) {
-            const uint32_t sp = L.span[k * SP + r];
-            const int xl = span_xl(sp), w = span_xr(sp) - xl + 1;
-            if (w <= 0) continue;
-            const uint32_t base = L.rpref[k * SP + r];
-            const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
-            for (uint32_t c = c0; c <= c1; ++c) {
-                const uint32_t l = c * R;
-                tab[L.cpref[k] + c] = (uint32_t)k | (r << 6) | ((uint32_t)(xl + (int)(l - base)) << 11) |
-                                      ((L.tpref[k] + l) << 16) | (min((uint32_t)R, tot - l) << 27);
+    constexpr int LPT = NT >= 4 * B ? 4 : NT >= 2 * B ? 2 : 1;
+    constexpr int QMAX = (TS + LPT - 1) / LPT;
+    static_assert(LPT * B <= NT && B <= 64, "one pass over the batch");
+    static_assert(B * TS * TS < 65536, "packed scan fields");
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int k = t / LPT, j = t % LPT;
+    const bool valid = k < nb;
+    uint32_t sp[QMAX];
+    int ra = 0, nr = 0;  // this lane's first row and row count
+    uint32_t s = 0;      // covered pixels in those rows
+    if (valid) {
+        const uint32_t rr = L.rows[k], r0 = rr & 0xFFu, r1 = rr >> 8;
+        if (r0 <= r1) {
+            const int q = ((int)(r1 - r0) + LPT) / LPT;
+            ra = (int)r0 + j * q;
+            nr = min(q, (int)r1 + 1 - ra);
+        }
+        if (nr > 0) {
+            const float px_[4] = {L.pxy[0][k], L.pxy[1][k], L.pxy[2][k], L.pxy[3][k]};
+            const float py_[4] = {L.pxy[4][k], L.pxy[5][k], L.pxy[6][k], L.pxy[7][k]};
+            float ry[6];
+            edge_rcp(py_, ry);
+#pragma unroll
+            for (int i = 0; i < QMAX; ++i) {
+                if (i < nr) {
+                    sp[i] = row_span<FORWARD_RULE>(px_, py_, ry, (uint32_t)(ra + i), TS);
+                    L.span[k * SP + ra + i] = sp[i];
+                    s += (uint32_t)max(span_xr(sp[i]) - span_xl(sp[i]) + 1, 0);
+                }
             }
-            L.rpref[k * SP + r] = L.tpref[k] + base - (uint32_t)xl;  // read only by this thread above
         }
     }
+    if (t == 0) L.nbp = 0u;
+    // pixels of the tet before this lane, and the tet's total
+    uint32_t g = s;
+    if constexpr (LPT >= 2) {
+        const uint32_t u = dpp_u<0x111, 0xF>(g);  // row_shr:1
+        if (j >= 1) g += u;
+    }
+    if constexpr (LPT >= 4) {
+        const uint32_t u = dpp_u<0x112, 0xF>(g);  // row_shr:2
+        if (j >= 2) g += u;
+    }
+    const uint32_t tot = group_last<LPT>(g);
+    const uint32_t pl = g - s;
+    const uint32_t cc = (tot + R - 1) / R;
+    const bool last = j == LPT - 1;
+    uint32_t inc = wave_scan_dpp(s | (valid && last ? cc << 16 : 0u));
+    if (lane == 63) L.wsum[wid] = inc;
+    __syncthreads();
+    for (int w = 0; w < wid; ++w) inc += L.wsum[w];
+    const uint32_t texc = (inc & 0xFFFFu) - s - pl;               // records of the tets before k
+    const uint32_t cexc = (inc >> 16) - (valid && last ? cc : 0u);  // chunks of the tets before k
+    const bool kept = valid && texc + tot <= (uint32_t)CAP;
+    if (kept) {
+        uint32_t base = pl;  // tet-local record index of the row's first pixel
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i) {
+            if (i < nr) {
+                const uint32_t row = (uint32_t)(ra + i);
+                const int xl = span_xl(sp[i]), w = span_xr(sp[i]) - xl + 1;
+                if (w > 0) {
+                    const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
+                    for (uint32_t c = c0; c <= c1; ++c) {
+                        const uint32_t l = c * R;
+                        tab[cexc + c] = (uint32_t)k | (row << 6) | ((uint32_t)(xl + (int)(l - base)) << 11) |
+                                        ((texc + l) << 16) | (min((uint32_t)R, tot - l) << 27);
+                    }
+                    L.rpref[k * SP + row] = texc + base - (uint32_t)xl;
+                    base += (uint32_t)w;
+                }
+            }
+        }
+    }
+    // kept tets are a prefix of the batch: the highest one carries the totals
+    const uint64_t m = __ballot(kept && last);
+    if (m && lane == 63 - __clzll((long long)m)) atomicMax(&L.nbp, ((cexc + cc) << 8) | (uint32_t)(k + 1));
 }
 
 struct ChunkRef {
@@ -574,11 +631,10 @@ __global__ __launch_bounds__(C::NT) void k_forward(
         else if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
         if (tid < TS * TS) s_mask[tid] = 0ull;
         __syncthreads();
-        batch_layout<TS, B, false, true, CAP, R, NT>(L, nb);
-        const uint32_t nbp = L.nbp;
-        const uint32_t nchunks = L.cpref[nbp];
-        build_chunks<TS, B, false, R, NT>(L, nbp, s_chunk);
+        batch_layout<TS, B, false, true, CAP, R, NT>(L, nb, s_chunk);
         __syncthreads();
+        const uint32_t nbp = L.nbp & 0xFFu;
+        const uint32_t nchunks = L.nbp >> 8;
         // prefetch the next batch's tet records while this one is processed
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
@@ -981,11 +1037,10 @@ __global__ __launch_bounds__(C::NT) void k_backward(
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
         __syncthreads();
-        batch_layout<TS, B, true, false, CAP, R, NT>(L, nb);
-        const uint32_t nbp = L.nbp;
-        const uint32_t nchunks = L.cpref[nbp];
-        build_chunks<TS, B, true, R, NT>(L, nbp, s_chunk);
+        batch_layout<TS, B, true, false, CAP, R, NT>(L, nb, s_chunk);
         __syncthreads();
+        const uint32_t nbp = L.nbp & 0xFFu;
+        const uint32_t nchunks = L.nbp >> 8;
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
         if (C::PF == 1 && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
