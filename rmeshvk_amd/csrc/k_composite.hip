@@ -33,6 +33,8 @@
 #include "rmesh_common.hpp"
 #include "rmesh_kernels.hpp"
 
+#include <type_traits>
+
 namespace rmesh {
 
 struct TileCtx {
@@ -62,7 +64,8 @@ __device__ __forceinline__ void edge_rcp(const float* py_, float* ry) {
 
 // Row span of the tet in tile-local pixels (rasterize_compute.wgsl:170-275;
 // FORWARD adds the vertex-in-row rule, the adjoint omits it:
-// backward_tiled_compute.wgsl:242-299).  Packed as (xl & 0xffff) | (xr << 16).
+// backward_tiled_compute.wgsl:242-299).  Packed in a byte as xl | xr << 4
+// (0x0F, xl > xr: empty).
 // The edge parameter (yc - yi) / (yj - yi) is formed with the edge's 1-ulp
 // reciprocal ry: the span is widened by 0.001 px on both sides, so the two
 // roundings can only disagree about a pixel centre lying within ~1e-6 px of
@@ -93,21 +96,17 @@ __device__ __forceinline__ uint32_t row_span(const float* px_, const float* py_,
             }
         }
     }
-    int32_t xl = 0, xr = -1;
     if (xl_f <= xr_f) {
         const float eps = 0.001f;
         int32_t l = max(sat_i32(ceilf(xl_f - 0.5f - eps)), 0);
         int32_t r = min(sat_i32(floorf(xr_f - 0.5f + eps)), TS - 1);
-        if (l <= r) {
-            xl = l;
-            xr = r;
-        }
+        if (l <= r) return (uint32_t)l | ((uint32_t)r << 4);
     }
-    return ((uint32_t)xl & 0xffffu) | ((uint32_t)xr << 16);
+    return 0x0Fu;
 }
 
-__device__ __forceinline__ int32_t span_xl(uint32_t sp) { return (int32_t)(int16_t)(sp & 0xffffu); }
-__device__ __forceinline__ int32_t span_xr(uint32_t sp) { return (int32_t)(int16_t)(sp >> 16); }
+__device__ __forceinline__ int32_t span_xl(uint32_t sp) { return (int32_t)(sp & 15u); }
+__device__ __forceinline__ int32_t span_xr(uint32_t sp) { return (int32_t)(sp >> 4); }
 
 // LDS image of one batch of tets.  Row arrays use an odd stride (TS + 1) so
 // the pixel-sequential phase (fixed tet, lanes on different rows) is free of
@@ -123,15 +122,15 @@ struct BatchLDS {
     float pxy[8][B];        // tile-local projected vertices (x0..x3, y0..y3)
     uint32_t behind[B];
     uint32_t rows[B];       // tile rows the tet's projected y-range touches: r0 | r1 << 8 (r0 > r1: none)
-    uint32_t span[B * SP];  // [tet][row] packed span
-    uint32_t rpref[B * SP]; // [tet][row] record base of the row - its first column
+    uint8_t span[B * SP];    // [tet][row] span: first column | last column << 4 (0x0F: empty)
+    uint16_t rpref[B * SP];  // [tet][row] record base of the row - its first column (mod 2^16)
     uint32_t wsum[4];       // layout scan carry per wave
     uint32_t nbp;           // chunk count << 8 | tets processed this batch (capacity cut)
-    // adjoint only: per face (grad lambda_a, grad lambda_b, grad lambda_c, raw
-    // normal, lambda_{a,b,c}(v0), pad) in face order, 4 faces x 16 floats padded
-    // to 68 floats so that lanes reading different tets start 4 banks apart;
-    // vertex ids; tet id
-    static constexpr int FS = 4 * 16 + 4;
+    // adjoint only: per face (grad lambda_a, lambda_a(v0), grad lambda_b,
+    // lambda_b(v0)) in face order, then per face the sign s of the stored
+    // inward normal n = s * raw normal; 36 floats per tet so that lanes reading
+    // different tets start on different banks; vertex ids; tet id
+    static constexpr int FS = 4 * 8 + 4;
     alignas(16) float face[BWD ? B * FS : 1];
     uint32_t vi[BWD ? 4 : 1][B];
     uint32_t tid[BWD ? B : 1];
@@ -201,20 +200,16 @@ __device__ __forceinline__ void store_face(BatchLDS<TS, B, BWD>& L, int k, int f
     if (BWD) {
         // barycentric weights of a point h of the face plane, affine in h:
         // lambda_a(h) = lambda_a(v0) + grad_a . (h - v0), grad_a = ((c - b) x raw) / |raw|^2
-        // (and cyclic: (a - c) for b, (b - a) for c)
+        // (and cyclic: (a - c) for b); the three gradients sum to zero, so
+        // lambda_c = 1 - lambda_a - lambda_b everywhere
         const float inv = 1.0f / dot3(raw, raw);
         const V3 ga = scl3(cross3(sub3(vc, vb), raw), inv);
         const V3 gb = scl3(cross3(sub3(va, vc), raw), inv);
-        const V3 gc = scl3(cross3(sub3(vb, va), raw), inv);
         const V3 pa = sub3(va, v0), pb = sub3(vb, v0), pc = sub3(vc, v0);
-        float* F = &L.face[k * L.FS + fi * 16];
-        F[0] = ga.x; F[1] = ga.y; F[2] = ga.z;
-        F[3] = gb.x; F[4] = gb.y; F[5] = gb.z;
-        F[6] = gc.x; F[7] = gc.y; F[8] = gc.z;
-        F[9] = raw.x; F[10] = raw.y; F[11] = raw.z;
-        F[12] = dot3(cross3(pc, pb), raw) * inv;  // lambda_a(v0)
-        F[13] = dot3(cross3(pa, pc), raw) * inv;  // lambda_b(v0)
-        F[14] = dot3(cross3(pb, pa), raw) * inv;  // lambda_c(v0)
+        float4* F = reinterpret_cast<float4*>(&L.face[k * L.FS + fi * 8]);
+        F[0] = make_float4(ga.x, ga.y, ga.z, dot3(cross3(pc, pb), raw) * inv);
+        F[1] = make_float4(gb.x, gb.y, gb.z, dot3(cross3(pa, pc), raw) * inv);
+        L.face[k * L.FS + 32 + fi] = nn.x == raw.x && nn.y == raw.y && nn.z == raw.z ? 1.0f : -1.0f;
     }
 }
 
@@ -302,6 +297,17 @@ __device__ __forceinline__ void store_batch_staged(BatchLDS<TS, B, BWD>& L, cons
     }
 }
 
+// Entry i < 128 of a wave's two-register id window (entries 0-63 in w0, 64-127 in w1).
+template <int B>
+__device__ __forceinline__ uint32_t window_id(uint32_t w0, uint32_t w1, int i) {
+    const uint32_t a = (uint32_t)__shfl((int)w0, i & 63, 64);
+    if constexpr (B > 32) {
+        const uint32_t b = (uint32_t)__shfl((int)w1, i & 63, 64);
+        return i < 64 ? a : b;
+    }
+    return a;
+}
+
 template <int CTRL, int ROWS>
 __device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, ROWS == 0xF);
@@ -329,8 +335,18 @@ __device__ __forceinline__ uint32_t group_last(uint32_t v) {
 
 // Chunk table entry: up to R consecutive covered pixels of one tet (in row,
 // col order), packed as
-//   tet k (6 bits) | row (5) | first column (5) | first record slot (11) | count (5).
+//   tet k (6 bits) | row (5) | first column (5) | first record slot (11) | count (5);
+// with R = 1 chunk j is record slot j and the entry keeps the low 16 bits.
 constexpr int chunk_capacity(int CAP, int R, int B) { return CAP / R + B; }
+template <int R>
+using chunk_t = typename std::conditional<R == 1, uint16_t, uint32_t>::type;
+template <int R>
+__device__ __forceinline__ chunk_t<R> chunk_entry(uint32_t k, uint32_t row, uint32_t col, uint32_t slot0,
+                                                  uint32_t cnt) {
+    const uint32_t lo = k | (row << 6) | (col << 11);
+    if constexpr (R == 1) return (uint16_t)lo;
+    else return lo | (slot0 << 16) | (cnt << 27);
+}
 
 // Layout of a batch in one pass: spans, record slots, chunk table and the
 // capacity cut (records <= CAP).  LPT lanes per tet, each over a contiguous
@@ -343,7 +359,7 @@ constexpr int chunk_capacity(int CAP, int R, int B) { return CAP / R + B; }
 // published by the caller's barrier.  Contains one barrier; every thread of
 // the block calls it.
 template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int R, int NT>
-__device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, uint32_t* __restrict__ tab) {
+__device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, chunk_t<R>* __restrict__ tab) {
     constexpr int SP = TS + 1;
     constexpr int LPT = NT >= 4 * B ? 4 : NT >= 2 * B ? 2 : 1;
     constexpr int QMAX = (TS + LPT - 1) / LPT;
@@ -371,7 +387,7 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ui
             for (int i = 0; i < QMAX; ++i) {
                 if (i < nr) {
                     sp[i] = row_span<FORWARD_RULE>(px_, py_, ry, (uint32_t)(ra + i), TS);
-                    L.span[k * SP + ra + i] = sp[i];
+                    L.span[k * SP + ra + i] = (uint8_t)sp[i];
                     s += (uint32_t)max(span_xr(sp[i]) - span_xl(sp[i]) + 1, 0);
                 }
             }
@@ -410,10 +426,10 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ui
                     const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
                     for (uint32_t c = c0; c <= c1; ++c) {
                         const uint32_t l = c * R;
-                        tab[cexc + c] = (uint32_t)k | (row << 6) | ((uint32_t)(xl + (int)(l - base)) << 11) |
-                                        ((texc + l) << 16) | (min((uint32_t)R, tot - l) << 27);
+                        tab[cexc + c] = chunk_entry<R>((uint32_t)k, row, (uint32_t)(xl + (int)(l - base)), texc + l,
+                                                       min((uint32_t)R, tot - l));
                     }
-                    L.rpref[k * SP + row] = texc + base - (uint32_t)xl;
+                    L.rpref[k * SP + row] = (uint16_t)(texc + base - (uint32_t)xl);
                     base += (uint32_t)w;
                 }
             }
@@ -428,8 +444,10 @@ struct ChunkRef {
     int k, row, col;
     uint32_t slot0, cnt;
 };
-__device__ __forceinline__ ChunkRef chunk_at(const uint32_t* tab, uint32_t j) {
+template <int R>
+__device__ __forceinline__ ChunkRef chunk_at(const chunk_t<R>* tab, uint32_t j) {
     const uint32_t e = tab[j];
+    if constexpr (R == 1) return ChunkRef{(int)(e & 63u), (int)((e >> 6) & 31u), (int)((e >> 11) & 31u), j, 1u};
     return ChunkRef{(int)(e & 63u), (int)((e >> 6) & 31u), (int)((e >> 11) & 31u), (e >> 16) & 2047u, e >> 27};
 }
 
@@ -555,7 +573,9 @@ __device__ __forceinline__ void integral(const TetRegs& T, V3 rd, float t_min, f
 // Batch geometry of the forward: B tets per LDS batch (<= 64, one mask bit
 // each), CAP (pixel, tet) records per batch, R pixels per lane in the pair
 // phase, PF = fetch the next batch's records while this batch runs.
-template <int TS_, int B_, int CAP_, int R_, int PF_>
+// WPE_: minimum waves per SIMD the register allocation must allow (0: the
+// compiler's choice); a cap on VGPRs that trades recomputation for occupancy.
+template <int TS_, int B_, int CAP_, int R_, int PF_, int WPE_ = 0>
 struct FwdCfg {
     static constexpr int TS = TS_;
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
@@ -563,10 +583,11 @@ struct FwdCfg {
     static constexpr int CAP = CAP_;
     static constexpr int R = R_;
     static constexpr int PF = PF_;  // next batch's records: 0 at batch start, 1 in registers, 2 LDS DMA
+    static constexpr int WPE = WPE_ > 0 ? WPE_ : 1;
 };
 
 template <class C, bool AUX>
-__global__ __launch_bounds__(C::NT) void k_forward(
+__global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_forward(
     const float4* __restrict__ rec, const float4* __restrict__ colors, uint64_t color_view_stride,
     const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges, const ViewParams* __restrict__ views,
     uint32_t tiles_per_view, float* __restrict__ rgba, float* __restrict__ aux) {
@@ -582,7 +603,7 @@ __global__ __launch_bounds__(C::NT) void k_forward(
     __shared__ float s_ray[3][TS * TS];
     __shared__ uint32_t s_live[TS * TS];  // pixel inside the image and not yet saturated (log T >= -13.8)
     __shared__ unsigned long long s_mask[TS * TS];  // covered tets of the batch per pixel
-    __shared__ uint32_t s_chunk[chunk_capacity(CAP, R, B)];
+    __shared__ chunk_t<R> s_chunk[chunk_capacity(CAP, R, B)];
     static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
 
     const uint32_t gtile = blockIdx.x;
@@ -612,7 +633,7 @@ __global__ __launch_bounds__(C::NT) void k_forward(
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
     TetFetch f;
-    static_assert(C::PF != 2 || B <= 32, "LDS-DMA staging: one wave, next batch inside a 64-entry index window");
+    static_assert(C::PF != 2 || B <= 64, "LDS-DMA staging: one wave, next batch inside a 128-entry id window");
     __shared__ Staging<C::PF == 2 ? B : 1> S;
     const int wid = tid / 64, lane = (int)__lane_id();
     if (C::PF == 1 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, col_v);
@@ -624,8 +645,13 @@ __global__ __launch_bounds__(C::NT) void k_forward(
         // staged records of this batch have landed (and none is in flight at an early exit)
         if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);
         if (__syncthreads_and(!active || log_t < -13.8f)) break;
-        uint32_t win = 0;  // wave 0: tet ids of this and the next batch (back to front of the sorted list)
-        if (C::PF == 2 && wid == 0) win = done + lane < count ? sorted_vals[re - 1 - done - lane] : 0u;
+        // wave 0: tet ids of this and the next batch (back to front of the sorted
+        // list), 64 per register
+        uint32_t win = 0, win1 = 0;
+        if (C::PF == 2 && wid == 0) {
+            win = done + lane < count ? sorted_vals[re - 1 - done - lane] : 0u;
+            if constexpr (B > 32) win1 = done + 64 + lane < count ? sorted_vals[re - 1 - done - 64 - lane] : 0u;
+        }
         if (C::PF == 0 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - done - tid], rec, nullptr, col_v);
         if constexpr (C::PF == 2) store_batch_staged<TS, B, false, NT>(L, S, nb, vw, tc);
         else if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
@@ -640,12 +666,12 @@ __global__ __launch_bounds__(C::NT) void k_forward(
         const int nb_next = (int)min<uint32_t>(B, count - next);
         if (C::PF == 1 && tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
         if (C::PF == 2 && wid == 0) {
-            const uint32_t tn = (uint32_t)__shfl((int)win, (int)nbp + lane, 64);
+            const uint32_t tn = window_id<B>(win, win1, (int)nbp + lane);
             stage_async<false>(S, tn, lane < nb_next, rec, nullptr, col_v);
         }
         // chunk-parallel: intersection + volume integral of R pixels of one tet
         for (uint32_t j = tid; j < nchunks; j += NT) {
-            const ChunkRef c = chunk_at(s_chunk, j);
+            const ChunkRef c = chunk_at<R>(s_chunk, j);
             const int k = c.k;
             const uint32_t cnt = c.cnt, slot0 = c.slot0;
             TetRegs T;
@@ -688,7 +714,7 @@ __global__ __launch_bounds__(C::NT) void k_forward(
                 const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
                 m &= m - 1ull;
                 if (log_t < -13.8f) break;
-                const uint32_t slot = L.rpref[k * SP + row] + (uint32_t)colp;
+                const uint32_t slot = (L.rpref[k * SP + row] + (uint32_t)colp) & 0xFFFFu;
                 const float od = s_od[slot];
                 const float Tj = fast_safe_exp(log_t);
                 if (Tj >= 1e-6f) {
@@ -749,7 +775,7 @@ constexpr int kTerms = 19;  // d_density, d_v[4][3], d_grad[3], d_colour[3]
 // Batch geometry of the adjoint (B <= 32, one mask bit each); SCAN = how a
 // wave sums each tet's terms before the LDS adds: 2 segmented scan on DPP,
 // 1 segmented scan on ds_bpermute, 0 none (every lane adds its own terms).
-template <int TS_, int B_, int CAP_, int R_, int PF_, int SCAN_>
+template <int TS_, int B_, int CAP_, int R_, int PF_, int SCAN_, int WPE_ = 0>
 struct BwdCfg {
     static constexpr int TS = TS_;
     static constexpr int NT = (TS * TS + 63) / 64 * 64;
@@ -758,6 +784,7 @@ struct BwdCfg {
     static constexpr int R = R_;
     static constexpr int PF = PF_;  // next batch's records: 0 at batch start, 1 in registers, 2 LDS DMA
     static constexpr int SCAN = SCAN_;
+    static constexpr int WPE = WPE_ > 0 ? WPE_ : 1;
 };
 
 // Segmented inclusive sum across the wave; lanes with equal `key` form
@@ -891,15 +918,17 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
         const int fi = side == 0 ? min_face : max_face;
         const float tt = side == 0 ? t_min : t_max;
         const float dt = side == 0 ? d_tmin : d_tmax;
-        const float4* F = reinterpret_cast<const float4*>(&L.face[k * L.FS + fi * 16]);
-        const float4 f0 = F[0], f1 = F[1], f2 = F[2], f3 = F[3];
-        const V3 ga = mk3(f0.x, f0.y, f0.z), gb = mk3(f0.w, f1.x, f1.y), gc = mk3(f1.z, f1.w, f2.x);
-        const V3 nn = mk3(f2.y, f2.z, f2.w);
+        const float4* F = reinterpret_cast<const float4*>(&L.face[k * L.FS + fi * 8]);
+        const float4 f0 = F[0], f1 = F[1];
+        const float4 nq = L.tr[k * L.TR + fi];
+        const V3 ga = mk3(f0.x, f0.y, f0.z), gb = mk3(f1.x, f1.y, f1.z);
+        const float sg = L.face[k * L.FS + 32 + fi];
+        const V3 nn = mk3(nq.x * sg, nq.y * sg, nq.z * sg);  // raw normal
         const float inv_den = fast_safe_div(1.0f, dot3(nn, rd));
         const V3 h = add3(cam_v0, scl3(rd, tt));  // hit - v0
-        const float la = f3.x + dot3(ga, h);
-        const float lb = f3.y + dot3(gb, h);
-        const float lc = f3.z + dot3(gc, h);
+        const float la = f0.w + dot3(ga, h);
+        const float lb = f1.w + dot3(gb, h);
+        const float lc = (1.0f - la) - lb;
         G[side] = scl3(nn, dt * inv_den);
         const int sa = face_slot(fi, 0), sb = face_slot(fi, 1), sc = face_slot(fi, 2);
 #pragma unroll
@@ -926,7 +955,7 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
 }
 
 template <class C>
-__global__ __launch_bounds__(C::NT) void k_backward(
+__global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_backward(
     const float4* __restrict__ rec, const uint32_t* __restrict__ idx, const float4* __restrict__ colors,
     uint64_t color_view_stride, const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges,
     const ViewParams* __restrict__ views, uint32_t tiles_per_view, const float* __restrict__ rgba,
@@ -939,12 +968,12 @@ __global__ __launch_bounds__(C::NT) void k_backward(
     // (dc . cp, the pixel's image gradient dotted with the pair's premultiplied
     // colour, is all the d(log T) chain needs of the colour)
     __shared__ float s_r[4][CAP];
-    __shared__ uint32_t s_meta[CAP];   // fmin | fmax << 2 | valid << 4 | did << 5
+    __shared__ uint8_t s_meta[CAP];    // fmin | fmax << 2 | valid << 4 | did << 5
     __shared__ float s_ray[3][TS * TS];
     __shared__ float s_dc[3][TS * TS];
     __shared__ float s_acc[B][kTerms];
     __shared__ unsigned int s_mask[TS * TS];  // covered tets of the batch per pixel (B <= 32)
-    __shared__ uint32_t s_chunk[chunk_capacity(CAP, R, B)];
+    __shared__ chunk_t<R> s_chunk[chunk_capacity(CAP, R, B)];
     static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
     __shared__ float s_loss[NW];
     static_assert(B <= 32, "pixel masks hold 32 tets");
@@ -1019,7 +1048,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
     TetFetch f;
-    static_assert(C::PF != 2 || B <= 32, "LDS-DMA staging: one wave, next batch inside a 64-entry index window");
+    static_assert(C::PF != 2 || B <= 64, "LDS-DMA staging: one wave, next batch inside a 128-entry id window");
     __shared__ Staging<C::PF == 2 ? B : 1> S;
     if (C::PF == 1 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
     if (C::PF == 2 && wid == 0) {
@@ -1029,8 +1058,11 @@ __global__ __launch_bounds__(C::NT) void k_backward(
     while (done < count) {
         if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // staged records of this batch have landed
         __syncthreads();
-        uint32_t win = 0;  // wave 0: tet ids of this and the next batch
-        if (C::PF == 2 && wid == 0) win = done + lane < count ? sorted_vals[rs + done + lane] : 0u;
+        uint32_t win = 0, win1 = 0;  // wave 0: tet ids of this and the next batch, 64 per register
+        if (C::PF == 2 && wid == 0) {
+            win = done + lane < count ? sorted_vals[rs + done + lane] : 0u;
+            if constexpr (B > 32) win1 = done + 64 + lane < count ? sorted_vals[rs + done + 64 + lane] : 0u;
+        }
         if (C::PF == 0 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + done + tid], rec, idx, col_v);
         if constexpr (C::PF == 2) store_batch_staged<TS, B, true, NT>(L, S, nb, vw, tc);
         else if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
@@ -1045,12 +1077,12 @@ __global__ __launch_bounds__(C::NT) void k_backward(
         const int nb_next = (int)min<uint32_t>(B, count - next);
         if (C::PF == 1 && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
         if (C::PF == 2 && wid == 0) {
-            const uint32_t tn = (uint32_t)__shfl((int)win, (int)nbp + lane, 64);
+            const uint32_t tn = window_id<B>(win, win1, (int)nbp + lane);
             stage_async<true>(S, tn, lane < nb_next, rec, idx, col_v);
         }
         // phase A (chunk-parallel): intersection + integral
         for (uint32_t j = tid; j < nchunks; j += NT) {
-            const ChunkRef c = chunk_at(s_chunk, j);
+            const ChunkRef c = chunk_at<R>(s_chunk, j);
             const int k = c.k;
             const uint32_t cnt = c.cnt, slot0 = c.slot0;
             TetRegs T;
@@ -1079,7 +1111,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
                         atomicOr(&s_mask[p], 1u << k);
                     }
                 }
-                s_meta[slot] = meta;
+                s_meta[slot] = (uint8_t)meta;
                 w.next(L, k);
             }
         }
@@ -1091,7 +1123,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
             while (m) {
                 const uint32_t k = (uint32_t)__ffs((int)m) - 1u;
                 m &= m - 1u;
-                const uint32_t slot = L.rpref[k * SP + row] + (uint32_t)colp;
+                const uint32_t slot = (L.rpref[k * SP + row] + (uint32_t)colp) & 0xFFFFu;
                 const float od = s_r[0][slot];
                 const float prev_log_t = fminf(log_t + od, 0.0f);
                 const float t_prev = fast_safe_exp(prev_log_t);
@@ -1100,7 +1132,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
                 const float dcp = s_r[1][slot];
                 s_r[0][slot] = t_prev;
                 s_r[1][slot] = d_log_t;
-                s_meta[slot] |= 32u;
+                s_meta[slot] = (uint8_t)(s_meta[slot] | 32u);
                 d_log_t = safe_clip(d_log_t + dcp * t_prev, kMinVal, kMaxVal);
             }
         }
@@ -1114,7 +1146,7 @@ __global__ __launch_bounds__(C::NT) void k_backward(
             for (int q = 0; q < kTerms; ++q) acc[q] = 0.0f;
             int key = 0x7fffffff;
             if (j < nchunks) {
-                const ChunkRef c = chunk_at(s_chunk, j);
+                const ChunkRef c = chunk_at<R>(s_chunk, j);
                 const int k = c.k;
                 const uint32_t cnt = c.cnt, slot0 = c.slot0;
                 key = k;
@@ -1199,9 +1231,14 @@ static void fwd_launch(const FwdArgs& a, hipStream_t s) {
 template <int TS>
 static bool fwd_variant(const FwdArgs& a, int v, hipStream_t s) {
     switch (v) {
-        case 0: fwd_launch<FwdCfg<TS, 32, 768, 1, 2>>(a, s); return true;
-        case 1: fwd_launch<FwdCfg<TS, 48, 768, 1, 1>>(a, s); return true;
-        case 2: fwd_launch<FwdCfg<TS, 32, 640, 1, 2>>(a, s); return true;
+        case 0: fwd_launch<FwdCfg<TS, 48, 768, 1, 1>>(a, s); return true;
+        case 1: fwd_launch<FwdCfg<TS, 48, 768, 2, 1>>(a, s); return true;
+        case 2: fwd_launch<FwdCfg<TS, 32, 704, 1, 2, 5>>(a, s); return true;
+        case 3: fwd_launch<FwdCfg<TS, 40, 640, 1, 2, 5>>(a, s); return true;
+        case 4: fwd_launch<FwdCfg<TS, 56, 896, 1, 1>>(a, s); return true;
+        case 5: fwd_launch<FwdCfg<TS, 32, 768, 1, 2, 5>>(a, s); return true;
+        case 6: fwd_launch<FwdCfg<TS, 48, 704, 1, 2, 5>>(a, s); return true;
+        case 7: fwd_launch<FwdCfg<TS, 64, 1024, 1, 1>>(a, s); return true;
         default: return false;
     }
 }
@@ -1226,8 +1263,11 @@ template <int TS>
 static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
     switch (v) {
         case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2>>(a, s); return true;
-        case 1: bwd_launch<BwdCfg<TS, 32, 512, 3, 0, 2>>(a, s); return true;
-        case 2: bwd_launch<BwdCfg<TS, 32, 576, 3, 2, 2>>(a, s); return true;
+        case 1: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2, 4>>(a, s); return true;
+        case 2: bwd_launch<BwdCfg<TS, 32, 512, 2, 2, 2, 4>>(a, s); return true;
+        case 3: bwd_launch<BwdCfg<TS, 32, 576, 3, 2, 2, 4>>(a, s); return true;
+        case 4: bwd_launch<BwdCfg<TS, 32, 640, 3, 2, 2, 4>>(a, s); return true;
+        case 5: bwd_launch<BwdCfg<TS, 32, 448, 3, 2, 2, 4>>(a, s); return true;
         default: return false;
     }
 }

@@ -43,8 +43,8 @@ struct FwdArgs {
     float* aux;  // nullable: [V][H][W][8]
 };
 // Batch-geometry variants of the compositing kernels (rmesh_autotune picks one).
-constexpr int kFwdVariants = 3;
-constexpr int kBwdVariants = 3;
+constexpr int kFwdVariants = 8;
+constexpr int kBwdVariants = 6;
 bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
 
 struct BwdArgs {
