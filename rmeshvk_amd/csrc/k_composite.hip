@@ -335,8 +335,10 @@ __device__ __forceinline__ uint32_t group_last(uint32_t v) {
 
 // Chunk table entry: up to R consecutive covered pixels of one tet (in row,
 // col order), packed as
-//   tet k (6 bits) | row (5) | first column (5) | first record slot (11) | count (5);
-// with R = 1 chunk j is record slot j and the entry keeps the low 16 bits.
+//   tet k (6 bits) | row (5) | first column (5) | first record slot (11) | count (5).
+// With R = 1 the table holds one 16-bit entry (tet, row, first column) per
+// row segment, at the segment's first record slot, and a bit mask marks those
+// slots: record j belongs to the segment starting at the last marked slot <= j.
 constexpr int chunk_capacity(int CAP, int R, int B) { return CAP / R + B; }
 template <int R>
 using chunk_t = typename std::conditional<R == 1, uint16_t, uint32_t>::type;
@@ -359,7 +361,8 @@ __device__ __forceinline__ chunk_t<R> chunk_entry(uint32_t k, uint32_t row, uint
 // published by the caller's barrier.  Contains one barrier; every thread of
 // the block calls it.
 template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int R, int NT>
-__device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, chunk_t<R>* __restrict__ tab) {
+__device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, chunk_t<R>* __restrict__ tab,
+                                             uint32_t* __restrict__ starts) {
     constexpr int SP = TS + 1;
     constexpr int LPT = NT >= 4 * B ? 4 : NT >= 2 * B ? 2 : 1;
     constexpr int QMAX = (TS + LPT - 1) / LPT;
@@ -423,11 +426,17 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ch
                 const uint32_t row = (uint32_t)(ra + i);
                 const int xl = span_xl(sp[i]), w = span_xr(sp[i]) - xl + 1;
                 if (w > 0) {
-                    const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
-                    for (uint32_t c = c0; c <= c1; ++c) {
-                        const uint32_t l = c * R;
-                        tab[cexc + c] = chunk_entry<R>((uint32_t)k, row, (uint32_t)(xl + (int)(l - base)), texc + l,
-                                                       min((uint32_t)R, tot - l));
+                    if constexpr (R == 1) {
+                        const uint32_t b = texc + base;  // the row segment's first record slot
+                        tab[b] = chunk_entry<1>((uint32_t)k, row, (uint32_t)xl, 0u, 0u);
+                        atomicOr(&starts[b >> 5], 1u << (b & 31u));
+                    } else {
+                        const uint32_t c0 = (base + R - 1) / R, c1 = (base + (uint32_t)w - 1) / R;
+                        for (uint32_t c = c0; c <= c1; ++c) {
+                            const uint32_t l = c * R;
+                            tab[cexc + c] = chunk_entry<R>((uint32_t)k, row, (uint32_t)(xl + (int)(l - base)),
+                                                           texc + l, min((uint32_t)R, tot - l));
+                        }
                     }
                     L.rpref[k * SP + row] = (uint16_t)(texc + base - (uint32_t)xl);
                     base += (uint32_t)w;
@@ -444,6 +453,16 @@ struct ChunkRef {
     int k, row, col;
     uint32_t slot0, cnt;
 };
+// R = 1: record slot j.  A row segment spans at most 16 slots, so its start
+// lies in j's mask word or the one before.
+__device__ __forceinline__ ChunkRef record_at(const uint16_t* seg, const uint32_t* starts, uint32_t j) {
+    const uint32_t wd = j >> 5;
+    const uint32_t m = starts[wd] & ((2u << (j & 31u)) - 1u);
+    const uint32_t s0 = m ? (wd << 5) + 31u - (uint32_t)__clz((int)m)
+                          : ((wd - 1u) << 5) + 31u - (uint32_t)__clz((int)starts[wd - 1u]);
+    const uint32_t e = seg[s0];
+    return ChunkRef{(int)(e & 63u), (int)((e >> 6) & 31u), (int)((e >> 11) & 31u) + (int)(j - s0), j, 1u};
+}
 template <int R>
 __device__ __forceinline__ ChunkRef chunk_at(const chunk_t<R>* tab, uint32_t j) {
     const uint32_t e = tab[j];
@@ -604,7 +623,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     __shared__ uint32_t s_live[TS * TS];  // pixel inside the image and not yet saturated (log T >= -13.8)
     __shared__ unsigned long long s_mask[TS * TS];  // covered tets of the batch per pixel
     __shared__ chunk_t<R> s_chunk[chunk_capacity(CAP, R, B)];
+    __shared__ uint32_t s_start[R == 1 ? CAP / 32 : 1];  // R = 1: record slots that start a row segment
     static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
+    static_assert(R != 1 || (CAP % 32 == 0 && CAP / 32 <= NT), "segment start mask");
 
     const uint32_t gtile = blockIdx.x;
     const uint32_t view = gtile / tiles_per_view;
@@ -656,8 +677,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         if constexpr (C::PF == 2) store_batch_staged<TS, B, false, NT>(L, S, nb, vw, tc);
         else if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
         if (tid < TS * TS) s_mask[tid] = 0ull;
+        if (R == 1 && tid < CAP / 32) s_start[tid] = 0u;
         __syncthreads();
-        batch_layout<TS, B, false, true, CAP, R, NT>(L, nb, s_chunk);
+        batch_layout<TS, B, false, true, CAP, R, NT>(L, nb, s_chunk, s_start);
         __syncthreads();
         const uint32_t nbp = L.nbp & 0xFFu;
         const uint32_t nchunks = L.nbp >> 8;
@@ -669,18 +691,16 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
             const uint32_t tn = window_id<B>(win, win1, (int)nbp + lane);
             stage_async<false>(S, tn, lane < nb_next, rec, nullptr, col_v);
         }
-        // chunk-parallel: intersection + volume integral of R pixels of one tet
+        // record-parallel: intersection + volume integral of one (pixel, tet)
+        // record per lane (R = 1), or of a chunk of R pixels of one tet
         for (uint32_t j = tid; j < nchunks; j += NT) {
-            const ChunkRef c = chunk_at<R>(s_chunk, j);
+            ChunkRef c;
+            if constexpr (R == 1) c = record_at(s_chunk, s_start, j);
+            else c = chunk_at<R>(s_chunk, j);
             const int k = c.k;
-            const uint32_t cnt = c.cnt, slot0 = c.slot0;
             TetRegs T;
             tet_regs(L, k, T);
-            PixWalk<TS> w;
-            w.init(L, c);
-            for (uint32_t q = 0; q < cnt; ++q) {
-                const uint32_t slot = slot0 + q;
-                const int p = w.row * TS + w.col;
+            auto one = [&](uint32_t slot, int p) {
                 // saturated pixels skip the rest of the tile's tets (their compositing loop would)
                 if (s_live[p]) {
                     V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
@@ -702,7 +722,16 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                         atomicOr(&s_mask[p], 1ull << k);
                     }
                 }
-                w.next(L, k);
+            };
+            if constexpr (R == 1) {
+                one(j, c.row * TS + c.col);
+            } else {
+                PixWalk<TS> w;
+                w.init(L, c);
+                for (uint32_t q = 0; q < c.cnt; ++q) {
+                    one(c.slot0 + q, w.row * TS + w.col);
+                    w.next(L, k);
+                }
             }
         }
         __syncthreads();
@@ -977,6 +1006,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
     __shared__ float s_loss[NW];
     static_assert(B <= 32, "pixel masks hold 32 tets");
+    static_assert(R > 1, "adjoint chunks hold several records");
     static_assert(CAP >= TS * TS, "one tet's pixels of a tile must fit a batch");
 
     const uint32_t gtile = blockIdx.x;
@@ -1069,7 +1099,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
         __syncthreads();
-        batch_layout<TS, B, true, false, CAP, R, NT>(L, nb, s_chunk);
+        batch_layout<TS, B, true, false, CAP, R, NT>(L, nb, s_chunk, nullptr);
         __syncthreads();
         const uint32_t nbp = L.nbp & 0xFFu;
         const uint32_t nchunks = L.nbp >> 8;
