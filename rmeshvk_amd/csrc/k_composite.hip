@@ -1006,7 +1006,8 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
     __shared__ float s_loss[NW];
     static_assert(B <= 32, "pixel masks hold 32 tets");
-    static_assert(R > 1, "adjoint chunks hold several records");
+    __shared__ uint32_t s_start[R == 1 ? CAP / 32 : 1];  // R = 1: record slots that start a row segment
+    static_assert(R != 1 || (CAP % 32 == 0 && CAP / 32 <= NT), "segment start mask");
     static_assert(CAP >= TS * TS, "one tet's pixels of a tile must fit a batch");
 
     const uint32_t gtile = blockIdx.x;
@@ -1098,8 +1099,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         else if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
+        if (R == 1 && tid < CAP / 32) s_start[tid] = 0u;
         __syncthreads();
-        batch_layout<TS, B, true, false, CAP, R, NT>(L, nb, s_chunk, nullptr);
+        batch_layout<TS, B, true, false, CAP, R, NT>(L, nb, s_chunk, s_start);
         __syncthreads();
         const uint32_t nbp = L.nbp & 0xFFu;
         const uint32_t nchunks = L.nbp >> 8;
@@ -1110,21 +1112,19 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
             const uint32_t tn = window_id<B>(win, win1, (int)nbp + lane);
             stage_async<true>(S, tn, lane < nb_next, rec, idx, col_v);
         }
-        // phase A (chunk-parallel): intersection + integral
+        // phase A (record- or chunk-parallel): intersection + integral
         for (uint32_t j = tid; j < nchunks; j += NT) {
-            const ChunkRef c = chunk_at<R>(s_chunk, j);
+            ChunkRef c;
+            if constexpr (R == 1) c = record_at(s_chunk, s_start, j);
+            else c = chunk_at<R>(s_chunk, j);
             const int k = c.k;
-            const uint32_t cnt = c.cnt, slot0 = c.slot0;
             TetRegs T;
             tet_regs(L, k, T);
-            PixWalk<TS> w;
-            w.init(L, c);
-            for (uint32_t q = 0; q < cnt; ++q) {
-                const uint32_t slot = slot0 + q;
-                const uint32_t gx = tc.tile_x * TS + w.col, gy = tc.tile_y * TS + w.row;
+            auto one = [&](uint32_t slot, int prow, int pcol) {
+                const uint32_t gx = tc.tile_x * TS + pcol, gy = tc.tile_y * TS + prow;
                 uint32_t meta = 0;
                 if (gx < vw.width && gy < vw.height) {
-                    const int p = w.row * TS + w.col;
+                    const int p = prow * TS + pcol;
                     V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
                     float t_min, t_max;
                     int fmin, fmax;
@@ -1142,7 +1142,16 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                     }
                 }
                 s_meta[slot] = (uint8_t)meta;
-                w.next(L, k);
+            };
+            if constexpr (R == 1) {
+                one(j, c.row, c.col);
+            } else {
+                PixWalk<TS> w;
+                w.init(L, c);
+                for (uint32_t q = 0; q < c.cnt; ++q) {
+                    one(c.slot0 + q, w.row, w.col);
+                    w.next(L, k);
+                }
             }
         }
         __syncthreads();
@@ -1176,27 +1185,33 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
             for (int q = 0; q < kTerms; ++q) acc[q] = 0.0f;
             int key = 0x7fffffff;
             if (j < nchunks) {
-                const ChunkRef c = chunk_at<R>(s_chunk, j);
+                ChunkRef c;
+                if constexpr (R == 1) c = record_at(s_chunk, s_start, j);
+                else c = chunk_at<R>(s_chunk, j);
                 const int k = c.k;
-                const uint32_t cnt = c.cnt, slot0 = c.slot0;
                 key = k;
                 TetRegs T;
                 tet_regs(L, k, T);
                 const float4 v0q = L.tr[k * L.TR + 6];
                 const V3 v0 = mk3(v0q.x, v0q.y, v0q.z);
-                PixWalk<TS> w;
-                w.init(L, c);
-                for (uint32_t q = 0; q < cnt; ++q) {
-                    const uint32_t slot = slot0 + q;
+                auto one = [&](uint32_t slot, int p) {
                     const uint32_t meta = s_meta[slot];
                     if (meta & 32u) {
-                        const int p = w.row * TS + w.col;
                         const V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
                         pair_grad<TS, B>(L, k, T, v0, cam, rd, s_dc[0][p], s_dc[1][p], s_dc[2][p], s_r[0][slot],
                                          s_r[1][slot], s_r[2][slot], s_r[3][slot], (int)(meta & 3u),
                                          (int)((meta >> 2) & 3u), acc);
                     }
-                    w.next(L, k);
+                };
+                if constexpr (R == 1) {
+                    one(j, c.row * TS + c.col);
+                } else {
+                    PixWalk<TS> w;
+                    w.init(L, c);
+                    for (uint32_t q = 0; q < c.cnt; ++q) {
+                        one(c.slot0 + q, w.row * TS + w.col);
+                        w.next(L, k);
+                    }
                 }
             }
             if (C::SCAN == 2) {
@@ -1292,12 +1307,12 @@ static void bwd_launch(const BwdArgs& a, hipStream_t s) {
 template <int TS>
 static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
     switch (v) {
-        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2>>(a, s); return true;
-        case 1: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2, 4>>(a, s); return true;
-        case 2: bwd_launch<BwdCfg<TS, 32, 512, 2, 2, 2, 4>>(a, s); return true;
-        case 3: bwd_launch<BwdCfg<TS, 32, 576, 3, 2, 2, 4>>(a, s); return true;
-        case 4: bwd_launch<BwdCfg<TS, 32, 640, 3, 2, 2, 4>>(a, s); return true;
-        case 5: bwd_launch<BwdCfg<TS, 32, 448, 3, 2, 2, 4>>(a, s); return true;
+        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2, 4>>(a, s); return true;
+        case 1: bwd_launch<BwdCfg<TS, 32, 512, 2, 2, 2, 4>>(a, s); return true;
+        case 2: bwd_launch<BwdCfg<TS, 32, 576, 3, 2, 2, 4>>(a, s); return true;
+        case 3: bwd_launch<BwdCfg<TS, 32, 512, 1, 2, 2, 4>>(a, s); return true;
+        case 4: bwd_launch<BwdCfg<TS, 32, 576, 1, 2, 2, 4>>(a, s); return true;
+        case 5: bwd_launch<BwdCfg<TS, 32, 640, 1, 2, 2, 4>>(a, s); return true;
         default: return false;
     }
 }
