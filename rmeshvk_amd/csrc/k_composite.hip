@@ -885,6 +885,34 @@ __device__ __forceinline__ void seg_scan_dpp(float* v, const SegMasksDpp& s) {
     for (int q = 0; q < N; ++q) v[q] = fmaf(dpp_f<0x143, 0xC>(v[q]), s.m[5], v[q]);
 }
 
+// The same scan of 19 sums with the shift folded into the accumulate: one
+// v_fmac_f32_dpp (v += dpp(v) * m) per sum and level.  Each level is one asm
+// block opened by s_nop 1, the two wait states a DPP read needs after a VALU
+// write of its source; inside a level every register is written once, and
+// its next DPP read is 18 instructions later.
+#define RMESH_FMAC_DPP(i, CTRL) "v_fmac_f32_dpp %" #i ", %" #i ", %19 " CTRL "\n"
+#define RMESH_SCAN_LEVEL(M, CTRL)                                                                              \
+    asm volatile("s_nop 1\n" RMESH_FMAC_DPP(0, CTRL) RMESH_FMAC_DPP(1, CTRL) RMESH_FMAC_DPP(2, CTRL)            \
+                     RMESH_FMAC_DPP(3, CTRL) RMESH_FMAC_DPP(4, CTRL) RMESH_FMAC_DPP(5, CTRL) RMESH_FMAC_DPP(6, CTRL) \
+                         RMESH_FMAC_DPP(7, CTRL) RMESH_FMAC_DPP(8, CTRL) RMESH_FMAC_DPP(9, CTRL)                    \
+                             RMESH_FMAC_DPP(10, CTRL) RMESH_FMAC_DPP(11, CTRL) RMESH_FMAC_DPP(12, CTRL)             \
+                                 RMESH_FMAC_DPP(13, CTRL) RMESH_FMAC_DPP(14, CTRL) RMESH_FMAC_DPP(15, CTRL)         \
+                                     RMESH_FMAC_DPP(16, CTRL) RMESH_FMAC_DPP(17, CTRL) RMESH_FMAC_DPP(18, CTRL)     \
+                 : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]), "+v"(v[7]),  \
+                   "+v"(v[8]), "+v"(v[9]), "+v"(v[10]), "+v"(v[11]), "+v"(v[12]), "+v"(v[13]), "+v"(v[14]),         \
+                   "+v"(v[15]), "+v"(v[16]), "+v"(v[17]), "+v"(v[18])                                             \
+                 : "v"(M))
+__device__ __forceinline__ void seg_scan_dpp19(float* v, const SegMasksDpp& s) {
+    RMESH_SCAN_LEVEL(s.m[0], "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1");
+    RMESH_SCAN_LEVEL(s.m[1], "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1");
+    RMESH_SCAN_LEVEL(s.m[2], "row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1");
+    RMESH_SCAN_LEVEL(s.m[3], "row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1");
+    RMESH_SCAN_LEVEL(s.m[4], "row_bcast:15 row_mask:0xa bank_mask:0xf");
+    RMESH_SCAN_LEVEL(s.m[5], "row_bcast:31 row_mask:0xc bank_mask:0xf");
+}
+#undef RMESH_SCAN_LEVEL
+#undef RMESH_FMAC_DPP
+
 // Vertex slot of each face role (a, b, c) for face fi; the fourth slot is the
 // opposite vertex.  FACES = {0,2,1}, {1,2,3}, {0,3,2}, {3,0,1}.
 __device__ __forceinline__ int face_slot(int fi, int role) {
@@ -1214,13 +1242,18 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                     }
                 }
             }
-            if (C::SCAN == 2) {
+            if (C::SCAN >= 2) {
                 // lanes of one tet are contiguous: a segmented wave scan leaves
                 // each tet's sum in its last lane, which adds it to LDS
                 const SegMasksDpp sm = seg_masks_dpp(key);
                 const int next_key = dpp_i<0x130, 0xF>(key, -1);  // wave_shl:1 (lane 63 reads -1)
                 const bool tail = key != 0x7fffffff && next_key != key;
-                seg_scan_dpp<kTerms>(acc, sm);
+                if constexpr (C::SCAN == 3) {
+                    static_assert(kTerms == 19, "the asm scan holds 19 sums");
+                    seg_scan_dpp19(acc, sm);
+                } else {
+                    seg_scan_dpp<kTerms>(acc, sm);
+                }
 #pragma unroll
                 for (int q = 0; q < kTerms; ++q)
                     if (tail && acc[q] != 0.0f) atomicAdd(&s_acc[key][q], acc[q]);
@@ -1309,9 +1342,9 @@ static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
     switch (v) {
         case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2, 4>>(a, s); return true;
         case 1: bwd_launch<BwdCfg<TS, 32, 512, 2, 2, 2, 4>>(a, s); return true;
-        case 2: bwd_launch<BwdCfg<TS, 32, 576, 3, 2, 2, 4>>(a, s); return true;
+        case 2: bwd_launch<BwdCfg<TS, 32, 512, 1, 2, 3, 4>>(a, s); return true;
         case 3: bwd_launch<BwdCfg<TS, 32, 512, 1, 2, 2, 4>>(a, s); return true;
-        case 4: bwd_launch<BwdCfg<TS, 32, 576, 1, 2, 2, 4>>(a, s); return true;
+        case 4: bwd_launch<BwdCfg<TS, 32, 640, 1, 2, 3, 4>>(a, s); return true;
         case 5: bwd_launch<BwdCfg<TS, 32, 640, 1, 2, 2, 4>>(a, s); return true;
         default: return false;
     }
