@@ -272,6 +272,91 @@ __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const 
     store_rest(L, k, f, v, vw, tc);
 }
 
+// Quad prefetch: lane j of the four lanes 4k..4k+3 of tet slot k holds the
+// tet's vertex record j (vertex j; w = density for j = 0, else g component
+// j - 1), its colour, vertex id j (adjoint) and the tet id.
+struct QuadFetch {
+    float4 r;
+    float4 col;
+    uint32_t vid;
+    uint32_t t;
+};
+
+template <bool BWD>
+__device__ __forceinline__ void fetch_quad(QuadFetch& f, uint32_t t, int j, const float4* __restrict__ rec,
+                                           const uint32_t* __restrict__ idx, const float4* __restrict__ colors) {
+    f.t = t;
+    f.r = rec[4ull * t + j];
+    f.col = colors[t];
+    if (BWD) f.vid = idx[4ull * t + j];
+}
+
+// quad_perm: lane j of each quad reads lane P_j of the quad (CTRL = P0 | P1 << 2 | P2 << 4 | P3 << 6).
+template <int CTRL>
+__device__ __forceinline__ float qperm(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ V3 qperm3(float4 v) {
+    return mk3(qperm<CTRL>(v.x), qperm<CTRL>(v.y), qperm<CTRL>(v.z));
+}
+
+// Four lanes per tet store the batch from quad-prefetched records: lane j
+// builds face j (its vertices a, b, c, o come from the quad by one quad_perm
+// each: FACES = {0,2,1,3}, {1,2,3,0}, {0,3,2,1}, {3,0,1,2}) and projects
+// vertex j; lanes 0-2 write the per-tet rows.  Every lane of the block calls
+// it (DPP reads the whole quad); `on` marks the lanes of the batch's tets.
+template <int TS, int B, bool BWD>
+__device__ __forceinline__ void store_quad(BatchLDS<TS, B, BWD>& L, const QuadFetch& f, int k, int j, bool on,
+                                           const ViewParams& vw, const TileCtx& tc) {
+    const V3 va = qperm3<0xC4>(f.r);  // [0,1,0,3]
+    const V3 vb = qperm3<0x3A>(f.r);  // [2,2,3,0]
+    const V3 vc = qperm3<0x6D>(f.r);  // [1,3,2,1]
+    const V3 vo = qperm3<0x93>(f.r);  // [3,0,1,2]
+    const V3 v0 = qperm3<0x00>(f.r);
+    const float g1 = qperm<0x55>(f.r.w), g2 = qperm<0xAA>(f.r.w), g3 = qperm<0xFF>(f.r.w);
+    // vertex j's projection; the quad's behind flag and y-range
+    const V3 vj = mk3(f.r.x, f.r.y, f.r.z);
+    float clip[4];
+    mul_vp(vw.vp, vj, clip);
+    const float pxj = ((clip[0] / clip[3] + 1.0f) * 0.5f) * vw.Wf - tc.tile_ox;
+    const float pyj = ((1.0f - clip[1] / clip[3]) * 0.5f) * vw.Hf - tc.tile_oy;
+    float bh = clip[3] <= 0.0f ? 1.0f : 0.0f;
+    bh = fmaxf(bh, qperm<0xB1>(bh));  // [1,0,3,2]
+    bh = fmaxf(bh, qperm<0x4E>(bh));  // [2,3,0,1]
+    float ymin = fminf(pyj, qperm<0xB1>(pyj)), ymax = fmaxf(pyj, qperm<0xB1>(pyj));
+    ymin = fminf(ymin, qperm<0x4E>(ymin));
+    ymax = fmaxf(ymax, qperm<0x4E>(ymax));
+    if (!on) return;
+    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
+    store_face(L, k, j, va, vb, vc, vo, v0, cam);
+    const bool behind = bh != 0.0f;
+    if (!behind) {
+        L.pxy[j][k] = pxj;
+        L.pxy[4 + j][k] = pyj;
+    }
+    if (j == 0) {
+        const V3 g = mk3(g1, g2, g3);
+        const float bo = dot3(g, sub3(cam, v0));
+        L.tr[k * L.TR + 4] = make_float4(f.col.x + bo, f.col.y + bo, f.col.z + bo, f.r.w);
+        L.behind[k] = behind ? 1u : 0u;
+        uint32_t rr = 0xFFu;
+        if (!behind) {
+            // rows whose pixel band [row, row + 1] meets [ymin, ymax] (as store_rest)
+            const float r0 = fmaxf(ceilf(ymin) - 1.0f, 0.0f);
+            const float r1 = fminf(floorf(ymax), (float)(TS - 1));
+            if (r0 <= r1) rr = (uint32_t)r0 | ((uint32_t)r1 << 8);
+        }
+        L.rows[k] = rr;
+        if (BWD) L.tid[k] = f.t;
+    } else if (j == 1) {
+        L.tr[k * L.TR + 5] = make_float4(g1, g2, g3, 0.0f);
+    } else if (j == 2) {
+        L.tr[k * L.TR + 6] = make_float4(v0.x, v0.y, v0.z, 0.0f);
+    }
+    if (BWD) L.vi[j][k] = f.vid;
+}
+
 // Five lanes per tet stage the batch from the LDS staging area: lanes 0-3 a
 // face each, lane 4 the per-tet data.
 template <int TS, int B, bool BWD, int NT>
@@ -601,7 +686,9 @@ struct FwdCfg {
     static constexpr int B = B_;
     static constexpr int CAP = CAP_;
     static constexpr int R = R_;
-    static constexpr int PF = PF_;  // next batch's records: 0 at batch start, 1 in registers, 2 LDS DMA
+    // next batch's records: 0 at batch start, 1 in registers, 2 LDS DMA, 3 in
+    // registers by quads (4 lanes per tet: falls back to 1 where 4 B > NT)
+    static constexpr int PF = (PF_ == 3 && 4 * B_ > NT) ? 1 : PF_;
     static constexpr int WPE = WPE_ > 0 ? WPE_ : 1;
 };
 
@@ -657,7 +744,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     static_assert(C::PF != 2 || B <= 64, "LDS-DMA staging: one wave, next batch inside a 128-entry id window");
     __shared__ Staging<C::PF == 2 ? B : 1> S;
     const int wid = tid / 64, lane = (int)__lane_id();
+    QuadFetch qf{};
     if (C::PF == 1 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, col_v);
+    if (C::PF == 3 && tid < 4 * nb) fetch_quad<false>(qf, sorted_vals[re - 1 - (tid >> 2)], tid & 3, rec, nullptr, col_v);
     if (C::PF == 2 && wid == 0) {
         const uint32_t t = lane < nb ? sorted_vals[re - 1 - lane] : 0u;
         stage_async<false>(S, t, lane < nb, rec, nullptr, col_v);
@@ -675,6 +764,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         }
         if (C::PF == 0 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - done - tid], rec, nullptr, col_v);
         if constexpr (C::PF == 2) store_batch_staged<TS, B, false, NT>(L, S, nb, vw, tc);
+        else if constexpr (C::PF == 3) store_quad<TS, B, false>(L, qf, tid >> 2, tid & 3, tid < 4 * nb, vw, tc);
         else if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
         if (tid < TS * TS) s_mask[tid] = 0ull;
         if (R == 1 && tid < CAP / 32) s_start[tid] = 0u;
@@ -687,6 +777,8 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
         if (C::PF == 1 && tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
+        if (C::PF == 3 && tid < 4 * nb_next)
+            fetch_quad<false>(qf, sorted_vals[re - 1 - next - (tid >> 2)], tid & 3, rec, nullptr, col_v);
         if (C::PF == 2 && wid == 0) {
             const uint32_t tn = window_id<B>(win, win1, (int)nbp + lane);
             stage_async<false>(S, tn, lane < nb_next, rec, nullptr, col_v);
@@ -811,7 +903,7 @@ struct BwdCfg {
     static constexpr int B = B_;
     static constexpr int CAP = CAP_;
     static constexpr int R = R_;
-    static constexpr int PF = PF_;  // next batch's records: 0 at batch start, 1 in registers, 2 LDS DMA
+    static constexpr int PF = (PF_ == 3 && 4 * B_ > NT) ? 1 : PF_;  // as FwdCfg::PF
     static constexpr int SCAN = SCAN_;
     static constexpr int WPE = WPE_ > 0 ? WPE_ : 1;
 };
@@ -1109,7 +1201,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     TetFetch f;
     static_assert(C::PF != 2 || B <= 64, "LDS-DMA staging: one wave, next batch inside a 128-entry id window");
     __shared__ Staging<C::PF == 2 ? B : 1> S;
+    QuadFetch qf{};
     if (C::PF == 1 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
+    if (C::PF == 3 && tid < 4 * nb) fetch_quad<true>(qf, sorted_vals[rs + (tid >> 2)], tid & 3, rec, idx, col_v);
     if (C::PF == 2 && wid == 0) {
         const uint32_t t = lane < nb ? sorted_vals[rs + lane] : 0u;
         stage_async<true>(S, t, lane < nb, rec, idx, col_v);
@@ -1124,6 +1218,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         }
         if (C::PF == 0 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + done + tid], rec, idx, col_v);
         if constexpr (C::PF == 2) store_batch_staged<TS, B, true, NT>(L, S, nb, vw, tc);
+        else if constexpr (C::PF == 3) store_quad<TS, B, true>(L, qf, tid >> 2, tid & 3, tid < 4 * nb, vw, tc);
         else if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
         for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
@@ -1136,6 +1231,8 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
         if (C::PF == 1 && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
+        if (C::PF == 3 && tid < 4 * nb_next)
+            fetch_quad<true>(qf, sorted_vals[rs + next + (tid >> 2)], tid & 3, rec, idx, col_v);
         if (C::PF == 2 && wid == 0) {
             const uint32_t tn = window_id<B>(win, win1, (int)nbp + lane);
             stage_async<true>(S, tn, lane < nb_next, rec, idx, col_v);
@@ -1310,12 +1407,12 @@ template <int TS>
 static bool fwd_variant(const FwdArgs& a, int v, hipStream_t s) {
     switch (v) {
         case 0: fwd_launch<FwdCfg<TS, 48, 768, 1, 1>>(a, s); return true;
-        case 1: fwd_launch<FwdCfg<TS, 48, 768, 2, 1>>(a, s); return true;
+        case 1: fwd_launch<FwdCfg<TS, 48, 768, 1, 3>>(a, s); return true;
         case 2: fwd_launch<FwdCfg<TS, 32, 704, 1, 2, 5>>(a, s); return true;
-        case 3: fwd_launch<FwdCfg<TS, 40, 640, 1, 2, 5>>(a, s); return true;
+        case 3: fwd_launch<FwdCfg<TS, 48, 704, 1, 3, 5>>(a, s); return true;
         case 4: fwd_launch<FwdCfg<TS, 56, 896, 1, 1>>(a, s); return true;
-        case 5: fwd_launch<FwdCfg<TS, 32, 768, 1, 2, 5>>(a, s); return true;
-        case 6: fwd_launch<FwdCfg<TS, 48, 704, 1, 2, 5>>(a, s); return true;
+        case 5: fwd_launch<FwdCfg<TS, 64, 1024, 1, 3>>(a, s); return true;
+        case 6: fwd_launch<FwdCfg<TS, 32, 704, 1, 3, 5>>(a, s); return true;
         case 7: fwd_launch<FwdCfg<TS, 64, 1024, 1, 1>>(a, s); return true;
         default: return false;
     }
@@ -1340,12 +1437,12 @@ static void bwd_launch(const BwdArgs& a, hipStream_t s) {
 template <int TS>
 static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
     switch (v) {
-        case 0: bwd_launch<BwdCfg<TS, 32, 512, 3, 2, 2, 4>>(a, s); return true;
-        case 1: bwd_launch<BwdCfg<TS, 32, 512, 2, 2, 2, 4>>(a, s); return true;
+        case 0: bwd_launch<BwdCfg<TS, 32, 480, 1, 3, 3, 5>>(a, s); return true;
+        case 1: bwd_launch<BwdCfg<TS, 32, 512, 1, 3, 3, 4>>(a, s); return true;
         case 2: bwd_launch<BwdCfg<TS, 32, 512, 1, 2, 3, 4>>(a, s); return true;
-        case 3: bwd_launch<BwdCfg<TS, 32, 512, 1, 2, 2, 4>>(a, s); return true;
+        case 3: bwd_launch<BwdCfg<TS, 32, 576, 1, 3, 3, 4>>(a, s); return true;
         case 4: bwd_launch<BwdCfg<TS, 32, 640, 1, 2, 3, 4>>(a, s); return true;
-        case 5: bwd_launch<BwdCfg<TS, 32, 640, 1, 2, 2, 4>>(a, s); return true;
+        case 5: bwd_launch<BwdCfg<TS, 32, 640, 1, 3, 3, 4>>(a, s); return true;
         default: return false;
     }
 }
