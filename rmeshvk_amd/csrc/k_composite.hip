@@ -410,10 +410,11 @@ __device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
     return v;
 }
 
-// Value of the last lane of each aligned group of G lanes (quad_perm).
+// Value of the last lane of each aligned group of G lanes (quad_perm; G = 8: bpermute).
 template <int G>
 __device__ __forceinline__ uint32_t group_last(uint32_t v) {
-    if constexpr (G == 4) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xFF, 0xF, 0xF, false);
+    if constexpr (G == 8) return (uint32_t)__shfl((int)v, (int)(__lane_id() | 7u), 64);
+    else if constexpr (G == 4) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xFF, 0xF, 0xF, false);
     else if constexpr (G == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xF5, 0xF, 0xF, false);
     else return v;
 }
@@ -449,7 +450,7 @@ template <int TS, int B, bool BWD, bool FORWARD_RULE, int CAP, int R, int NT>
 __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, chunk_t<R>* __restrict__ tab,
                                              uint32_t* __restrict__ starts) {
     constexpr int SP = TS + 1;
-    constexpr int LPT = NT >= 4 * B ? 4 : NT >= 2 * B ? 2 : 1;
+    constexpr int LPT = NT >= 8 * B ? 8 : NT >= 4 * B ? 4 : NT >= 2 * B ? 2 : 1;
     constexpr int QMAX = (TS + LPT - 1) / LPT;
     static_assert(LPT * B <= NT && B <= 64, "one pass over the batch");
     static_assert(B * TS * TS < 65536, "packed scan fields");
@@ -491,6 +492,10 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ch
     if constexpr (LPT >= 4) {
         const uint32_t u = dpp_u<0x112, 0xF>(g);  // row_shr:2
         if (j >= 2) g += u;
+    }
+    if constexpr (LPT >= 8) {
+        const uint32_t u = dpp_u<0x114, 0xF>(g);  // row_shr:4
+        if (j >= 4) g += u;
     }
     const uint32_t tot = group_last<LPT>(g);
     const uint32_t pl = g - s;
