@@ -8,11 +8,12 @@
 // The SH adjoint has no counterpart in the reference (it never differentiates
 // SH); it is derived from the forward above and checked by finite differences.
 //
-// MI355X design: one thread per tet loops over all views of the batch so the
-// 192-byte SH block of a tet is read once per step (not once per view); the
-// SH adjoint accumulates d_sh over views in registers and applies Adam to the
-// tet's 48 coefficients in the same pass, so d_sh never touches HBM in a
-// training step.  The other parameter groups are updated by one launch.
+// MI355X design: the SH block of a tet is read once per step (not once per
+// view): k_sh_colors loops over all views of the batch in one thread, and the
+// SH adjoint (four lanes per tet, each over a quarter of the views)
+// accumulates d_sh in registers and applies Adam to the tet's coefficients in
+// the same pass, so d_sh never touches HBM in a training step.  The other
+// parameter groups are updated by one launch.
 #include "rmesh_common.hpp"
 #include "rmesh_kernels.hpp"
 
@@ -174,23 +175,40 @@ __device__ __forceinline__ float adam_update(float& p, float gr, float& m, float
     return p;
 }
 
-// SH adjoint (+ optional fused Adam on the SH coefficients): one lane per
-// (tet, colour channel), 64 tets per 192-lane workgroup.  The workgroup's
-// coefficients (and Adam's m, v) are one contiguous block in HBM: they are
-// moved between HBM and LDS with coalesced dword accesses, and each lane works
-// on its channel's row in LDS (rows padded to NC + 1 floats: conflict-free).
-// The three channel partials of a tet's direction/offset adjoint are summed in
-// a fixed order through LDS before one lane scatters them to the vertices.
-constexpr int kShTets = 64;
+// SH adjoint (+ optional fused Adam on the SH coefficients): four lanes per
+// tet, 32 tets per 128-lane workgroup; lane j of a tet takes views j, j + 4,
+// ... for all three channels, so the view direction, the basis and the
+// direction adjoint (one vjp of the channel-summed weights: the vjp is linear)
+// are formed once per (tet, view).  The workgroup's coefficients (and Adam's
+// m, v) are one contiguous block in HBM, moved between HBM and LDS with
+// coalesced dword accesses; the four lanes' partial d_sh are summed with two
+// quad_perm butterflies, staged in LDS rows (padded, see SR) and
+// applied (Adam) or stored element-parallel.  One lane per tet scatters the
+// direction/offset adjoint to the vertices.
+constexpr int kShTets = 32;
+constexpr int kShLanes = 4;
+
+template <int CTRL>
+__device__ __forceinline__ float sh_qperm(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// Sum over the quad, left in every lane: quad_perm [1,0,3,2] then [2,3,0,1].
+__device__ __forceinline__ float quad_sum(float v) {
+    v = v + sh_qperm<0xB1>(v);
+    return v + sh_qperm<0x4E>(v);
+}
 
 template <int DEG, bool ADAM>
-__global__ __launch_bounds__(3 * kShTets) void k_sh_backward(ShBwdArgs a) {
+__global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a) {
     constexpr int NC = (DEG + 1) * (DEG + 1);
-    constexpr int SR = NC + 1;                      // LDS row stride
-    constexpr int NIO = ADAM ? 3 : 1;               // sh (+ m, v)
-    __shared__ float s_io[NIO][3 * kShTets * SR];
-    __shared__ float red[kShTets][3][4];  // per channel: d(centroid).xyz, d(offset)
-    const uint32_t lt = threadIdx.x / 3, ch = threadIdx.x % 3;
+    // LDS row stride: whole float4s, and 16 tets' rows (3 SR floats apart) on
+    // distinct banks for ds_read_b128
+    constexpr int SR = (NC + 3) / 4 * 4 + 4;
+    constexpr int NIO = ADAM ? 3 : 1;  // sh (+ m, v)
+    constexpr int NT = kShLanes * kShTets;
+    __shared__ __align__(16) float s_io[NIO][3 * kShTets * SR];
+    __shared__ float s_g[3 * kShTets * SR];  // summed d_sh rows
+    const uint32_t lt = threadIdx.x / kShLanes, j = threadIdx.x % kShLanes;
     const uint32_t t0 = blockIdx.x * kShTets;
     const uint32_t t = t0 + lt;
     const uint32_t nflt = min((uint32_t)kShTets, a.M - t0) * 3u * NC;  // floats of this workgroup
@@ -198,40 +216,73 @@ __global__ __launch_bounds__(3 * kShTets) void k_sh_backward(ShBwdArgs a) {
                       ADAM ? a.v_sh + (uint64_t)t0 * 3 * NC : nullptr};
 #pragma unroll
     for (int q = 0; q < NIO; ++q)
-        for (uint32_t i = threadIdx.x; i < nflt; i += 3 * kShTets) s_io[q][(i / NC) * SR + i % NC] = io_g[q][i];
+        for (uint32_t i = threadIdx.x; i < nflt; i += NT) s_io[q][(i / NC) * SR + i % NC] = io_g[q][i];
     __syncthreads();
     const bool live = t < a.M;
-    const int row = (int)(lt * 3 + ch) * SR;
-    V3 v[4], g, cen, off;
-    float c[NC], dsh[NC];
+    const int row0 = (int)(lt * 3) * SR;
+    V3 v[4], g = mk3(0.f, 0.f, 0.f), cen, off = mk3(0.f, 0.f, 0.f);
+    float dsh[3][NC];
+    float s_all = 0.0f;
+    V3 dcen = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+#pragma unroll
+        for (int k = 0; k < NC; ++k) dsh[ch][k] = 0.0f;
     if (live) {
         tet_geom(a.rec, t, v, g, cen);
         off = sub3(v[0], cen);
         const float offset = dot3(g, off);
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            c[k] = s_io[0][row + k];
-            dsh[k] = 0.0f;
+        // colour gradients one view ahead (their load latency overlaps a view's work)
+        float dn[3] = {0.f, 0.f, 0.f};
+        if (j < a.V) {
+            const float* dcp = a.d_colors + ((uint64_t)j * a.M + t) * 3;
+            dn[0] = dcp[0];
+            dn[1] = dcp[1];
+            dn[2] = dcp[2];
         }
-        float s_all = 0.0f;
-        V3 dcen = mk3(0.f, 0.f, 0.f);
-        for (uint32_t vv = 0; vv < a.V; ++vv) {
-            const float dc = a.d_colors[((uint64_t)vv * a.M + t) * 3 + ch];
-            if (dc == 0.0f) continue;
+        for (uint32_t vv = j; vv < a.V; vv += kShLanes) {
+            const float dc[3] = {dn[0], dn[1], dn[2]};
+            if (vv + kShLanes < a.V) {
+                const float* dcp = a.d_colors + ((uint64_t)(vv + kShLanes) * a.M + t) * 3;
+                dn[0] = dcp[0];
+                dn[1] = dcp[1];
+                dn[2] = dcp[2];
+            }
+            if (dc[0] == 0.0f && dc[1] == 0.0f && dc[2] == 0.0f) continue;
             float len;
             const V3 d = view_dir(cen, a.views[vv], len);
-            float Y[NC];
+            float Y[NC], w[NC];
             sh_basis<DEG>(d, Y);
-            float val = Y[0] * c[0];
 #pragma unroll
-            for (int k = 1; k < NC; ++k) val = val + Y[k] * c[k];
-            const float d_pre = dc * dsoftplus_f((val + 0.5f) + offset);
-            s_all += d_pre;
-            float w[NC];
+            for (int k = 0; k < NC; ++k) w[k] = 0.0f;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                dsh[k] += d_pre * Y[k];
-                w[k] = d_pre * c[k];
+            for (int ch = 0; ch < 3; ++ch) {
+                if (dc[ch] == 0.0f) continue;
+                float c[NC];  // the channel's coefficients, from the LDS row
+                if constexpr (NC % 4 == 0) {
+                    const float4* cr = reinterpret_cast<const float4*>(&s_io[0][row0 + ch * SR]);
+#pragma unroll
+                    for (int q = 0; q < NC / 4; ++q) {
+                        const float4 x = cr[q];
+                        c[4 * q] = x.x;
+                        c[4 * q + 1] = x.y;
+                        c[4 * q + 2] = x.z;
+                        c[4 * q + 3] = x.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) c[k] = s_io[0][row0 + ch * SR + k];
+                }
+                float val = Y[0] * c[0];
+#pragma unroll
+                for (int k = 1; k < NC; ++k) val = val + Y[k] * c[k];
+                const float d_pre = dc[ch] * dsoftplus_f((val + 0.5f) + offset);
+                s_all += d_pre;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                    dsh[ch][k] += d_pre * Y[k];
+                    w[k] += d_pre * c[k];
+                }
             }
             if (len > 0.0f) {
                 const V3 d_dir = sh_basis_vjp<DEG>(d, w);
@@ -239,16 +290,15 @@ __global__ __launch_bounds__(3 * kShTets) void k_sh_backward(ShBwdArgs a) {
                 dcen = add3(dcen, scl3(sub3(d_dir, scl3(d, dd)), 1.0f / len));
             }
         }
-        red[lt][ch][0] = dcen.x;
-        red[lt][ch][1] = dcen.y;
-        red[lt][ch][2] = dcen.z;
-        red[lt][ch][3] = s_all;
     }
-    __syncthreads();
-    if (live && ch == 0) {
-        const float s_all = (red[lt][0][3] + red[lt][1][3]) + red[lt][2][3];
-        V3 dcen = mk3((red[lt][0][0] + red[lt][1][0]) + red[lt][2][0], (red[lt][0][1] + red[lt][1][1]) + red[lt][2][1],
-                      (red[lt][0][2] + red[lt][1][2]) + red[lt][2][2]);
+    // sums over the tet's four lanes (every lane of the wave takes part)
+    s_all = quad_sum(s_all);
+    dcen = mk3(quad_sum(dcen.x), quad_sum(dcen.y), quad_sum(dcen.z));
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+#pragma unroll
+        for (int k = 0; k < NC; ++k) dsh[ch][k] = quad_sum(dsh[ch][k]);
+    if (live && j == 0) {
         // offset = g . (v0 - centroid)
         a.d_grads[3ull * t] += s_all * off.x;
         a.d_grads[3ull * t + 1] += s_all * off.y;
@@ -265,33 +315,31 @@ __global__ __launch_bounds__(3 * kShTets) void k_sh_backward(ShBwdArgs a) {
             if (dvv.z != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 2, dvv.z);
         }
     }
+    // lane j < 3 stages channel j's row
+    if (live && j < 3) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+            if (ch == (int)j)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) s_g[row0 + ch * SR + k] = dsh[ch][k];
+    }
+    __syncthreads();
     if (ADAM) {
         if (*a.st.skip) return;  // uniform: no barrier follows on this path
-        if (live) {
-            const float stepf = (float)*a.st.step;
-            const float b1p = expf(stepf * logf(a.st.beta1));
-            const float b2p = expf(stepf * logf(a.st.beta2));
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                float p = c[k], m = s_io[NIO > 1 ? 1 : 0][row + k], v2 = s_io[NIO > 2 ? 2 : 0][row + k];
-                adam_update(p, dsh[k], m, v2, a.lr, a.st.beta1, a.st.beta2, a.st.eps, b1p, b2p);
-                s_io[0][row + k] = p;
-                s_io[NIO > 1 ? 1 : 0][row + k] = m;
-                s_io[NIO > 2 ? 2 : 0][row + k] = v2;
-            }
+        const float stepf = (float)*a.st.step;
+        const float b1p = expf(stepf * logf(a.st.beta1));
+        const float b2p = expf(stepf * logf(a.st.beta2));
+        for (uint32_t i = threadIdx.x; i < nflt; i += NT) {
+            const uint32_t r = (i / NC) * SR + i % NC;
+            float p = s_io[0][r], m = s_io[NIO > 1 ? 1 : 0][r], v2 = s_io[NIO > 2 ? 2 : 0][r];
+            adam_update(p, s_g[r], m, v2, a.lr, a.st.beta1, a.st.beta2, a.st.eps, b1p, b2p);
+            io_g[0][i] = p;
+            if (NIO > 1) io_g[1][i] = m;
+            if (NIO > 2) io_g[2][i] = v2;
         }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < NIO; ++q)
-            for (uint32_t i = threadIdx.x; i < nflt; i += 3 * kShTets) io_g[q][i] = s_io[q][(i / NC) * SR + i % NC];
     } else {
-        if (live) {
-#pragma unroll
-            for (int k = 0; k < NC; ++k) s_io[0][row + k] = dsh[k];
-        }
-        __syncthreads();
         float* o = a.d_sh + (uint64_t)t0 * 3 * NC;
-        for (uint32_t i = threadIdx.x; i < nflt; i += 3 * kShTets) o[i] = s_io[0][(i / NC) * SR + i % NC];
+        for (uint32_t i = threadIdx.x; i < nflt; i += NT) o[i] = s_g[(i / NC) * SR + i % NC];
     }
 }
 
@@ -350,7 +398,7 @@ void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t
 
 template <int DEG>
 static void shb(const ShBwdArgs& a, hipStream_t s) {
-    dim3 gr((a.M + kShTets - 1) / kShTets), bl(3 * kShTets);
+    dim3 gr((a.M + kShTets - 1) / kShTets), bl(kShLanes * kShTets);
     if (a.adam) hipLaunchKernelGGL((k_sh_backward<DEG, true>), gr, bl, 0, s, a);
     else hipLaunchKernelGGL((k_sh_backward<DEG, false>), gr, bl, 0, s, a);
 }
