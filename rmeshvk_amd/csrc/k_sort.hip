@@ -11,7 +11,7 @@
 //  * stable in-tile ranking with wave64 ballots: each wave matches the 8 digit
 //    bits of its 64 keys (8 ballots), ranks by popcount, and keeps a private
 //    256-entry running count in LDS — no barriers inside the ranking rounds;
-//  * the tile (4096 keys) is re-ordered by digit in LDS before the scatter,
+//  * the tile (2048 keys) is re-ordered by digit in LDS before the scatter,
 //    so the global writes are runs per digit instead of 8-byte scatters.
 #include "rmesh_common.hpp"
 #include "rmesh_kernels.hpp"
@@ -20,9 +20,9 @@ namespace rmesh {
 
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / kWave;
-constexpr int kRounds = 16;                         // 64-key rounds per wave per tile
-constexpr int kWaveKeys = kRounds * kWave;          // 1024
-constexpr int kTileKeys = kWaveKeys * kSortWaves;   // 4096
+constexpr int kRounds = 8;                          // 64-key rounds per wave per tile
+constexpr int kWaveKeys = kRounds * kWave;          // 512
+constexpr int kTileKeys = kWaveKeys * kSortWaves;   // 2048 (LDS 31 KB: 5 waves/SIMD)
 
 __device__ __forceinline__ void chunk_of(uint32_t n, uint32_t G, uint32_t b, uint32_t& lo, uint32_t& hi) {
     uint32_t per = (n + G - 1) / G;
