@@ -52,58 +52,90 @@ __device__ __forceinline__ TileCtx tile_ctx(const ViewParams& vw, uint32_t gtile
     return c;
 }
 
-// Edges of a tet in the order of rasterize_compute.wgsl:176-231.
-constexpr int kEdgeI[6] = {0, 0, 0, 1, 1, 2};
-constexpr int kEdgeJ[6] = {1, 2, 3, 2, 3, 3};
-
-// Reciprocal y extents of the six projected edges (one v_rcp_f32 each).
-__device__ __forceinline__ void edge_rcp(const float* py_, float* ry) {
-#pragma unroll
-    for (int e = 0; e < 6; ++e) ry[e] = __builtin_amdgcn_rcpf(py_[kEdgeJ[e]] - py_[kEdgeI[e]]);
-}
-
-// Row span of the tet in tile-local pixels (rasterize_compute.wgsl:170-275;
-// FORWARD adds the vertex-in-row rule, the adjoint omits it:
-// backward_tiled_compute.wgsl:242-299).  Packed in a byte as xl | xr << 4
+// Scanline spans of one projected tet (rasterize_compute.wgsl:170-275; FORWARD
+// adds the vertex-in-row rule, the adjoint omits it:
+// backward_tiled_compute.wgsl:242-299), packed in a byte as xl | xr << 4
 // (0x0F, xl > xr: empty).
-// The edge parameter (yc - yi) / (yj - yi) is formed with the edge's 1-ulp
-// reciprocal ry: the span is widened by 0.001 px on both sides, so the two
-// roundings can only disagree about a pixel centre lying within ~1e-6 px of
-// the widened boundary, 0.001 px outside the silhouette, whose ray misses the
-// tet (the pixel is tested, not composited, by virtue of the span).
-template <bool FORWARD>
-__device__ __forceinline__ uint32_t row_span(const float* px_, const float* py_, const float* ry, uint32_t row,
-                                             int32_t TS) {
-    float xl_f = 1e10f, xr_f = -1e10f;
-    float yc = (float)row + 0.5f;
-#pragma unroll
-    for (int e = 0; e < 6; ++e) {
-        float yi = py_[kEdgeI[e]], yj = py_[kEdgeJ[e]];
-        if ((yi <= yc && yj > yc) || (yj <= yc && yi > yc)) {
-            float t = (yc - yi) * ry[e];
-            float x = px_[kEdgeI[e]] + t * (px_[kEdgeJ[e]] - px_[kEdgeI[e]]);
-            xl_f = fminf(xl_f, x);
-            xr_f = fmaxf(xr_f, x);
-        }
+//
+// The reference tests each of the six edges, min(yi, yj) <= yc < max(yi, yj),
+// and takes the min / max of the crossing edges' x.  With the vertices sorted
+// by y (y0 <= y1 <= y2 <= y3) that test selects exactly the edges
+//   {01, 02, 03} for yc in [y0, y1), {02, 03, 12, 13} in [y1, y2),
+//   {03, 13, 23} in [y2, y3), and none elsewhere,
+// so a row's extent is a min3 / max3 over a fixed set, chosen by two compares.
+// An edge's x is formed from its lower vertex with a 1-ulp reciprocal slope;
+// the span is widened by 0.001 px on both sides, so this rounding can only
+// disagree with the reference's about a pixel centre within ~1e-6 px of the
+// widened boundary, 0.001 px outside the silhouette, whose ray misses the tet
+// (such a pixel is intersection-tested or not; it never contributes).
+struct TetRows {
+    float y[4], x[4];  // projected vertices sorted by y
+    float s[6];        // dx/dy of the sorted edges 01 02 03 12 13 23
+
+    __device__ __forceinline__ void cswap(int a, int b) {
+        const bool sw = y[b] < y[a];
+        const float ya = y[a], xa = x[a];
+        y[a] = sw ? y[b] : ya;
+        x[a] = sw ? x[b] : xa;
+        y[b] = sw ? ya : y[b];
+        x[b] = sw ? xa : x[b];
     }
-    if (FORWARD) {
+    __device__ __forceinline__ float slope(int a, int b) const {
+        return (x[b] - x[a]) * __builtin_amdgcn_rcpf(y[b] - y[a]);
+    }
+    __device__ __forceinline__ void init(const float* px_, const float* py_) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            float vy = py_[k];
-            if (vy >= yc - 0.5f && vy < yc + 0.5f) {
-                xl_f = fminf(xl_f, px_[k]);
-                xr_f = fmaxf(xr_f, px_[k]);
+            y[k] = py_[k];
+            x[k] = px_[k];
+        }
+        cswap(0, 1);
+        cswap(2, 3);
+        cswap(0, 2);
+        cswap(1, 3);
+        cswap(1, 2);
+        s[0] = slope(0, 1);
+        s[1] = slope(0, 2);
+        s[2] = slope(0, 3);
+        s[3] = slope(1, 2);
+        s[4] = slope(1, 3);
+        s[5] = slope(2, 3);
+    }
+    template <bool FORWARD>
+    __device__ __forceinline__ uint32_t span(uint32_t row, int32_t TS) const {
+        const float yc = (float)row + 0.5f;
+        const float d0 = yc - y[0], d1 = yc - y[1], d2 = yc - y[2];
+        const float x01 = x[0] + d0 * s[0], x02 = x[0] + d0 * s[1], x03 = x[0] + d0 * s[2];
+        const float x12 = x[1] + d1 * s[3], x13 = x[1] + d1 * s[4], x23 = x[2] + d2 * s[5];
+        const bool lo = yc < y[1], mid = yc < y[2];
+        float xl_f = lo ? fminf(fminf(x01, x02), x03) : mid ? fminf(fminf(fminf(x02, x03), x12), x13)
+                                                            : fminf(fminf(x03, x13), x23);
+        float xr_f = lo ? fmaxf(fmaxf(x01, x02), x03) : mid ? fmaxf(fmaxf(fmaxf(x02, x03), x12), x13)
+                                                            : fmaxf(fmaxf(x03, x13), x23);
+        if (!(y[0] <= yc && yc < y[3])) {
+            xl_f = 1e10f;
+            xr_f = -1e10f;
+        }
+        if (FORWARD) {
+            // vertices inside the row's band [row, row + 1)
+            const float r0 = (float)row, r1 = r0 + 1.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (y[k] >= r0 && y[k] < r1) {
+                    xl_f = fminf(xl_f, x[k]);
+                    xr_f = fmaxf(xr_f, x[k]);
+                }
             }
         }
+        if (xl_f <= xr_f) {
+            const float eps = 0.001f;
+            int32_t l = max(sat_i32(ceilf(xl_f - 0.5f - eps)), 0);
+            int32_t r = min(sat_i32(floorf(xr_f - 0.5f + eps)), TS - 1);
+            if (l <= r) return (uint32_t)l | ((uint32_t)r << 4);
+        }
+        return 0x0Fu;
     }
-    if (xl_f <= xr_f) {
-        const float eps = 0.001f;
-        int32_t l = max(sat_i32(ceilf(xl_f - 0.5f - eps)), 0);
-        int32_t r = min(sat_i32(floorf(xr_f - 0.5f + eps)), TS - 1);
-        if (l <= r) return (uint32_t)l | ((uint32_t)r << 4);
-    }
-    return 0x0Fu;
-}
+};
 
 __device__ __forceinline__ int32_t span_xl(uint32_t sp) { return (int32_t)(sp & 15u); }
 __device__ __forceinline__ int32_t span_xr(uint32_t sp) { return (int32_t)(sp >> 4); }
@@ -470,12 +502,12 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ch
         if (nr > 0) {
             const float px_[4] = {L.pxy[0][k], L.pxy[1][k], L.pxy[2][k], L.pxy[3][k]};
             const float py_[4] = {L.pxy[4][k], L.pxy[5][k], L.pxy[6][k], L.pxy[7][k]};
-            float ry[6];
-            edge_rcp(py_, ry);
+            TetRows tr;
+            tr.init(px_, py_);
 #pragma unroll
             for (int i = 0; i < QMAX; ++i) {
                 if (i < nr) {
-                    sp[i] = row_span<FORWARD_RULE>(px_, py_, ry, (uint32_t)(ra + i), TS);
+                    sp[i] = tr.span<FORWARD_RULE>((uint32_t)(ra + i), TS);
                     L.span[k * SP + ra + i] = (uint8_t)sp[i];
                     s += (uint32_t)max(span_xr(sp[i]) - span_xl(sp[i]) + 1, 0);
                 }
