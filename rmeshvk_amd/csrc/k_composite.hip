@@ -1443,14 +1443,14 @@ static void fwd_launch(const FwdArgs& a, hipStream_t s) {
 template <int TS>
 static bool fwd_variant(const FwdArgs& a, int v, hipStream_t s) {
     switch (v) {
-        case 0: fwd_launch<FwdCfg<TS, 48, 768, 1, 1>>(a, s); return true;
-        case 1: fwd_launch<FwdCfg<TS, 48, 768, 1, 3>>(a, s); return true;
-        case 2: fwd_launch<FwdCfg<TS, 32, 704, 1, 2, 5>>(a, s); return true;
-        case 3: fwd_launch<FwdCfg<TS, 48, 704, 1, 3, 5>>(a, s); return true;
-        case 4: fwd_launch<FwdCfg<TS, 56, 896, 1, 1>>(a, s); return true;
-        case 5: fwd_launch<FwdCfg<TS, 64, 1024, 1, 3>>(a, s); return true;
-        case 6: fwd_launch<FwdCfg<TS, 32, 704, 1, 3, 5>>(a, s); return true;
-        case 7: fwd_launch<FwdCfg<TS, 64, 1024, 1, 1>>(a, s); return true;
+        case 0: fwd_launch<FwdCfg<TS, 48, 768, 1, 3>>(a, s); return true;
+        case 1: fwd_launch<FwdCfg<TS, 48, 704, 1, 3, 5>>(a, s); return true;
+        case 2: fwd_launch<FwdCfg<TS, 32, 704, 1, 3, 5>>(a, s); return true;
+        case 3: fwd_launch<FwdCfg<TS, 64, 1024, 1, 3>>(a, s); return true;
+        case 4: fwd_launch<FwdCfg<TS, 64, 768, 1, 3, 5>>(a, s); return true;
+        case 5: fwd_launch<FwdCfg<TS, 56, 896, 1, 3>>(a, s); return true;
+        case 6: fwd_launch<FwdCfg<TS, 56, 768, 1, 3, 5>>(a, s); return true;
+        case 7: fwd_launch<FwdCfg<TS, 64, 896, 1, 3>>(a, s); return true;
         default: return false;
     }
 }
