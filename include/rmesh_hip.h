@@ -77,6 +77,14 @@ rmesh_status rmesh_forward(rmesh_renderer* h, const float* cam, const float* vp,
 /* RMeshRenderer.read_instance_count (lib.rs:1879-1884): tets that passed
  * culling in the last forward (first view). */
 rmesh_status rmesh_read_instance_count(rmesh_renderer* h, uint32_t* out);
+/* RMeshRenderer.forward_error (lib.rs:1621-1711, error_tiled_compute.wgsl):
+ * per-tet error statistics over the tile lists and camera of the last
+ * forward_tiled.  pixel_err, ssim_err [H*W] host; outputs host: tet_err
+ * [M*16] (weight, weighted L1, L1^2, -, -, SSIM, entry xyz, exit xyz, weight,
+ * centroid xyz; all weighted by T_val = exp(log T) * od) and tet_count [M*2]
+ * (covered pixels, max of i32(65535 * T_val)). */
+rmesh_status rmesh_forward_error(rmesh_renderer* h, const float* pixel_err, const float* ssim_err, float* tet_err,
+                                 int32_t* tet_count);
 
 /* RMeshRenderer.backward (lib.rs:1894-2031).  Uses the sorted tile lists and
  * image of the last forward_tiled.  dl_d_image [H*W*4] host; outputs host:

@@ -39,6 +39,13 @@ orc_sh_colors     basis == scipy real spherical harmonics (Condon-Shortley
                   phase), hence orthonormal; degree-0 closed form.
 orc_sh_backward   float64 finite differences, degrees 0-3.
 orc_circumspheres the regular-tet known answer; equidistance of the vertices.
+orc_error         closed forms on one tet (weight = sum of the pixels' optical
+                  depths, which agree with the forward's alpha = 1 - e^-od;
+                  weighted entry/exit points on the tet's planes; centroid
+                  moment / weight = centroid); pe = 1 gives weighted L1 ==
+                  weighted L1^2 == weight; pixel count and max weight by
+                  brute force; a front tet shields a back one exactly as the
+                  -5.5 log-transmittance rule and exp(log T) say.
 ================  ==========================================================
 """
 
@@ -58,6 +65,7 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _u32 = ctypes.c_uint32
 _u64 = ctypes.c_uint64
 _f32 = ctypes.c_float
@@ -91,6 +99,8 @@ def lib():
                                                           _u32, _u32, _u32, rp, rp, _u32, _u32, _u32, rp, rp, rp, rp]
             getattr(L, "orc_sh_colors" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp]
             getattr(L, "orc_sh_backward" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp, rp, rp, rp]
+            getattr(L, "orc_error" + sfx).argtypes = [rp, _u32p, rp, _u32p, _u32p, _f32p, _f32p, _f32p, _u32, _u32,
+                                                       _u32, _f32, rp, rp, rp, _i32p]
         L.orc_sorting_bits.argtypes = [_u32]
         L.orc_sorting_bits.restype = _u32
         L.orc_sort.argtypes = [_u64p, _u32p, _u64, _u32]
@@ -208,6 +218,21 @@ def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, 
         _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
         w, h, ts, _a(rgba, dtype).ravel(), _a(dl, dtype).ravel(), t0, t1, tile_step, dV, dD, dG, dC)
     return {"d_vertices": dV, "d_densities": dD, "d_color_grads": dG, "d_base_colors": dC}
+
+
+def error_stats(scene, cam, w, h, sorted_vals, rng_, pixel_err, ssim_err, ts=12, min_t=0.0, dtype=np.float32):
+    """Per-tet error statistics of the last forward's tile lists:
+    (tet_err [M, 16], tet_count [M, 2] int32)."""
+    cp, vp, ci = cam
+    M = len(scene["indices"]) // 4
+    err = np.zeros(M * 16, dtype=dtype)
+    cnt = np.zeros(M * 2, dtype=np.int32)
+    getattr(lib(), "orc_error" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(scene["densities"], dtype),
+        _a(sorted_vals, np.uint32), _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32),
+        _a(ci, np.float32), w, h, ts, float(min_t), _a(pixel_err, dtype).ravel(), _a(ssim_err, dtype).ravel(),
+        err, cnt)
+    return err.reshape(M, 16), cnt.reshape(M, 2)
 
 
 def adam(params, grads, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):

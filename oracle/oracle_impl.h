@@ -506,6 +506,94 @@ void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const
 }
 
 /* ------------------------------------------------------------------ */
+/* Per-tet error statistics for densification (forward_error)           */
+/* crates/rmesh-error/src/wgsl/error_tiled_compute.wgsl:88-373,          */
+/* crates/rmesh-python/src/lib.rs:1621-1711.  Same tile traversal and    */
+/* forward spans as orc_forward; per covered pixel with log T >= -5.5 and */
+/* a valid slab, T_val = exp(log T) * od (when exp(log T) >= 1e-6) adds   */
+/* into tet_err[16 t + k]: 0 T_val, 1 T_val pe, 2 T_val pe^2, 5 T_val se, */
+/* 6-8 T_val entry, 9-11 T_val exit, 12 T_val, 13-15 T_val centroid        */
+/* (3, 4 stay 0); tet_count[2 t] += 1, tet_count[2 t + 1] = max(...,       */
+/* i32(65535 T_val)); then log T -= od.  Outputs accumulate.              */
+/* ------------------------------------------------------------------ */
+void FN(orc_error)(const R* verts, const uint32_t* idx, const R* dens, const uint32_t* sorted_vals,
+                   const uint32_t* ranges, const float* cam_pos, const float* vp, const float* c2w_intr, uint32_t w,
+                   uint32_t h, uint32_t ts, float min_t, const R* pixel_err, const R* ssim_err, R* tet_err,
+                   int32_t* tet_count) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, min_t);
+    uint32_t tiles_x = (w + ts - 1) / ts, tiles_y = (h + ts - 1) / ts;
+    uint32_t TS = ts;
+    R* log_t = (R*)calloc((size_t)TS * TS, sizeof(R));
+    V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
+    for (uint32_t tile = 0; tile < tiles_x * tiles_y; ++tile) {
+        uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
+        R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
+        memset(log_t, 0, sizeof(R) * TS * TS);
+        uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+        uint32_t cursor = re;
+        while (cursor > rs) {
+            cursor -= 1;
+            uint32_t t = sorted_vals[cursor];
+            V3 v[4];
+            for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+            R proj[4][2];
+            int any_behind = FN(tile_proj)(&c, v, tile_ox, tile_oy, proj);
+            V3 cen = FN(scl)(FN(add)(FN(add)(FN(add)(v[0], v[1]), v[2]), v[3]), (R)0.25);
+            V3 fn[4];
+            for (int fi = 0; fi < 4; ++fi) {
+                const int* f = FN(FACES)[fi];
+                V3 n = FN(cross)(FN(sub)(v[f[2]], v[f[0]]), FN(sub)(v[f[1]], v[f[0]]));
+                if (FN(dot)(n, FN(sub)(v[f[3]], v[f[0]])) < 0) n = FN(neg)(n);
+                fn[fi] = n;
+            }
+            for (uint32_t row = 0; row < TS; ++row) {
+                int32_t xl_i = 0, xr_i = -1;
+                if (!any_behind) FN(row_span)(proj, row, TS, 1, &xl_i, &xr_i);
+                for (int32_t col_i = xl_i; col_i <= xr_i; ++col_i) {
+                    uint32_t colu = (uint32_t)col_i;
+                    uint32_t pl = row * TS + colu;
+                    uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+                    if (px >= w || py >= h) continue;
+                    if (log_t[pl] < (R)-5.5) continue;
+                    V3 rd = FN(pixel_ray)(&c, px, py);
+                    V3 ro = FN(add)(cam, FN(scl)(rd, c.min_t));
+                    R t_min = (R)-3.402823e38f, t_max = (R)3.402823e38f;
+                    int valid = 1;
+                    for (int fi = 0; fi < 4; ++fi) {
+                        const int* f = FN(FACES)[fi];
+                        R num = FN(dot)(fn[fi], FN(sub)(v[f[0]], ro));
+                        R den = FN(dot)(fn[fi], rd);
+                        if (RABS(den) < (R)1e-20f) { if (num > 0) valid = 0; continue; }
+                        R tt = num / den;
+                        if (den > 0) { if (tt > t_min) t_min = tt; }
+                        else { if (tt < t_max) t_max = tt; }
+                    }
+                    if (!(valid && t_min < t_max)) continue;
+                    R od = FN(clampr)(dens[t] * (t_max - t_min), (R)1e-8f, (R)88.0);
+                    R trans = FN(safe_exp)(log_t[pl]);
+                    if (trans < (R)1e-6f) continue;
+                    R tv = trans * od;
+                    V3 en = FN(add)(ro, FN(scl)(rd, t_min)), ex = FN(add)(ro, FN(scl)(rd, t_max));
+                    size_t pix = (size_t)py * w + px;
+                    R pe = pixel_err[pix], se = ssim_err[pix];
+                    R* e = tet_err + 16 * (size_t)t;
+                    e[0] += tv; e[1] += tv * pe; e[2] += tv * pe * pe; e[5] += tv * se;
+                    e[6] += tv * en.x; e[7] += tv * en.y; e[8] += tv * en.z;
+                    e[9] += tv * ex.x; e[10] += tv * ex.y; e[11] += tv * ex.z;
+                    e[12] += tv; e[13] += tv * cen.x; e[14] += tv * cen.y; e[15] += tv * cen.z;
+                    tet_count[2 * (size_t)t] += 1;
+                    int32_t mx = sat_i32((double)((R)65535.0 * tv));
+                    if (mx > tet_count[2 * (size_t)t + 1]) tet_count[2 * (size_t)t + 1] = mx;
+                    log_t[pl] = log_t[pl] - od;
+                }
+            }
+        }
+    }
+    free(log_t);
+}
+
+/* ------------------------------------------------------------------ */
 /* K14: per-pixel loss gradient (+ loss value, an extension: the         */
 /* reference never writes its loss buffer).                             */
 /* crates/rmesh-train/src/wgsl/loss_compute.wgsl:24-72                  */

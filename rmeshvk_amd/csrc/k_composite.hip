@@ -1493,4 +1493,230 @@ bool launch_backward(const BwdArgs& a, uint32_t tile_size, int variant, hipStrea
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Per-tet error statistics for densification (forward_error):
+//   crates/rmesh-error/src/wgsl/error_tiled_compute.wgsl:88-373
+// The forward's tiled traversal on the last forward's tile lists (one view):
+// batches of B tets front to back, the forward's spans and slab test.  Phase A
+// (record-parallel) keeps each covered pair's slab and optical depth; phase B
+// (pixel-sequential, depth order) applies the -5.5 log-transmittance cut-off
+// and turns a record into its weight T_val = exp(log T) od (0: no
+// contribution); phase C (record-parallel) forms the 14 weighted moments and
+// the pixel count, sums them per tet with the segmented DPP scan, and the batch
+// adds one atomic per statistic per (tile, tet); the max weight goes through an
+// LDS atomicMax.
+// ---------------------------------------------------------------------------
+constexpr int kErrStats = 15;  // tet_err columns 0 1 2 5 6..15 (14), then the pixel count
+__device__ __forceinline__ int err_column(int q) { return q < 3 ? q : q + 2; }
+
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_error(const float4* __restrict__ rec, const uint32_t* __restrict__ sorted_vals,
+                                                 const uint32_t* __restrict__ ranges,
+                                                 const ViewParams* __restrict__ views, const float* __restrict__ pixel_err,
+                                                 const float* __restrict__ ssim_err, float* __restrict__ tet_err,
+                                                 int32_t* __restrict__ tet_count) {
+    constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, SP = TS + 1;
+    static_assert(C::R == 1 && B <= 64 && CAP % 32 == 0 && CAP / 32 <= NT && CAP >= TS * TS, "error pass geometry");
+    __shared__ BatchLDS<TS, B, false> L;
+    __shared__ float s_r[3][CAP];  // t_min, t_max, optical depth -> weight (0: none)
+    __shared__ float s_ray[3][TS * TS];
+    __shared__ float s_pe[TS * TS], s_se[TS * TS];
+    __shared__ uint32_t s_live[TS * TS];            // inside the image, log T >= -5.5
+    __shared__ unsigned long long s_mask[TS * TS];  // covered tets of the batch per pixel
+    __shared__ chunk_t<1> s_chunk[chunk_capacity(CAP, 1, B)];
+    __shared__ uint32_t s_start[CAP / 32];
+    __shared__ float4 s_cen[B];
+    __shared__ uint32_t s_tid[B];  // tet ids of the batch (the forward's LDS image has none)
+    __shared__ float s_acc[B][kErrStats];
+    __shared__ int32_t s_max[B];
+
+    const uint32_t gtile = blockIdx.x;
+    const ViewParams& vw = views[0];
+    const TileCtx tc = tile_ctx(vw, gtile);
+    const int tid = threadIdx.x, wid = tid / 64, lane = (int)__lane_id();
+    const int row = tid / TS, colp = tid % TS;
+    const uint32_t px = tc.tile_x * TS + colp, py = tc.tile_y * TS + row;
+    const bool active = tid < TS * TS && px < vw.width && py < vw.height;
+    const float min_t = vw.min_t;
+    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
+
+    if (tid < TS * TS) {
+        V3 rd = mk3(0.f, 0.f, 1.f);
+        float pe = 0.f, se = 0.f;
+        if (active) {
+            rd = pixel_ray(vw, px, py);
+            const uint64_t pix = (uint64_t)py * vw.width + px;
+            pe = pixel_err[pix];
+            se = ssim_err[pix];
+        }
+        s_ray[0][tid] = rd.x;
+        s_ray[1][tid] = rd.y;
+        s_ray[2][tid] = rd.z;
+        s_pe[tid] = pe;
+        s_se[tid] = se;
+        s_live[tid] = active ? 1u : 0u;
+    }
+    float log_t = 0.f;
+
+    const uint32_t rs = ranges[2ull * gtile], re = ranges[2ull * gtile + 1];
+    const uint32_t count = re > rs ? re - rs : 0u;
+    uint32_t done = 0;
+    int nb = (int)min<uint32_t>(B, count);
+    TetFetch f;
+    QuadFetch qf{};
+    if (C::PF == 1 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - tid], rec, nullptr, rec);
+    if (C::PF == 3 && tid < 4 * nb) fetch_quad<false>(qf, sorted_vals[re - 1 - (tid >> 2)], tid & 3, rec, nullptr, rec);
+    while (done < count) {
+        if (__syncthreads_and(!active || log_t < -5.5f)) break;
+        if constexpr (C::PF == 3) {
+            store_quad<TS, B, false>(L, qf, tid >> 2, tid & 3, tid < 4 * nb, vw, tc);
+            const V3 a = qperm3<0x00>(qf.r), b = qperm3<0x55>(qf.r), c = qperm3<0xAA>(qf.r), d = qperm3<0xFF>(qf.r);
+            const V3 cen = scl3(add3(add3(add3(a, b), c), d), 0.25f);
+            if (tid < 4 * nb && (tid & 3) == 0) {
+                s_cen[tid >> 2] = make_float4(cen.x, cen.y, cen.z, 0.0f);
+                s_tid[tid >> 2] = qf.t;
+            }
+        } else if (tid < nb) {
+            store_tet<TS, B, false>(L, tid, f, vw, tc);
+            const V3 a = mk3(f.r[0].x, f.r[0].y, f.r[0].z), b = mk3(f.r[1].x, f.r[1].y, f.r[1].z),
+                     c = mk3(f.r[2].x, f.r[2].y, f.r[2].z), d = mk3(f.r[3].x, f.r[3].y, f.r[3].z);
+            const V3 cen = scl3(add3(add3(add3(a, b), c), d), 0.25f);
+            s_cen[tid] = make_float4(cen.x, cen.y, cen.z, 0.0f);
+            s_tid[tid] = f.t;
+        }
+        if (tid < TS * TS) s_mask[tid] = 0ull;
+        if (tid < CAP / 32) s_start[tid] = 0u;
+        for (int i = tid; i < B * kErrStats; i += NT) (&s_acc[0][0])[i] = 0.0f;
+        if (tid < B) s_max[tid] = 0;
+        __syncthreads();
+        batch_layout<TS, B, false, true, CAP, 1, NT>(L, nb, s_chunk, s_start);
+        __syncthreads();
+        const uint32_t nbp = L.nbp & 0xFFu;
+        const uint32_t nrec = L.nbp >> 8;
+        const uint32_t next = done + nbp;
+        const int nb_next = (int)min<uint32_t>(B, count - next);
+        if (C::PF == 1 && tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, rec);
+        if (C::PF == 3 && tid < 4 * nb_next)
+            fetch_quad<false>(qf, sorted_vals[re - 1 - next - (tid >> 2)], tid & 3, rec, nullptr, rec);
+        // phase A: slab and optical depth of every covered pair
+        for (uint32_t j = tid; j < nrec; j += NT) {
+            const ChunkRef c = record_at(s_chunk, s_start, j);
+            const int p = c.row * TS + c.col;
+            if (s_live[p]) {
+                TetRegs T;
+                tet_regs(L, c.k, T);
+                const V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
+                float t_min, t_max;
+                int fmin, fmax;
+                if (intersect(T, rd, min_t, t_min, t_max, fmin, fmax)) {
+                    s_r[0][j] = t_min;
+                    s_r[1][j] = t_max;
+                    s_r[2][j] = med_clip(T.dens * (t_max - t_min), 1e-8f, 88.0f);
+                    atomicOr(&s_mask[p], 1ull << c.k);
+                }
+            }
+        }
+        __syncthreads();
+        // phase B: depth order per pixel; weight or 0 for every covered pair
+        if (active) {
+            unsigned long long m = s_mask[tid];
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                const uint32_t slot = (L.rpref[k * SP + row] + (uint32_t)colp) & 0xFFFFu;
+                float tv = 0.0f;
+                if (log_t >= -5.5f) {
+                    const float trans = safe_exp(log_t);
+                    if (trans >= 1e-6f) {
+                        const float od = s_r[2][slot];
+                        tv = trans * od;
+                        log_t = log_t - od;
+                    }
+                }
+                s_r[2][slot] = tv;
+            }
+            if (log_t < -5.5f) s_live[tid] = 0u;
+        }
+        __syncthreads();
+        // phase C: moments of the weighted pairs, summed per tet
+        for (uint32_t base = (uint32_t)wid * 64; base < nrec; base += NT) {
+            const uint32_t j = base + lane;
+            float acc[kErrStats];
+#pragma unroll
+            for (int q = 0; q < kErrStats; ++q) acc[q] = 0.0f;
+            int key = 0x7fffffff;
+            if (j < nrec) {
+                const ChunkRef c = record_at(s_chunk, s_start, j);
+                key = c.k;
+                const int p = c.row * TS + c.col;
+                const float tv = (s_mask[p] >> c.k) & 1ull ? s_r[2][j] : 0.0f;
+                if (tv > 0.0f) {
+                    const V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
+                    const V3 ro = add3(cam, scl3(rd, min_t));
+                    const V3 en = add3(ro, scl3(rd, s_r[0][j])), ex = add3(ro, scl3(rd, s_r[1][j]));
+                    const float pe = s_pe[p], se = s_se[p];
+                    const float4 cq = s_cen[c.k];
+                    acc[0] = tv;
+                    acc[1] = tv * pe;
+                    acc[2] = (tv * pe) * pe;
+                    acc[3] = tv * se;
+                    acc[4] = tv * en.x;
+                    acc[5] = tv * en.y;
+                    acc[6] = tv * en.z;
+                    acc[7] = tv * ex.x;
+                    acc[8] = tv * ex.y;
+                    acc[9] = tv * ex.z;
+                    acc[10] = tv;
+                    acc[11] = tv * cq.x;
+                    acc[12] = tv * cq.y;
+                    acc[13] = tv * cq.z;
+                    acc[14] = 1.0f;
+                    atomicMax(&s_max[c.k], sat_i32(65535.0f * tv));
+                }
+            }
+            const SegMasksDpp sm = seg_masks_dpp(key);
+            const int next_key = dpp_i<0x130, 0xF>(key, -1);  // wave_shl:1 (lane 63 reads -1)
+            const bool tail = key != 0x7fffffff && next_key != key;
+            seg_scan_dpp<kErrStats>(acc, sm);
+#pragma unroll
+            for (int q = 0; q < kErrStats; ++q)
+                if (tail && acc[q] != 0.0f) atomicAdd(&s_acc[key][q], acc[q]);
+        }
+        __syncthreads();
+        // flush: one atomic per non-zero statistic per (tile, tet)
+        for (int it = tid; it < (int)nbp * (kErrStats + 1); it += NT) {
+            const int k = it / (kErrStats + 1), q = it % (kErrStats + 1);
+            const uint32_t t = s_tid[k];
+            if (q < kErrStats - 1) {
+                const float v = s_acc[k][q];
+                if (v != 0.0f) unsafeAtomicAdd(tet_err + 16ull * t + err_column(q), v);
+            } else if (q == kErrStats - 1) {
+                const int32_t n = (int32_t)s_acc[k][q];
+                if (n) atomicAdd(tet_count + 2ull * t, n);
+            } else if (s_max[k] > 0) {
+                atomicMax(tet_count + 2ull * t + 1, s_max[k]);
+            }
+        }
+        done = next;
+        nb = nb_next;
+    }
+}
+
+template <int TS>
+static void err_launch(const ErrArgs& a, hipStream_t s) {
+    using C = FwdCfg<TS, 32, 512, 1, 3>;
+    hipLaunchKernelGGL((k_error<C>), dim3(a.num_tiles), dim3(C::NT), 0, s, a.rec, a.sorted_vals, a.ranges, a.views,
+                       a.pixel_err, a.ssim_err, a.tet_err, a.tet_count);
+}
+
+bool launch_error(const ErrArgs& a, uint32_t tile_size, hipStream_t s) {
+    switch (tile_size) {
+        case 8: err_launch<8>(a, s); return true;
+        case 12: err_launch<12>(a, s); return true;
+        case 16: err_launch<16>(a, s); return true;
+        default: return false;
+    }
+}
+
 }  // namespace rmesh

@@ -91,6 +91,9 @@ struct rmesh_renderer {
     // images
     float *rgba = nullptr, *aux = nullptr, *dl = nullptr, *gt = nullptr, *loss_partial = nullptr, *loss_out = nullptr;
     bool aux_alloc = false;
+    // forward_error inputs and outputs (allocated on first use)
+    float *err_pe = nullptr, *err_se = nullptr, *tet_err = nullptr;
+    int32_t* tet_cnt = nullptr;
     // last forward
     uint32_t last_views = 0;
     bool have_forward = false;
@@ -419,7 +422,8 @@ void free_all(rmesh_renderer* h) {
                     h->m_g, h->v_g, h->m_sh, h->v_sh, h->m_b, h->v_b, h->d_verts, h->d_dens, h->d_grads, h->d_colors,
                     h->d_sh, h->rec, h->colors, h->d_views, h->counts, h->block_sums, h->vis_count, h->n_pairs,
                     h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a, h->vals_b, h->hist, h->ranges, h->rgba,
-                    h->aux, h->dl, h->gt, h->loss_partial, h->loss_out};
+                    h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
+                    h->tet_cnt};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -591,6 +595,42 @@ extern "C" rmesh_status rmesh_forward_tiled(rmesh_renderer* h, const float* cam,
 extern "C" rmesh_status rmesh_forward(rmesh_renderer* h, const float* cam, const float* vp, const float* ci,
                                       float* out_rgba) {
     return rmesh_forward_tiled(h, cam, vp, ci, out_rgba, nullptr);
+}
+
+extern "C" rmesh_status rmesh_forward_error(rmesh_renderer* h, const float* pixel_err, const float* ssim_err,
+                                            float* tet_err, int32_t* tet_count) {
+    if (!h || !pixel_err || !ssim_err || !tet_err || !tet_count) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->have_forward || h->last_views != 1)
+        return fail(RMESH_ERR_INVALID, "forward_error requires a preceding forward_tiled");
+    DeviceGuard g(h->device);
+    hipStream_t s = h->stream;
+    const size_t npx = (size_t)h->W * h->H;
+    if (!h->tet_err) {
+        HIP_TRY(dalloc(&h->err_pe, npx));
+        HIP_TRY(dalloc(&h->err_se, npx));
+        HIP_TRY(dalloc(&h->tet_err, 16ull * h->M));
+        HIP_TRY(dalloc(&h->tet_cnt, 2ull * h->M));
+    }
+    HIP_TRY(hipMemcpyAsync(h->err_pe, pixel_err, npx * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(h->err_se, ssim_err, npx * 4, hipMemcpyHostToDevice, s));
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->tet_err), 0u, 16ull * h->M, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->tet_cnt), 0u, 2ull * h->M, s);
+    ErrArgs a;
+    a.rec = h->rec;
+    a.sorted_vals = sorted_vals(h);
+    a.ranges = h->ranges;
+    a.views = h->d_views;
+    a.num_tiles = h->T;
+    a.pixel_err = h->err_pe;
+    a.ssim_err = h->err_se;
+    a.tet_err = h->tet_err;
+    a.tet_count = h->tet_cnt;
+    if (!launch_error(a, h->TS, s)) return fail(RMESH_ERR_UNSUPPORTED, "tile size");
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(tet_err, h->tet_err, 64ull * h->M, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(tet_count, h->tet_cnt, 8ull * h->M, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RMESH_OK;
 }
 
 extern "C" rmesh_status rmesh_read_instance_count(rmesh_renderer* h, uint32_t* out) {

@@ -47,6 +47,20 @@ constexpr int kFwdVariants = 8;
 constexpr int kBwdVariants = 6;
 bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
 
+// Per-tet error statistics (forward_error) over one view's tile lists.
+struct ErrArgs {
+    const float4* rec;
+    const uint32_t* sorted_vals;
+    const uint32_t* ranges;
+    const ViewParams* views;  // views[0]
+    uint32_t num_tiles;
+    const float* pixel_err;   // [H][W]
+    const float* ssim_err;    // [H][W]
+    float* tet_err;           // [M][16], accumulated
+    int32_t* tet_count;       // [M][2], accumulated (count, max of i32(65535 T_val))
+};
+bool launch_error(const ErrArgs& a, uint32_t tile_size, hipStream_t s);
+
 struct BwdArgs {
     const float4* rec;
     const uint32_t* idx;

@@ -22,7 +22,7 @@ from . import _lib
 
 _OUT_OF_SCOPE = (
     "forward_interval_tiled", "backward_interval_tiled", "forward_raytrace", "backward_raytrace",
-    "backward_hw", "forward_error", "locate_tets", "read_aux", "set_vertex_normals",
+    "backward_hw", "locate_tets", "read_aux", "set_vertex_normals",
     "get_vertex_normals", "set_aux_data", "get_aux_data",
 )
 
@@ -146,6 +146,21 @@ class RMeshRenderer:
         out = ctypes.c_uint32()
         _lib.check(self._lib.rmesh_read_instance_count(self._h, ctypes.byref(out)))
         return int(out.value)
+
+    # ---- forward_error (lib.rs:1621-1711) ----
+    def forward_error(self, pixel_err, ssim_err):
+        """Per-tet error statistics of the last forward_tiled (densification):
+        returns (tet_err float32 [M, 16], tet_count int32 [M, 2])."""
+        n = self._hgt * self._w
+        for name, a in (("pixel_err", pixel_err), ("ssim_err", ssim_err)):
+            if isinstance(a, np.ndarray) and a.dtype == np.float32 and a.size != n:
+                raise ValueError(f"{name} must have {n} elements, got {a.size}")
+        _f32("pixel_err", pixel_err, n)
+        _f32("ssim_err", ssim_err, n)
+        err = np.empty((self._m, 16), dtype=np.float32)
+        cnt = np.empty((self._m, 2), dtype=np.int32)
+        _lib.check(self._lib.rmesh_forward_error(self._h, _p(pixel_err), _p(ssim_err), _p(err), _p(cnt)))
+        return err, cnt
 
     # ---- backward (lib.rs:1894-2031) ----
     def backward(self, dl_d_image):
