@@ -149,6 +149,7 @@ def main():
     ap.add_argument("--tuning", default="", help="TS,F,B: tile size and kernel variants, no autotune")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true", help="skip the forward_error (densification statistics) pass")
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB (HBM counter calibration)")
@@ -237,6 +238,27 @@ def main():
                  "target upload included", "speedup_batched_graph": value / (V / tpl)}
         log(f"plain per-view path: {V / tpl:.1f} views/s")
 
+    err_pass = None
+    if not args.no_error:
+        # per-tet error statistics of one view (outside the timed step: a
+        # densification-time call; the synchronous C ABI copies the two error
+        # images in and the [M, 16] + [M, 2] statistics out)
+        r.set_graph_mode(False)
+        r.set_timing(False)
+        r.forward_tiled(*cams[0])
+        erng = np.random.default_rng(7)
+        pe = erng.random(Wd * Wd, dtype=np.float32)
+        se = erng.random(Wd * Wd, dtype=np.float32)
+        times = []
+        for _ in range(3):
+            r.synchronize()
+            te = time.perf_counter()
+            _, cnt = r.forward_error(pe, se)
+            times.append(time.perf_counter() - te)
+        err_pass = {"ms": round(1000.0 * min(times), 3), "tets_with_stats": int((cnt[:, 0] > 0).sum()),
+                    "note": "forward_error on view 0, host copies included (kernel time: profiles/)"}
+        log(f"forward_error: {err_pass['ms']:.2f} ms (view 0, {err_pass['tets_with_stats']} tets with statistics)")
+
     cpu = None
     if not args.no_cpu:
         tc = time.perf_counter()
@@ -282,6 +304,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_per_view": plain,
+        "forward_error": err_pass,
     }
     if copy_probe:
         out["copy_probe"] = copy_probe

@@ -374,6 +374,42 @@ void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewP
     hipLaunchKernelGGL(k_pairs, dim3(project_blocks(M, V)), dim3(256), 0, s, rec, circ, M, views, V, counts, block_off,
                        keys, vals, cap, overflow);
 }
+// Tiles heaviest first: the compositing grids take their tiles in this order,
+// so the few very dense tiles (tens of times the median pair count) start
+// early instead of running alone at the end of the grid (longest-processing-
+// time-first).  A bucket sort by bit length of the pair count, descending;
+// the order inside a bucket is whatever the LDS atomics give (every tile's
+// result is independent of when it runs).
+__device__ __forceinline__ uint32_t tile_bucket(const uint32_t* __restrict__ ranges, uint32_t t) {
+    const uint32_t rs = ranges[2ull * t], re = ranges[2ull * t + 1];
+    const uint32_t c = re > rs ? re - rs : 0u;
+    return c ? 32u - (uint32_t)__clz((int)c) : 0u;
+}
+
+__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ ranges, uint32_t n,
+                                                      uint32_t* __restrict__ order) {
+    __shared__ uint32_t cnt[33];
+    __shared__ uint32_t off[33];
+    const uint32_t tid = threadIdx.x;
+    if (tid < 33) cnt[tid] = 0u;
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += blockDim.x) atomicAdd(&cnt[tile_bucket(ranges, t)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int b = 32; b >= 0; --b) {
+            off[b] = run;
+            run += cnt[b];
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += blockDim.x) order[atomicAdd(&off[tile_bucket(ranges, t)], 1u)] = t;
+}
+
+void launch_tile_order(const uint32_t* ranges, uint32_t n_tiles, uint32_t* order, hipStream_t s) {
+    if (n_tiles) hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, ranges, n_tiles, order);
+}
+
 void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges, hipStream_t s) {
     uint32_t blocks = (cap + 255) / 256;
     if (blocks > 8192) blocks = 8192;

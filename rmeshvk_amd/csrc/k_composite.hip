@@ -733,7 +733,8 @@ template <class C, bool AUX>
 __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_forward(
     const float4* __restrict__ rec, const float4* __restrict__ colors, uint64_t color_view_stride,
     const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges, const ViewParams* __restrict__ views,
-    uint32_t tiles_per_view, float* __restrict__ rgba, float* __restrict__ aux) {
+    uint32_t tiles_per_view, const uint32_t* __restrict__ tile_order, float* __restrict__ rgba,
+    float* __restrict__ aux) {
     constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1;
     constexpr int NA = AUX ? CAP : 1;
     static_assert(B <= 64, "pixel masks hold 64 tets; the layout scan is one wave");
@@ -751,7 +752,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
     static_assert(R != 1 || (CAP % 32 == 0 && CAP / 32 <= NT), "segment start mask");
 
-    const uint32_t gtile = blockIdx.x;
+    const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
     const uint32_t view = gtile / tiles_per_view;
     const ViewParams& vw = views[view];
     const TileCtx tc = tile_ctx(vw, gtile);
@@ -1144,8 +1145,8 @@ template <class C>
 __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_backward(
     const float4* __restrict__ rec, const uint32_t* __restrict__ idx, const float4* __restrict__ colors,
     uint64_t color_view_stride, const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges,
-    const ViewParams* __restrict__ views, uint32_t tiles_per_view, const float* __restrict__ rgba,
-    const float* __restrict__ dl_img, const float* __restrict__ gt, uint32_t loss_type,
+    const ViewParams* __restrict__ views, uint32_t tiles_per_view, const uint32_t* __restrict__ tile_order,
+    const float* __restrict__ rgba, const float* __restrict__ dl_img, const float* __restrict__ gt, uint32_t loss_type,
     float* __restrict__ loss_partial, float* __restrict__ d_verts, float* __restrict__ d_dens,
     float* __restrict__ d_grads, float* __restrict__ d_colors, uint64_t dcolor_view_stride) {
     constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1, NW = NT / 64;
@@ -1167,7 +1168,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     static_assert(R != 1 || (CAP % 32 == 0 && CAP / 32 <= NT), "segment start mask");
     static_assert(CAP >= TS * TS, "one tet's pixels of a tile must fit a batch");
 
-    const uint32_t gtile = blockIdx.x;
+    const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
     const uint32_t view = gtile / tiles_per_view;
     const ViewParams& vw = views[view];
     const TileCtx tc = tile_ctx(vw, gtile);
@@ -1434,10 +1435,12 @@ template <class C>
 static void fwd_launch(const FwdArgs& a, hipStream_t s) {
     if (a.aux)
         hipLaunchKernelGGL((k_forward<C, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
-                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.aux);
+                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.tile_order,
+                           a.rgba, a.aux);
     else
         hipLaunchKernelGGL((k_forward<C, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
-                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.aux);
+                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.tile_order,
+                           a.rgba, a.aux);
 }
 
 template <int TS>
@@ -1467,8 +1470,8 @@ bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream
 template <class C>
 static void bwd_launch(const BwdArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_backward<C>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
-                       a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.rgba, a.dl_img, a.gt,
-                       a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads, a.d_colors, a.dcolor_view_stride);
+                       a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.tile_order, a.rgba,
+                       a.dl_img, a.gt, a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads, a.d_colors, a.dcolor_view_stride);
 }
 
 template <int TS>

@@ -87,6 +87,7 @@ struct rmesh_renderer {
     uint32_t cap = 0;
     uint32_t* hist = nullptr;
     uint32_t* ranges = nullptr;
+    uint32_t* tile_order = nullptr;  // compositing workgroup -> global tile, heaviest first
     bool sorted_in_b = false;
     // images
     float *rgba = nullptr, *aux = nullptr, *dl = nullptr, *gt = nullptr, *loss_partial = nullptr, *loss_out = nullptr;
@@ -246,6 +247,7 @@ rmesh_status stage_binning_back(rmesh_renderer* h, uint32_t V) {
                                        h->sorting_bits(V), h->hist, s);
     launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
     launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
+    launch_tile_order(h->ranges, h->gtiles(V), h->tile_order, s);
     return RMESH_OK;
 }
 
@@ -263,6 +265,7 @@ rmesh_status stage_forward(rmesh_renderer* h, uint32_t V, bool want_aux) {
     a.views = h->d_views;
     a.tiles_per_view = h->T;
     a.num_gtiles = h->gtiles(V);
+    a.tile_order = h->tile_order;
     a.rgba = h->rgba;
     a.aux = want_aux ? h->aux : nullptr;
     if (!launch_forward(a, h->TS, h->fwd_var, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
@@ -288,6 +291,7 @@ rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, cons
     a.views = h->d_views;
     a.tiles_per_view = h->T;
     a.num_gtiles = h->gtiles(V);
+    a.tile_order = h->tile_order;
     a.rgba = h->rgba;
     a.dl_img = dl;
     a.gt = gt;
@@ -397,6 +401,7 @@ rmesh_status record_stage(rmesh_renderer* h, int stage, uint32_t V, const float*
         case 3:
             launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
             launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
+            launch_tile_order(h->ranges, h->gtiles(V), h->tile_order, s);
             return RMESH_OK;
         case 4: return stage_forward(h, V, false);
         case 5: return stage_backward(h, V, nullptr, gt, loss_type);
@@ -423,7 +428,7 @@ void free_all(rmesh_renderer* h) {
                     h->d_sh, h->rec, h->colors, h->d_views, h->counts, h->block_sums, h->vis_count, h->n_pairs,
                     h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a, h->vals_b, h->hist, h->ranges, h->rgba,
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
-                    h->tet_cnt};
+                    h->tet_cnt, h->tile_order};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -521,6 +526,7 @@ extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer**
     CREATE_TRY(dalloc(&h->d_step, 1));
     CREATE_TRY(hipMemset(h->d_step, 0, 4));
     CREATE_TRY(dalloc(&h->ranges, 2ull * h->T_alloc * V));
+    CREATE_TRY(dalloc(&h->tile_order, (size_t)h->T_alloc * V));
     size_t npx = (size_t)h->W * h->H;
     CREATE_TRY(dalloc(&h->rgba, 4 * npx * V));
     CREATE_TRY(dalloc(&h->dl, 4 * npx));

@@ -23,6 +23,8 @@ void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewP
                   const uint32_t* counts, const uint32_t* block_off, uint64_t* keys, uint32_t* vals, uint32_t cap,
                   uint32_t* overflow, hipStream_t s);
 void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges, hipStream_t s);
+// Permutation of the global tiles, heaviest (most pairs) first.
+void launch_tile_order(const uint32_t* ranges, uint32_t n_tiles, uint32_t* order, hipStream_t s);
 
 // sort (k_sort.hip)
 bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
@@ -39,6 +41,7 @@ struct FwdArgs {
     const ViewParams* views;
     uint32_t tiles_per_view;
     uint32_t num_gtiles;
+    const uint32_t* tile_order;  // nullable: workgroup -> global tile (identity when null)
     float* rgba;
     float* aux;  // nullable: [V][H][W][8]
 };
@@ -71,6 +74,7 @@ struct BwdArgs {
     const ViewParams* views;
     uint32_t tiles_per_view;
     uint32_t num_gtiles;
+    const uint32_t* tile_order;  // nullable, as FwdArgs::tile_order
     const float* rgba;
     const float* dl_img;      // used when gt == nullptr
     const float* gt;          // [V][H][W][3] or nullptr
