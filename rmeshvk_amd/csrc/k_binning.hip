@@ -377,27 +377,31 @@ void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewP
 // Tiles heaviest first: the compositing grids take their tiles in this order,
 // so the few very dense tiles (tens of times the median pair count) start
 // early instead of running alone at the end of the grid (longest-processing-
-// time-first).  A bucket sort by bit length of the pair count, descending;
-// the order inside a bucket is whatever the LDS atomics give (every tile's
-// result is independent of when it runs).
+// time-first).  A bucket sort by the pair count's bit length and its next two
+// bits (four buckets per octave), descending; the order inside a bucket is
+// whatever the LDS atomics give (every tile's result is independent of when
+// it runs).
+constexpr int kOrderBuckets = 33 * 4;
 __device__ __forceinline__ uint32_t tile_bucket(const uint32_t* __restrict__ ranges, uint32_t t) {
     const uint32_t rs = ranges[2ull * t], re = ranges[2ull * t + 1];
     const uint32_t c = re > rs ? re - rs : 0u;
-    return c ? 32u - (uint32_t)__clz((int)c) : 0u;
+    if (c < 4u) return c;
+    const uint32_t len = 32u - (uint32_t)__clz((int)c);  // >= 3
+    return len * 4u + ((c >> (len - 3u)) & 3u);           // 3 leading bits: 1xx
 }
 
 __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ ranges, uint32_t n,
                                                       uint32_t* __restrict__ order) {
-    __shared__ uint32_t cnt[33];
-    __shared__ uint32_t off[33];
+    __shared__ uint32_t cnt[kOrderBuckets];
+    __shared__ uint32_t off[kOrderBuckets];
     const uint32_t tid = threadIdx.x;
-    if (tid < 33) cnt[tid] = 0u;
+    if (tid < kOrderBuckets) cnt[tid] = 0u;
     __syncthreads();
     for (uint32_t t = tid; t < n; t += blockDim.x) atomicAdd(&cnt[tile_bucket(ranges, t)], 1u);
     __syncthreads();
     if (tid == 0) {
         uint32_t run = 0;
-        for (int b = 32; b >= 0; --b) {
+        for (int b = kOrderBuckets - 1; b >= 0; --b) {
             off[b] = run;
             run += cnt[b];
         }
