@@ -93,6 +93,28 @@ def profiled_traffic(kernel, tile_size):
             "source": "profiles/" + best[4]}
 
 
+def profiled_valu(kernel, tile_size):
+    """VALU issue rate and lane utilisation of `kernel` at this tile size from
+    the committed rocprofv3 SQ-counter summary (profiles/*_valu_lds.md); None
+    when not profiled."""
+    import glob
+    import re
+
+    best = None
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_valu_lds.md"))):
+        for line in open(path):
+            m = re.match(r"\| `(.+?)` \| (\d+) \| ([\d.]+) \| [\d.]+ \| [\d.]+ \| (\d+)% \| (\d+)% \|", line)
+            if not m or kernel not in m.group(1) or f"<{tile_size}," not in m.group(1).replace(" ", ""):
+                continue
+            n = int(m.group(2))
+            if best is None or n > best[0]:
+                best = (n, m.group(1), float(m.group(3)), int(m.group(4)), int(m.group(5)), os.path.basename(path))
+    if best is None:
+        return None
+    return {"issue": min(best[3], 100) / 100.0, "lane_util": best[4] / 100.0, "valu_instr_G": best[2],
+            "kernel": best[1], "source": "profiles/" + best[5]}
+
+
 def cpu_baseline(scene, cam, width, stride, tile_size):
     """Oracle (C, one thread) on one view: full binning, SH colours/adjoint and
     Adam over every parameter, forward/backward on every `stride`-th tile."""
@@ -216,6 +238,11 @@ def main():
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(prof["bytes"]) if prof else None,
                 "algorithmic_bytes": int(alg), "kernel_ms": round(stages[dom], 4), "pairs": P}
+    valu = profiled_valu(kname, tile_size)
+    if valu:
+        # the binding resource of the compositing kernels: SQ_INSTS_VALU x 4 cycles
+        # per SIMD over the GPU cycles (issue), and active lanes per VALU op
+        roofline["valu"] = valu
     if prof:
         roofline["traffic_detail"] = {k: prof[k] for k in ("read", "written", "kernel", "source")}
         roofline["traffic_note"] = ("FETCH_SIZE x2 + WRITE_SIZE x1 per launch (copy-kernel calibration); "
