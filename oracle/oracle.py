@@ -39,6 +39,21 @@ orc_sh_colors     basis == scipy real spherical harmonics (Condon-Shortley
                   phase), hence orthonormal; degree-0 closed form.
 orc_sh_backward   float64 finite differences, degrees 0-3.
 orc_circumspheres the regular-tet known answer; equidistance of the vertices.
+orc_interval_forward
+                  with near/far 0.01/1000 in the camera (the values the
+                  reference linearizes with), the image of single tets and
+                  two tets equals the float64 quadrature renderer (NDC z is
+                  affine over a planar face, so the linearized chord is the
+                  ray chord), depth x ray scale equals the quadrature's
+                  alpha-weighted chord depth; distortion moments of one
+                  tet against float64 quadrature of the weight
+                  sigma e^-sigma(t - t0) (m0, m1, self term) and of two tets
+                  (the cross term).  All with constant colour per tet.
+                  PARITY UNPINNED: the screen-space (not perspective-
+                  correct) barycentric interpolation of the colour offsets
+                  when g != 0 (interval_generate.wgsl:290-330,
+                  interval_tiled_rasterize.wgsl:327-330) has no independent
+                  check; only GPU vs this restatement.
 orc_error         closed forms on one tet (weight = sum of the pixels' optical
                   depths, which agree with the forward's alpha = 1 - e^-od;
                   weighted entry/exit points on the tet's planes; centroid
@@ -101,6 +116,8 @@ def lib():
             getattr(L, "orc_sh_backward" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp, rp, rp, rp]
             getattr(L, "orc_error" + sfx).argtypes = [rp, _u32p, rp, _u32p, _u32p, _f32p, _f32p, _f32p, _u32, _u32,
                                                        _u32, _f32, rp, rp, rp, _i32p]
+            getattr(L, "orc_interval_forward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p,
+                                                                  _f32p, _u32, _u32, _u32, rp, rp, rp]
         L.orc_sorting_bits.argtypes = [_u32]
         L.orc_sorting_bits.restype = _u32
         L.orc_sort.argtypes = [_u64p, _u32p, _u64, _u32]
@@ -218,6 +235,21 @@ def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, 
         _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
         w, h, ts, _a(rgba, dtype).ravel(), _a(dl, dtype).ravel(), t0, t1, tile_step, dV, dD, dG, dC)
     return {"d_vertices": dV, "d_densities": dD, "d_color_grads": dG, "d_base_colors": dC}
+
+
+def interval_forward(scene, cam, w, h, sorted_vals, rng_, ts=12, colors=None, dtype=np.float32):
+    """Interval-tiled forward over the regular binning's tile lists:
+    {"image" [h, w, 4], "xyzd" [h, w, 4], "distortion" [h, w, 5]}."""
+    cp, vp, ci = cam
+    img = np.zeros(h * w * 4, dtype=dtype)
+    xyzd = np.zeros(h * w * 4, dtype=dtype)
+    dist = np.zeros(h * w * 5, dtype=dtype)
+    col = scene["base_colors"] if colors is None else colors
+    getattr(lib(), "orc_interval_forward" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
+        _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
+        _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32), w, h, ts, img, xyzd, dist)
+    return {"image": img.reshape(h, w, 4), "xyzd": xyzd.reshape(h, w, 4), "distortion": dist.reshape(h, w, 5)}
 
 
 def error_stats(scene, cam, w, h, sorted_vals, rng_, pixel_err, ssim_err, ts=12, min_t=0.0, dtype=np.float32):

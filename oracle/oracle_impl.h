@@ -506,6 +506,275 @@ void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const
 }
 
 /* ------------------------------------------------------------------ */
+/* Interval-tiled forward (forward_interval_tiled, lib.rs:1218-1390):    */
+/* per-tet silhouette decomposition (interval_generate.wgsl:174-408)     */
+/* and tiled compositing over it (interval_tiled_rasterize.wgsl:96-441).  */
+/* Vertex normals are zero (set_vertex_normals is outside this path), so  */
+/* xyzd carries depth only.  near/far are the reference's 0.01 / 1000.    */
+/* Parity unpinned: the screen-space interpolation of the colour offsets  */
+/* for g != 0 (only the GPU is compared with it).                         */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    R px[5], py[5], zf[5], zb[5], of[5], ob[5]; /* slots 0-3 silhouette, 4 centre */
+    V3 base;                                   /* base colour at the camera */
+    R dens;
+    uint32_t nsil;                             /* 0: skipped */
+} FN(ivl_);
+#define IVL FN(ivl_)
+
+static const int FN(OPPF)[4] = {1, 2, 3, 0};
+static const int FN(TE)[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+static R FN(mixr)(R a, R b, R t) { return a * ((R)1.0 - t) + b * t; }
+
+/* interval_generate.wgsl:89-109 */
+static uint32_t FN(classify_sil)(const R p[4][2]) {
+    for (int v = 0; v < 4; ++v) {
+        const int* f = FN(FACES)[v];
+        R ax = p[f[0]][0], ay = p[f[0]][1], bx = p[f[1]][0], by = p[f[1]][1], cx = p[f[2]][0], cy = p[f[2]][1];
+        R tx = p[f[3]][0], ty = p[f[3]][1];
+        R d0 = (bx - ax) * (ty - ay) - (by - ay) * (tx - ax);
+        R d1 = (cx - bx) * (ty - by) - (cy - by) * (tx - bx);
+        R d2 = (ax - cx) * (ty - cy) - (ay - cy) * (tx - cx);
+        int all_pos = d0 >= 0 && d1 >= 0 && d2 >= 0;
+        int all_neg = d0 <= 0 && d1 <= 0 && d2 <= 0;
+        if (all_pos || all_neg) return (uint32_t)f[3];
+    }
+    return 4u;
+}
+
+/* interval_generate.wgsl:112-126 (also interval_tiled_rasterize.wgsl:74-87) */
+static void FN(bary2d)(R ptx, R pty, R ax, R ay, R bx, R by, R cx, R cy, R out[3]) {
+    R v0x = bx - ax, v0y = by - ay, v1x = cx - ax, v1y = cy - ay, v2x = ptx - ax, v2y = pty - ay;
+    R d00 = v0x * v0x + v0y * v0y, d01 = v0x * v1x + v0y * v1y, d11 = v1x * v1x + v1y * v1y;
+    R d20 = v2x * v0x + v2y * v0y, d21 = v2x * v1x + v2y * v1y;
+    R denom = d00 * d11 - d01 * d01;
+    R inv = RABS(denom) > (R)1e-10f ? (R)1.0 / denom : (R)0.0;
+    R v = (d11 * d20 - d01 * d21) * inv;
+    R w = (d00 * d21 - d01 * d20) * inv;
+    out[0] = (R)1.0 - v - w;
+    out[1] = v;
+    out[2] = w;
+}
+
+/* interval_generate.wgsl:129-139 */
+static R FN(line_t)(R a1x, R a1y, R a2x, R a2y, R b1x, R b1y, R b2x, R b2y) {
+    R d1x = a2x - a1x, d1y = a2y - a1y, d2x = b2x - b1x, d2y = b2y - b1y;
+    R denom = d1x * d2y - d1y * d2x;
+    if (RABS(denom) < (R)1e-10f) return (R)-1.0;
+    R dx = b1x - a1x, dy = b1y - a1y;
+    return (dx * d2y - dy * d2x) / denom;
+}
+
+static void FN(ivl_generate)(const CAM* c, const V3 v[4], V3 col, V3 g, R dens, IVL* o) {
+    memset(o, 0, sizeof(*o));
+    R clip[4][4];
+    for (int i = 0; i < 4; ++i) FN(mul_vp)(c->vp, v[i], clip[i]);
+    if (clip[0][3] <= 0 || clip[1][3] <= 0 || clip[2][3] <= 0 || clip[3][3] <= 0) return;
+    R W = (R)c->width, H = (R)c->height;
+    R p[4][2], nz[4];
+    for (int i = 0; i < 4; ++i) {
+        R w = clip[i][3];
+        R inv_w = RABS(w) > (R)1e-10f ? (R)1.0 / w : (R)0.0;
+        R nx = clip[i][0] * inv_w, ny = clip[i][1] * inv_w;
+        nz[i] = clip[i][2] * inv_w;
+        p[i][0] = ((nx + (R)1.0) * (R)0.5) * W;
+        p[i][1] = (((R)1.0 - ny) * (R)0.5) * H;
+    }
+    V3 cam = FN(mk)(c->cam[0], c->cam[1], c->cam[2]);
+    R bo = FN(dot)(g, FN(sub)(cam, v[0]));
+    o->base = FN(mk)(col.x + bo, col.y + bo, col.z + bo);
+    o->dens = dens;
+    R zmax = FN(rmax)(FN(rmax)(nz[0], nz[1]), FN(rmax)(nz[2], nz[3]));
+    R zmin = FN(rmin)(FN(rmin)(nz[0], nz[1]), FN(rmin)(nz[2], nz[3]));
+    if (zmax - zmin < (R)1e-7f || dens < (R)1e-6f) return;
+    R off[4];
+    for (int i = 0; i < 4; ++i) off[i] = FN(dot)(g, FN(sub)(v[i], cam));
+    uint32_t iv = FN(classify_sil)((const R(*)[2])p);
+    if (iv < 4u) {
+        const int* f = FN(FACES)[FN(OPPF)[iv]];
+        int si[3] = {f[0], f[1], f[2]}, ce = f[3];
+        R b[3];
+        FN(bary2d)(p[ce][0], p[ce][1], p[si[0]][0], p[si[0]][1], p[si[1]][0], p[si[1]][1], p[si[2]][0], p[si[2]][1], b);
+        R z_face = b[0] * nz[si[0]] + b[1] * nz[si[1]] + b[2] * nz[si[2]];
+        R z_c = nz[ce];
+        R face_off = b[0] * off[si[0]] + b[1] * off[si[1]] + b[2] * off[si[2]];
+        int face_first = z_face <= z_c;
+        for (int k = 0; k < 4; ++k) {
+            int vi = si[k == 3 ? 0 : k];
+            o->px[k] = p[vi][0]; o->py[k] = p[vi][1];
+            o->zf[k] = o->zb[k] = nz[vi];
+            o->of[k] = o->ob[k] = off[vi];
+        }
+        o->px[4] = p[ce][0]; o->py[4] = p[ce][1];
+        o->zf[4] = FN(rmin)(z_face, z_c); o->zb[4] = FN(rmax)(z_face, z_c);
+        o->of[4] = face_first ? face_off : off[ce];
+        o->ob[4] = face_first ? off[ce] : face_off;
+        o->nsil = 3u;
+    } else {
+        static const int PAIRS[3][2] = {{0, 5}, {1, 4}, {2, 3}};
+        int ea = 0, eb = 5;
+        for (int i = 0; i < 3; ++i) {
+            const int* A = FN(TE)[PAIRS[i][0]];
+            const int* Bv = FN(TE)[PAIRS[i][1]];
+            R ta = FN(line_t)(p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1], p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1]);
+            R tb = FN(line_t)(p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1], p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1]);
+            if (ta > 0 && ta < 1 && tb > 0 && tb < 1) { ea = PAIRS[i][0]; eb = PAIRS[i][1]; break; }
+        }
+        const int* A = FN(TE)[ea];
+        const int* Bv = FN(TE)[eb];
+        R ta = FN(line_t)(p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1], p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1]);
+        R tb = FN(line_t)(p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1], p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1]);
+        R cxp = FN(mixr)(p[A[0]][0], p[A[1]][0], ta), cyp = FN(mixr)(p[A[0]][1], p[A[1]][1], ta);
+        R z_a = FN(mixr)(nz[A[0]], nz[A[1]], ta), z_b = FN(mixr)(nz[Bv[0]], nz[Bv[1]], tb);
+        R off_a = FN(mixr)(off[A[0]], off[A[1]], ta), off_b = FN(mixr)(off[Bv[0]], off[Bv[1]], tb);
+        int sv[4] = {A[0], Bv[0], A[1], Bv[1]};
+        for (int k = 0; k < 4; ++k) {
+            o->px[k] = p[sv[k]][0]; o->py[k] = p[sv[k]][1];
+            o->zf[k] = o->zb[k] = nz[sv[k]];
+            o->of[k] = o->ob[k] = off[sv[k]];
+        }
+        o->px[4] = cxp; o->py[4] = cyp;
+        o->zf[4] = FN(rmin)(z_a, z_b); o->zb[4] = FN(rmax)(z_a, z_b);
+        o->of[4] = z_a <= z_b ? off_a : off_b;
+        o->ob[4] = z_a <= z_b ? off_b : off_a;
+        o->nsil = 4u;
+    }
+}
+
+/* interval_tiled_rasterize.wgsl:90-97 */
+static int FN(in_tri)(R ptx, R pty, R ax, R ay, R bx, R by, R cx, R cy) {
+    R d0 = (bx - ax) * (pty - ay) - (by - ay) * (ptx - ax);
+    R d1 = (cx - bx) * (pty - by) - (cy - by) * (ptx - bx);
+    R d2 = (ax - cx) * (pty - cy) - (ay - cy) * (ptx - cx);
+    int has_neg = d0 < 0 || d1 < 0 || d2 < 0;
+    int has_pos = d0 > 0 || d1 > 0 || d2 > 0;
+    return !(has_neg && has_pos);
+}
+
+/* rgba [H*W*4], xyzd [H*W*4], dist [H*W*5] (overwritten for the tiles visited). */
+void FN(orc_interval_forward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                              const uint32_t* sorted_vals, const uint32_t* ranges, const float* cam_pos,
+                              const float* vp, const float* c2w_intr, uint32_t w, uint32_t h, uint32_t ts,
+                              R* rgba, R* xyzd, R* dist) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    const R near_p = (R)0.01f, far_p = (R)1000.0f, range_nf = far_p - near_p;
+    uint32_t tiles_x = (w + ts - 1) / ts, tiles_y = (h + ts - 1) / ts, TS = ts;
+    R* st = (R*)calloc((size_t)TS * TS * 13, sizeof(R)); /* colour 3, log T, xyzd 4, dist 5 */
+    for (uint32_t tile = 0; tile < tiles_x * tiles_y; ++tile) {
+        uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
+        R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
+        memset(st, 0, sizeof(R) * TS * TS * 13);
+        uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+        uint32_t cursor = re;
+        while (cursor > rs) {
+            cursor -= 1;
+            uint32_t t = sorted_vals[cursor];
+            V3 v[4];
+            for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+            IVL iv;
+            FN(ivl_generate)(&c, v, FN(load3)(colors, t), FN(load3)(grads, t), dens[t], &iv);
+            if (iv.nsil == 0) continue;
+            uint32_t ns = iv.nsil;
+            for (uint32_t row = 0; row < TS; ++row) {
+                /* scanline over the silhouette polygon (interval_tiled_rasterize.wgsl:192-243) */
+                R xl_f = (R)1e10f, xr_f = (R)-1e10f, yc = (R)row + (R)0.5;
+                for (uint32_t e = 0; e < ns; ++e) {
+                    uint32_t ej = (e + 1) % ns;
+                    R yi = iv.py[e] - tile_oy, yj = iv.py[ej] - tile_oy;
+                    R xi = iv.px[e] - tile_ox, xj = iv.px[ej] - tile_ox;
+                    if ((yi <= yc && yj > yc) || (yj <= yc && yi > yc)) {
+                        R tt = (yc - yi) / (yj - yi);
+                        R x = xi + tt * (xj - xi);
+                        xl_f = FN(rmin)(xl_f, x); xr_f = FN(rmax)(xr_f, x);
+                    }
+                }
+                for (uint32_t k = 0; k < ns; ++k) {
+                    R vy = iv.py[k] - tile_oy;
+                    if (vy >= yc - (R)0.5 && vy < yc + (R)0.5) {
+                        xl_f = FN(rmin)(xl_f, iv.px[k] - tile_ox); xr_f = FN(rmax)(xr_f, iv.px[k] - tile_ox);
+                    }
+                }
+                if (!(xl_f <= xr_f)) continue;
+                R eps = (R)0.001f;
+                int32_t xl_i = imax32(sat_i32((double)RCEIL(xl_f - (R)0.5 - eps)), 0);
+                int32_t xr_i = imin32(sat_i32((double)RFLOOR(xr_f - (R)0.5 + eps)), (int32_t)TS - 1);
+                for (int32_t ci = xl_i; ci <= xr_i; ++ci) {
+                    uint32_t colu = (uint32_t)ci, pl = row * TS + colu;
+                    uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+                    if (px >= w || py >= h) continue;
+                    R* s = st + 13 * pl;
+                    if (s[3] < (R)-13.8) continue;
+                    R pcx = (R)px + (R)0.5, pcy = (R)py + (R)0.5;
+                    int found = 0;
+                    uint32_t ai = 0, bi = 0;
+                    R b[3] = {0, 0, 0};
+                    for (uint32_t tri = 0; tri < ns; ++tri) {
+                        uint32_t ta = tri, tb = (tri + 1) % ns;
+                        if (FN(in_tri)(pcx, pcy, iv.px[ta], iv.py[ta], iv.px[tb], iv.py[tb], iv.px[4], iv.py[4])) {
+                            ai = ta; bi = tb;
+                            FN(bary2d)(pcx, pcy, iv.px[ai], iv.py[ai], iv.px[bi], iv.py[bi], iv.px[4], iv.py[4], b);
+                            found = 1;
+                            break;
+                        }
+                    }
+                    if (!found) continue;
+                    R zfn = b[0] * iv.zf[ai] + b[1] * iv.zf[bi] + b[2] * iv.zf[4];
+                    R zbn = b[0] * iv.zb[ai] + b[1] * iv.zb[bi] + b[2] * iv.zb[4];
+                    R ofn = b[0] * iv.of[ai] + b[1] * iv.of[bi] + b[2] * iv.of[4];
+                    R obn = b[0] * iv.ob[ai] + b[1] * iv.ob[bi] + b[2] * iv.ob[4];
+                    R zf = (near_p * far_p) / (far_p - FN(clampr)(zfn, (R)0.0, (R)1.0) * range_nf);
+                    R zb = (near_p * far_p) / (far_p - FN(clampr)(zbn, (R)0.0, (R)1.0) * range_nf);
+                    R xc = (pcx - c.intr[2]) / c.intr[0], yc2 = (pcy - c.intr[3]) / c.intr[1];
+                    R rsc = RSQRT((xc * xc + yc2 * yc2) + (R)1.0);
+                    R dd = RABS(zb - zf) * rsc;
+                    V3 cf = FN(mk)(FN(rmax)(iv.base.x + ofn, 0), FN(rmax)(iv.base.y + ofn, 0), FN(rmax)(iv.base.z + ofn, 0));
+                    V3 cb = FN(mk)(FN(rmax)(iv.base.x + obn, 0), FN(rmax)(iv.base.y + obn, 0), FN(rmax)(iv.base.z + obn, 0));
+                    R od = FN(clampr)(iv.dens * dd, (R)1e-8f, (R)88.0);
+                    R alpha_t = FN(safe_exp)(-od);
+                    R phv = FN(phi)(od);
+                    R w0 = phv - alpha_t, w1 = (R)1.0 - phv;
+                    V3 cp = FN(safe_clip3)(FN(mk)(cb.x * w0 + cf.x * w1, cb.y * w0 + cf.y * w1, cb.z * w0 + cf.z * w1),
+                                           (R)0.0, MAX_VAL_R);
+                    R Tj = FN(safe_exp)(s[3]);
+                    if (Tj < (R)1e-6f) continue;
+                    s[0] += cp.x * Tj; s[1] += cp.y * Tj; s[2] += cp.z * Tj;
+                    s[3] = s[3] - od;
+                    /* xyzd: zero vertex normals, depth only */
+                    s[7] += Tj * (w0 * zb + w1 * zf);
+                    /* distortion state (interval_tiled_rasterize.wgsl:383-400) */
+                    R enter = zf * rsc, exit_d = zb * rsc;
+                    R u = iv.dens * (exit_d - enter);
+                    R en = FN(safe_exp)(-u);
+                    R ad = FN(rmax)((R)1.0 - en, 0);
+                    R m0 = Tj * ad;
+                    R m1 = Tj * (enter * ad + FN(safe_div)((R)1.0 - ((R)1.0 + u) * en, iv.dens));
+                    R sd = Tj * Tj * FN(safe_div)((R)1.0 - (R)2.0 * u * en - en * en, iv.dens);
+                    R* ds = s + 8;
+                    ds[0] += (R)2.0 * m1 * ds[2];
+                    ds[1] += (R)2.0 * m0 * ds[3];
+                    ds[2] += m0;
+                    ds[3] += m1;
+                    ds[4] += sd;
+                }
+            }
+        }
+        for (uint32_t i = 0; i < TS * TS; ++i) {
+            uint32_t row = i / TS, colu = i % TS, px = tile_x * TS + colu, py = tile_y * TS + row;
+            if (px >= w || py >= h) continue;
+            size_t pix = (size_t)py * w + px;
+            R* s = st + 13 * i;
+            rgba[4 * pix] = s[0]; rgba[4 * pix + 1] = s[1]; rgba[4 * pix + 2] = s[2];
+            rgba[4 * pix + 3] = (R)1.0 - FN(safe_exp)(s[3]);
+            for (int k = 0; k < 4; ++k) xyzd[4 * pix + k] = s[4 + k];
+            for (int k = 0; k < 5; ++k) dist[5 * pix + k] = s[8 + k];
+        }
+    }
+    free(st);
+}
+
+/* ------------------------------------------------------------------ */
 /* Per-tet error statistics for densification (forward_error)           */
 /* crates/rmesh-error/src/wgsl/error_tiled_compute.wgsl:88-373,          */
 /* crates/rmesh-python/src/lib.rs:1621-1711.  Same tile traversal and    */
