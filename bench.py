@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true", help="skip the forward_error (densification statistics) pass")
+    ap.add_argument("--no-interval", action="store_true", help="skip the forward_interval_tiled pass")
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB (HBM counter calibration)")
@@ -286,6 +287,27 @@ def main():
                     "note": "forward_error on view 0, host copies included (kernel time: profiles/)"}
         log(f"forward_error: {err_pass['ms']:.2f} ms (view 0, {err_pass['tets_with_stats']} tets with statistics)")
 
+    ivl_pass = None
+    if not args.no_interval:
+        # interval-tiled forward of view 0 beside the tiled forward of the same
+        # view (each: binning, compositing and the synchronous host copies)
+        r.set_graph_mode(False)
+        r.set_timing(False)
+        ti, tf = [], []
+        for _ in range(3):
+            r.synchronize()
+            t0 = time.perf_counter()
+            iv = r.forward_interval_tiled(*cams[0])
+            ti.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            r.forward_tiled(*cams[0])
+            tf.append(time.perf_counter() - t0)
+        ivl_pass = {"ms": round(1000.0 * min(ti), 3), "forward_tiled_ms": round(1000.0 * min(tf), 3),
+                    "mean_alpha": round(float(iv["image"][..., 3].mean()), 4),
+                    "note": "forward_interval_tiled on view 0 (image, depth, distortion) beside forward_tiled of "
+                            "the same view; binning and host copies included in both (kernel times: profiles/)"}
+        log(f"forward_interval_tiled: {ivl_pass['ms']:.2f} ms (forward_tiled {ivl_pass['forward_tiled_ms']:.2f} ms)")
+
     cpu = None
     if not args.no_cpu:
         tc = time.perf_counter()
@@ -332,6 +354,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_per_view": plain,
         "forward_error": err_pass,
+        "interval_forward": ivl_pass,
     }
     if copy_probe:
         out["copy_probe"] = copy_probe

@@ -85,6 +85,16 @@ rmesh_status rmesh_read_instance_count(rmesh_renderer* h, uint32_t* out);
  * (covered pixels, max of i32(65535 * T_val)). */
 rmesh_status rmesh_forward_error(rmesh_renderer* h, const float* pixel_err, const float* ssim_err, float* tet_err,
                                  int32_t* tet_count);
+/* RMeshRenderer.forward_interval_tiled (lib.rs:1218-1390,
+ * interval_generate.wgsl, interval_tiled_rasterize.wgsl): silhouette fan
+ * triangles per tet, NDC z interpolated in screen space and linearized with
+ * near 0.01 / far 1000, composited over the regular tile lists.  Outputs
+ * host: rgba [H*W*4], xyzd [H*W*4] (x, y, z = 0: no vertex normals; w =
+ * depth), distortion [H*W*5].  Leaves no forward state for backward or
+ * forward_error. */
+rmesh_status rmesh_forward_interval_tiled(rmesh_renderer* h, const float* cam, const float* vp,
+                                          const float* c2w_intrinsics, float* out_rgba, float* out_xyzd,
+                                          float* out_distortion);
 
 /* RMeshRenderer.backward (lib.rs:1894-2031).  Uses the sorted tile lists and
  * image of the last forward_tiled.  dl_d_image [H*W*4] host; outputs host:

@@ -1722,4 +1722,405 @@ bool launch_error(const ErrArgs& a, uint32_t tile_size, hipStream_t s) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Interval-tiled forward (forward_interval_tiled, crates/rmesh-python/src/
+// lib.rs:1218-1390).  Each tet's projection is split into screen-space fan
+// triangles around one interior point (interval_generate.wgsl:174-408): with
+// one vertex inside the triangle of the other three (case 1) the silhouette
+// is that triangle and the interior point is the inner vertex, paired with
+// the opposite face's depth at it; otherwise (case 2) the silhouette is the
+// quad of the four vertices and the interior point is the crossing of its
+// diagonals, carrying both diagonals' depths.  Per pixel the covering fan
+// triangle's screen barycentrics interpolate front/back NDC z and colour
+// offsets; the chord is the linearized z difference times the ray scale
+// (interval_tiled_rasterize.wgsl:96-441).  Outputs: image, xyzd (x, y, z
+// stay 0: no vertex normals on this path; w = depth) and the five-float
+// distortion state.
+// ---------------------------------------------------------------------------
+constexpr float kIvlNear = 0.01f, kIvlFar = 1000.0f;  // lib.rs:1241-1242
+constexpr int kIvlRec = 7;                            // float4 per tet
+
+__device__ __forceinline__ float ivl_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// interval_generate.wgsl:112-126
+__device__ __forceinline__ void ivl_bary(float ptx, float pty, float ax, float ay, float bx, float by, float cx,
+                                         float cy, float& b0, float& b1, float& b2) {
+    const float v0x = bx - ax, v0y = by - ay, v1x = cx - ax, v1y = cy - ay, v2x = ptx - ax, v2y = pty - ay;
+    const float d00 = v0x * v0x + v0y * v0y, d01 = v0x * v1x + v0y * v1y, d11 = v1x * v1x + v1y * v1y;
+    const float d20 = v2x * v0x + v2y * v0y, d21 = v2x * v1x + v2y * v1y;
+    const float denom = d00 * d11 - d01 * d01;
+    const float inv = fabsf(denom) > 1e-10f ? 1.0f / denom : 0.0f;
+    b1 = (d11 * d20 - d01 * d21) * inv;
+    b2 = (d00 * d21 - d01 * d20) * inv;
+    b0 = 1.0f - b1 - b2;
+}
+
+// interval_generate.wgsl:129-139: parameter along a of the crossing with line b (-1: parallel)
+__device__ __forceinline__ float ivl_line_t(float a1x, float a1y, float a2x, float a2y, float b1x, float b1y,
+                                            float b2x, float b2y) {
+    const float d1x = a2x - a1x, d1y = a2y - a1y, d2x = b2x - b1x, d2y = b2y - b1y;
+    const float denom = d1x * d2y - d1y * d2x;
+    if (fabsf(denom) < 1e-10f) return -1.0f;
+    const float dx = b1x - a1x, dy = b1y - a1y;
+    return (dx * d2y - dy * d2x) / denom;
+}
+
+// interval_tiled_rasterize.wgsl:90-97 (edges inclusive)
+__device__ __forceinline__ bool ivl_in_tri(float ptx, float pty, float ax, float ay, float bx, float by, float cx,
+                                           float cy) {
+    const float d0 = (bx - ax) * (pty - ay) - (by - ay) * (ptx - ax);
+    const float d1 = (cx - bx) * (pty - by) - (cy - by) * (ptx - bx);
+    const float d2 = (ax - cx) * (pty - cy) - (ay - cy) * (ptx - cx);
+    const bool has_neg = d0 < 0.0f || d1 < 0.0f || d2 < 0.0f;
+    const bool has_pos = d0 > 0.0f || d1 > 0.0f || d2 > 0.0f;
+    return !(has_neg && has_pos);
+}
+
+__device__ __forceinline__ float sel4(const float* a, int i) {
+    return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
+}
+
+// One thread per tet: the silhouette decomposition, as 7 float4
+//   [0..3] (x, y, ndc z, colour offset) of silhouette points 0-3 (3 points:
+//          point 3 repeats point 0), pixel coordinates of the full image,
+//   [4]    (x, y, front z, back z) of the interior point,
+//   [5]    (front offset, back offset, density, point count: 0 = skipped),
+//   [6]    (colour + g . (cam - v0), 0).
+__global__ __launch_bounds__(256) void k_interval_generate(const float4* __restrict__ rec,
+                                                           const float4* __restrict__ colors,
+                                                           const ViewParams* __restrict__ views, uint32_t M,
+                                                           float4* __restrict__ out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const ViewParams& vw = views[0];
+    float4 r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = rec[4ull * t + i];
+    const float4 col = colors[t];
+    const V3 v[4] = {mk3(r[0].x, r[0].y, r[0].z), mk3(r[1].x, r[1].y, r[1].z), mk3(r[2].x, r[2].y, r[2].z),
+                     mk3(r[3].x, r[3].y, r[3].z)};
+    const float dens = r[0].w;
+    const V3 g = mk3(r[1].w, r[2].w, r[3].w);
+    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
+    float4* o = out + (uint64_t)kIvlRec * t;
+    const float bo = dot3(g, sub3(cam, v[0]));
+    const float4 base = make_float4(col.x + bo, col.y + bo, col.z + bo, 0.0f);
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float clip[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mul_vp(vw.vp, v[i], clip[i]);
+    bool skip = clip[0][3] <= 0.0f || clip[1][3] <= 0.0f || clip[2][3] <= 0.0f || clip[3][3] <= 0.0f;
+    float px[4], py[4], nz[4], off[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float w = clip[i][3];
+        const float inv_w = fabsf(w) > 1e-10f ? 1.0f / w : 0.0f;
+        const float nx = clip[i][0] * inv_w, ny = clip[i][1] * inv_w;
+        nz[i] = clip[i][2] * inv_w;
+        px[i] = ((nx + 1.0f) * 0.5f) * vw.Wf;
+        py[i] = ((1.0f - ny) * 0.5f) * vw.Hf;
+        off[i] = dot3(g, sub3(v[i], cam));
+    }
+    const float zmax = fmaxf(fmaxf(nz[0], nz[1]), fmaxf(nz[2], nz[3]));
+    const float zmin = fminf(fminf(nz[0], nz[1]), fminf(nz[2], nz[3]));
+    skip = skip || zmax - zmin < 1e-7f || dens < 1e-6f;
+    if (skip) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) o[i] = zero;
+        o[6] = base;
+        return;
+    }
+    // interval_generate.wgsl:89-109: a vertex inside the triangle of the other three
+    int inner = 4;
+#pragma unroll
+    for (int fi = 3; fi >= 0; --fi) {
+        int a, b, c, q;
+        face_idx(fi, a, b, c, q);
+        const float d0 = (px[b] - px[a]) * (py[q] - py[a]) - (py[b] - py[a]) * (px[q] - px[a]);
+        const float d1 = (px[c] - px[b]) * (py[q] - py[b]) - (py[c] - py[b]) * (px[q] - px[b]);
+        const float d2 = (px[a] - px[c]) * (py[q] - py[c]) - (py[a] - py[c]) * (px[q] - px[c]);
+        const bool all_pos = d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f;
+        const bool all_neg = d0 <= 0.0f && d1 <= 0.0f && d2 <= 0.0f;
+        if (all_pos || all_neg) inner = q;  // descending order: the first face in table order wins
+    }
+    int s0, s1, s2, s3;  // silhouette points
+    float cx, cy, zf4, zb4, of4, ob4;
+    if (inner < 4) {
+        int a, b, c, q;
+        face_idx((inner + 1) & 3, a, b, c, q);  // the face opposite the inner vertex
+        float w0, w1, w2;
+        ivl_bary(sel4(px, q), sel4(py, q), sel4(px, a), sel4(py, a), sel4(px, b), sel4(py, b), sel4(px, c),
+                 sel4(py, c), w0, w1, w2);
+        const float z_face = w0 * sel4(nz, a) + w1 * sel4(nz, b) + w2 * sel4(nz, c);
+        const float z_c = sel4(nz, q);
+        const float face_off = w0 * sel4(off, a) + w1 * sel4(off, b) + w2 * sel4(off, c);
+        const bool face_first = z_face <= z_c;
+        s0 = a; s1 = b; s2 = c; s3 = a;
+        cx = sel4(px, q);
+        cy = sel4(py, q);
+        zf4 = fminf(z_face, z_c);
+        zb4 = fmaxf(z_face, z_c);
+        of4 = face_first ? face_off : sel4(off, q);
+        ob4 = face_first ? sel4(off, q) : face_off;
+    } else {
+        // the first pair of opposite edges whose segments cross (01/23, 02/13,
+        // 03/12), else 01/23
+        int a0 = 0, a1 = 1, b0 = 2, b1 = 3;
+        float ta = ivl_line_t(px[0], py[0], px[1], py[1], px[2], py[2], px[3], py[3]);
+        float tb = ivl_line_t(px[2], py[2], px[3], py[3], px[0], py[0], px[1], py[1]);
+        if (!(ta > 0.0f && ta < 1.0f && tb > 0.0f && tb < 1.0f)) {
+            const float ua = ivl_line_t(px[0], py[0], px[2], py[2], px[1], py[1], px[3], py[3]);
+            const float ub = ivl_line_t(px[1], py[1], px[3], py[3], px[0], py[0], px[2], py[2]);
+            if (ua > 0.0f && ua < 1.0f && ub > 0.0f && ub < 1.0f) {
+                a1 = 2; b0 = 1; ta = ua; tb = ub;
+            } else {
+                const float wa = ivl_line_t(px[0], py[0], px[3], py[3], px[1], py[1], px[2], py[2]);
+                const float wb = ivl_line_t(px[1], py[1], px[2], py[2], px[0], py[0], px[3], py[3]);
+                if (wa > 0.0f && wa < 1.0f && wb > 0.0f && wb < 1.0f) {
+                    a1 = 3; b0 = 1; b1 = 2; ta = wa; tb = wb;
+                }
+            }
+        }
+        const float pa0x = sel4(px, a0), pa1x = sel4(px, a1), pa0y = sel4(py, a0), pa1y = sel4(py, a1);
+        cx = ivl_mix(pa0x, pa1x, ta);
+        cy = ivl_mix(pa0y, pa1y, ta);
+        const float z_a = ivl_mix(sel4(nz, a0), sel4(nz, a1), ta), z_b = ivl_mix(sel4(nz, b0), sel4(nz, b1), tb);
+        const float off_a = ivl_mix(sel4(off, a0), sel4(off, a1), ta);
+        const float off_b = ivl_mix(sel4(off, b0), sel4(off, b1), tb);
+        s0 = a0; s1 = b0; s2 = a1; s3 = b1;
+        zf4 = fminf(z_a, z_b);
+        zb4 = fmaxf(z_a, z_b);
+        of4 = z_a <= z_b ? off_a : off_b;
+        ob4 = z_a <= z_b ? off_b : off_a;
+    }
+    const int sv[4] = {s0, s1, s2, s3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        o[k] = make_float4(sel4(px, sv[k]), sel4(py, sv[k]), sel4(nz, sv[k]), sel4(off, sv[k]));
+    o[4] = make_float4(cx, cy, zf4, zb4);
+    o[5] = make_float4(of4, ob4, dens, inner < 4 ? 3.0f : 4.0f);
+    o[6] = base;
+}
+
+// Tiled compositing over the fan triangles: the batch machinery of the
+// tiled forward (spans of the tet's four projected points, which bound the
+// silhouette; pixels of a span outside it find no fan triangle), with one
+// record per covered pixel holding the colour, optical depth, depth and the
+// distortion factors, composited per pixel in depth order.
+template <class C>
+__global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_interval_forward(const float4* __restrict__ ivr,
+                                                           const uint32_t* __restrict__ sorted_vals,
+                                                           const uint32_t* __restrict__ ranges,
+                                                           const ViewParams* __restrict__ views,
+                                                           const uint32_t* __restrict__ tile_order,
+                                                           float* __restrict__ rgba, float* __restrict__ xyzd,
+                                                           float* __restrict__ dist) {
+    constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, SP = TS + 1;
+    static_assert(B <= 64 && CAP >= TS * TS && CAP % 32 == 0 && CAP / 32 <= NT && CAP <= 2048, "batch geometry");
+    static_assert(BatchLDS<TS, B, false>::TR == kIvlRec, "records live in the face rows");
+    __shared__ BatchLDS<TS, B, false> L;
+    __shared__ float s_r[8][CAP];  // colour rgb, optical depth, depth, 1 - e^-u, m1 / T, self term / T^2
+    __shared__ float s_rsc[TS * TS];
+    __shared__ uint32_t s_live[TS * TS];
+    __shared__ unsigned long long s_mask[TS * TS];
+    __shared__ uint16_t s_chunk[chunk_capacity(CAP, 1, B)];
+    __shared__ uint32_t s_start[CAP / 32];
+
+    const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
+    const ViewParams& vw = views[0];
+    const TileCtx tc = tile_ctx(vw, gtile);
+    const int tid = threadIdx.x;
+    const int row = tid / TS, colp = tid % TS;
+    const uint32_t px = tc.tile_x * TS + colp, py = tc.tile_y * TS + row;
+    const bool active = tid < TS * TS && px < vw.width && py < vw.height;
+    if (tid < TS * TS) {
+        float rs = 1.0f;
+        if (active) {
+            // interval_tiled_rasterize.wgsl:300-306: |(x_cam, y_cam, 1)|
+            const float xc = ((float)px + 0.5f - vw.intr[2]) / vw.intr[0];
+            const float yc = ((float)py + 0.5f - vw.intr[3]) / vw.intr[1];
+            rs = sqrtf((xc * xc + yc * yc) + 1.0f);
+        }
+        s_rsc[tid] = rs;
+        s_live[tid] = active ? 1u : 0u;
+    }
+    float cr = 0.f, cg = 0.f, cb = 0.f, log_t = 0.f, dep = 0.f;
+    float ds0 = 0.f, ds1 = 0.f, ds2 = 0.f, ds3 = 0.f, ds4 = 0.f;
+    const float nf = kIvlNear * kIvlFar, range_nf = kIvlFar - kIvlNear;
+
+    const uint32_t rs_ = ranges[2ull * gtile], re = ranges[2ull * gtile + 1];
+    const uint32_t count = re > rs_ ? re - rs_ : 0u;
+    uint32_t done = 0;
+    int nb = (int)min<uint32_t>(B, count);
+    float4 f[kIvlRec];
+    auto fetch = [&](uint32_t t) {
+#pragma unroll
+        for (int i = 0; i < kIvlRec; ++i) f[i] = ivr[(uint64_t)kIvlRec * t + i];
+    };
+    if (tid < nb) fetch(sorted_vals[re - 1 - tid]);
+    while (done < count) {
+        if (__syncthreads_and(!active || log_t < -13.8f)) break;
+        if (tid < nb) {
+            const int k = tid;
+#pragma unroll
+            for (int i = 0; i < kIvlRec; ++i) L.tr[k * kIvlRec + i] = f[i];
+            // the tet's four projected points: silhouette 0-2 and the inner
+            // vertex (3 points) or silhouette 0-3
+            const bool tri = f[5].w == 3.0f;
+            const float qx[4] = {f[0].x, f[1].x, f[2].x, tri ? f[4].x : f[3].x};
+            const float qy[4] = {f[0].y, f[1].y, f[2].y, tri ? f[4].y : f[3].y};
+            uint32_t rr = 0xFFu;
+            if (f[5].w != 0.0f) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    L.pxy[i][k] = qx[i] - tc.tile_ox;
+                    L.pxy[4 + i][k] = qy[i] - tc.tile_oy;
+                }
+                const float ymin = fminf(fminf(qy[0], qy[1]), fminf(qy[2], qy[3])) - tc.tile_oy;
+                const float ymax = fmaxf(fmaxf(qy[0], qy[1]), fmaxf(qy[2], qy[3])) - tc.tile_oy;
+                const float r0 = fmaxf(ceilf(ymin) - 1.0f, 0.0f);
+                const float r1 = fminf(floorf(ymax), (float)(TS - 1));
+                if (r0 <= r1) rr = (uint32_t)r0 | ((uint32_t)r1 << 8);
+            }
+            L.rows[k] = rr;
+        }
+        if (tid < TS * TS) s_mask[tid] = 0ull;
+        if (tid < CAP / 32) s_start[tid] = 0u;
+        __syncthreads();
+        batch_layout<TS, B, false, true, CAP, 1, NT>(L, nb, s_chunk, s_start);
+        __syncthreads();
+        const uint32_t nbp = L.nbp & 0xFFu;
+        const uint32_t nrec = L.nbp >> 8;
+        const uint32_t next = done + nbp;
+        const int nb_next = (int)min<uint32_t>(B, count - next);
+        if (tid < nb_next) fetch(sorted_vals[re - 1 - next - tid]);
+        // record-parallel: fan triangle, interpolation and the per-pair terms
+        // (interval_tiled_rasterize.wgsl:248-400)
+        for (uint32_t j = tid; j < nrec; j += NT) {
+            const ChunkRef c = record_at(s_chunk, s_start, j);
+            const int p = c.row * TS + c.col;
+            if (!s_live[p]) continue;
+            const float4* T = &L.tr[c.k * kIvlRec];
+            const float4 q0 = T[0], q1 = T[1], q2 = T[2], q3 = T[3], q4 = T[4], q5 = T[5];
+            const bool tri3 = q5.w == 3.0f;
+            const float pcx = (float)(tc.tile_x * TS + (uint32_t)c.col) + 0.5f;
+            const float pcy = (float)(tc.tile_y * TS + (uint32_t)c.row) + 0.5f;
+            // first fan triangle (point i, point i + 1 mod n, interior point) holding the pixel centre
+            float4 ea = q0, eb = q1;
+            bool found = ivl_in_tri(pcx, pcy, q0.x, q0.y, q1.x, q1.y, q4.x, q4.y);
+            if (!found && ivl_in_tri(pcx, pcy, q1.x, q1.y, q2.x, q2.y, q4.x, q4.y)) {
+                found = true;
+                ea = q1;
+                eb = q2;
+            }
+            const float4 e3 = tri3 ? q0 : q3;
+            if (!found && ivl_in_tri(pcx, pcy, q2.x, q2.y, e3.x, e3.y, q4.x, q4.y)) {
+                found = true;
+                ea = q2;
+                eb = e3;
+            }
+            if (!found && !tri3 && ivl_in_tri(pcx, pcy, q3.x, q3.y, q0.x, q0.y, q4.x, q4.y)) {
+                found = true;
+                ea = q3;
+                eb = q0;
+            }
+            if (!found) continue;
+            float b0, b1, b2;
+            ivl_bary(pcx, pcy, ea.x, ea.y, eb.x, eb.y, q4.x, q4.y, b0, b1, b2);
+            const float zfn = b0 * ea.z + b1 * eb.z + b2 * q4.z;
+            const float zbn = b0 * ea.z + b1 * eb.z + b2 * q4.w;
+            const float ofn = b0 * ea.w + b1 * eb.w + b2 * q5.x;
+            const float obn = b0 * ea.w + b1 * eb.w + b2 * q5.y;
+            const float zf = nf / (kIvlFar - clampf(zfn, 0.0f, 1.0f) * range_nf);
+            const float zb = nf / (kIvlFar - clampf(zbn, 0.0f, 1.0f) * range_nf);
+            const float rsc = s_rsc[p];
+            const float dd = fabsf(zb - zf) * rsc;
+            const float4 base = T[6];
+            const float dens = q5.z;
+            const V3 cf = mk3(fmaxf(base.x + ofn, 0.0f), fmaxf(base.y + ofn, 0.0f), fmaxf(base.z + ofn, 0.0f));
+            const V3 cbk = mk3(fmaxf(base.x + obn, 0.0f), fmaxf(base.y + obn, 0.0f), fmaxf(base.z + obn, 0.0f));
+            const float od = clampf(dens * dd, 1e-8f, 88.0f);
+            const float alpha_t = safe_exp(-od);
+            const float phv = phi_f(od);
+            const float w0 = phv - alpha_t, w1 = 1.0f - phv;
+            const V3 cp = safe_clip3(mk3(cbk.x * w0 + cf.x * w1, cbk.y * w0 + cf.y * w1, cbk.z * w0 + cf.z * w1),
+                                     0.0f, kMaxVal);
+            // distortion factors (interval_tiled_rasterize.wgsl:383-400)
+            const float enter = zf * rsc, exit_d = zb * rsc;
+            const float u = dens * (exit_d - enter);
+            const float en = safe_exp(-u);
+            const float ad = fmaxf(1.0f - en, 0.0f);
+            s_r[0][j] = cp.x;
+            s_r[1][j] = cp.y;
+            s_r[2][j] = cp.z;
+            s_r[3][j] = od;
+            s_r[4][j] = w0 * zb + w1 * zf;
+            s_r[5][j] = ad;
+            s_r[6][j] = enter * ad + safe_div(1.0f - (1.0f + u) * en, dens);
+            s_r[7][j] = safe_div(1.0f - 2.0f * u * en - en * en, dens);
+            atomicOr(&s_mask[p], 1ull << c.k);
+        }
+        __syncthreads();
+        // pixel-sequential compositing in depth order (interval_tiled_rasterize.wgsl:350-401)
+        if (active) {
+            unsigned long long m = s_mask[tid];
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                if (log_t < -13.8f) break;
+                const uint32_t slot = (L.rpref[k * SP + row] + (uint32_t)colp) & 0xFFFFu;
+                const float Tj = safe_exp(log_t);
+                if (Tj >= 1e-6f) {
+                    cr = cr + s_r[0][slot] * Tj;
+                    cg = cg + s_r[1][slot] * Tj;
+                    cb = cb + s_r[2][slot] * Tj;
+                    log_t = log_t - s_r[3][slot];
+                    dep = dep + Tj * s_r[4][slot];
+                    const float m0 = Tj * s_r[5][slot];
+                    const float m1 = Tj * s_r[6][slot];
+                    const float sd = (Tj * Tj) * s_r[7][slot];
+                    ds0 = ds0 + 2.0f * m1 * ds2;
+                    ds1 = ds1 + 2.0f * m0 * ds3;
+                    ds2 = ds2 + m0;
+                    ds3 = ds3 + m1;
+                    ds4 = ds4 + sd;
+                }
+            }
+            if (log_t < -13.8f) s_live[tid] = 0u;
+        }
+        done = next;
+        nb = nb_next;
+    }
+    if (active) {
+        const uint64_t pix = (uint64_t)py * vw.width + px;
+        reinterpret_cast<float4*>(rgba)[pix] = make_float4(cr, cg, cb, 1.0f - safe_exp(log_t));
+        reinterpret_cast<float4*>(xyzd)[pix] = make_float4(0.0f, 0.0f, 0.0f, dep);
+        float* d = dist + 5ull * pix;
+        d[0] = ds0;
+        d[1] = ds1;
+        d[2] = ds2;
+        d[3] = ds3;
+        d[4] = ds4;
+    }
+}
+
+template <int TS>
+static void ivl_launch(const IvlArgs& a, hipStream_t s) {
+    using C = FwdCfg<TS, 32, 512, 1, 1>;
+    hipLaunchKernelGGL((k_interval_forward<C>), dim3(a.num_tiles), dim3(C::NT), 0, s, a.ivr, a.sorted_vals, a.ranges,
+                       a.views, a.tile_order, a.rgba, a.xyzd, a.dist);
+}
+
+bool launch_interval(const IvlArgs& a, uint32_t tile_size, hipStream_t s) {
+    if (tile_size != 8 && tile_size != 12 && tile_size != 16) return false;
+    if (a.M) hipLaunchKernelGGL(k_interval_generate, dim3((a.M + 255) / 256), dim3(256), 0, s, a.rec, a.colors, a.views,
+                                a.M, a.ivr);
+    switch (tile_size) {
+        case 8: ivl_launch<8>(a, s); break;
+        case 12: ivl_launch<12>(a, s); break;
+        default: ivl_launch<16>(a, s); break;
+    }
+    return true;
+}
+
 }  // namespace rmesh

@@ -95,6 +95,9 @@ struct rmesh_renderer {
     // forward_error inputs and outputs (allocated on first use)
     float *err_pe = nullptr, *err_se = nullptr, *tet_err = nullptr;
     int32_t* tet_cnt = nullptr;
+    // interval forward: per-tet decomposition and the depth / distortion images (allocated on first use)
+    float4* ivr = nullptr;
+    float *ivl_xyzd = nullptr, *ivl_dist = nullptr;
     // last forward
     uint32_t last_views = 0;
     bool have_forward = false;
@@ -428,7 +431,7 @@ void free_all(rmesh_renderer* h) {
                     h->d_sh, h->rec, h->colors, h->d_views, h->counts, h->block_sums, h->vis_count, h->n_pairs,
                     h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a, h->vals_b, h->hist, h->ranges, h->rgba,
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
-                    h->tet_cnt, h->tile_order};
+                    h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -636,6 +639,51 @@ extern "C" rmesh_status rmesh_forward_error(rmesh_renderer* h, const float* pixe
     HIP_TRY(hipMemcpyAsync(tet_err, h->tet_err, 64ull * h->M, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(tet_count, h->tet_cnt, 8ull * h->M, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_forward_interval_tiled(rmesh_renderer* h, const float* cam, const float* vp,
+                                                     const float* ci, float* out_rgba, float* out_xyzd,
+                                                     float* out_dist) {
+    if (!h || !cam || !vp || !ci || !out_rgba || !out_xyzd || !out_dist) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    hipStream_t s = h->stream;
+    const size_t npx = (size_t)h->W * h->H;
+    if (!h->ivr) {
+        HIP_TRY(dalloc(&h->ivr, 7ull * h->M));
+        HIP_TRY(dalloc(&h->ivl_xyzd, 4 * npx));
+        HIP_TRY(dalloc(&h->ivl_dist, 5 * npx));
+    }
+    rmesh_status st = upload_views(h, 1, cam, vp, ci);
+    if (st != RMESH_OK) return st;
+    st = stage_binning_front(h, 1, false);
+    if (st != RMESH_OK) return st;
+    st = stage_binning_back(h, 1);
+    if (st != RMESH_OK) return st;
+    IvlArgs a;
+    a.rec = h->rec;
+    a.colors = h->colors;
+    a.views = h->d_views;
+    a.M = h->M;
+    a.ivr = h->ivr;
+    a.sorted_vals = sorted_vals(h);
+    a.ranges = h->ranges;
+    a.tile_order = h->tile_order;
+    a.num_tiles = h->T;
+    a.rgba = h->rgba;
+    a.xyzd = h->ivl_xyzd;
+    a.dist = h->ivl_dist;
+    if (!launch_interval(a, h->TS, s)) return fail(RMESH_ERR_UNSUPPORTED, "tile size");
+    HIP_TRY(hipGetLastError());
+    // the tiled forward's state (image, aux) is not this path's
+    h->have_forward = false;
+    HIP_TRY(hipMemcpyAsync(out_rgba, h->rgba, npx * 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out_xyzd, h->ivl_xyzd, npx * 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out_dist, h->ivl_dist, npx * 20, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpy(&ovf, h->overflow, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(RMESH_ERR_CAPACITY, "pair buffer overflow");
     return RMESH_OK;
 }
 

@@ -64,6 +64,24 @@ struct ErrArgs {
 };
 bool launch_error(const ErrArgs& a, uint32_t tile_size, hipStream_t s);
 
+// Interval-tiled forward of view 0: silhouette decomposition of every tet,
+// then compositing over the regular binning's tile lists.
+struct IvlArgs {
+    const float4* rec;
+    const float4* colors;        // view 0's colours
+    const ViewParams* views;
+    uint32_t M;
+    float4* ivr;                 // [M][7] decomposition (written, then read)
+    const uint32_t* sorted_vals;
+    const uint32_t* ranges;
+    const uint32_t* tile_order;  // nullable, as FwdArgs::tile_order
+    uint32_t num_tiles;
+    float* rgba;                 // [H][W][4]
+    float* xyzd;                 // [H][W][4]
+    float* dist;                 // [H][W][5]
+};
+bool launch_interval(const IvlArgs& a, uint32_t tile_size, hipStream_t s);
+
 struct BwdArgs {
     const float4* rec;
     const uint32_t* idx;

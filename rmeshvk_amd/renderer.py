@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib
 
 _OUT_OF_SCOPE = (
-    "forward_interval_tiled", "backward_interval_tiled", "forward_raytrace", "backward_raytrace",
+    "backward_interval_tiled", "forward_raytrace", "backward_raytrace",
     "backward_hw", "locate_tets", "read_aux", "set_vertex_normals",
     "get_vertex_normals", "set_aux_data", "get_aux_data",
 )
@@ -146,6 +146,18 @@ class RMeshRenderer:
         out = ctypes.c_uint32()
         _lib.check(self._lib.rmesh_read_instance_count(self._h, ctypes.byref(out)))
         return int(out.value)
+
+    # ---- forward_interval_tiled (lib.rs:1218-1390) ----
+    def forward_interval_tiled(self, cam_pos, vp, c2w_intrinsics):
+        """Interval-tiled forward: {'image' [H, W, 4], 'xyzd' [H, W, 4],
+        'distortion' [H, W, 5]} (no 'aux': aux_dim is always 0 here)."""
+        c, v, ci = self._cam(cam_pos, vp, c2w_intrinsics)
+        rgba = np.empty((self._hgt, self._w, 4), dtype=np.float32)
+        xyzd = np.empty((self._hgt, self._w, 4), dtype=np.float32)
+        dist = np.empty((self._hgt, self._w, 5), dtype=np.float32)
+        _lib.check(self._lib.rmesh_forward_interval_tiled(self._h, _p(c), _p(v), _p(ci), _p(rgba), _p(xyzd),
+                                                          _p(dist)))
+        return {"image": rgba, "xyzd": xyzd, "distortion": dist}
 
     # ---- forward_error (lib.rs:1621-1711) ----
     def forward_error(self, pixel_err, ssim_err):
