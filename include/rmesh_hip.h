@@ -95,6 +95,17 @@ rmesh_status rmesh_forward_error(rmesh_renderer* h, const float* pixel_err, cons
 rmesh_status rmesh_forward_interval_tiled(rmesh_renderer* h, const float* cam, const float* vp,
                                           const float* c2w_intrinsics, float* out_rgba, float* out_xyzd,
                                           float* out_distortion);
+/* RMeshRenderer.backward_interval_tiled (lib.rs:2049-2240,
+ * backward_interval_tiled.wgsl, interval_chain_back.wgsl): adjoint of the
+ * last forward_interval_tiled.  dl_d_image [H*W*4] host; dl_d_xyzd [H*W*4]
+ * and dl_d_distortion [H*W*5] host or NULL (zero).  Outputs host:
+ * d_vertices [N*3], d_densities [M], d_base_colors [M*3], d_color_grads
+ * [M*3], d_vertex_normals [N*3].  As the reference, gradients flow through
+ * each silhouette slot's NDC depths and colour offsets with the screen
+ * geometry fixed, and the depth and distortion terms follow its adjoint. */
+rmesh_status rmesh_backward_interval_tiled(rmesh_renderer* h, const float* dl_d_image, const float* dl_d_xyzd,
+                                           const float* dl_d_distortion, float* d_vertices, float* d_densities,
+                                           float* d_base_colors, float* d_color_grads, float* d_vertex_normals);
 
 /* RMeshRenderer.backward (lib.rs:1894-2031).  Uses the sorted tile lists and
  * image of the last forward_tiled.  dl_d_image [H*W*4] host; outputs host:

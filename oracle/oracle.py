@@ -117,7 +117,17 @@ def lib():
             getattr(L, "orc_error" + sfx).argtypes = [rp, _u32p, rp, _u32p, _u32p, _f32p, _f32p, _f32p, _u32, _u32,
                                                        _u32, _f32, rp, rp, rp, _i32p]
             getattr(L, "orc_interval_forward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p,
-                                                                  _f32p, _u32, _u32, _u32, rp, rp, rp]
+                                                                  _f32p, _u32, _u32, _u32, _u32, rp, rp, rp]
+            getattr(L, "orc_interval_generate" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _vp, _f32p, _f32p, _f32p,
+                                                                   _u32, _u32, _u32, rp]
+            getattr(L, "orc_interval_attrs" + sfx).argtypes = [rp, _u32p, rp, rp, _vp, _f32p, _f32p, _f32p, _u32,
+                                                                _u32, _u32, rp, rp]
+            getattr(L, "orc_interval_composite" + sfx).argtypes = [rp, _u32p, _u32p, _f32p, _f32p, _f32p, _u32, _u32,
+                                                                    _u32, _u32, rp, rp, rp]
+            getattr(L, "orc_interval_backward" + sfx).argtypes = [rp, _u32p, _u32p, _f32p, _f32p, _f32p, _u32, _u32,
+                                                                   _u32, _u32, rp, rp, rp, _vp, _vp, rp]
+            getattr(L, "orc_interval_chain" + sfx).argtypes = [rp, _u32p, rp, _f32p, _f32p, _f32p, _u32, _u32, _u32,
+                                                                rp, rp, rp, rp, rp, rp, rp]
         L.orc_sorting_bits.argtypes = [_u32]
         L.orc_sorting_bits.restype = _u32
         L.orc_sort.argtypes = [_u64p, _u32p, _u64, _u32]
@@ -237,10 +247,18 @@ def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, 
     return {"d_vertices": dV, "d_densities": dD, "d_color_grads": dG, "d_base_colors": dC}
 
 
+IVL_NF = 80  # floats per tet decomposition (flat layout of oracle_impl.h)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
 def interval_forward(scene, cam, w, h, sorted_vals, rng_, ts=12, colors=None, dtype=np.float32):
     """Interval-tiled forward over the regular binning's tile lists:
     {"image" [h, w, 4], "xyzd" [h, w, 4], "distortion" [h, w, 5]}."""
     cp, vp, ci = cam
+    M = len(scene["indices"]) // 4
     img = np.zeros(h * w * 4, dtype=dtype)
     xyzd = np.zeros(h * w * 4, dtype=dtype)
     dist = np.zeros(h * w * 5, dtype=dtype)
@@ -248,8 +266,96 @@ def interval_forward(scene, cam, w, h, sorted_vals, rng_, ts=12, colors=None, dt
     getattr(lib(), "orc_interval_forward" + _sfx(dtype))(
         _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
         _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
-        _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32), w, h, ts, img, xyzd, dist)
+        _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32), w, h, ts, M, img, xyzd,
+        dist)
     return {"image": img.reshape(h, w, 4), "xyzd": xyzd.reshape(h, w, 4), "distortion": dist.reshape(h, w, 5)}
+
+
+def interval_generate(scene, cam, w, h, colors=None, vertex_normals=None, dtype=np.float32):
+    """Per-tet silhouette decompositions, [M, IVL_NF] (flat layout)."""
+    cp, vp, ci = cam
+    M = len(scene["indices"]) // 4
+    out = np.zeros(M * IVL_NF, dtype=dtype)
+    col = scene["base_colors"] if colors is None else colors
+    vn = None if vertex_normals is None else _a(vertex_normals, dtype)
+    getattr(lib(), "orc_interval_generate" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
+        _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _ptr(vn), _a(cp, np.float32),
+        _a(vp, np.float32), _a(ci, np.float32), w, h, M, out)
+    return out.reshape(M, IVL_NF)
+
+
+def interval_attrs(scene, cam, w, h, frozen, colors=None, vertex_normals=None, dtype=np.float32):
+    """Attributes of the scene's (moved) vertices under frozen discrete
+    choices and weights: [M, IVL_NF]."""
+    cp, vp, ci = cam
+    M = len(scene["indices"]) // 4
+    out = np.zeros(M * IVL_NF, dtype=dtype)
+    col = scene["base_colors"] if colors is None else colors
+    vn = None if vertex_normals is None else _a(vertex_normals, dtype)
+    getattr(lib(), "orc_interval_attrs" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
+        _a(scene["color_grads"], dtype), _ptr(vn), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
+        w, h, M, _a(frozen, dtype).ravel(), out)
+    return out.reshape(M, IVL_NF)
+
+
+def interval_composite(ivl, cam, w, h, sorted_vals, rng_, ts=12, dtype=np.float32):
+    """Compositing over given decompositions [M, IVL_NF]."""
+    cp, vp, ci = cam
+    M = len(ivl)
+    img = np.zeros(h * w * 4, dtype=dtype)
+    xyzd = np.zeros(h * w * 4, dtype=dtype)
+    dist = np.zeros(h * w * 5, dtype=dtype)
+    getattr(lib(), "orc_interval_composite" + _sfx(dtype))(
+        _a(ivl, dtype).ravel(), _a(sorted_vals, np.uint32), _a(rng_, np.uint32), _a(cp, np.float32),
+        _a(vp, np.float32), _a(ci, np.float32), w, h, ts, M, img, xyzd, dist)
+    return {"image": img.reshape(h, w, 4), "xyzd": xyzd.reshape(h, w, 4), "distortion": dist.reshape(h, w, 5)}
+
+
+def interval_backward(ivl, cam, w, h, sorted_vals, rng_, fwd, dl_image, dl_xyzd=None, dl_distortion=None, ts=12,
+                      dtype=np.float32):
+    """Per-tile adjoint: d_ivl [M, IVL_NF] (slot attributes, normals, base
+    colour, density) from the forward's image and distortion."""
+    cp, vp, ci = cam
+    M = len(ivl)
+    d = np.zeros(M * IVL_NF, dtype=dtype)
+    dx = None if dl_xyzd is None else _a(dl_xyzd, dtype).ravel()
+    dd = None if dl_distortion is None else _a(dl_distortion, dtype).ravel()
+    getattr(lib(), "orc_interval_backward" + _sfx(dtype))(
+        _a(ivl, dtype).ravel(), _a(sorted_vals, np.uint32), _a(rng_, np.uint32), _a(cp, np.float32),
+        _a(vp, np.float32), _a(ci, np.float32), w, h, ts, M, _a(fwd["image"], dtype).ravel(),
+        _a(fwd["distortion"], dtype).ravel(), _a(dl_image, dtype).ravel(), _ptr(dx), _ptr(dd), d)
+    return d.reshape(M, IVL_NF)
+
+
+def interval_chain(scene, cam, w, h, ivl, d_ivl, dtype=np.float32):
+    """Per-tet chain back of d_ivl to d_vertices, d_densities,
+    d_base_colors, d_color_grads, d_vertex_normals."""
+    cp, vp, ci = cam
+    M = len(ivl)
+    N = len(scene["vertices"]) // 3
+    dV = np.zeros(N * 3, dtype=dtype)
+    dD = np.zeros(M, dtype=dtype)
+    dB = np.zeros(M * 3, dtype=dtype)
+    dG = np.zeros(M * 3, dtype=dtype)
+    dN = np.zeros(N * 3, dtype=dtype)
+    getattr(lib(), "orc_interval_chain" + _sfx(dtype))(
+        _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(scene["color_grads"], dtype),
+        _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32), w, h, M, _a(ivl, dtype).ravel(),
+        _a(d_ivl, dtype).ravel(), dV, dD, dB, dG, dN)
+    return {"d_vertices": dV, "d_densities": dD, "d_base_colors": dB, "d_color_grads": dG,
+            "d_vertex_normals": dN}
+
+
+def interval_backward_full(scene, cam, w, h, sorted_vals, rng_, dl_image, dl_xyzd=None, dl_distortion=None, ts=12,
+                           colors=None, dtype=np.float32):
+    """backward_interval_tiled end to end (decomposition, forward, adjoint, chain back)."""
+    ivl = interval_generate(scene, cam, w, h, colors=colors, dtype=dtype)
+    fwd = interval_composite(ivl, cam, w, h, sorted_vals, rng_, ts=ts, dtype=dtype)
+    d_ivl = interval_backward(ivl, cam, w, h, sorted_vals, rng_, fwd, dl_image, dl_xyzd, dl_distortion, ts=ts,
+                              dtype=dtype)
+    return interval_chain(scene, cam, w, h, ivl, d_ivl, dtype=dtype)
 
 
 def error_stats(scene, cam, w, h, sorted_vals, rng_, pixel_err, ssim_err, ts=12, min_t=0.0, dtype=np.float32):

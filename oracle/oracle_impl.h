@@ -506,26 +506,63 @@ void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const
 }
 
 /* ------------------------------------------------------------------ */
-/* Interval-tiled forward (forward_interval_tiled, lib.rs:1218-1390):    */
-/* per-tet silhouette decomposition (interval_generate.wgsl:174-408)     */
-/* and tiled compositing over it (interval_tiled_rasterize.wgsl:96-441).  */
-/* Vertex normals are zero (set_vertex_normals is outside this path), so  */
-/* xyzd carries depth only.  near/far are the reference's 0.01 / 1000.    */
-/* Parity unpinned: the screen-space interpolation of the colour offsets  */
-/* for g != 0 (only the GPU is compared with it).                         */
+/* Interval-tiled forward and adjoint (forward_interval_tiled,          */
+/* lib.rs:1218-1390; backward_interval_tiled, lib.rs:2049-2240):         */
+/* per-tet silhouette decomposition (interval_generate.wgsl:174-408),    */
+/* tiled compositing over it (interval_tiled_rasterize.wgsl:96-441), the */
+/* per-tile adjoint (backward_interval_tiled.wgsl:165-720) and the per-  */
+/* tet chain back to the scene (interval_chain_back.wgsl:160-545).       */
+/* near/far are the reference's 0.01 / 1000.  The decomposition is kept  */
+/* per tet id (the reference stores it per compact visible index but     */
+/* reads it back through the tile lists' tet ids: the two agree when the */
+/* compaction is the identity).                                          */
+/* Parity unpinned: the screen-space interpolation of the colour offsets */
+/* for g != 0 (forward), and the adjoint's depth and distortion terms,   */
+/* which the reference does not form as derivatives of its forward       */
+/* (only the GPU is compared with them).                                 */
 /* ------------------------------------------------------------------ */
 typedef struct {
     R px[5], py[5], zf[5], zb[5], of[5], ob[5]; /* slots 0-3 silhouette, 4 centre */
+    R nf[5][3], nb[5][3];                      /* interpolated vertex normals */
     V3 base;                                   /* base colour at the camera */
     R dens;
     uint32_t nsil;                             /* 0: skipped */
+    /* the decomposition's discrete choices and weights */
+    int inner;                                 /* inner vertex (3 points), 4: quad */
+    int ea, eb;                                /* crossing edges (quad) */
+    R bw[3];                                   /* inner vertex in its opposite face (3 points) */
+    R ta, tb;                                  /* crossing parameters (quad) */
 } FN(ivl_);
 #define IVL FN(ivl_)
+/* flat layout of one decomposition (exported to the tests) */
+#define IVL_NF 80
 
 static const int FN(OPPF)[4] = {1, 2, 3, 0};
 static const int FN(TE)[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
 
 static R FN(mixr)(R a, R b, R t) { return a * ((R)1.0 - t) + b * t; }
+
+static void FN(ivl_pack)(const IVL* v, R* o) {
+    for (int k = 0; k < 5; ++k) {
+        o[k] = v->px[k]; o[5 + k] = v->py[k]; o[10 + k] = v->zf[k]; o[15 + k] = v->zb[k];
+        o[20 + k] = v->of[k]; o[25 + k] = v->ob[k];
+        for (int c = 0; c < 3; ++c) { o[30 + 3 * k + c] = v->nf[k][c]; o[45 + 3 * k + c] = v->nb[k][c]; }
+    }
+    o[60] = v->base.x; o[61] = v->base.y; o[62] = v->base.z; o[63] = v->dens;
+    o[64] = (R)v->nsil; o[65] = (R)v->inner; o[66] = (R)v->ea; o[67] = (R)v->eb;
+    o[68] = v->bw[0]; o[69] = v->bw[1]; o[70] = v->bw[2]; o[71] = v->ta; o[72] = v->tb;
+    for (int k = 73; k < IVL_NF; ++k) o[k] = 0;
+}
+static void FN(ivl_unpack)(const R* o, IVL* v) {
+    for (int k = 0; k < 5; ++k) {
+        v->px[k] = o[k]; v->py[k] = o[5 + k]; v->zf[k] = o[10 + k]; v->zb[k] = o[15 + k];
+        v->of[k] = o[20 + k]; v->ob[k] = o[25 + k];
+        for (int c = 0; c < 3; ++c) { v->nf[k][c] = o[30 + 3 * k + c]; v->nb[k][c] = o[45 + 3 * k + c]; }
+    }
+    v->base = FN(mk)(o[60], o[61], o[62]); v->dens = o[63];
+    v->nsil = (uint32_t)o[64]; v->inner = (int)o[65]; v->ea = (int)o[66]; v->eb = (int)o[67];
+    v->bw[0] = o[68]; v->bw[1] = o[69]; v->bw[2] = o[70]; v->ta = o[71]; v->tb = o[72];
+}
 
 /* interval_generate.wgsl:89-109 */
 static uint32_t FN(classify_sil)(const R p[4][2]) {
@@ -566,50 +603,97 @@ static R FN(line_t)(R a1x, R a1y, R a2x, R a2y, R b1x, R b1y, R b2x, R b2y) {
     return (dx * d2y - dy * d2x) / denom;
 }
 
-static void FN(ivl_generate)(const CAM* c, const V3 v[4], V3 col, V3 g, R dens, IVL* o) {
-    memset(o, 0, sizeof(*o));
+/* Per-vertex NDC z, pixel position and colour offset. */
+static int FN(ivl_project)(const CAM* c, const V3 v[4], V3 g, R p[4][2], R nz[4], R off[4], R inv_w[4]) {
     R clip[4][4];
     for (int i = 0; i < 4; ++i) FN(mul_vp)(c->vp, v[i], clip[i]);
-    if (clip[0][3] <= 0 || clip[1][3] <= 0 || clip[2][3] <= 0 || clip[3][3] <= 0) return;
+    int behind = clip[0][3] <= 0 || clip[1][3] <= 0 || clip[2][3] <= 0 || clip[3][3] <= 0;
     R W = (R)c->width, H = (R)c->height;
-    R p[4][2], nz[4];
+    V3 cam = FN(mk)(c->cam[0], c->cam[1], c->cam[2]);
     for (int i = 0; i < 4; ++i) {
         R w = clip[i][3];
-        R inv_w = RABS(w) > (R)1e-10f ? (R)1.0 / w : (R)0.0;
-        R nx = clip[i][0] * inv_w, ny = clip[i][1] * inv_w;
-        nz[i] = clip[i][2] * inv_w;
+        inv_w[i] = RABS(w) > (R)1e-10f ? (R)1.0 / w : (R)0.0;
+        R nx = clip[i][0] * inv_w[i], ny = clip[i][1] * inv_w[i];
+        nz[i] = clip[i][2] * inv_w[i];
         p[i][0] = ((nx + (R)1.0) * (R)0.5) * W;
         p[i][1] = (((R)1.0 - ny) * (R)0.5) * H;
+        off[i] = FN(dot)(g, FN(sub)(v[i], cam));
     }
+    return behind;
+}
+
+/* The attributes of a decomposition whose discrete choices and weights
+ * (inner vertex / crossing edges, bw / ta, tb) are given in o: front/back
+ * NDC z, colour offsets, normals of the five slots and the base colour.
+ * vn: vertex normals or NULL (zero). */
+static void FN(ivl_attrs)(const CAM* c, const V3 v[4], const V3* vn, V3 col, V3 g, IVL* o) {
+    R p[4][2], nz[4], off[4], iw[4];
+    FN(ivl_project)(c, v, g, p, nz, off, iw);
     V3 cam = FN(mk)(c->cam[0], c->cam[1], c->cam[2]);
     R bo = FN(dot)(g, FN(sub)(cam, v[0]));
     o->base = FN(mk)(col.x + bo, col.y + bo, col.z + bo);
-    o->dens = dens;
-    R zmax = FN(rmax)(FN(rmax)(nz[0], nz[1]), FN(rmax)(nz[2], nz[3]));
-    R zmin = FN(rmin)(FN(rmin)(nz[0], nz[1]), FN(rmin)(nz[2], nz[3]));
-    if (zmax - zmin < (R)1e-7f || dens < (R)1e-6f) return;
-    R off[4];
-    for (int i = 0; i < 4; ++i) off[i] = FN(dot)(g, FN(sub)(v[i], cam));
-    uint32_t iv = FN(classify_sil)((const R(*)[2])p);
-    if (iv < 4u) {
-        const int* f = FN(FACES)[FN(OPPF)[iv]];
-        int si[3] = {f[0], f[1], f[2]}, ce = f[3];
-        R b[3];
-        FN(bary2d)(p[ce][0], p[ce][1], p[si[0]][0], p[si[0]][1], p[si[1]][0], p[si[1]][1], p[si[2]][0], p[si[2]][1], b);
-        R z_face = b[0] * nz[si[0]] + b[1] * nz[si[1]] + b[2] * nz[si[2]];
+    V3 n[4];
+    for (int i = 0; i < 4; ++i) n[i] = vn ? vn[i] : FN(mk)(0, 0, 0);
+    int sv[4];
+    V3 cnf, cnb;
+    if (o->inner < 4) {
+        const int* f = FN(FACES)[FN(OPPF)[o->inner]];
+        int ce = f[3];
+        R z_face = o->bw[0] * nz[f[0]] + o->bw[1] * nz[f[1]] + o->bw[2] * nz[f[2]];
         R z_c = nz[ce];
-        R face_off = b[0] * off[si[0]] + b[1] * off[si[1]] + b[2] * off[si[2]];
+        R face_off = o->bw[0] * off[f[0]] + o->bw[1] * off[f[1]] + o->bw[2] * off[f[2]];
+        V3 n_face = FN(add)(FN(add)(FN(scl)(n[f[0]], o->bw[0]), FN(scl)(n[f[1]], o->bw[1])), FN(scl)(n[f[2]], o->bw[2]));
         int face_first = z_face <= z_c;
-        for (int k = 0; k < 4; ++k) {
-            int vi = si[k == 3 ? 0 : k];
-            o->px[k] = p[vi][0]; o->py[k] = p[vi][1];
-            o->zf[k] = o->zb[k] = nz[vi];
-            o->of[k] = o->ob[k] = off[vi];
-        }
+        sv[0] = f[0]; sv[1] = f[1]; sv[2] = f[2]; sv[3] = f[0];
         o->px[4] = p[ce][0]; o->py[4] = p[ce][1];
         o->zf[4] = FN(rmin)(z_face, z_c); o->zb[4] = FN(rmax)(z_face, z_c);
         o->of[4] = face_first ? face_off : off[ce];
         o->ob[4] = face_first ? off[ce] : face_off;
+        cnf = face_first ? n_face : n[ce];
+        cnb = face_first ? n[ce] : n_face;
+    } else {
+        const int* A = FN(TE)[o->ea];
+        const int* Bv = FN(TE)[o->eb];
+        R z_a = FN(mixr)(nz[A[0]], nz[A[1]], o->ta), z_b = FN(mixr)(nz[Bv[0]], nz[Bv[1]], o->tb);
+        R off_a = FN(mixr)(off[A[0]], off[A[1]], o->ta), off_b = FN(mixr)(off[Bv[0]], off[Bv[1]], o->tb);
+        V3 n_a = FN(add)(FN(scl)(n[A[0]], (R)1.0 - o->ta), FN(scl)(n[A[1]], o->ta));
+        V3 n_b = FN(add)(FN(scl)(n[Bv[0]], (R)1.0 - o->tb), FN(scl)(n[Bv[1]], o->tb));
+        sv[0] = A[0]; sv[1] = Bv[0]; sv[2] = A[1]; sv[3] = Bv[1];
+        o->px[4] = FN(mixr)(p[A[0]][0], p[A[1]][0], o->ta);
+        o->py[4] = FN(mixr)(p[A[0]][1], p[A[1]][1], o->ta);
+        o->zf[4] = FN(rmin)(z_a, z_b); o->zb[4] = FN(rmax)(z_a, z_b);
+        o->of[4] = z_a <= z_b ? off_a : off_b;
+        o->ob[4] = z_a <= z_b ? off_b : off_a;
+        cnf = z_a <= z_b ? n_a : n_b;
+        cnb = z_a <= z_b ? n_b : n_a;
+    }
+    for (int k = 0; k < 4; ++k) {
+        o->px[k] = p[sv[k]][0]; o->py[k] = p[sv[k]][1];
+        o->zf[k] = o->zb[k] = nz[sv[k]];
+        o->of[k] = o->ob[k] = off[sv[k]];
+        o->nf[k][0] = o->nb[k][0] = n[sv[k]].x;
+        o->nf[k][1] = o->nb[k][1] = n[sv[k]].y;
+        o->nf[k][2] = o->nb[k][2] = n[sv[k]].z;
+    }
+    o->nf[4][0] = cnf.x; o->nf[4][1] = cnf.y; o->nf[4][2] = cnf.z;
+    o->nb[4][0] = cnb.x; o->nb[4][1] = cnb.y; o->nb[4][2] = cnb.z;
+}
+
+/* interval_generate.wgsl:174-408: the discrete choices, then the attributes. */
+static void FN(ivl_generate)(const CAM* c, const V3 v[4], const V3* vn, V3 col, V3 g, R dens, IVL* o) {
+    memset(o, 0, sizeof(*o));
+    R p[4][2], nz[4], off[4], iw[4];
+    if (FN(ivl_project)(c, v, g, p, nz, off, iw)) return;
+    o->dens = dens;
+    R zmax = FN(rmax)(FN(rmax)(nz[0], nz[1]), FN(rmax)(nz[2], nz[3]));
+    R zmin = FN(rmin)(FN(rmin)(nz[0], nz[1]), FN(rmin)(nz[2], nz[3]));
+    if (zmax - zmin < (R)1e-7f || dens < (R)1e-6f) return;
+    uint32_t iv = FN(classify_sil)((const R(*)[2])p);
+    o->inner = (int)iv;
+    if (iv < 4u) {
+        const int* f = FN(FACES)[FN(OPPF)[iv]];
+        FN(bary2d)(p[f[3]][0], p[f[3]][1], p[f[0]][0], p[f[0]][1], p[f[1]][0], p[f[1]][1], p[f[2]][0], p[f[2]][1],
+                   o->bw);
         o->nsil = 3u;
     } else {
         static const int PAIRS[3][2] = {{0, 5}, {1, 4}, {2, 3}};
@@ -623,23 +707,12 @@ static void FN(ivl_generate)(const CAM* c, const V3 v[4], V3 col, V3 g, R dens, 
         }
         const int* A = FN(TE)[ea];
         const int* Bv = FN(TE)[eb];
-        R ta = FN(line_t)(p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1], p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1]);
-        R tb = FN(line_t)(p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1], p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1]);
-        R cxp = FN(mixr)(p[A[0]][0], p[A[1]][0], ta), cyp = FN(mixr)(p[A[0]][1], p[A[1]][1], ta);
-        R z_a = FN(mixr)(nz[A[0]], nz[A[1]], ta), z_b = FN(mixr)(nz[Bv[0]], nz[Bv[1]], tb);
-        R off_a = FN(mixr)(off[A[0]], off[A[1]], ta), off_b = FN(mixr)(off[Bv[0]], off[Bv[1]], tb);
-        int sv[4] = {A[0], Bv[0], A[1], Bv[1]};
-        for (int k = 0; k < 4; ++k) {
-            o->px[k] = p[sv[k]][0]; o->py[k] = p[sv[k]][1];
-            o->zf[k] = o->zb[k] = nz[sv[k]];
-            o->of[k] = o->ob[k] = off[sv[k]];
-        }
-        o->px[4] = cxp; o->py[4] = cyp;
-        o->zf[4] = FN(rmin)(z_a, z_b); o->zb[4] = FN(rmax)(z_a, z_b);
-        o->of[4] = z_a <= z_b ? off_a : off_b;
-        o->ob[4] = z_a <= z_b ? off_b : off_a;
+        o->ea = ea; o->eb = eb;
+        o->ta = FN(line_t)(p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1], p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1]);
+        o->tb = FN(line_t)(p[Bv[0]][0], p[Bv[0]][1], p[Bv[1]][0], p[Bv[1]][1], p[A[0]][0], p[A[0]][1], p[A[1]][0], p[A[1]][1]);
         o->nsil = 4u;
     }
+    FN(ivl_attrs)(c, v, vn, col, g, o);
 }
 
 /* interval_tiled_rasterize.wgsl:90-97 */
@@ -652,14 +725,95 @@ static int FN(in_tri)(R ptx, R pty, R ax, R ay, R bx, R by, R cx, R cy) {
     return !(has_neg && has_pos);
 }
 
-/* rgba [H*W*4], xyzd [H*W*4], dist [H*W*5] (overwritten for the tiles visited). */
-void FN(orc_interval_forward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
-                              const uint32_t* sorted_vals, const uint32_t* ranges, const float* cam_pos,
-                              const float* vp, const float* c2w_intr, uint32_t w, uint32_t h, uint32_t ts,
-                              R* rgba, R* xyzd, R* dist) {
-    CAM c;
-    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+static void FN(ivl_all)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                        const CAM* c, uint32_t M, IVL* out) {
+    for (uint32_t t = 0; t < M; ++t) {
+        V3 v[4];
+        for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+        FN(ivl_generate)(c, v, NULL, FN(load3)(colors, t), FN(load3)(grads, t), dens[t], &out[t]);
+    }
+}
+
+/* Covered columns of one tile row of the silhouette polygon
+ * (interval_tiled_rasterize.wgsl:192-243): 0 when empty. */
+static int FN(ivl_row_span)(const IVL* iv, uint32_t row, R tile_ox, R tile_oy, uint32_t TS, int32_t* xl, int32_t* xr) {
+    uint32_t ns = iv->nsil;
+    R xl_f = (R)1e10f, xr_f = (R)-1e10f, yc = (R)row + (R)0.5;
+    for (uint32_t e = 0; e < ns; ++e) {
+        uint32_t ej = (e + 1) % ns;
+        R yi = iv->py[e] - tile_oy, yj = iv->py[ej] - tile_oy;
+        R xi = iv->px[e] - tile_ox, xj = iv->px[ej] - tile_ox;
+        if ((yi <= yc && yj > yc) || (yj <= yc && yi > yc)) {
+            R tt = (yc - yi) / (yj - yi);
+            R x = xi + tt * (xj - xi);
+            xl_f = FN(rmin)(xl_f, x); xr_f = FN(rmax)(xr_f, x);
+        }
+    }
+    for (uint32_t k = 0; k < ns; ++k) {
+        R vy = iv->py[k] - tile_oy;
+        if (vy >= yc - (R)0.5 && vy < yc + (R)0.5) {
+            xl_f = FN(rmin)(xl_f, iv->px[k] - tile_ox); xr_f = FN(rmax)(xr_f, iv->px[k] - tile_ox);
+        }
+    }
+    if (!(xl_f <= xr_f)) return 0;
+    R eps = (R)0.001f;
+    *xl = imax32(sat_i32((double)RCEIL(xl_f - (R)0.5 - eps)), 0);
+    *xr = imin32(sat_i32((double)RFLOOR(xr_f - (R)0.5 + eps)), (int32_t)TS - 1);
+    return *xl <= *xr;
+}
+
+/* The first fan triangle (i, i + 1 mod n, centre) holding the pixel centre. */
+static int FN(ivl_find)(const IVL* iv, R pcx, R pcy, uint32_t* ai, uint32_t* bi, R b[3]) {
+    uint32_t ns = iv->nsil;
+    for (uint32_t tri = 0; tri < ns; ++tri) {
+        uint32_t ta = tri, tb = (tri + 1) % ns;
+        if (FN(in_tri)(pcx, pcy, iv->px[ta], iv->py[ta], iv->px[tb], iv->py[tb], iv->px[4], iv->py[4])) {
+            *ai = ta; *bi = tb;
+            FN(bary2d)(pcx, pcy, iv->px[ta], iv->py[ta], iv->px[tb], iv->py[tb], iv->px[4], iv->py[4], b);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* Per-pair quantities shared by the forward and the adjoint
+ * (interval_tiled_rasterize.wgsl:300-400, backward_interval_tiled.wgsl:400-440). */
+typedef struct {
+    R zfn, zbn, zfc, zbc, z_f, z_b, rsc, dist, od, alpha_t, phv, w0, w1;
+    V3 cf_raw, cb_raw, cf, cb, cp;
+} FN(ivl_pair_);
+#define IPAIR FN(ivl_pair_)
+
+static void FN(ivl_pair)(const CAM* c, const IVL* iv, uint32_t ai, uint32_t bi, const R b[3], R pcx, R pcy, IPAIR* q) {
     const R near_p = (R)0.01f, far_p = (R)1000.0f, range_nf = far_p - near_p;
+    q->zfn = b[0] * iv->zf[ai] + b[1] * iv->zf[bi] + b[2] * iv->zf[4];
+    q->zbn = b[0] * iv->zb[ai] + b[1] * iv->zb[bi] + b[2] * iv->zb[4];
+    R ofn = b[0] * iv->of[ai] + b[1] * iv->of[bi] + b[2] * iv->of[4];
+    R obn = b[0] * iv->ob[ai] + b[1] * iv->ob[bi] + b[2] * iv->ob[4];
+    q->zfc = FN(clampr)(q->zfn, (R)0.0, (R)1.0);
+    q->zbc = FN(clampr)(q->zbn, (R)0.0, (R)1.0);
+    q->z_f = (near_p * far_p) / (far_p - q->zfc * range_nf);
+    q->z_b = (near_p * far_p) / (far_p - q->zbc * range_nf);
+    R xc = (pcx - c->intr[2]) / c->intr[0], yc2 = (pcy - c->intr[3]) / c->intr[1];
+    q->rsc = RSQRT((xc * xc + yc2 * yc2) + (R)1.0);
+    q->dist = RABS(q->z_b - q->z_f) * q->rsc;
+    q->cf_raw = FN(mk)(iv->base.x + ofn, iv->base.y + ofn, iv->base.z + ofn);
+    q->cb_raw = FN(mk)(iv->base.x + obn, iv->base.y + obn, iv->base.z + obn);
+    q->cf = FN(mk)(FN(rmax)(q->cf_raw.x, 0), FN(rmax)(q->cf_raw.y, 0), FN(rmax)(q->cf_raw.z, 0));
+    q->cb = FN(mk)(FN(rmax)(q->cb_raw.x, 0), FN(rmax)(q->cb_raw.y, 0), FN(rmax)(q->cb_raw.z, 0));
+    q->od = FN(clampr)(iv->dens * q->dist, (R)1e-8f, (R)88.0);
+    q->alpha_t = FN(safe_exp)(-q->od);
+    q->phv = FN(phi)(q->od);
+    q->w0 = q->phv - q->alpha_t;
+    q->w1 = (R)1.0 - q->phv;
+    q->cp = FN(safe_clip3)(FN(mk)(q->cb.x * q->w0 + q->cf.x * q->w1, q->cb.y * q->w0 + q->cf.y * q->w1,
+                                  q->cb.z * q->w0 + q->cf.z * q->w1), (R)0.0, MAX_VAL_R);
+}
+
+/* Tiled compositing over given decompositions.  rgba [H*W*4], xyzd [H*W*4],
+ * dist [H*W*5] (overwritten for the tiles visited). */
+static void FN(ivl_composite)(const IVL* ivs, const uint32_t* sorted_vals, const uint32_t* ranges, const CAM* c,
+                              uint32_t w, uint32_t h, uint32_t ts, R* rgba, R* xyzd, R* dist) {
     uint32_t tiles_x = (w + ts - 1) / ts, tiles_y = (h + ts - 1) / ts, TS = ts;
     R* st = (R*)calloc((size_t)TS * TS * 13, sizeof(R)); /* colour 3, log T, xyzd 4, dist 5 */
     for (uint32_t tile = 0; tile < tiles_x * tiles_y; ++tile) {
@@ -670,36 +824,11 @@ void FN(orc_interval_forward)(const R* verts, const uint32_t* idx, const R* colo
         uint32_t cursor = re;
         while (cursor > rs) {
             cursor -= 1;
-            uint32_t t = sorted_vals[cursor];
-            V3 v[4];
-            for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
-            IVL iv;
-            FN(ivl_generate)(&c, v, FN(load3)(colors, t), FN(load3)(grads, t), dens[t], &iv);
-            if (iv.nsil == 0) continue;
-            uint32_t ns = iv.nsil;
+            const IVL* iv = &ivs[sorted_vals[cursor]];
+            if (iv->nsil == 0) continue;
             for (uint32_t row = 0; row < TS; ++row) {
-                /* scanline over the silhouette polygon (interval_tiled_rasterize.wgsl:192-243) */
-                R xl_f = (R)1e10f, xr_f = (R)-1e10f, yc = (R)row + (R)0.5;
-                for (uint32_t e = 0; e < ns; ++e) {
-                    uint32_t ej = (e + 1) % ns;
-                    R yi = iv.py[e] - tile_oy, yj = iv.py[ej] - tile_oy;
-                    R xi = iv.px[e] - tile_ox, xj = iv.px[ej] - tile_ox;
-                    if ((yi <= yc && yj > yc) || (yj <= yc && yi > yc)) {
-                        R tt = (yc - yi) / (yj - yi);
-                        R x = xi + tt * (xj - xi);
-                        xl_f = FN(rmin)(xl_f, x); xr_f = FN(rmax)(xr_f, x);
-                    }
-                }
-                for (uint32_t k = 0; k < ns; ++k) {
-                    R vy = iv.py[k] - tile_oy;
-                    if (vy >= yc - (R)0.5 && vy < yc + (R)0.5) {
-                        xl_f = FN(rmin)(xl_f, iv.px[k] - tile_ox); xr_f = FN(rmax)(xr_f, iv.px[k] - tile_ox);
-                    }
-                }
-                if (!(xl_f <= xr_f)) continue;
-                R eps = (R)0.001f;
-                int32_t xl_i = imax32(sat_i32((double)RCEIL(xl_f - (R)0.5 - eps)), 0);
-                int32_t xr_i = imin32(sat_i32((double)RFLOOR(xr_f - (R)0.5 + eps)), (int32_t)TS - 1);
+                int32_t xl_i, xr_i;
+                if (!FN(ivl_row_span)(iv, row, tile_ox, tile_oy, TS, &xl_i, &xr_i)) continue;
                 for (int32_t ci = xl_i; ci <= xr_i; ++ci) {
                     uint32_t colu = (uint32_t)ci, pl = row * TS + colu;
                     uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
@@ -707,50 +836,31 @@ void FN(orc_interval_forward)(const R* verts, const uint32_t* idx, const R* colo
                     R* s = st + 13 * pl;
                     if (s[3] < (R)-13.8) continue;
                     R pcx = (R)px + (R)0.5, pcy = (R)py + (R)0.5;
-                    int found = 0;
-                    uint32_t ai = 0, bi = 0;
-                    R b[3] = {0, 0, 0};
-                    for (uint32_t tri = 0; tri < ns; ++tri) {
-                        uint32_t ta = tri, tb = (tri + 1) % ns;
-                        if (FN(in_tri)(pcx, pcy, iv.px[ta], iv.py[ta], iv.px[tb], iv.py[tb], iv.px[4], iv.py[4])) {
-                            ai = ta; bi = tb;
-                            FN(bary2d)(pcx, pcy, iv.px[ai], iv.py[ai], iv.px[bi], iv.py[bi], iv.px[4], iv.py[4], b);
-                            found = 1;
-                            break;
-                        }
-                    }
-                    if (!found) continue;
-                    R zfn = b[0] * iv.zf[ai] + b[1] * iv.zf[bi] + b[2] * iv.zf[4];
-                    R zbn = b[0] * iv.zb[ai] + b[1] * iv.zb[bi] + b[2] * iv.zb[4];
-                    R ofn = b[0] * iv.of[ai] + b[1] * iv.of[bi] + b[2] * iv.of[4];
-                    R obn = b[0] * iv.ob[ai] + b[1] * iv.ob[bi] + b[2] * iv.ob[4];
-                    R zf = (near_p * far_p) / (far_p - FN(clampr)(zfn, (R)0.0, (R)1.0) * range_nf);
-                    R zb = (near_p * far_p) / (far_p - FN(clampr)(zbn, (R)0.0, (R)1.0) * range_nf);
-                    R xc = (pcx - c.intr[2]) / c.intr[0], yc2 = (pcy - c.intr[3]) / c.intr[1];
-                    R rsc = RSQRT((xc * xc + yc2 * yc2) + (R)1.0);
-                    R dd = RABS(zb - zf) * rsc;
-                    V3 cf = FN(mk)(FN(rmax)(iv.base.x + ofn, 0), FN(rmax)(iv.base.y + ofn, 0), FN(rmax)(iv.base.z + ofn, 0));
-                    V3 cb = FN(mk)(FN(rmax)(iv.base.x + obn, 0), FN(rmax)(iv.base.y + obn, 0), FN(rmax)(iv.base.z + obn, 0));
-                    R od = FN(clampr)(iv.dens * dd, (R)1e-8f, (R)88.0);
-                    R alpha_t = FN(safe_exp)(-od);
-                    R phv = FN(phi)(od);
-                    R w0 = phv - alpha_t, w1 = (R)1.0 - phv;
-                    V3 cp = FN(safe_clip3)(FN(mk)(cb.x * w0 + cf.x * w1, cb.y * w0 + cf.y * w1, cb.z * w0 + cf.z * w1),
-                                           (R)0.0, MAX_VAL_R);
+                    uint32_t ai, bi;
+                    R b[3];
+                    if (!FN(ivl_find)(iv, pcx, pcy, &ai, &bi, b)) continue;
+                    IPAIR q;
+                    FN(ivl_pair)(c, iv, ai, bi, b, pcx, pcy, &q);
                     R Tj = FN(safe_exp)(s[3]);
                     if (Tj < (R)1e-6f) continue;
-                    s[0] += cp.x * Tj; s[1] += cp.y * Tj; s[2] += cp.z * Tj;
-                    s[3] = s[3] - od;
-                    /* xyzd: zero vertex normals, depth only */
-                    s[7] += Tj * (w0 * zb + w1 * zf);
+                    s[0] += q.cp.x * Tj; s[1] += q.cp.y * Tj; s[2] += q.cp.z * Tj;
+                    s[3] = s[3] - q.od;
+                    /* xyzd (interval_tiled_rasterize.wgsl:360-375): normals, depth */
+                    R nfv[3], nbv[3];
+                    for (int k = 0; k < 3; ++k) {
+                        nfv[k] = b[0] * iv->nf[ai][k] + b[1] * iv->nf[bi][k] + b[2] * iv->nf[4][k];
+                        nbv[k] = b[0] * iv->nb[ai][k] + b[1] * iv->nb[bi][k] + b[2] * iv->nb[4][k];
+                        s[4 + k] += Tj * (nbv[k] * q.w0 + nfv[k] * q.w1);
+                    }
+                    s[7] += Tj * (q.w0 * q.z_b + q.w1 * q.z_f);
                     /* distortion state (interval_tiled_rasterize.wgsl:383-400) */
-                    R enter = zf * rsc, exit_d = zb * rsc;
-                    R u = iv.dens * (exit_d - enter);
+                    R enter = q.z_f * q.rsc, exit_d = q.z_b * q.rsc;
+                    R u = iv->dens * (exit_d - enter);
                     R en = FN(safe_exp)(-u);
                     R ad = FN(rmax)((R)1.0 - en, 0);
                     R m0 = Tj * ad;
-                    R m1 = Tj * (enter * ad + FN(safe_div)((R)1.0 - ((R)1.0 + u) * en, iv.dens));
-                    R sd = Tj * Tj * FN(safe_div)((R)1.0 - (R)2.0 * u * en - en * en, iv.dens);
+                    R m1 = Tj * (enter * ad + FN(safe_div)((R)1.0 - ((R)1.0 + u) * en, iv->dens));
+                    R sd = Tj * Tj * FN(safe_div)((R)1.0 - (R)2.0 * u * en - en * en, iv->dens);
                     R* ds = s + 8;
                     ds[0] += (R)2.0 * m1 * ds[2];
                     ds[1] += (R)2.0 * m0 * ds[3];
@@ -772,6 +882,332 @@ void FN(orc_interval_forward)(const R* verts, const uint32_t* idx, const R* colo
         }
     }
     free(st);
+}
+
+/* rgba [H*W*4], xyzd [H*W*4], dist [H*W*5] (overwritten for the tiles visited). */
+void FN(orc_interval_forward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                              const uint32_t* sorted_vals, const uint32_t* ranges, const float* cam_pos,
+                              const float* vp, const float* c2w_intr, uint32_t w, uint32_t h, uint32_t ts,
+                              uint32_t M, R* rgba, R* xyzd, R* dist) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    IVL* ivs = (IVL*)malloc(sizeof(IVL) * (M ? M : 1));
+    FN(ivl_all)(verts, idx, colors, dens, grads, &c, M, ivs);
+    FN(ivl_composite)(ivs, sorted_vals, ranges, &c, w, h, ts, rgba, xyzd, dist);
+    free(ivs);
+}
+
+/* The decompositions [M][IVL_NF] (vn [N*3] vertex normals or NULL). */
+void FN(orc_interval_generate)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                               const R* vn, const float* cam_pos, const float* vp, const float* c2w_intr, uint32_t w,
+                               uint32_t h, uint32_t M, R* out) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, 12, 0.0f);
+    for (uint32_t t = 0; t < M; ++t) {
+        V3 v[4], n[4];
+        for (int i = 0; i < 4; ++i) {
+            v[i] = FN(load3)(verts, idx[4 * t + i]);
+            if (vn) n[i] = FN(load3)(vn, idx[4 * t + i]);
+        }
+        IVL iv;
+        FN(ivl_generate)(&c, v, vn ? n : NULL, FN(load3)(colors, t), FN(load3)(grads, t), dens[t], &iv);
+        FN(ivl_pack)(&iv, out + (size_t)IVL_NF * t);
+    }
+}
+
+/* The attributes of (possibly moved) vertices under the discrete choices and
+ * weights of the given decompositions (in: [M][IVL_NF]; out likewise). */
+void FN(orc_interval_attrs)(const R* verts, const uint32_t* idx, const R* colors, const R* grads, const R* vn,
+                            const float* cam_pos, const float* vp, const float* c2w_intr, uint32_t w, uint32_t h,
+                            uint32_t M, const R* in, R* out) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, 12, 0.0f);
+    for (uint32_t t = 0; t < M; ++t) {
+        IVL iv;
+        FN(ivl_unpack)(in + (size_t)IVL_NF * t, &iv);
+        if (iv.nsil) {
+            V3 v[4], n[4];
+            for (int i = 0; i < 4; ++i) {
+                v[i] = FN(load3)(verts, idx[4 * t + i]);
+                if (vn) n[i] = FN(load3)(vn, idx[4 * t + i]);
+            }
+            FN(ivl_attrs)(&c, v, vn ? n : NULL, FN(load3)(colors, t), FN(load3)(grads, t), &iv);
+        }
+        FN(ivl_pack)(&iv, out + (size_t)IVL_NF * t);
+    }
+}
+
+/* Compositing over given decompositions [M][IVL_NF]. */
+void FN(orc_interval_composite)(const R* ivf, const uint32_t* sorted_vals, const uint32_t* ranges,
+                                const float* cam_pos, const float* vp, const float* c2w_intr, uint32_t w, uint32_t h,
+                                uint32_t ts, uint32_t M, R* rgba, R* xyzd, R* dist) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    IVL* ivs = (IVL*)malloc(sizeof(IVL) * (M ? M : 1));
+    for (uint32_t t = 0; t < M; ++t) FN(ivl_unpack)(ivf + (size_t)IVL_NF * t, &ivs[t]);
+    FN(ivl_composite)(ivs, sorted_vals, ranges, &c, w, h, ts, rgba, xyzd, dist);
+    free(ivs);
+}
+
+/* Per-tile adjoint (backward_interval_tiled.wgsl:165-720) over given
+ * decompositions: accumulates d_ivl [M][IVL_NF] in the flat layout (slot
+ * attributes, normals, base colour, density).  rgba / dist: the forward's
+ * outputs; dl_xyzd / dl_dist may be NULL (zero). */
+void FN(orc_interval_backward)(const R* ivf, const uint32_t* sorted_vals, const uint32_t* ranges,
+                               const float* cam_pos, const float* vp, const float* c2w_intr, uint32_t w, uint32_t h,
+                               uint32_t ts, uint32_t M, const R* rgba, const R* dist_img, const R* dl_img,
+                               const R* dl_xyzd, const R* dl_dist, R* d_ivl) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    const R near_p = (R)0.01f, far_p = (R)1000.0f, range_nf = far_p - near_p;
+    IVL* ivs = (IVL*)malloc(sizeof(IVL) * (M ? M : 1));
+    for (uint32_t t = 0; t < M; ++t) FN(ivl_unpack)(ivf + (size_t)IVL_NF * t, &ivs[t]);
+    uint32_t tiles_x = (w + ts - 1) / ts, tiles_y = (h + ts - 1) / ts, TS = ts;
+    /* per pixel: colour 3, log T, d log T, dist z, dist w, d dist z, d dist w */
+    R* st = (R*)calloc((size_t)TS * TS * 9, sizeof(R));
+    for (uint32_t tile = 0; tile < tiles_x * tiles_y; ++tile) {
+        uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
+        R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
+        for (uint32_t i = 0; i < TS * TS; ++i) {
+            uint32_t row = i / TS, colu = i % TS, px = tile_x * TS + colu, py = tile_y * TS + row;
+            R* s = st + 9 * i;
+            memset(s, 0, sizeof(R) * 9);
+            if (px >= w || py >= h) continue;
+            size_t pix = (size_t)py * w + px;
+            R a = rgba[4 * pix + 3];
+            s[0] = rgba[4 * pix]; s[1] = rgba[4 * pix + 1]; s[2] = rgba[4 * pix + 2];
+            s[3] = RLOG(FN(rmax)((R)1.0 - a, (R)1e-20f));
+            s[4] = -dl_img[4 * pix + 3] * ((R)1.0 - a);
+            s[5] = dist_img[5 * pix + 2];
+            s[6] = dist_img[5 * pix + 3];
+            s[7] = dl_dist ? dl_dist[5 * pix + 2] : 0;
+            s[8] = dl_dist ? dl_dist[5 * pix + 3] : 0;
+        }
+        uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+        for (uint32_t cursor = rs; cursor < re; ++cursor) {
+            uint32_t t = sorted_vals[cursor];
+            const IVL* iv = &ivs[t];
+            if (iv->nsil == 0) continue;
+            R* D = d_ivl + (size_t)IVL_NF * t;
+            for (uint32_t row = 0; row < TS; ++row) {
+                int32_t xl_i, xr_i;
+                if (!FN(ivl_row_span)(iv, row, tile_ox, tile_oy, TS, &xl_i, &xr_i)) continue;
+                for (int32_t ci = xl_i; ci <= xr_i; ++ci) {
+                    uint32_t colu = (uint32_t)ci, pl = row * TS + colu;
+                    uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+                    if (px >= w || py >= h) continue;
+                    size_t pix = (size_t)py * w + px;
+                    R* s = st + 9 * pl;
+                    R pcx = (R)px + (R)0.5, pcy = (R)py + (R)0.5;
+                    uint32_t ai, bi;
+                    R b[3];
+                    if (!FN(ivl_find)(iv, pcx, pcy, &ai, &bi, b)) continue;
+                    IPAIR q;
+                    FN(ivl_pair)(&c, iv, ai, bi, b, pcx, pcy, &q);
+                    /* undo (backward_interval_tiled.wgsl:432-447) */
+                    R d_log_t = s[4];
+                    R prev_log_t = FN(rmin)(s[3] + q.od, (R)0.0);
+                    R t_prev = FN(safe_exp)(prev_log_t);
+                    s[0] -= q.cp.x * t_prev; s[1] -= q.cp.y * t_prev; s[2] -= q.cp.z * t_prev;
+                    s[3] = prev_log_t;
+                    if (t_prev < (R)1e-6f) continue;
+                    V3 d_color = FN(mk)(dl_img[4 * pix], dl_img[4 * pix + 1], dl_img[4 * pix + 2]);
+                    V3 d_cp = FN(safe_clip3)(FN(scl)(d_color, t_prev), MIN_VAL_R, MAX_VAL_R);
+                    R d_od_state = -d_log_t;
+                    s[4] = FN(safe_clip)(d_log_t + FN(dot)(d_color, q.cp) * t_prev, MIN_VAL_R, MAX_VAL_R);
+                    R dphi = FN(dphi_dx)(q.od);
+                    R dw0 = dphi + q.alpha_t, dw1 = -dphi;
+                    V3 d_cb_int = FN(scl)(d_cp, q.w0), d_cf_int = FN(scl)(d_cp, q.w1);
+                    R d_od_int = FN(dot)(d_cp, FN(add)(FN(scl)(q.cb, dw0), FN(scl)(q.cf, dw1)));
+                    R d_od = FN(safe_clip)(d_od_state + d_od_int, MIN_VAL_R, MAX_VAL_R);
+                    R od_raw = iv->dens * q.dist;
+                    R d_od_raw = (od_raw >= (R)1e-8f && od_raw <= (R)88.0) ? d_od : (R)0.0;
+                    R d_dens_local = FN(safe_clip)(d_od_raw * q.dist, MIN_VAL_R, MAX_VAL_R);
+                    R d_dist = FN(safe_clip)(d_od_raw * iv->dens, MIN_VAL_R, MAX_VAL_R);
+                    R sgn = q.z_b >= q.z_f ? (R)1.0 : (R)-1.0;
+                    R d_zf_view = -d_dist * q.rsc * sgn, d_zb_view = d_dist * q.rsc * sgn;
+                    R den_f = far_p - q.zfc * range_nf, den_b = far_p - q.zbc * range_nf;
+                    R dz_f = near_p * far_p * range_nf / (den_f * den_f);
+                    R dz_b = near_p * far_p * range_nf / (den_b * den_b);
+                    int zf_act = q.zfn >= 0 && q.zfn <= 1, zb_act = q.zbn >= 0 && q.zbn <= 1;
+                    R d_zfn = zf_act ? d_zf_view * dz_f : (R)0.0;
+                    R d_zbn = zb_act ? d_zb_view * dz_b : (R)0.0;
+                    V3 d_cb_raw = FN(mk)(q.cb_raw.x > 0 ? d_cb_int.x : 0, q.cb_raw.y > 0 ? d_cb_int.y : 0,
+                                         q.cb_raw.z > 0 ? d_cb_int.z : 0);
+                    V3 d_cf_raw = FN(mk)(q.cf_raw.x > 0 ? d_cf_int.x : 0, q.cf_raw.y > 0 ? d_cf_int.y : 0,
+                                         q.cf_raw.z > 0 ? d_cf_int.z : 0);
+                    V3 d_bc = FN(add)(d_cf_raw, d_cb_raw);
+                    R d_of = (d_cf_raw.x + d_cf_raw.y) + d_cf_raw.z;
+                    R d_ob = (d_cb_raw.x + d_cb_raw.y) + d_cb_raw.z;
+                    const uint32_t sl[3] = {ai, bi, 4u};
+                    for (int k = 0; k < 3; ++k) {
+                        D[10 + sl[k]] += b[k] * d_zfn;
+                        D[15 + sl[k]] += b[k] * d_zbn;
+                        D[20 + sl[k]] += b[k] * d_of;
+                        D[25 + sl[k]] += b[k] * d_ob;
+                    }
+                    V3 cbc = FN(safe_clip3)(d_bc, MIN_VAL_R, MAX_VAL_R);
+                    D[60] += cbc.x; D[61] += cbc.y; D[62] += cbc.z;
+                    D[63] += FN(safe_clip)(d_dens_local, MIN_VAL_R, MAX_VAL_R);
+                    /* xyzd (backward_interval_tiled.wgsl:544-600) */
+                    R dx4[4] = {0, 0, 0, 0};
+                    if (dl_xyzd) for (int k = 0; k < 4; ++k) dx4[k] = dl_xyzd[4 * pix + k];
+                    R nfv[3], nbv[3], ncon[3];
+                    for (int k = 0; k < 3; ++k) {
+                        nfv[k] = b[0] * iv->nf[ai][k] + b[1] * iv->nf[bi][k] + b[2] * iv->nf[4][k];
+                        nbv[k] = b[0] * iv->nb[ai][k] + b[1] * iv->nb[bi][k] + b[2] * iv->nb[4][k];
+                        ncon[k] = nbv[k] * q.w0 + nfv[k] * q.w1;
+                    }
+                    R depth_c = q.w0 * q.z_b + q.w1 * q.z_f;
+                    R d_od_x = -(((dx4[0] * ncon[0] + dx4[1] * ncon[1]) + dx4[2] * ncon[2]) + dx4[3] * depth_c) * t_prev;
+                    R dn[3] = {dx4[0] * t_prev, dx4[1] * t_prev, dx4[2] * t_prev};
+                    R d_depth = dx4[3] * t_prev;
+                    R d_od_w = (dn[0] * (nbv[0] * dw0 + nfv[0] * dw1) + dn[1] * (nbv[1] * dw0 + nfv[1] * dw1)) +
+                               dn[2] * (nbv[2] * dw0 + nfv[2] * dw1) + d_depth * (q.z_b * dw0 + q.z_f * dw1);
+                    D[63] += FN(safe_clip)((d_od_x + d_od_w) * q.dist, MIN_VAL_R, MAX_VAL_R);
+                    R d_zf_x = zf_act ? (d_depth * q.w1) * dz_f : (R)0.0;
+                    R d_zb_x = zb_act ? (d_depth * q.w0) * dz_b : (R)0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        D[10 + sl[k]] += b[k] * d_zf_x;
+                        D[15 + sl[k]] += b[k] * d_zb_x;
+                        for (int j = 0; j < 3; ++j) {
+                            D[30 + 3 * sl[k] + j] += b[k] * (dn[j] * q.w1);
+                            D[45 + 3 * sl[k] + j] += b[k] * (dn[j] * q.w0);
+                        }
+                    }
+                    /* distortion (backward_interval_tiled.wgsl:602-650) */
+                    R enter = q.z_f * q.rsc, exit_d = q.z_b * q.rsc, dt_d = exit_d - enter;
+                    R u = iv->dens * dt_d;
+                    R en = FN(safe_exp)(-u);
+                    R ad = FN(rmax)((R)1.0 - en, 0);
+                    R m0 = t_prev * ad;
+                    R m1 = t_prev * (enter * ad + FN(safe_div)((R)1.0 - ((R)1.0 + u) * en, iv->dens));
+                    R old_z = s[5] - m0, old_w = s[6] - m1;
+                    s[5] = old_z; s[6] = old_w;
+                    R d_x = dl_dist ? dl_dist[5 * pix] : 0, d_y = dl_dist ? dl_dist[5 * pix + 1] : 0;
+                    R d_z = s[7], d_w = s[8];
+                    R d_m0 = d_z + (R)2.0 * old_w * d_y;
+                    s[7] = d_z + (R)2.0 * m1 * d_x;
+                    s[8] = d_w + (R)2.0 * m0 * d_y;
+                    R d_u = (d_m0 * t_prev) * en;
+                    D[63] += FN(safe_clip)(d_u * dt_d, MIN_VAL_R, MAX_VAL_R);
+                }
+            }
+        }
+    }
+    free(st);
+    free(ivs);
+}
+
+/* d(ndc z)/d(v) of vertex v with 1/w = iw: (vp row z - ndc z vp row w) / w */
+static V3 FN(ivl_dz)(const CAM* c, R nz, R iw, R g) {
+    V3 rz = FN(mk)(c->vp[2], c->vp[6], c->vp[10]), rw = FN(mk)(c->vp[3], c->vp[7], c->vp[11]);
+    return FN(scl)(FN(scl)(FN(sub)(rz, FN(scl)(rw, nz)), iw), g);
+}
+
+/* Per-tet chain back (interval_chain_back.wgsl:160-545) of d_ivl
+ * [M][IVL_NF] to the scene: d_verts [N*3], d_dens [M], d_base [M*3],
+ * d_grads [M*3], d_vn [N*3] (accumulated). */
+void FN(orc_interval_chain)(const R* verts, const uint32_t* idx, const R* grads, const float* cam_pos,
+                            const float* vp, const float* c2w_intr, uint32_t w, uint32_t h, uint32_t M,
+                            const R* ivf, const R* d_ivl, R* d_verts, R* d_dens, R* d_base, R* d_grads, R* d_vn) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, 12, 0.0f);
+    V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
+    for (uint32_t t = 0; t < M; ++t) {
+        IVL iv, d;
+        FN(ivl_unpack)(ivf + (size_t)IVL_NF * t, &iv);
+        if (iv.nsil == 0) continue;
+        FN(ivl_unpack)(d_ivl + (size_t)IVL_NF * t, &d);
+        V3 v[4];
+        for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+        V3 g = FN(load3)(grads, t);
+        R p[4][2], nz[4], off[4], iw[4];
+        FN(ivl_project)(&c, v, g, p, nz, off, iw);
+        V3 dv[4], dn[4];
+        for (int i = 0; i < 4; ++i) { dv[i] = FN(mk)(0, 0, 0); dn[i] = FN(mk)(0, 0, 0); }
+        V3 dg = FN(mk)(0, 0, 0);
+        R dbs = (d.base.x + d.base.y) + d.base.z;
+        dg = FN(add)(dg, FN(scl)(FN(sub)(cam, v[0]), dbs));
+        dv[0] = FN(sub)(dv[0], FN(scl)(g, dbs));
+#define CLIP3(x) FN(safe_clip3)((x), MIN_VAL_R, MAX_VAL_R)
+        int sv[4];
+        if (iv.inner < 4) {
+            const int* f = FN(FACES)[FN(OPPF)[iv.inner]];
+            sv[0] = f[0]; sv[1] = f[1]; sv[2] = f[2]; sv[3] = f[0];
+        } else {
+            const int* A = FN(TE)[iv.ea];
+            const int* Bv = FN(TE)[iv.eb];
+            sv[0] = A[0]; sv[1] = Bv[0]; sv[2] = A[1]; sv[3] = Bv[1];
+        }
+        for (int i = 0; i < 4; ++i) {
+            int vi = sv[i];
+            R dz = d.zf[i] + d.zb[i];
+            dv[vi] = FN(add)(dv[vi], CLIP3(FN(ivl_dz)(&c, nz[vi], iw[vi], dz)));
+            R doff = d.of[i] + d.ob[i];
+            dv[vi] = FN(add)(dv[vi], CLIP3(FN(scl)(g, doff)));
+            dg = FN(add)(dg, CLIP3(FN(scl)(FN(sub)(v[vi], cam), doff)));
+            dn[vi] = FN(add)(dn[vi], FN(add)(FN(mk)(d.nf[i][0], d.nf[i][1], d.nf[i][2]),
+                                             FN(mk)(d.nb[i][0], d.nb[i][1], d.nb[i][2])));
+        }
+        V3 cnf = FN(mk)(d.nf[4][0], d.nf[4][1], d.nf[4][2]), cnb = FN(mk)(d.nb[4][0], d.nb[4][1], d.nb[4][2]);
+        if (iv.inner < 4) {
+            const int* f = FN(FACES)[FN(OPPF)[iv.inner]];
+            int ce = f[3];
+            R z_face = iv.bw[0] * nz[f[0]] + iv.bw[1] * nz[f[1]] + iv.bw[2] * nz[f[2]];
+            int ff = z_face <= nz[ce];
+            R dzf = ff ? d.zf[4] : d.zb[4], dzc = ff ? d.zb[4] : d.zf[4];
+            R dof = ff ? d.of[4] : d.ob[4], doc = ff ? d.ob[4] : d.of[4];
+            for (int i = 0; i < 3; ++i)
+                dv[f[i]] = FN(add)(dv[f[i]], CLIP3(FN(ivl_dz)(&c, nz[f[i]], iw[f[i]], iv.bw[i] * dzf)));
+            dv[ce] = FN(add)(dv[ce], CLIP3(FN(ivl_dz)(&c, nz[ce], iw[ce], dzc)));
+            for (int i = 0; i < 3; ++i) {
+                R doi = iv.bw[i] * dof;
+                dv[f[i]] = FN(add)(dv[f[i]], CLIP3(FN(scl)(g, doi)));
+                dg = FN(add)(dg, CLIP3(FN(scl)(FN(sub)(v[f[i]], cam), doi)));
+            }
+            dv[ce] = FN(add)(dv[ce], CLIP3(FN(scl)(g, doc)));
+            dg = FN(add)(dg, CLIP3(FN(scl)(FN(sub)(v[ce], cam), doc)));
+            V3 dnf = ff ? cnf : cnb, dnc = ff ? cnb : cnf;
+            for (int i = 0; i < 3; ++i) dn[f[i]] = FN(add)(dn[f[i]], FN(scl)(dnf, iv.bw[i]));
+            dn[ce] = FN(add)(dn[ce], dnc);
+        } else {
+            const int* A = FN(TE)[iv.ea];
+            const int* Bv = FN(TE)[iv.eb];
+            R ta = iv.ta, tb = iv.tb;
+            R z_a = FN(mixr)(nz[A[0]], nz[A[1]], ta), z_b = FN(mixr)(nz[Bv[0]], nz[Bv[1]], tb);
+            int af = z_a <= z_b;
+            R dza = af ? d.zf[4] : d.zb[4], dzb = af ? d.zb[4] : d.zf[4];
+            R doa = af ? d.of[4] : d.ob[4], dob = af ? d.ob[4] : d.of[4];
+            dv[A[0]] = FN(add)(dv[A[0]], CLIP3(FN(ivl_dz)(&c, nz[A[0]], iw[A[0]], ((R)1.0 - ta) * dza)));
+            dv[A[1]] = FN(add)(dv[A[1]], CLIP3(FN(ivl_dz)(&c, nz[A[1]], iw[A[1]], ta * dza)));
+            dv[Bv[0]] = FN(add)(dv[Bv[0]], CLIP3(FN(ivl_dz)(&c, nz[Bv[0]], iw[Bv[0]], ((R)1.0 - tb) * dzb)));
+            dv[Bv[1]] = FN(add)(dv[Bv[1]], CLIP3(FN(ivl_dz)(&c, nz[Bv[1]], iw[Bv[1]], tb * dzb)));
+            R oa0 = ((R)1.0 - ta) * doa, oa1 = ta * doa, ob0 = ((R)1.0 - tb) * dob, ob1 = tb * dob;
+            dv[A[0]] = FN(add)(dv[A[0]], CLIP3(FN(scl)(g, oa0)));
+            dv[A[1]] = FN(add)(dv[A[1]], CLIP3(FN(scl)(g, oa1)));
+            dg = FN(add)(dg, CLIP3(FN(add)(FN(scl)(FN(sub)(v[A[0]], cam), oa0), FN(scl)(FN(sub)(v[A[1]], cam), oa1))));
+            dv[Bv[0]] = FN(add)(dv[Bv[0]], CLIP3(FN(scl)(g, ob0)));
+            dv[Bv[1]] = FN(add)(dv[Bv[1]], CLIP3(FN(scl)(g, ob1)));
+            dg = FN(add)(dg, CLIP3(FN(add)(FN(scl)(FN(sub)(v[Bv[0]], cam), ob0), FN(scl)(FN(sub)(v[Bv[1]], cam), ob1))));
+            V3 dna = af ? cnf : cnb, dnb = af ? cnb : cnf;
+            dn[A[0]] = FN(add)(dn[A[0]], FN(scl)(dna, (R)1.0 - ta));
+            dn[A[1]] = FN(add)(dn[A[1]], FN(scl)(dna, ta));
+            dn[Bv[0]] = FN(add)(dn[Bv[0]], FN(scl)(dnb, (R)1.0 - tb));
+            dn[Bv[1]] = FN(add)(dn[Bv[1]], FN(scl)(dnb, tb));
+        }
+        d_dens[t] += d.dens;
+        for (int i = 0; i < 4; ++i) {
+            uint32_t gi = 3 * idx[4 * t + i];
+            V3 a = CLIP3(dv[i]), b = CLIP3(dn[i]);
+            d_verts[gi] += a.x; d_verts[gi + 1] += a.y; d_verts[gi + 2] += a.z;
+            d_vn[gi] += b.x; d_vn[gi + 1] += b.y; d_vn[gi + 2] += b.z;
+        }
+        V3 a = CLIP3(dg), b = CLIP3(d.base);
+        d_grads[3 * t] += a.x; d_grads[3 * t + 1] += a.y; d_grads[3 * t + 2] += a.z;
+        d_base[3 * t] += b.x; d_base[3 * t + 1] += b.y; d_base[3 * t + 2] += b.z;
+#undef CLIP3
+    }
 }
 
 /* ------------------------------------------------------------------ */

@@ -52,6 +52,7 @@ SIGNATURES = {
     "rmesh_read_instance_count": (_st, [_vp, ctypes.POINTER(_u32)]),
     "rmesh_forward_error": (_st, [_vp, _vp, _vp, _vp, _vp]),
     "rmesh_forward_interval_tiled": (_st, [_vp] * 7),
+    "rmesh_backward_interval_tiled": (_st, [_vp] * 9),
     "rmesh_backward": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rmesh_get_sh_grad": (_st, [_vp, _vp]),
     "rmesh_train_step": (_st, [_vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _u32, ctypes.POINTER(_f32)]),

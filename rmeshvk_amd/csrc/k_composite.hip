@@ -1780,6 +1780,80 @@ __device__ __forceinline__ float sel4(const float* a, int i) {
     return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
 }
 
+// Screen-space geometry of one tet's decomposition: pixel positions, NDC z,
+// 1 / w and colour offsets of the vertices, and the discrete choices with
+// their weights (interval_generate.wgsl:174-300).  False: the tet is skipped
+// (a vertex behind the camera, a flat depth range or density < 1e-6).
+struct IvlGeo {
+    float px[4], py[4], nz[4], iw[4], off[4];
+    int inner;               // vertex inside the triangle of the other three, 4: none (quad)
+    int f0, f1, f2, ce;      // 3 points: the face opposite the inner vertex, and that vertex
+    float w0, w1, w2;        // the inner vertex's barycentrics in that face
+    int a0, a1, b0, b1;      // quad: crossing edges a0-a1 and b0-b1
+    float ta, tb;            // the crossing's parameter along each edge
+};
+
+__device__ __forceinline__ bool ivl_geometry(const V3 v[4], V3 g, float dens, const ViewParams& vw, IvlGeo& G) {
+    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
+    float clip[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mul_vp(vw.vp, v[i], clip[i]);
+    bool skip = clip[0][3] <= 0.0f || clip[1][3] <= 0.0f || clip[2][3] <= 0.0f || clip[3][3] <= 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float w = clip[i][3];
+        G.iw[i] = fabsf(w) > 1e-10f ? 1.0f / w : 0.0f;
+        const float nx = clip[i][0] * G.iw[i], ny = clip[i][1] * G.iw[i];
+        G.nz[i] = clip[i][2] * G.iw[i];
+        G.px[i] = ((nx + 1.0f) * 0.5f) * vw.Wf;
+        G.py[i] = ((1.0f - ny) * 0.5f) * vw.Hf;
+        G.off[i] = dot3(g, sub3(v[i], cam));
+    }
+    const float zmax = fmaxf(fmaxf(G.nz[0], G.nz[1]), fmaxf(G.nz[2], G.nz[3]));
+    const float zmin = fminf(fminf(G.nz[0], G.nz[1]), fminf(G.nz[2], G.nz[3]));
+    if (skip || zmax - zmin < 1e-7f || dens < 1e-6f) return false;
+    const float* px = G.px;
+    const float* py = G.py;
+    // interval_generate.wgsl:89-109: a vertex inside the triangle of the other three
+    G.inner = 4;
+#pragma unroll
+    for (int fi = 3; fi >= 0; --fi) {
+        int a, b, c, q;
+        face_idx(fi, a, b, c, q);
+        const float d0 = (px[b] - px[a]) * (py[q] - py[a]) - (py[b] - py[a]) * (px[q] - px[a]);
+        const float d1 = (px[c] - px[b]) * (py[q] - py[b]) - (py[c] - py[b]) * (px[q] - px[b]);
+        const float d2 = (px[a] - px[c]) * (py[q] - py[c]) - (py[a] - py[c]) * (px[q] - px[c]);
+        const bool all_pos = d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f;
+        const bool all_neg = d0 <= 0.0f && d1 <= 0.0f && d2 <= 0.0f;
+        if (all_pos || all_neg) G.inner = q;  // descending order: the first face in table order wins
+    }
+    if (G.inner < 4) {
+        face_idx((G.inner + 1) & 3, G.f0, G.f1, G.f2, G.ce);  // the face opposite the inner vertex
+        ivl_bary(sel4(px, G.ce), sel4(py, G.ce), sel4(px, G.f0), sel4(py, G.f0), sel4(px, G.f1), sel4(py, G.f1),
+                 sel4(px, G.f2), sel4(py, G.f2), G.w0, G.w1, G.w2);
+    } else {
+        // the first pair of opposite edges whose segments cross (01/23, 02/13,
+        // 03/12), else 01/23
+        G.a0 = 0; G.a1 = 1; G.b0 = 2; G.b1 = 3;
+        G.ta = ivl_line_t(px[0], py[0], px[1], py[1], px[2], py[2], px[3], py[3]);
+        G.tb = ivl_line_t(px[2], py[2], px[3], py[3], px[0], py[0], px[1], py[1]);
+        if (!(G.ta > 0.0f && G.ta < 1.0f && G.tb > 0.0f && G.tb < 1.0f)) {
+            const float ua = ivl_line_t(px[0], py[0], px[2], py[2], px[1], py[1], px[3], py[3]);
+            const float ub = ivl_line_t(px[1], py[1], px[3], py[3], px[0], py[0], px[2], py[2]);
+            if (ua > 0.0f && ua < 1.0f && ub > 0.0f && ub < 1.0f) {
+                G.a1 = 2; G.b0 = 1; G.ta = ua; G.tb = ub;
+            } else {
+                const float wa = ivl_line_t(px[0], py[0], px[3], py[3], px[1], py[1], px[2], py[2]);
+                const float wb = ivl_line_t(px[1], py[1], px[2], py[2], px[0], py[0], px[3], py[3]);
+                if (wa > 0.0f && wa < 1.0f && wb > 0.0f && wb < 1.0f) {
+                    G.a1 = 3; G.b0 = 1; G.b1 = 2; G.ta = wa; G.tb = wb;
+                }
+            }
+        }
+    }
+    return true;
+}
+
 // One thread per tet: the silhouette decomposition, as 7 float4
 //   [0..3] (x, y, ndc z, colour offset) of silhouette points 0-3 (3 points:
 //          point 3 repeats point 0), pixel coordinates of the full image,
@@ -1805,89 +1879,40 @@ __global__ __launch_bounds__(256) void k_interval_generate(const float4* __restr
     float4* o = out + (uint64_t)kIvlRec * t;
     const float bo = dot3(g, sub3(cam, v[0]));
     const float4 base = make_float4(col.x + bo, col.y + bo, col.z + bo, 0.0f);
-    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float clip[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) mul_vp(vw.vp, v[i], clip[i]);
-    bool skip = clip[0][3] <= 0.0f || clip[1][3] <= 0.0f || clip[2][3] <= 0.0f || clip[3][3] <= 0.0f;
-    float px[4], py[4], nz[4], off[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const float w = clip[i][3];
-        const float inv_w = fabsf(w) > 1e-10f ? 1.0f / w : 0.0f;
-        const float nx = clip[i][0] * inv_w, ny = clip[i][1] * inv_w;
-        nz[i] = clip[i][2] * inv_w;
-        px[i] = ((nx + 1.0f) * 0.5f) * vw.Wf;
-        py[i] = ((1.0f - ny) * 0.5f) * vw.Hf;
-        off[i] = dot3(g, sub3(v[i], cam));
-    }
-    const float zmax = fmaxf(fmaxf(nz[0], nz[1]), fmaxf(nz[2], nz[3]));
-    const float zmin = fminf(fminf(nz[0], nz[1]), fminf(nz[2], nz[3]));
-    skip = skip || zmax - zmin < 1e-7f || dens < 1e-6f;
-    if (skip) {
+    IvlGeo G;
+    if (!ivl_geometry(v, g, dens, vw, G)) {
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
         for (int i = 0; i < 6; ++i) o[i] = zero;
         o[6] = base;
         return;
     }
-    // interval_generate.wgsl:89-109: a vertex inside the triangle of the other three
-    int inner = 4;
-#pragma unroll
-    for (int fi = 3; fi >= 0; --fi) {
-        int a, b, c, q;
-        face_idx(fi, a, b, c, q);
-        const float d0 = (px[b] - px[a]) * (py[q] - py[a]) - (py[b] - py[a]) * (px[q] - px[a]);
-        const float d1 = (px[c] - px[b]) * (py[q] - py[b]) - (py[c] - py[b]) * (px[q] - px[b]);
-        const float d2 = (px[a] - px[c]) * (py[q] - py[c]) - (py[a] - py[c]) * (px[q] - px[c]);
-        const bool all_pos = d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f;
-        const bool all_neg = d0 <= 0.0f && d1 <= 0.0f && d2 <= 0.0f;
-        if (all_pos || all_neg) inner = q;  // descending order: the first face in table order wins
-    }
+    const float* px = G.px;
+    const float* py = G.py;
+    const float* nz = G.nz;
+    const float* off = G.off;
     int s0, s1, s2, s3;  // silhouette points
     float cx, cy, zf4, zb4, of4, ob4;
-    if (inner < 4) {
-        int a, b, c, q;
-        face_idx((inner + 1) & 3, a, b, c, q);  // the face opposite the inner vertex
-        float w0, w1, w2;
-        ivl_bary(sel4(px, q), sel4(py, q), sel4(px, a), sel4(py, a), sel4(px, b), sel4(py, b), sel4(px, c),
-                 sel4(py, c), w0, w1, w2);
-        const float z_face = w0 * sel4(nz, a) + w1 * sel4(nz, b) + w2 * sel4(nz, c);
-        const float z_c = sel4(nz, q);
-        const float face_off = w0 * sel4(off, a) + w1 * sel4(off, b) + w2 * sel4(off, c);
+    if (G.inner < 4) {
+        const float z_face = G.w0 * sel4(nz, G.f0) + G.w1 * sel4(nz, G.f1) + G.w2 * sel4(nz, G.f2);
+        const float z_c = sel4(nz, G.ce);
+        const float face_off = G.w0 * sel4(off, G.f0) + G.w1 * sel4(off, G.f1) + G.w2 * sel4(off, G.f2);
         const bool face_first = z_face <= z_c;
-        s0 = a; s1 = b; s2 = c; s3 = a;
-        cx = sel4(px, q);
-        cy = sel4(py, q);
+        s0 = G.f0; s1 = G.f1; s2 = G.f2; s3 = G.f0;
+        cx = sel4(px, G.ce);
+        cy = sel4(py, G.ce);
         zf4 = fminf(z_face, z_c);
         zb4 = fmaxf(z_face, z_c);
-        of4 = face_first ? face_off : sel4(off, q);
-        ob4 = face_first ? sel4(off, q) : face_off;
+        of4 = face_first ? face_off : sel4(off, G.ce);
+        ob4 = face_first ? sel4(off, G.ce) : face_off;
     } else {
-        // the first pair of opposite edges whose segments cross (01/23, 02/13,
-        // 03/12), else 01/23
-        int a0 = 0, a1 = 1, b0 = 2, b1 = 3;
-        float ta = ivl_line_t(px[0], py[0], px[1], py[1], px[2], py[2], px[3], py[3]);
-        float tb = ivl_line_t(px[2], py[2], px[3], py[3], px[0], py[0], px[1], py[1]);
-        if (!(ta > 0.0f && ta < 1.0f && tb > 0.0f && tb < 1.0f)) {
-            const float ua = ivl_line_t(px[0], py[0], px[2], py[2], px[1], py[1], px[3], py[3]);
-            const float ub = ivl_line_t(px[1], py[1], px[3], py[3], px[0], py[0], px[2], py[2]);
-            if (ua > 0.0f && ua < 1.0f && ub > 0.0f && ub < 1.0f) {
-                a1 = 2; b0 = 1; ta = ua; tb = ub;
-            } else {
-                const float wa = ivl_line_t(px[0], py[0], px[3], py[3], px[1], py[1], px[2], py[2]);
-                const float wb = ivl_line_t(px[1], py[1], px[2], py[2], px[0], py[0], px[3], py[3]);
-                if (wa > 0.0f && wa < 1.0f && wb > 0.0f && wb < 1.0f) {
-                    a1 = 3; b0 = 1; b1 = 2; ta = wa; tb = wb;
-                }
-            }
-        }
-        const float pa0x = sel4(px, a0), pa1x = sel4(px, a1), pa0y = sel4(py, a0), pa1y = sel4(py, a1);
-        cx = ivl_mix(pa0x, pa1x, ta);
-        cy = ivl_mix(pa0y, pa1y, ta);
-        const float z_a = ivl_mix(sel4(nz, a0), sel4(nz, a1), ta), z_b = ivl_mix(sel4(nz, b0), sel4(nz, b1), tb);
-        const float off_a = ivl_mix(sel4(off, a0), sel4(off, a1), ta);
-        const float off_b = ivl_mix(sel4(off, b0), sel4(off, b1), tb);
-        s0 = a0; s1 = b0; s2 = a1; s3 = b1;
+        cx = ivl_mix(sel4(px, G.a0), sel4(px, G.a1), G.ta);
+        cy = ivl_mix(sel4(py, G.a0), sel4(py, G.a1), G.ta);
+        const float z_a = ivl_mix(sel4(nz, G.a0), sel4(nz, G.a1), G.ta);
+        const float z_b = ivl_mix(sel4(nz, G.b0), sel4(nz, G.b1), G.tb);
+        const float off_a = ivl_mix(sel4(off, G.a0), sel4(off, G.a1), G.ta);
+        const float off_b = ivl_mix(sel4(off, G.b0), sel4(off, G.b1), G.tb);
+        s0 = G.a0; s1 = G.b0; s2 = G.a1; s3 = G.b1;
         zf4 = fminf(z_a, z_b);
         zb4 = fmaxf(z_a, z_b);
         of4 = z_a <= z_b ? off_a : off_b;
@@ -1898,7 +1923,7 @@ __global__ __launch_bounds__(256) void k_interval_generate(const float4* __restr
     for (int k = 0; k < 4; ++k)
         o[k] = make_float4(sel4(px, sv[k]), sel4(py, sv[k]), sel4(nz, sv[k]), sel4(off, sv[k]));
     o[4] = make_float4(cx, cy, zf4, zb4);
-    o[5] = make_float4(of4, ob4, dens, inner < 4 ? 3.0f : 4.0f);
+    o[5] = make_float4(of4, ob4, dens, G.inner < 4 ? 3.0f : 4.0f);
     o[6] = base;
 }
 
@@ -2102,6 +2127,533 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         d[3] = ds3;
         d[4] = ds4;
     }
+}
+
+// Per-pair quantities of one covered pixel (interval_tiled_rasterize.wgsl:
+// 248-340, backward_interval_tiled.wgsl:360-430): the fan triangle (slots ai,
+// bi of the silhouette and the centre) and barycentrics, the interpolated and
+// linearized depths, colours, optical depth and the integral weights.
+struct IvlPair {
+    float b0, b1, b2;
+    int ai, bi;
+    float zfn, zbn, zfc, zbc, z_f, z_b, dist, od, alpha_t, w0, w1;
+    V3 cf_raw, cb_raw, cf, cb, cp;
+};
+
+__device__ __forceinline__ bool ivl_pair(const float4* __restrict__ T, float pcx, float pcy, float rsc, IvlPair& q) {
+    const float4 q0 = T[0], q1 = T[1], q2 = T[2], q3 = T[3], q4 = T[4], q5 = T[5];
+    const bool tri3 = q5.w == 3.0f;
+    float4 ea = q0, eb = q1;
+    q.ai = 0;
+    q.bi = 1;
+    bool found = ivl_in_tri(pcx, pcy, q0.x, q0.y, q1.x, q1.y, q4.x, q4.y);
+    if (!found && ivl_in_tri(pcx, pcy, q1.x, q1.y, q2.x, q2.y, q4.x, q4.y)) {
+        found = true;
+        ea = q1; eb = q2; q.ai = 1; q.bi = 2;
+    }
+    const float4 e3 = tri3 ? q0 : q3;
+    if (!found && ivl_in_tri(pcx, pcy, q2.x, q2.y, e3.x, e3.y, q4.x, q4.y)) {
+        found = true;
+        ea = q2; eb = e3; q.ai = 2; q.bi = tri3 ? 0 : 3;
+    }
+    if (!found && !tri3 && ivl_in_tri(pcx, pcy, q3.x, q3.y, q0.x, q0.y, q4.x, q4.y)) {
+        found = true;
+        ea = q3; eb = q0; q.ai = 3; q.bi = 0;
+    }
+    if (!found) return false;
+    ivl_bary(pcx, pcy, ea.x, ea.y, eb.x, eb.y, q4.x, q4.y, q.b0, q.b1, q.b2);
+    q.zfn = q.b0 * ea.z + q.b1 * eb.z + q.b2 * q4.z;
+    q.zbn = q.b0 * ea.z + q.b1 * eb.z + q.b2 * q4.w;
+    const float ofn = q.b0 * ea.w + q.b1 * eb.w + q.b2 * q5.x;
+    const float obn = q.b0 * ea.w + q.b1 * eb.w + q.b2 * q5.y;
+    const float nf = kIvlNear * kIvlFar, range_nf = kIvlFar - kIvlNear;
+    q.zfc = clampf(q.zfn, 0.0f, 1.0f);
+    q.zbc = clampf(q.zbn, 0.0f, 1.0f);
+    q.z_f = nf / (kIvlFar - q.zfc * range_nf);
+    q.z_b = nf / (kIvlFar - q.zbc * range_nf);
+    q.dist = fabsf(q.z_b - q.z_f) * rsc;
+    const float4 base = T[6];
+    const float dens = q5.z;
+    q.cf_raw = mk3(base.x + ofn, base.y + ofn, base.z + ofn);
+    q.cb_raw = mk3(base.x + obn, base.y + obn, base.z + obn);
+    q.cf = mk3(fmaxf(q.cf_raw.x, 0.0f), fmaxf(q.cf_raw.y, 0.0f), fmaxf(q.cf_raw.z, 0.0f));
+    q.cb = mk3(fmaxf(q.cb_raw.x, 0.0f), fmaxf(q.cb_raw.y, 0.0f), fmaxf(q.cb_raw.z, 0.0f));
+    q.od = clampf(dens * q.dist, 1e-8f, 88.0f);
+    q.alpha_t = safe_exp(-q.od);
+    const float phv = phi_f(q.od);
+    q.w0 = phv - q.alpha_t;
+    q.w1 = 1.0f - phv;
+    q.cp = safe_clip3(mk3(q.cb.x * q.w0 + q.cf.x * q.w1, q.cb.y * q.w0 + q.cf.y * q.w1, q.cb.z * q.w0 + q.cf.z * q.w1),
+                      0.0f, kMaxVal);
+    return true;
+}
+
+// Per-tet adjoint accumulators of the interval decomposition, in floats:
+//   [0..19]  slot s = 0..4: d front z, d back z, d front offset, d back offset
+//   [20..22] d base colour, [23] d density,
+//   [24..35] silhouette slots 0-3: d normal (front + back),
+//   [36..41] centre: d front normal, d back normal.
+constexpr int kIvlGradTerms = 24, kIvlNormalTerms = 18, kIvlGradStride = 48;
+
+// Per-tile adjoint (backward_interval_tiled.wgsl:165-720), back to front in
+// batches: phase A evaluates each covered pair, phase B undoes the pixel's
+// colour / transmittance / distortion state in order and keeps the pair's
+// T_prev, the transmittance gradient before it and the m0 gradient, phase C
+// forms the pair's terms and reduces them per tet (segmented wave scan, LDS
+// adds, one global add per term per (tile, tet)).
+template <class C>
+__global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_interval_backward(
+    const float4* __restrict__ ivr, const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges,
+    const ViewParams* __restrict__ views, const uint32_t* __restrict__ tile_order, const float* __restrict__ rgba,
+    const float* __restrict__ dist_img, const float* __restrict__ dl_img, const float* __restrict__ dl_xyzd,
+    const float* __restrict__ dl_dist, float* __restrict__ d_ivr) {
+    constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, SP = TS + 1;
+    constexpr int KT = kIvlGradTerms + kIvlNormalTerms;
+    static_assert(B <= 64 && CAP >= TS * TS && CAP % 32 == 0 && CAP / 32 <= NT && CAP <= 2048, "batch geometry");
+    __shared__ BatchLDS<TS, B, false> L;
+    __shared__ float s_r[6][CAP];  // A: colour rgb, od, 1 - e^-u, m1 / T; B: T_prev, -d log T, d m0
+    __shared__ float s_rsc[TS * TS];
+    __shared__ float s_dl[7][TS * TS];  // d colour rgb, d xyzd
+    __shared__ uint32_t s_live[TS * TS];
+    __shared__ unsigned long long s_mask[TS * TS];
+    __shared__ uint16_t s_chunk[chunk_capacity(CAP, 1, B)];
+    __shared__ uint32_t s_start[CAP / 32];
+    __shared__ float s_acc[B][KT];
+    __shared__ uint32_t s_tid[B];
+
+    const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
+    const ViewParams& vw = views[0];
+    const TileCtx tc = tile_ctx(vw, gtile);
+    const int tid = threadIdx.x, lane = (int)__lane_id(), wid = tid / 64;
+    const int row = tid / TS, colp = tid % TS;
+    const uint32_t px = tc.tile_x * TS + colp, py = tc.tile_y * TS + row;
+    const bool active = tid < TS * TS && px < vw.width && py < vw.height;
+    // pixel state of the forward's end (backward_interval_tiled.wgsl:196-225)
+    float log_t = 0.f, d_log_t = 0.f, st_z = 0.f, st_w = 0.f, d_z = 0.f, d_w = 0.f, d_x = 0.f, d_y = 0.f;
+    float dl_r = 0.f, dl_g = 0.f, dl_b = 0.f;
+    bool has_dn = false;
+    if (tid < TS * TS) {
+        float rs = 1.0f;
+        float dx[4] = {0.f, 0.f, 0.f, 0.f};
+        if (active) {
+            const uint64_t pix = (uint64_t)py * vw.width + px;
+            const float xc = ((float)px + 0.5f - vw.intr[2]) / vw.intr[0];
+            const float yc = ((float)py + 0.5f - vw.intr[3]) / vw.intr[1];
+            rs = sqrtf((xc * xc + yc * yc) + 1.0f);
+            const float4 img = reinterpret_cast<const float4*>(rgba)[pix];
+            const float4 dl = reinterpret_cast<const float4*>(dl_img)[pix];
+            log_t = logf(fmaxf(1.0f - img.w, 1e-20f));
+            d_log_t = -dl.w * (1.0f - img.w);
+            dl_r = dl.x;
+            dl_g = dl.y;
+            dl_b = dl.z;
+            st_z = dist_img[5 * pix + 2];
+            st_w = dist_img[5 * pix + 3];
+            if (dl_dist) {
+                d_x = dl_dist[5 * pix];
+                d_y = dl_dist[5 * pix + 1];
+                d_z = dl_dist[5 * pix + 2];
+                d_w = dl_dist[5 * pix + 3];
+            }
+            if (dl_xyzd) {
+                const float4 q = reinterpret_cast<const float4*>(dl_xyzd)[pix];
+                dx[0] = q.x; dx[1] = q.y; dx[2] = q.z; dx[3] = q.w;
+            }
+        }
+        s_rsc[tid] = rs;
+        s_live[tid] = active ? 1u : 0u;
+        s_dl[0][tid] = dl_r;
+        s_dl[1][tid] = dl_g;
+        s_dl[2][tid] = dl_b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s_dl[3 + i][tid] = dx[i];
+        has_dn = dx[0] != 0.0f || dx[1] != 0.0f || dx[2] != 0.0f;
+    }
+    // the normal terms are formed only where some pixel of the tile has a normal gradient
+    has_dn = __syncthreads_or(has_dn);
+
+    const uint32_t rs_ = ranges[2ull * gtile], re = ranges[2ull * gtile + 1];
+    const uint32_t count = re > rs_ ? re - rs_ : 0u;
+    uint32_t done = 0;
+    int nb = (int)min<uint32_t>(B, count);
+    float4 f[kIvlRec];
+    uint32_t ft = 0;
+    auto fetch = [&](uint32_t t) {
+        ft = t;
+#pragma unroll
+        for (int i = 0; i < kIvlRec; ++i) f[i] = ivr[(uint64_t)kIvlRec * t + i];
+    };
+    if (tid < nb) fetch(sorted_vals[rs_ + tid]);
+    while (done < count) {
+        __syncthreads();  // the previous batch's flush has read s_acc / s_tid
+        if (tid < nb) {
+            const int k = tid;
+#pragma unroll
+            for (int i = 0; i < kIvlRec; ++i) L.tr[k * kIvlRec + i] = f[i];
+            s_tid[k] = ft;
+            const bool tri = f[5].w == 3.0f;
+            const float qx[4] = {f[0].x, f[1].x, f[2].x, tri ? f[4].x : f[3].x};
+            const float qy[4] = {f[0].y, f[1].y, f[2].y, tri ? f[4].y : f[3].y};
+            uint32_t rr = 0xFFu;
+            if (f[5].w != 0.0f) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    L.pxy[i][k] = qx[i] - tc.tile_ox;
+                    L.pxy[4 + i][k] = qy[i] - tc.tile_oy;
+                }
+                const float ymin = fminf(fminf(qy[0], qy[1]), fminf(qy[2], qy[3])) - tc.tile_oy;
+                const float ymax = fmaxf(fmaxf(qy[0], qy[1]), fmaxf(qy[2], qy[3])) - tc.tile_oy;
+                const float r0 = fmaxf(ceilf(ymin) - 1.0f, 0.0f);
+                const float r1 = fminf(floorf(ymax), (float)(TS - 1));
+                if (r0 <= r1) rr = (uint32_t)r0 | ((uint32_t)r1 << 8);
+            }
+            L.rows[k] = rr;
+        }
+        if (tid < TS * TS) s_mask[tid] = 0ull;
+        if (tid < CAP / 32) s_start[tid] = 0u;
+        for (int i = tid; i < B * KT; i += NT) (&s_acc[0][0])[i] = 0.0f;
+        __syncthreads();
+        batch_layout<TS, B, false, true, CAP, 1, NT>(L, nb, s_chunk, s_start);
+        __syncthreads();
+        const uint32_t nbp = L.nbp & 0xFFu;
+        const uint32_t nrec = L.nbp >> 8;
+        const uint32_t next = done + nbp;
+        const int nb_next = (int)min<uint32_t>(B, count - next);
+        if (tid < nb_next) fetch(sorted_vals[rs_ + next + tid]);
+        // phase A: the pair's colour, optical depth and distortion factors
+        for (uint32_t j = tid; j < nrec; j += NT) {
+            const ChunkRef c = record_at(s_chunk, s_start, j);
+            const int p = c.row * TS + c.col;
+            if (!s_live[p]) continue;
+            const float4* T = &L.tr[c.k * kIvlRec];
+            const float pcx = (float)(tc.tile_x * TS + (uint32_t)c.col) + 0.5f;
+            const float pcy = (float)(tc.tile_y * TS + (uint32_t)c.row) + 0.5f;
+            const float rsc = s_rsc[p];
+            IvlPair q;
+            if (!ivl_pair(T, pcx, pcy, rsc, q)) continue;
+            const float dens = T[5].z;
+            const float enter = q.z_f * rsc, exit_d = q.z_b * rsc;
+            const float u = dens * (exit_d - enter);
+            const float en = safe_exp(-u);
+            const float ad = fmaxf(1.0f - en, 0.0f);
+            s_r[0][j] = q.cp.x;
+            s_r[1][j] = q.cp.y;
+            s_r[2][j] = q.cp.z;
+            s_r[3][j] = q.od;
+            s_r[4][j] = ad;
+            s_r[5][j] = enter * ad + safe_div(1.0f - (1.0f + u) * en, dens);
+            atomicOr(&s_mask[p], 1ull << c.k);
+        }
+        __syncthreads();
+        // phase B: undo back to front (backward_interval_tiled.wgsl:432-460, 602-630)
+        if (active) {
+            unsigned long long m = s_mask[tid];
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                const uint32_t slot = (L.rpref[k * SP + row] + (uint32_t)colp) & 0xFFFFu;
+                const float cr = s_r[0][slot], cg = s_r[1][slot], cb = s_r[2][slot], od = s_r[3][slot];
+                const float prev = fminf(log_t + od, 0.0f);
+                const float tp = safe_exp(prev);
+                log_t = prev;
+                if (tp < 1e-6f) {
+                    s_r[0][slot] = 0.0f;
+                    continue;
+                }
+                const float dos = -d_log_t;
+                d_log_t = safe_clip(d_log_t + ((dl_r * cr + dl_g * cg) + dl_b * cb) * tp, kMinVal, kMaxVal);
+                const float m0 = tp * s_r[4][slot];
+                const float m1 = tp * s_r[5][slot];
+                st_z = st_z - m0;
+                st_w = st_w - m1;
+                const float dm0 = d_z + 2.0f * st_w * d_y;
+                d_z = d_z + 2.0f * m1 * d_x;
+                d_w = d_w + 2.0f * m0 * d_y;
+                s_r[0][slot] = tp;
+                s_r[1][slot] = dos;
+                s_r[2][slot] = dm0;
+            }
+        }
+        __syncthreads();
+        // phase C: the pair's terms, summed per tet (backward_interval_tiled.wgsl:460-660)
+        for (uint32_t base = (uint32_t)wid * 64; base < nrec; base += NT) {
+            const uint32_t j = base + lane;
+            float acc[kIvlGradTerms], accn[kIvlNormalTerms];
+#pragma unroll
+            for (int i = 0; i < kIvlGradTerms; ++i) acc[i] = 0.0f;
+#pragma unroll
+            for (int i = 0; i < kIvlNormalTerms; ++i) accn[i] = 0.0f;
+            int key = 0x7fffffff;
+            if (j < nrec) {
+                const ChunkRef c = record_at(s_chunk, s_start, j);
+                key = c.k;
+                const int p = c.row * TS + c.col;
+                const float tp = (s_mask[p] >> c.k) & 1ull ? s_r[0][j] : 0.0f;
+                if (tp > 0.0f) {
+                    const float4* T = &L.tr[c.k * kIvlRec];
+                    const float pcx = (float)(tc.tile_x * TS + (uint32_t)c.col) + 0.5f;
+                    const float pcy = (float)(tc.tile_y * TS + (uint32_t)c.row) + 0.5f;
+                    const float rsc = s_rsc[p];
+                    IvlPair q;
+                    ivl_pair(T, pcx, pcy, rsc, q);
+                    const float dens = T[5].z;
+                    const float dos = s_r[1][j], dm0 = s_r[2][j];
+                    const V3 dcol = mk3(s_dl[0][p], s_dl[1][p], s_dl[2][p]);
+                    const V3 d_cp = safe_clip3(scl3(dcol, tp), kMinVal, kMaxVal);
+                    const float dphi = dphi_dx_f(q.od);
+                    const float dw0 = dphi + q.alpha_t, dw1 = -dphi;
+                    const V3 d_cb_int = scl3(d_cp, q.w0), d_cf_int = scl3(d_cp, q.w1);
+                    const float d_od_int = dot3(d_cp, add3(scl3(q.cb, dw0), scl3(q.cf, dw1)));
+                    const float d_od = safe_clip(dos + d_od_int, kMinVal, kMaxVal);
+                    const float od_raw = dens * q.dist;
+                    const float d_od_raw = (od_raw >= 1e-8f && od_raw <= 88.0f) ? d_od : 0.0f;
+                    const float d_dens_local = safe_clip(d_od_raw * q.dist, kMinVal, kMaxVal);
+                    const float d_dist = safe_clip(d_od_raw * dens, kMinVal, kMaxVal);
+                    const float sgn = q.z_b >= q.z_f ? 1.0f : -1.0f;
+                    const float d_zf_view = -d_dist * rsc * sgn, d_zb_view = d_dist * rsc * sgn;
+                    const float range_nf = kIvlFar - kIvlNear;
+                    const float den_f = kIvlFar - q.zfc * range_nf, den_b = kIvlFar - q.zbc * range_nf;
+                    const float dz_f = kIvlNear * kIvlFar * range_nf / (den_f * den_f);
+                    const float dz_b = kIvlNear * kIvlFar * range_nf / (den_b * den_b);
+                    const bool zf_act = q.zfn >= 0.0f && q.zfn <= 1.0f, zb_act = q.zbn >= 0.0f && q.zbn <= 1.0f;
+                    const float d_zfn = zf_act ? d_zf_view * dz_f : 0.0f;
+                    const float d_zbn = zb_act ? d_zb_view * dz_b : 0.0f;
+                    const V3 d_cb_raw = mk3(q.cb_raw.x > 0.0f ? d_cb_int.x : 0.0f, q.cb_raw.y > 0.0f ? d_cb_int.y : 0.0f,
+                                            q.cb_raw.z > 0.0f ? d_cb_int.z : 0.0f);
+                    const V3 d_cf_raw = mk3(q.cf_raw.x > 0.0f ? d_cf_int.x : 0.0f, q.cf_raw.y > 0.0f ? d_cf_int.y : 0.0f,
+                                            q.cf_raw.z > 0.0f ? d_cf_int.z : 0.0f);
+                    const V3 d_bc = safe_clip3(add3(d_cf_raw, d_cb_raw), kMinVal, kMaxVal);
+                    const float d_of = (d_cf_raw.x + d_cf_raw.y) + d_cf_raw.z;
+                    const float d_ob = (d_cb_raw.x + d_cb_raw.y) + d_cb_raw.z;
+                    // xyzd (backward_interval_tiled.wgsl:544-600); vertex normals are zero
+                    const float dxd = s_dl[6][p];
+                    const float depth_c = q.w0 * q.z_b + q.w1 * q.z_f;
+                    const float d_od_x = -(dxd * depth_c) * tp;
+                    const float d_depth = dxd * tp;
+                    const float d_od_w = d_depth * (q.z_b * dw0 + q.z_f * dw1);
+                    const float d_zf_x = zf_act ? (d_depth * q.w1) * dz_f : 0.0f;
+                    const float d_zb_x = zb_act ? (d_depth * q.w0) * dz_b : 0.0f;
+                    // distortion (backward_interval_tiled.wgsl:602-650): through 1 - e^-u only
+                    const float enter = q.z_f * rsc, dt_d = q.z_b * rsc - enter;
+                    const float en = safe_exp(-(dens * dt_d));
+                    const float d_u = (dm0 * tp) * en;
+                    const float tzf = d_zfn + d_zf_x, tzb = d_zbn + d_zb_x;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const float ws = (s == q.ai ? q.b0 : 0.0f) + (s == q.bi ? q.b1 : 0.0f);
+                        acc[4 * s] = ws * tzf;
+                        acc[4 * s + 1] = ws * tzb;
+                        acc[4 * s + 2] = ws * d_of;
+                        acc[4 * s + 3] = ws * d_ob;
+                    }
+                    acc[16] = q.b2 * tzf;
+                    acc[17] = q.b2 * tzb;
+                    acc[18] = q.b2 * d_of;
+                    acc[19] = q.b2 * d_ob;
+                    acc[20] = d_bc.x;
+                    acc[21] = d_bc.y;
+                    acc[22] = d_bc.z;
+                    acc[23] = safe_clip(d_dens_local, kMinVal, kMaxVal) +
+                              safe_clip((d_od_x + d_od_w) * q.dist, kMinVal, kMaxVal) +
+                              safe_clip(d_u * dt_d, kMinVal, kMaxVal);
+                    if (has_dn) {
+                        const V3 dn = mk3(s_dl[3][p] * tp, s_dl[4][p] * tp, s_dl[5][p] * tp);
+                        const V3 dnf = scl3(dn, q.w1), dnb = scl3(dn, q.w0);
+                        const V3 dns = add3(dnf, dnb);
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) {
+                            const float ws = (s == q.ai ? q.b0 : 0.0f) + (s == q.bi ? q.b1 : 0.0f);
+                            accn[3 * s] = ws * dns.x;
+                            accn[3 * s + 1] = ws * dns.y;
+                            accn[3 * s + 2] = ws * dns.z;
+                        }
+                        accn[12] = q.b2 * dnf.x;
+                        accn[13] = q.b2 * dnf.y;
+                        accn[14] = q.b2 * dnf.z;
+                        accn[15] = q.b2 * dnb.x;
+                        accn[16] = q.b2 * dnb.y;
+                        accn[17] = q.b2 * dnb.z;
+                    }
+                }
+            }
+            const SegMasksDpp sm = seg_masks_dpp(key);
+            const int next_key = dpp_i<0x130, 0xF>(key, -1);  // wave_shl:1 (lane 63 reads -1)
+            const bool tail = key != 0x7fffffff && next_key != key;
+            seg_scan_dpp<kIvlGradTerms>(acc, sm);
+#pragma unroll
+            for (int i = 0; i < kIvlGradTerms; ++i)
+                if (tail && acc[i] != 0.0f) atomicAdd(&s_acc[key][i], acc[i]);
+            if (has_dn) {
+                seg_scan_dpp<kIvlNormalTerms>(accn, sm);
+#pragma unroll
+                for (int i = 0; i < kIvlNormalTerms; ++i)
+                    if (tail && accn[i] != 0.0f) atomicAdd(&s_acc[key][kIvlGradTerms + i], accn[i]);
+            }
+        }
+        __syncthreads();
+        // flush: one global add per non-zero term per (tile, tet)
+        for (int it = tid; it < (int)nbp * KT; it += NT) {
+            const int k = it / KT, i = it % KT;
+            const float v = s_acc[k][i];
+            if (v != 0.0f) unsafeAtomicAdd(d_ivr + (uint64_t)kIvlGradStride * s_tid[k] + i, v);
+        }
+        done = next;
+        nb = nb_next;
+    }
+}
+
+__device__ __forceinline__ void add_at(V3* a, int i, V3 v) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k == i) a[k] = add3(a[k], v);
+}
+
+// Per-tet chain back (interval_chain_back.wgsl:160-545): the decomposition's
+// gradients through each vertex's NDC z and colour offset, with the screen
+// geometry (silhouette positions, barycentrics, crossing parameters) fixed.
+__global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict__ rec,
+                                                        const uint32_t* __restrict__ idx,
+                                                        const ViewParams* __restrict__ views, uint32_t M,
+                                                        const float* __restrict__ d_ivr, float* __restrict__ d_verts,
+                                                        float* __restrict__ d_dens, float* __restrict__ d_base,
+                                                        float* __restrict__ d_grads, float* __restrict__ d_vn) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const ViewParams& vw = views[0];
+    float4 r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = rec[4ull * t + i];
+    const V3 v[4] = {mk3(r[0].x, r[0].y, r[0].z), mk3(r[1].x, r[1].y, r[1].z), mk3(r[2].x, r[2].y, r[2].z),
+                     mk3(r[3].x, r[3].y, r[3].z)};
+    const V3 g = mk3(r[1].w, r[2].w, r[3].w);
+    IvlGeo G;
+    if (!ivl_geometry(v, g, r[0].w, vw, G)) return;
+    const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
+    float D[kIvlGradStride];
+    const float4* D4 = reinterpret_cast<const float4*>(d_ivr + (uint64_t)kIvlGradStride * t);
+#pragma unroll
+    for (int i = 0; i < kIvlGradStride / 4; ++i) {
+        const float4 q = D4[i];
+        D[4 * i] = q.x; D[4 * i + 1] = q.y; D[4 * i + 2] = q.z; D[4 * i + 3] = q.w;
+    }
+    const V3 rz = mk3(vw.vp[2], vw.vp[6], vw.vp[10]), rw = mk3(vw.vp[3], vw.vp[7], vw.vp[11]);
+    auto dz = [&](int vi, float d) {
+        return safe_clip3(scl3(scl3(sub3(rz, scl3(rw, sel4(G.nz, vi))), sel4(G.iw, vi)), d), kMinVal, kMaxVal);
+    };
+    auto vtx = [&](int vi) {
+        V3 a = v[0];
+        if (vi == 1) a = v[1];
+        if (vi == 2) a = v[2];
+        if (vi == 3) a = v[3];
+        return a;
+    };
+    V3 dv[4], dn[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dv[i] = dn[i] = mk3(0.f, 0.f, 0.f);
+    const float dbs = (D[20] + D[21]) + D[22];
+    V3 dg = scl3(sub3(cam, v[0]), dbs);
+    dv[0] = sub3(dv[0], scl3(g, dbs));
+    const bool tri = G.inner < 4;
+    const int sv[4] = {tri ? G.f0 : G.a0, tri ? G.f1 : G.b0, tri ? G.f2 : G.a1, tri ? G.f0 : G.b1};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int vi = sv[i];
+        add_at(dv, vi, dz(vi, D[4 * i] + D[4 * i + 1]));
+        const float doff = D[4 * i + 2] + D[4 * i + 3];
+        add_at(dv, vi, safe_clip3(scl3(g, doff), kMinVal, kMaxVal));
+        dg = add3(dg, safe_clip3(scl3(sub3(vtx(vi), cam), doff), kMinVal, kMaxVal));
+        add_at(dn, vi, mk3(D[24 + 3 * i], D[25 + 3 * i], D[26 + 3 * i]));
+    }
+    const V3 cnf = mk3(D[36], D[37], D[38]), cnb = mk3(D[39], D[40], D[41]);
+    if (tri) {
+        const float z_face = G.w0 * sel4(G.nz, G.f0) + G.w1 * sel4(G.nz, G.f1) + G.w2 * sel4(G.nz, G.f2);
+        const bool ff = z_face <= sel4(G.nz, G.ce);
+        const float dzf = ff ? D[16] : D[17], dzc = ff ? D[17] : D[16];
+        const float dof = ff ? D[18] : D[19], doc = ff ? D[19] : D[18];
+        const int fv[3] = {G.f0, G.f1, G.f2};
+        const float fw[3] = {G.w0, G.w1, G.w2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) add_at(dv, fv[i], dz(fv[i], fw[i] * dzf));
+        add_at(dv, G.ce, dz(G.ce, dzc));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float doi = fw[i] * dof;
+            add_at(dv, fv[i], safe_clip3(scl3(g, doi), kMinVal, kMaxVal));
+            dg = add3(dg, safe_clip3(scl3(sub3(vtx(fv[i]), cam), doi), kMinVal, kMaxVal));
+        }
+        add_at(dv, G.ce, safe_clip3(scl3(g, doc), kMinVal, kMaxVal));
+        dg = add3(dg, safe_clip3(scl3(sub3(vtx(G.ce), cam), doc), kMinVal, kMaxVal));
+        const V3 dnf = ff ? cnf : cnb, dnc = ff ? cnb : cnf;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) add_at(dn, fv[i], scl3(dnf, fw[i]));
+        add_at(dn, G.ce, dnc);
+    } else {
+        const float z_a = ivl_mix(sel4(G.nz, G.a0), sel4(G.nz, G.a1), G.ta);
+        const float z_b = ivl_mix(sel4(G.nz, G.b0), sel4(G.nz, G.b1), G.tb);
+        const bool af = z_a <= z_b;
+        const float dza = af ? D[16] : D[17], dzb = af ? D[17] : D[16];
+        const float doa = af ? D[18] : D[19], dob = af ? D[19] : D[18];
+        add_at(dv, G.a0, dz(G.a0, (1.0f - G.ta) * dza));
+        add_at(dv, G.a1, dz(G.a1, G.ta * dza));
+        add_at(dv, G.b0, dz(G.b0, (1.0f - G.tb) * dzb));
+        add_at(dv, G.b1, dz(G.b1, G.tb * dzb));
+        const float oa0 = (1.0f - G.ta) * doa, oa1 = G.ta * doa, ob0 = (1.0f - G.tb) * dob, ob1 = G.tb * dob;
+        add_at(dv, G.a0, safe_clip3(scl3(g, oa0), kMinVal, kMaxVal));
+        add_at(dv, G.a1, safe_clip3(scl3(g, oa1), kMinVal, kMaxVal));
+        dg = add3(dg, safe_clip3(add3(scl3(sub3(vtx(G.a0), cam), oa0), scl3(sub3(vtx(G.a1), cam), oa1)), kMinVal,
+                                 kMaxVal));
+        add_at(dv, G.b0, safe_clip3(scl3(g, ob0), kMinVal, kMaxVal));
+        add_at(dv, G.b1, safe_clip3(scl3(g, ob1), kMinVal, kMaxVal));
+        dg = add3(dg, safe_clip3(add3(scl3(sub3(vtx(G.b0), cam), ob0), scl3(sub3(vtx(G.b1), cam), ob1)), kMinVal,
+                                 kMaxVal));
+        const V3 dna = af ? cnf : cnb, dnb = af ? cnb : cnf;
+        add_at(dn, G.a0, scl3(dna, 1.0f - G.ta));
+        add_at(dn, G.a1, scl3(dna, G.ta));
+        add_at(dn, G.b0, scl3(dnb, 1.0f - G.tb));
+        add_at(dn, G.b1, scl3(dnb, G.tb));
+    }
+    d_dens[t] = D[23];
+    const uint4 q = reinterpret_cast<const uint4*>(idx)[t];
+    const uint32_t vid[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const V3 a = safe_clip3(dv[i], kMinVal, kMaxVal), b = safe_clip3(dn[i], kMinVal, kMaxVal);
+        unsafeAtomicAdd(d_verts + 3ull * vid[i], a.x);
+        unsafeAtomicAdd(d_verts + 3ull * vid[i] + 1, a.y);
+        unsafeAtomicAdd(d_verts + 3ull * vid[i] + 2, a.z);
+        if (b.x != 0.0f || b.y != 0.0f || b.z != 0.0f) {
+            unsafeAtomicAdd(d_vn + 3ull * vid[i], b.x);
+            unsafeAtomicAdd(d_vn + 3ull * vid[i] + 1, b.y);
+            unsafeAtomicAdd(d_vn + 3ull * vid[i] + 2, b.z);
+        }
+    }
+    const V3 a = safe_clip3(dg, kMinVal, kMaxVal), b = safe_clip3(mk3(D[20], D[21], D[22]), kMinVal, kMaxVal);
+    d_grads[3ull * t] = a.x;
+    d_grads[3ull * t + 1] = a.y;
+    d_grads[3ull * t + 2] = a.z;
+    d_base[3ull * t] = b.x;
+    d_base[3ull * t + 1] = b.y;
+    d_base[3ull * t + 2] = b.z;
+}
+
+template <int TS>
+static void ivl_bwd_launch(const IvlBwdArgs& a, hipStream_t s) {
+    using C = FwdCfg<TS, 32, 512, 1, 1>;
+    hipLaunchKernelGGL((k_interval_backward<C>), dim3(a.num_tiles), dim3(C::NT), 0, s, a.ivr, a.sorted_vals, a.ranges,
+                       a.views, a.tile_order, a.rgba, a.dist, a.dl_img, a.dl_xyzd, a.dl_dist, a.d_ivr);
+}
+
+bool launch_interval_backward(const IvlBwdArgs& a, uint32_t tile_size, hipStream_t s) {
+    if (tile_size != 8 && tile_size != 12 && tile_size != 16) return false;
+    switch (tile_size) {
+        case 8: ivl_bwd_launch<8>(a, s); break;
+        case 12: ivl_bwd_launch<12>(a, s); break;
+        default: ivl_bwd_launch<16>(a, s); break;
+    }
+    if (a.M)
+        hipLaunchKernelGGL(k_interval_chain, dim3((a.M + 255) / 256), dim3(256), 0, s, a.rec, a.idx, a.views, a.M,
+                           a.d_ivr, a.d_verts, a.d_dens, a.d_base, a.d_grads, a.d_vn);
+    return true;
 }
 
 template <int TS>

@@ -98,6 +98,9 @@ struct rmesh_renderer {
     // interval forward: per-tet decomposition and the depth / distortion images (allocated on first use)
     float4* ivr = nullptr;
     float *ivl_xyzd = nullptr, *ivl_dist = nullptr;
+    // interval adjoint: upstream gradients, per-tet decomposition gradients, vertex-normal gradients
+    float *ivl_dxyzd = nullptr, *ivl_ddist = nullptr, *ivl_dgrad = nullptr, *ivl_dvn = nullptr;
+    bool have_interval = false;  // the buffers above hold the last interval forward's state
     // last forward
     uint32_t last_views = 0;
     bool have_forward = false;
@@ -166,6 +169,7 @@ void make_view(const rmesh_renderer* h, uint32_t v, const float* cam, const floa
 }
 
 rmesh_status upload_views(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps, const float* c2ws) {
+    h->have_interval = false;
     h->hviews.resize(V);
     for (uint32_t v = 0; v < V; ++v) make_view(h, v, cams + 3 * v, vps + 16 * v, c2ws + 16 * v, h->hviews[v]);
     HIP_TRY(hipMemcpyAsync(h->d_views, h->hviews.data(), sizeof(ViewParams) * V, hipMemcpyHostToDevice, h->stream));
@@ -189,6 +193,7 @@ void set_tiles(rmesh_renderer* h, uint32_t ts) {
     h->tiles_y = (h->H + ts - 1) / ts;
     h->T = h->tiles_x * h->tiles_y;
     h->have_forward = false;
+    h->have_interval = false;
     drop_graph(h);
 }
 
@@ -431,7 +436,8 @@ void free_all(rmesh_renderer* h) {
                     h->d_sh, h->rec, h->colors, h->d_views, h->counts, h->block_sums, h->vis_count, h->n_pairs,
                     h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a, h->vals_b, h->hist, h->ranges, h->rgba,
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
-                    h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist};
+                    h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd,
+                    h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -684,6 +690,65 @@ extern "C" rmesh_status rmesh_forward_interval_tiled(rmesh_renderer* h, const fl
     uint32_t ovf = 0;
     HIP_TRY(hipMemcpy(&ovf, h->overflow, 4, hipMemcpyDeviceToHost));
     if (ovf) return fail(RMESH_ERR_CAPACITY, "pair buffer overflow");
+    h->have_interval = true;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_backward_interval_tiled(rmesh_renderer* h, const float* dl_img, const float* dl_xyzd,
+                                                      const float* dl_dist, float* d_vertices, float* d_densities,
+                                                      float* d_base_colors, float* d_color_grads,
+                                                      float* d_vertex_normals) {
+    if (!h || !dl_img || !d_vertices || !d_densities || !d_base_colors || !d_color_grads || !d_vertex_normals)
+        return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->have_interval)
+        return fail(RMESH_ERR_INVALID, "backward_interval_tiled requires a preceding forward_interval_tiled");
+    DeviceGuard g(h->device);
+    hipStream_t s = h->stream;
+    const size_t npx = (size_t)h->W * h->H;
+    if (!h->ivl_dgrad) {
+        HIP_TRY(dalloc(&h->ivl_dxyzd, 4 * npx));
+        HIP_TRY(dalloc(&h->ivl_ddist, 5 * npx));
+        HIP_TRY(dalloc(&h->ivl_dgrad, 48ull * h->M));
+        HIP_TRY(dalloc(&h->ivl_dvn, 3ull * h->N));
+    }
+    HIP_TRY(hipMemcpyAsync(h->dl, dl_img, npx * 16, hipMemcpyHostToDevice, s));
+    if (dl_xyzd) HIP_TRY(hipMemcpyAsync(h->ivl_dxyzd, dl_xyzd, npx * 16, hipMemcpyHostToDevice, s));
+    if (dl_dist) HIP_TRY(hipMemcpyAsync(h->ivl_ddist, dl_dist, npx * 20, hipMemcpyHostToDevice, s));
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->ivl_dgrad), 0u, 48ull * h->M, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_verts), 0u, 3ull * h->N, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_dens), 0u, h->M, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_colors), 0u, 3ull * h->M, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_grads), 0u, 3ull * h->M, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->ivl_dvn), 0u, 3ull * h->N, s);
+    IvlBwdArgs a;
+    a.rec = h->rec;
+    a.idx = h->idx;
+    a.views = h->d_views;
+    a.M = h->M;
+    a.ivr = h->ivr;
+    a.sorted_vals = sorted_vals(h);
+    a.ranges = h->ranges;
+    a.tile_order = h->tile_order;
+    a.num_tiles = h->T;
+    a.rgba = h->rgba;
+    a.dist = h->ivl_dist;
+    a.dl_img = h->dl;
+    a.dl_xyzd = dl_xyzd ? h->ivl_dxyzd : nullptr;
+    a.dl_dist = dl_dist ? h->ivl_ddist : nullptr;
+    a.d_ivr = h->ivl_dgrad;
+    a.d_verts = h->d_verts;
+    a.d_dens = h->d_dens;
+    a.d_base = h->d_colors;
+    a.d_grads = h->d_grads;
+    a.d_vn = h->ivl_dvn;
+    if (!launch_interval_backward(a, h->TS, s)) return fail(RMESH_ERR_UNSUPPORTED, "tile size");
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d_vertices, h->d_verts, 12ull * h->N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(d_densities, h->d_dens, 4ull * h->M, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(d_base_colors, h->d_colors, 12ull * h->M, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(d_color_grads, h->d_grads, 12ull * h->M, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(d_vertex_normals, h->ivl_dvn, 12ull * h->N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return RMESH_OK;
 }
 

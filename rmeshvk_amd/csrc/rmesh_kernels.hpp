@@ -82,6 +82,32 @@ struct IvlArgs {
 };
 bool launch_interval(const IvlArgs& a, uint32_t tile_size, hipStream_t s);
 
+// Interval-tiled adjoint of view 0 over the last interval forward's tile
+// lists and decomposition, then the per-tet chain back to the scene.
+struct IvlBwdArgs {
+    const float4* rec;
+    const uint32_t* idx;
+    const ViewParams* views;
+    uint32_t M;
+    const float4* ivr;           // [M][7] decomposition of the forward
+    const uint32_t* sorted_vals;
+    const uint32_t* ranges;
+    const uint32_t* tile_order;  // nullable
+    uint32_t num_tiles;
+    const float* rgba;           // forward image [H][W][4]
+    const float* dist;           // forward distortion [H][W][5]
+    const float* dl_img;         // [H][W][4]
+    const float* dl_xyzd;        // [H][W][4] or nullptr
+    const float* dl_dist;        // [H][W][5] or nullptr
+    float* d_ivr;                // [M][48] zeroed scratch
+    float* d_verts;              // [N*3] zeroed, accumulated
+    float* d_dens;               // [M] zeroed; written per decomposed tet
+    float* d_base;               // [M*3] zeroed; written per decomposed tet
+    float* d_grads;              // [M*3] zeroed; written per decomposed tet
+    float* d_vn;                 // [N*3] zeroed, accumulated
+};
+bool launch_interval_backward(const IvlBwdArgs& a, uint32_t tile_size, hipStream_t s);
+
 struct BwdArgs {
     const float4* rec;
     const uint32_t* idx;

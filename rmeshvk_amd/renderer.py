@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib
 
 _OUT_OF_SCOPE = (
-    "backward_interval_tiled", "forward_raytrace", "backward_raytrace",
+    "forward_raytrace", "backward_raytrace",
     "backward_hw", "locate_tets", "read_aux", "set_vertex_normals",
     "get_vertex_normals", "set_aux_data", "get_aux_data",
 )
@@ -158,6 +158,23 @@ class RMeshRenderer:
         _lib.check(self._lib.rmesh_forward_interval_tiled(self._h, _p(c), _p(v), _p(ci), _p(rgba), _p(xyzd),
                                                           _p(dist)))
         return {"image": rgba, "xyzd": xyzd, "distortion": dist}
+
+    # ---- backward_interval_tiled (lib.rs:2049-2240) ----
+    def backward_interval_tiled(self, dl_d_image, dl_d_xyzd=None, dl_d_distortion=None, dl_d_aux=None):
+        """Adjoint of the last forward_interval_tiled: {'d_vertices' [N*3],
+        'd_densities' [M], 'd_base_colors' [M*3], 'd_color_grads' [M*3],
+        'd_vertex_normals' [N*3]} (no 'd_aux_data': aux_dim is always 0 here)."""
+        n = self._hgt * self._w
+        dl = _f32("dl_d_image", dl_d_image, 4 * n)
+        dx = None if dl_d_xyzd is None else _f32("dl_d_xyzd", dl_d_xyzd, 4 * n)
+        dd = None if dl_d_distortion is None else _f32("dl_d_distortion", dl_d_distortion, 5 * n)
+        out = {"d_vertices": np.empty(3 * self._n, np.float32), "d_densities": np.empty(self._m, np.float32),
+               "d_base_colors": np.empty(3 * self._m, np.float32), "d_color_grads": np.empty(3 * self._m, np.float32),
+               "d_vertex_normals": np.empty(3 * self._n, np.float32)}
+        _lib.check(self._lib.rmesh_backward_interval_tiled(
+            self._h, _p(dl), _p(dx), _p(dd), _p(out["d_vertices"]), _p(out["d_densities"]), _p(out["d_base_colors"]),
+            _p(out["d_color_grads"]), _p(out["d_vertex_normals"])))
+        return out
 
     # ---- forward_error (lib.rs:1621-1711) ----
     def forward_error(self, pixel_err, ssim_err):

@@ -128,3 +128,101 @@ def test_distortion_moments():
     np.testing.assert_allclose(ds[:, 3], wsum, atol=2e-3 * max(1.0, np.abs(wsum).max()))
     np.testing.assert_allclose(ds[:, 4], selfd, atol=2e-3 * max(1.0, np.abs(selfd).max()))
     np.testing.assert_allclose(ds[:, 0] - ds[:, 1], cross, atol=2e-3 * max(1.0, np.abs(cross).max()))
+
+
+# ---------------------------------------------------------------------------
+# Adjoint (backward_interval_tiled.wgsl + interval_chain_back.wgsl).  The
+# reference differentiates the image with the screen-space geometry (pixel
+# positions of the silhouette, barycentric weights, crossing parameters)
+# held fixed: its per-tile stage yields the derivative with respect to each
+# slot's front/back NDC z and colour offsets, the base colour and the
+# density, and its chain back maps those through the NDC z and offsets of
+# the vertices.  Both stages are pinned by float64 central differences at
+# exactly that level: the compositing over frozen decompositions, and the
+# attributes of moved vertices under frozen choices and weights.
+# ---------------------------------------------------------------------------
+
+_ATTR = np.r_[10:30, 60:64]  # slot zf, zb, of, ob; base colour; density
+
+
+def _mesh_scene(seed, n=60):
+    from rmeshvk_amd.scenes import make_scene, uniform_points
+
+    rng = np.random.default_rng(seed)
+    s = make_scene(uniform_points(n, rng), rng)
+    s["densities"] = (0.5 * s["densities"]).astype(np.float32)
+    s["base_colors"] = np.abs(s["base_colors"]) + 0.2
+    return {k: (np.asarray(v, np.float64) if k in ("vertices", "base_colors", "densities", "color_grads")
+                else v) for k, v in s.items()}
+
+
+def test_interval_adjoint_colour_matches_finite_differences():
+    s = _mesh_scene(5)
+    w, h = 40, 32
+    cam = _cam((0.3, -2.8, 0.6), (0.0, 0.0, 0.0), math.radians(60), w, h)
+    ref = orc.render(s, cam, w, h, ts=8)
+    ivl = orc.interval_generate(s, cam, w, h, dtype=np.float64)
+    fwd = orc.interval_composite(ivl, cam, w, h, ref["sorted_vals"], ref["ranges"], ts=8, dtype=np.float64)
+    assert fwd["image"][..., 3].max() > 0.3
+    rng = np.random.default_rng(1)
+    dl = rng.normal(0.0, 1.0, (h, w, 4))
+    d = orc.interval_backward(ivl, cam, w, h, ref["sorted_vals"], ref["ranges"], fwd, dl, ts=8, dtype=np.float64)
+
+    def loss(v):
+        f = orc.interval_composite(v, cam, w, h, ref["sorted_vals"], ref["ranges"], ts=8, dtype=np.float64)
+        return float(np.sum(f["image"] * dl))
+
+    live = np.flatnonzero(np.abs(d[:, _ATTR]).sum(axis=1) > 0)
+    assert len(live) > 20
+    got, fd = [], []
+    eps = 1e-6
+    for t in rng.choice(live, 24, replace=False):
+        for k in _ATTR:
+            p = ivl.copy()
+            p[t, k] += eps
+            lp = loss(p)
+            p[t, k] -= 2 * eps
+            fd.append((lp - loss(p)) / (2 * eps))
+            got.append(d[t, k])
+    got, fd = np.array(got), np.array(fd)
+    assert np.linalg.norm(got - fd) / np.linalg.norm(fd) < 1e-5
+
+
+def test_interval_chain_back_matches_finite_differences():
+    s = _mesh_scene(6, n=30)
+    w, h = 40, 32
+    cam = _cam((0.4, -2.6, 0.8), (0.0, 0.0, 0.0), math.radians(60), w, h)
+    N, M = len(s["vertices"]) // 3, len(s["indices"]) // 4
+    vn = np.random.default_rng(2).normal(0.0, 1.0, N * 3)
+    ivl = orc.interval_generate(s, cam, w, h, vertex_normals=vn, dtype=np.float64)
+    assert np.count_nonzero(ivl[:, 64] == 3) > 0 and np.count_nonzero(ivl[:, 64] == 4) > 0
+    rng = np.random.default_rng(3)
+    d_ivl = np.zeros_like(ivl)
+    d_ivl[:, 10:64] = rng.normal(0.0, 1.0, (M, 54))
+    d_ivl[ivl[:, 64] == 0] = 0.0
+    g = orc.interval_chain(s, cam, w, h, ivl, d_ivl, dtype=np.float64)
+
+    def S(sc, normals):
+        a = orc.interval_attrs(sc, cam, w, h, ivl, vertex_normals=normals, dtype=np.float64)
+        return float(np.sum(a[:, 10:64] * d_ivl[:, 10:64]))
+
+    eps = 1e-6
+    for key, out in (("vertices", "d_vertices"), ("color_grads", "d_color_grads"), ("base_colors", "d_base_colors")):
+        fd = np.zeros_like(s[key])
+        for i in range(len(fd)):
+            sp = dict(s)
+            sp[key] = s[key].copy()
+            sp[key][i] += eps
+            lp = S(sp, vn)
+            sp[key][i] -= 2 * eps
+            fd[i] = (lp - S(sp, vn)) / (2 * eps)
+        assert np.linalg.norm(g[out] - fd) / np.linalg.norm(fd) < 1e-6, key
+    fdn = np.zeros(N * 3)
+    for i in range(N * 3):
+        p = vn.copy()
+        p[i] += eps
+        lp = S(s, p)
+        p[i] -= 2 * eps
+        fdn[i] = (lp - S(s, p)) / (2 * eps)
+    assert np.linalg.norm(g["d_vertex_normals"] - fdn) / np.linalg.norm(fdn) < 1e-6
+    np.testing.assert_array_equal(g["d_densities"], d_ivl[:, 63])
