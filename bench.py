@@ -293,20 +293,30 @@ def main():
         # view (each: binning, compositing and the synchronous host copies)
         r.set_graph_mode(False)
         r.set_timing(False)
-        ti, tf = [], []
+        ti, tb, tf = [], [], []
+        irng = np.random.default_rng(11)
+        dl_i = irng.normal(0.0, 1.0, (Wd, Wd, 4)).astype(np.float32)
+        dl_x = irng.normal(0.0, 1.0, (Wd, Wd, 4)).astype(np.float32)
+        dl_d = irng.normal(0.0, 1.0, (Wd, Wd, 5)).astype(np.float32)
         for _ in range(3):
             r.synchronize()
             t0 = time.perf_counter()
             iv = r.forward_interval_tiled(*cams[0])
             ti.append(time.perf_counter() - t0)
             t0 = time.perf_counter()
+            r.backward_interval_tiled(dl_i, dl_x, dl_d)
+            tb.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
             r.forward_tiled(*cams[0])
             tf.append(time.perf_counter() - t0)
-        ivl_pass = {"ms": round(1000.0 * min(ti), 3), "forward_tiled_ms": round(1000.0 * min(tf), 3),
+        ivl_pass = {"ms": round(1000.0 * min(ti), 3), "backward_ms": round(1000.0 * min(tb), 3),
+                    "forward_tiled_ms": round(1000.0 * min(tf), 3),
                     "mean_alpha": round(float(iv["image"][..., 3].mean()), 4),
-                    "note": "forward_interval_tiled on view 0 (image, depth, distortion) beside forward_tiled of "
-                            "the same view; binning and host copies included in both (kernel times: profiles/)"}
-        log(f"forward_interval_tiled: {ivl_pass['ms']:.2f} ms (forward_tiled {ivl_pass['forward_tiled_ms']:.2f} ms)")
+                    "note": "forward_interval_tiled on view 0 (image, depth, distortion), its adjoint with image, "
+                            "xyzd and distortion gradients, and forward_tiled of the same view; binning and host "
+                            "copies included (kernel times: profiles/)"}
+        log(f"forward_interval_tiled: {ivl_pass['ms']:.2f} ms, backward_interval_tiled {ivl_pass['backward_ms']:.2f} "
+            f"ms (forward_tiled {ivl_pass['forward_tiled_ms']:.2f} ms)")
 
     cpu = None
     if not args.no_cpu:

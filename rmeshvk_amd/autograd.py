@@ -2,9 +2,13 @@
 (crates/rmesh-python/python/rmesh_wgpu/autograd.py:15-151).
 
 ``RMeshForward`` / ``RMeshModule`` keep the reference's signatures: forward
-uploads the parameters and renders one view through the tiled pipeline;
-backward hands dL/d(image) to the renderer's hand-written adjoint and returns
-gradients for (vertices, base_colors, densities, color_grads).
+uploads the parameters and renders one view through the tiled pipeline
+(``render_mode="tiled"``) or the interval-tiled one
+(``render_mode="interval_tiled"``); backward hands dL/d(image) to the
+matching hand-written adjoint and returns gradients for (vertices,
+base_colors, densities, color_grads).  The reference's interval branch passes
+forward_interval_tiled's dict where an array is expected (autograd.py:40,49);
+here the branch uses the dict's image, so it runs.
 """
 
 from __future__ import annotations
@@ -26,10 +30,11 @@ class RMeshForward(torch.autograd.Function):
     @staticmethod
     def forward(ctx, renderer, cam_pos, vp, c2w_intrinsics, vertices, base_colors, densities, color_grads,
                 render_mode="tiled"):
-        if render_mode != "tiled":
-            raise NotImplementedError(f"render_mode={render_mode!r}: only 'tiled' is served")
         renderer.update_params(_np(vertices), _np(base_colors), _np(densities), _np(color_grads))
-        image_np = renderer.forward_tiled(_np(cam_pos), _np(vp), _np(c2w_intrinsics))
+        if render_mode == "interval_tiled":
+            image_np = renderer.forward_interval_tiled(_np(cam_pos), _np(vp), _np(c2w_intrinsics))["image"]
+        else:
+            image_np = renderer.forward_tiled(_np(cam_pos), _np(vp), _np(c2w_intrinsics))
         ctx.render_mode = render_mode
         ctx.renderer = renderer
         ctx.save_for_backward(vertices, base_colors, densities, color_grads)
@@ -38,7 +43,10 @@ class RMeshForward(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_output):
         dl = np.ascontiguousarray(grad_output.detach().cpu().numpy().astype(np.float32))
-        grads = ctx.renderer.backward(dl)
+        if ctx.render_mode == "interval_tiled":
+            grads = ctx.renderer.backward_interval_tiled(dl)
+        else:
+            grads = ctx.renderer.backward(dl)
         device = ctx.saved_tensors[0].device
         t = lambda k: torch.from_numpy(grads[k].copy()).to(device)  # noqa: E731
         return (None, None, None, None, t("d_vertices"), t("d_base_colors"), t("d_densities"),

@@ -1776,8 +1776,18 @@ __device__ __forceinline__ bool ivl_in_tri(float ptx, float pty, float ax, float
     return !(has_neg && has_pos);
 }
 
+// a[i] for i in 0..3 (and c ? a : b) as bit masks: a select chain over one
+// array's elements would be folded into an indexed load, which keeps the
+// array in scratch memory.
 __device__ __forceinline__ float sel4(const float* a, int i) {
-    return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
+    const uint32_t m0 = 0u - (uint32_t)(i == 0), m1 = 0u - (uint32_t)(i == 1), m2 = 0u - (uint32_t)(i == 2),
+                   m3 = 0u - (uint32_t)(i == 3);
+    return __uint_as_float((__float_as_uint(a[0]) & m0) | (__float_as_uint(a[1]) & m1) |
+                           (__float_as_uint(a[2]) & m2) | (__float_as_uint(a[3]) & m3));
+}
+__device__ __forceinline__ float bsel(bool c, float a, float b) {
+    const uint32_t m = 0u - (uint32_t)c;
+    return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
 }
 
 // Screen-space geometry of one tet's decomposition: pixel positions, NDC z,
@@ -2540,13 +2550,9 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
     auto dz = [&](int vi, float d) {
         return safe_clip3(scl3(scl3(sub3(rz, scl3(rw, sel4(G.nz, vi))), sel4(G.iw, vi)), d), kMinVal, kMaxVal);
     };
-    auto vtx = [&](int vi) {
-        V3 a = v[0];
-        if (vi == 1) a = v[1];
-        if (vi == 2) a = v[2];
-        if (vi == 3) a = v[3];
-        return a;
-    };
+    const float vx[4] = {v[0].x, v[1].x, v[2].x, v[3].x}, vy[4] = {v[0].y, v[1].y, v[2].y, v[3].y},
+                vz[4] = {v[0].z, v[1].z, v[2].z, v[3].z};
+    auto vtx = [&](int vi) { return mk3(sel4(vx, vi), sel4(vy, vi), sel4(vz, vi)); };
     V3 dv[4], dn[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) dv[i] = dn[i] = mk3(0.f, 0.f, 0.f);
@@ -2568,8 +2574,8 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
     if (tri) {
         const float z_face = G.w0 * sel4(G.nz, G.f0) + G.w1 * sel4(G.nz, G.f1) + G.w2 * sel4(G.nz, G.f2);
         const bool ff = z_face <= sel4(G.nz, G.ce);
-        const float dzf = ff ? D[16] : D[17], dzc = ff ? D[17] : D[16];
-        const float dof = ff ? D[18] : D[19], doc = ff ? D[19] : D[18];
+        const float dzf = bsel(ff, D[16], D[17]), dzc = bsel(ff, D[17], D[16]);
+        const float dof = bsel(ff, D[18], D[19]), doc = bsel(ff, D[19], D[18]);
         const int fv[3] = {G.f0, G.f1, G.f2};
         const float fw[3] = {G.w0, G.w1, G.w2};
 #pragma unroll
@@ -2583,7 +2589,8 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
         }
         add_at(dv, G.ce, safe_clip3(scl3(g, doc), kMinVal, kMaxVal));
         dg = add3(dg, safe_clip3(scl3(sub3(vtx(G.ce), cam), doc), kMinVal, kMaxVal));
-        const V3 dnf = ff ? cnf : cnb, dnc = ff ? cnb : cnf;
+        const V3 dnf = mk3(bsel(ff, cnf.x, cnb.x), bsel(ff, cnf.y, cnb.y), bsel(ff, cnf.z, cnb.z));
+        const V3 dnc = mk3(bsel(ff, cnb.x, cnf.x), bsel(ff, cnb.y, cnf.y), bsel(ff, cnb.z, cnf.z));
 #pragma unroll
         for (int i = 0; i < 3; ++i) add_at(dn, fv[i], scl3(dnf, fw[i]));
         add_at(dn, G.ce, dnc);
@@ -2591,8 +2598,8 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
         const float z_a = ivl_mix(sel4(G.nz, G.a0), sel4(G.nz, G.a1), G.ta);
         const float z_b = ivl_mix(sel4(G.nz, G.b0), sel4(G.nz, G.b1), G.tb);
         const bool af = z_a <= z_b;
-        const float dza = af ? D[16] : D[17], dzb = af ? D[17] : D[16];
-        const float doa = af ? D[18] : D[19], dob = af ? D[19] : D[18];
+        const float dza = bsel(af, D[16], D[17]), dzb = bsel(af, D[17], D[16]);
+        const float doa = bsel(af, D[18], D[19]), dob = bsel(af, D[19], D[18]);
         add_at(dv, G.a0, dz(G.a0, (1.0f - G.ta) * dza));
         add_at(dv, G.a1, dz(G.a1, G.ta * dza));
         add_at(dv, G.b0, dz(G.b0, (1.0f - G.tb) * dzb));
@@ -2606,7 +2613,8 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
         add_at(dv, G.b1, safe_clip3(scl3(g, ob1), kMinVal, kMaxVal));
         dg = add3(dg, safe_clip3(add3(scl3(sub3(vtx(G.b0), cam), ob0), scl3(sub3(vtx(G.b1), cam), ob1)), kMinVal,
                                  kMaxVal));
-        const V3 dna = af ? cnf : cnb, dnb = af ? cnb : cnf;
+        const V3 dna = mk3(bsel(af, cnf.x, cnb.x), bsel(af, cnf.y, cnb.y), bsel(af, cnf.z, cnb.z));
+        const V3 dnb = mk3(bsel(af, cnb.x, cnf.x), bsel(af, cnb.y, cnf.y), bsel(af, cnb.z, cnf.z));
         add_at(dn, G.a0, scl3(dna, 1.0f - G.ta));
         add_at(dn, G.a1, scl3(dna, G.ta));
         add_at(dn, G.b0, scl3(dnb, 1.0f - G.tb));

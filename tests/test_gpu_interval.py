@@ -157,3 +157,28 @@ def test_interval_backward_state_and_arguments(config):
     r.forward_tiled(*cams[0])
     with pytest.raises(ValueError):
         r.backward_interval_tiled(dl)
+
+
+def test_autograd_interval_mode(config):
+    """RMeshModule(render_mode="interval_tiled"): the image of
+    forward_interval_tiled and the gradients of backward_interval_tiled."""
+    import torch
+
+    from rmeshvk_amd.autograd import RMeshModule
+
+    name, scene, cams, w, h = config
+    if "sh" in scene:
+        pytest.skip("the autograd module renders base colours")
+    mod = RMeshModule(scene["vertices"], scene["indices"], scene["base_colors"], scene["densities"],
+                      scene["color_grads"], scene["circumdata"], w, h, device_id=0)
+    cam = [torch.from_numpy(np.asarray(c)) for c in cams[0]]
+    img = mod(*cam, render_mode="interval_tiled")
+    dl = torch.from_numpy(np.random.default_rng(3).normal(0.0, 1.0, (h, w, 4)).astype(np.float32))
+    (img * dl).sum().backward()
+    r = _renderer(scene, w, h)
+    exp_img = r.forward_interval_tiled(*cams[0])["image"]
+    g = r.backward_interval_tiled(dl.numpy())
+    np.testing.assert_array_equal(img.detach().numpy(), exp_img)
+    for p, k in ((mod.vertices, "d_vertices"), (mod.base_colors, "d_base_colors"), (mod.densities, "d_densities"),
+                 (mod.color_grads, "d_color_grads")):
+        assert _rel(p.grad.numpy(), g[k]) <= 1e-5, k
