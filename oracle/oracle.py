@@ -54,6 +54,19 @@ orc_interval_forward
                   when g != 0 (interval_generate.wgsl:290-330,
                   interval_tiled_rasterize.wgsl:327-330) has no independent
                   check; only GPU vs this restatement.
+orc_interval_backward (image terms), orc_interval_composite
+                  float64 central differences of the compositing over frozen
+                  decompositions (slot front/back NDC z, colour offsets, base
+                  colour, density of 24 tets): relative L2 < 1e-5.
+                  PARITY UNPINNED: the depth (xyzd) and distortion terms of
+                  the adjoint (backward_interval_tiled.wgsl:544-650), which
+                  the reference does not form as derivatives of its forward;
+                  only GPU vs this restatement.
+orc_interval_chain, orc_interval_attrs, orc_interval_generate
+                  float64 central differences of the attributes of moved
+                  vertices under frozen choices and weights, with respect to
+                  every vertex, colour gradient, base colour and vertex
+                  normal: relative L2 < 1e-6 (3- and 4-point silhouettes).
 orc_error         closed forms on one tet (weight = sum of the pixels' optical
                   depths, which agree with the forward's alpha = 1 - e^-od;
                   weighted entry/exit points on the tet's planes; centroid

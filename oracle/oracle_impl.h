@@ -6,9 +6,10 @@
  * binning/sort/ranges and the tolerance checker for images and gradients) and
  * once with R = double (used for finite-difference gradient checks).
  *
- * What pins each function is listed in oracle.py; one output is parity
- * unpinned: the aux entropy channels of orc_forward (aux[4..8]) have no
- * independent check.
+ * What pins each function is listed in oracle.py.  Parity unpinned: the aux
+ * entropy channels of orc_forward (aux[4..8]); the screen-space colour
+ * offset interpolation of orc_interval_forward for g != 0; the depth and
+ * distortion terms of orc_interval_backward.
  *
  * Every function names the reference kernel and lines it follows.  Float
  * semantics chosen where WGSL leaves them to the driver (documented in
