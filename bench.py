@@ -173,6 +173,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true", help="skip the forward_error (densification statistics) pass")
     ap.add_argument("--no-interval", action="store_true", help="skip the forward_interval_tiled pass")
+    ap.add_argument("--no-load", action="store_true", help="skip timing the .rmesh scene-file parser")
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB (HBM counter calibration)")
@@ -318,6 +319,42 @@ def main():
         log(f"forward_interval_tiled: {ivl_pass['ms']:.2f} ms, backward_interval_tiled {ivl_pass['backward_ms']:.2f} "
             f"ms (forward_tiled {ivl_pass['forward_tiled_ms']:.2f} ms)")
 
+    load_pass = None
+    if not args.no_load:
+        # the workload's scene as a .rmesh (SH degree 3, 12 PCA components, as
+        # the reference's converter writes it), parsed by the native loader:
+        # raw, and gzip level 1; host code, multi-threaded
+        import gzip
+
+        from rmeshvk_amd import data as sio
+
+        raw = sio.save_rmesh(None, scene["vertices"], scene["indices"], scene["densities"], scene["color_grads"],
+                             scene["sh"].reshape(M, 3, 16), 3, k_components=12, compress=False)
+        gz = gzip.compress(raw, compresslevel=1)
+        tr, tg = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sio.load_rmesh(raw)
+            tr.append(time.perf_counter() - t0)
+        for _ in range(2):
+            t0 = time.perf_counter()
+            sio.load_rmesh(gz)
+            tg.append(time.perf_counter() - t0)
+        load_pass = {"raw_ms": round(1000.0 * min(tr), 1), "gzip_ms": round(1000.0 * min(tg), 1),
+                     "raw_MB": round(len(raw) / 1e6, 1), "gzip_MB": round(len(gz) / 1e6, 1),
+                     "tets_per_s_raw": round(M / min(tr)),
+                     "note": "load_rmesh of this workload's scene (f16 decode, PCA decompression of 48 SH floats "
+                             "per tet from 12 components, float64 circumspheres) on the host, up to 16 threads"}
+        if not args.no_cpu:
+            from oracle import scene_io as sref
+
+            t0 = time.perf_counter()
+            sref.parse_rmesh(raw, raw=True)
+            load_pass["cpu_baseline_ms"] = round(1000.0 * (time.perf_counter() - t0), 1)
+            load_pass["cpu_baseline"] = "numpy restatement of parse_rmesh (oracle), same file"
+        log(f"load_rmesh: raw {load_pass['raw_ms']} ms ({load_pass['raw_MB']} MB), gzip {load_pass['gzip_ms']} ms")
+        del raw, gz
+
     cpu = None
     if not args.no_cpu:
         tc = time.perf_counter()
@@ -365,6 +402,7 @@ def main():
         "plain_per_view": plain,
         "forward_error": err_pass,
         "interval_forward": ivl_pass,
+        "scene_load": load_pass,
     }
     if copy_probe:
         out["copy_probe"] = copy_probe

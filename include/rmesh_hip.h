@@ -229,6 +229,37 @@ rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t n, uint32
 /* Bits sorted for a given tile count (crates/rmesh-sort/src/drs.rs:23-26). */
 uint32_t rmesh_sorting_bits(uint32_t num_tiles);
 
+/* ---- scene files (crates/rmesh-data/src/lib.rs) ----
+ * load_rmesh / load_rmesh_raw / load_ply (lib.rs:140-148, 243-394, 711-853):
+ * parse a file image held in host memory into the renderer's flat arrays.
+ * .rmesh: header (N, M, SH degree, PCA components), start pose, vertices,
+ * indices, u8 log densities (exp((v - 100) / 20)), f16 PCA mean / basis /
+ * weights of the SH coefficients (decompressed: mean + sum_k w_k basis_k),
+ * f16 colour gradients, optional tagged sections (counted, not decoded).
+ * PLY: binary little-endian vertices and tetrahedra with interleaved SH.
+ * The SH leave channel-major per tet ([M][3][(deg+1)^2], the layout of
+ * rmesh_scene_desc.sh_coeffs); circumdata is computed in float64 with the
+ * reference's degenerate-tet rules.  Host code, no device needed. */
+typedef struct rmesh_scene_file rmesh_scene_file;
+#define RMESH_FILE_RMESH 0      /* .rmesh, gzip-compressed or raw (detected) */
+#define RMESH_FILE_PLY 1        /* binary little-endian PLY */
+#define RMESH_FILE_RMESH_RAW 2  /* .rmesh, uncompressed only (load_rmesh_raw) */
+typedef struct rmesh_scene_info {
+    uint64_t n_vertices, n_tets;
+    uint32_t sh_degree;     /* SH coefficients per tet: 3 * (deg+1)^2 */
+    uint32_t k_components;  /* PCA components as stored (.rmesh), 0 for PLY */
+    float start_pose[7];    /* x, y, z, qx, qy, qz, qw (.rmesh), zeros for PLY */
+    uint32_t n_tagged;      /* tagged extension sections (.rmesh) */
+} rmesh_scene_info;
+rmesh_status rmesh_scene_parse(const uint8_t* data, uint64_t nbytes, int32_t format, rmesh_scene_file** out,
+                               rmesh_scene_info* info);
+/* Copies the parsed arrays out (any pointer may be NULL to skip it):
+ * vertices [N*3], indices [M*4], densities [M], color_grads [M*3],
+ * circumdata [M*4], sh_coeffs [M*3*(deg+1)^2]. */
+rmesh_status rmesh_scene_copy(const rmesh_scene_file* f, float* vertices, uint32_t* indices, float* densities,
+                              float* color_grads, float* circumdata, float* sh_coeffs);
+void rmesh_scene_free(rmesh_scene_file* f);
+
 #ifdef __cplusplus
 }
 #endif

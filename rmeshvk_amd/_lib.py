@@ -41,7 +41,19 @@ class SceneDesc(ctypes.Structure):
     ]
 
 
+class SceneInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_vertices", _u64), ("n_tets", _u64), ("sh_degree", _u32), ("k_components", _u32),
+        ("start_pose", _f32 * 7), ("n_tagged", _u32),
+    ]
+
+
+FILE_RMESH, FILE_PLY, FILE_RMESH_RAW = 0, 1, 2
+
 SIGNATURES = {
+    "rmesh_scene_parse": (_st, [_vp, _u64, _i32, ctypes.POINTER(_vp), ctypes.POINTER(SceneInfo)]),
+    "rmesh_scene_copy": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rmesh_scene_free": (None, [_vp]),
     "rmesh_create": (_st, [ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]),
     "rmesh_destroy": (None, [_vp]),
     "rmesh_last_error": (ctypes.c_char_p, []),

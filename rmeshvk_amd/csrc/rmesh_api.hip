@@ -25,6 +25,7 @@
 #include "../../include/rmesh_hip.h"
 #include "rmesh_common.hpp"
 #include "rmesh_kernels.hpp"
+#include "rmesh_error.hpp"
 
 using namespace rmesh;
 
@@ -52,6 +53,8 @@ hipError_t dalloc(T** p, size_t n) {
 }
 
 }  // namespace
+
+void rmesh::set_last_error(const std::string& msg) { g_err = msg; }
 
 struct rmesh_renderer {
     int device = 0;
