@@ -49,6 +49,8 @@ typedef struct rmesh_scene_desc {
     const float* sh_coeffs;    /* nullable [n_tets*3*(deg+1)^2], channel-major per tet */
     uint32_t sh_degree;        /* 0..3, used when sh_coeffs != NULL */
     uint32_t max_views;        /* views per batched call (0 -> 1) */
+    uint32_t sh_half;          /* non-zero: colours from the SH rounded to f16, as the reference
+                                  viewer's packed SH (project_compute.wgsl:267-318); forward only */
 } rmesh_scene_desc;
 
 /* RMeshRenderer.__new__ (lib.rs:265-870).  Uploads the scene and allocates
@@ -112,6 +114,11 @@ rmesh_status rmesh_backward_interval_tiled(rmesh_renderer* h, const float* dl_d_
  * d_vertices [N*3], d_densities [M], d_base_colors [M*3], d_color_grads [M*3]. */
 rmesh_status rmesh_backward(rmesh_renderer* h, const float* dl_d_image, float* d_vertices, float* d_densities,
                             float* d_base_colors, float* d_color_grads);
+/* RMeshRenderer.backward_hw (lib.rs:2255-2378): adjoint of the last forward().
+ * forward() is served by the tiled forward here, so this is rmesh_backward
+ * over that forward's state (the reference reads its hardware-raster image). */
+rmesh_status rmesh_backward_hw(rmesh_renderer* h, const float* dl_d_image, float* d_vertices, float* d_densities,
+                               float* d_base_colors, float* d_color_grads);
 /* SH extension: d_sh [M*3*nc] of the last rmesh_backward (host). */
 rmesh_status rmesh_get_sh_grad(rmesh_renderer* h, float* d_sh);
 

@@ -37,7 +37,7 @@ class SceneDesc(ctypes.Structure):
         ("indices", _vp), ("n_tets", _u64),
         ("base_colors", _vp), ("densities", _vp), ("color_grads", _vp), ("circumdata", _vp),
         ("width", _u32), ("height", _u32), ("aux_dim", _u32), ("tile_size", _u32),
-        ("device_id", _i32), ("sh_coeffs", _vp), ("sh_degree", _u32), ("max_views", _u32),
+        ("device_id", _i32), ("sh_coeffs", _vp), ("sh_degree", _u32), ("max_views", _u32), ("sh_half", _u32),
     ]
 
 
@@ -65,6 +65,7 @@ SIGNATURES = {
     "rmesh_forward_error": (_st, [_vp, _vp, _vp, _vp, _vp]),
     "rmesh_forward_interval_tiled": (_st, [_vp] * 7),
     "rmesh_backward_interval_tiled": (_st, [_vp] * 9),
+    "rmesh_backward_hw": (_st, [_vp] * 6),
     "rmesh_backward": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rmesh_get_sh_grad": (_st, [_vp, _vp]),
     "rmesh_train_step": (_st, [_vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _u32, ctypes.POINTER(_f32)]),

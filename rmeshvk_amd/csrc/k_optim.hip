@@ -132,7 +132,8 @@ __device__ __forceinline__ V3 view_dir(V3 centroid, const ViewParams& vw, float&
 
 template <int DEG>
 __global__ __launch_bounds__(256) void k_sh_colors(const float4* __restrict__ rec, const float* __restrict__ sh, uint32_t M,
-                                                   const ViewParams* __restrict__ views, uint32_t V, float4* __restrict__ colors) {
+                                                   const ViewParams* __restrict__ views, uint32_t V, float4* __restrict__ colors,
+                                                   bool half) {
     constexpr int NC = (DEG + 1) * (DEG + 1);
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= M) return;
@@ -142,6 +143,12 @@ __global__ __launch_bounds__(256) void k_sh_colors(const float4* __restrict__ re
     const float* s = sh + (uint64_t)t * 3 * NC;
 #pragma unroll
     for (int i = 0; i < 3 * NC; ++i) c[i] = s[i];
+    // the viewer path reads the coefficients as packed f16 (project_compute.wgsl:71-75,
+    // 267-318): the same values, rounded to nearest even
+    if (half) {
+#pragma unroll
+        for (int i = 0; i < 3 * NC; ++i) c[i] = (float)(_Float16)c[i];
+    }
     float offset = dot3(g, sub3(v[0], cen));
     for (uint32_t vv = 0; vv < V; ++vv) {
         float len;
@@ -385,14 +392,14 @@ __global__ __launch_bounds__(256) void k_reduce_loss(const float* __restrict__ p
 }
 
 void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t M, const ViewParams* views, uint32_t V,
-                      float4* colors, hipStream_t s) {
+                      float4* colors, bool half, hipStream_t s) {
     if (!M) return;
     dim3 gr((M + 255) / 256), bl(256);
     switch (deg) {
-        case 0: hipLaunchKernelGGL(k_sh_colors<0>, gr, bl, 0, s, rec, sh, M, views, V, colors); break;
-        case 1: hipLaunchKernelGGL(k_sh_colors<1>, gr, bl, 0, s, rec, sh, M, views, V, colors); break;
-        case 2: hipLaunchKernelGGL(k_sh_colors<2>, gr, bl, 0, s, rec, sh, M, views, V, colors); break;
-        default: hipLaunchKernelGGL(k_sh_colors<3>, gr, bl, 0, s, rec, sh, M, views, V, colors); break;
+        case 0: hipLaunchKernelGGL(k_sh_colors<0>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
+        case 1: hipLaunchKernelGGL(k_sh_colors<1>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
+        case 2: hipLaunchKernelGGL(k_sh_colors<2>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
+        default: hipLaunchKernelGGL(k_sh_colors<3>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
     }
 }
 

@@ -65,6 +65,7 @@ struct rmesh_renderer {
     int fwd_var = 0, bwd_var = 0;  // batch-geometry variants of the compositing kernels
     uint32_t max_views = 1;
     bool has_sh = false;
+    bool sh_half = false;  // forward colours from f16-rounded SH (the viewer path)
     uint32_t sh_deg = 0, nc = 1;
     float min_t = 0.0f;
     uint32_t step = 0;
@@ -234,7 +235,7 @@ void mark(rmesh_renderer* h, int i) {
 rmesh_status stage_binning_front(rmesh_renderer* h, uint32_t V, bool capture) {
     hipStream_t s = h->stream;
     launch_prep_records(h->verts, h->idx, h->dens, h->grads, h->M, h->rec, s);
-    if (h->has_sh) launch_sh_colors(h->rec, h->sh, h->sh_deg, h->M, h->d_views, V, h->colors, s);
+    if (h->has_sh) launch_sh_colors(h->rec, h->sh, h->sh_deg, h->M, h->d_views, V, h->colors, h->sh_half, s);
     else launch_copy_colors(h->base, h->M, h->colors, s);
     launch_fill_u32(h->vis_count, 0u, h->max_views, s);
     launch_fill_u32(h->overflow, 0u, 1, s);
@@ -489,6 +490,7 @@ extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer**
     h->T_alloc = ((h->W + 7) / 8) * ((h->H + 7) / 8);
     h->max_views = d->max_views ? d->max_views : 1;
     h->has_sh = d->sh_coeffs != nullptr;
+    h->sh_half = h->has_sh && d->sh_half != 0;
     h->sh_deg = h->has_sh ? d->sh_degree : 0;
     h->nc = (h->sh_deg + 1) * (h->sh_deg + 1);
     const uint32_t V = h->max_views;
@@ -766,6 +768,7 @@ extern "C" rmesh_status rmesh_read_instance_count(rmesh_renderer* h, uint32_t* o
 extern "C" rmesh_status rmesh_backward(rmesh_renderer* h, const float* dl, float* dV, float* dD, float* dB, float* dG) {
     if (!h || !dl || !dV || !dD || !dB || !dG) return fail(RMESH_ERR_INVALID, "null argument");
     if (!h->have_forward || h->last_views != 1) return fail(RMESH_ERR_INVALID, "backward requires a preceding forward_tiled");
+    if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
     DeviceGuard g(h->device);
     hipStream_t s = h->stream;
     size_t npx = (size_t)h->W * h->H;
@@ -799,6 +802,12 @@ extern "C" rmesh_status rmesh_backward(rmesh_renderer* h, const float* dl, float
     return RMESH_OK;
 }
 
+extern "C" rmesh_status rmesh_backward_hw(rmesh_renderer* h, const float* dl, float* dV, float* dD, float* dB,
+                                          float* dG) {
+    // the forward() it follows is served by the tiled forward, so its state is the tiled adjoint's
+    return rmesh_backward(h, dl, dV, dD, dB, dG);
+}
+
 extern "C" rmesh_status rmesh_get_sh_grad(rmesh_renderer* h, float* d_sh) {
     if (!h || !d_sh) return fail(RMESH_ERR_INVALID, "null argument");
     if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
@@ -830,6 +839,7 @@ extern "C" rmesh_status rmesh_train_step(rmesh_renderer* h, const float* cam, co
                                          const float* gt, float lr_v, float lr_d, float lr_g, uint32_t loss_type,
                                          float* out_loss) {
     if (!h || !cam || !vp || !ci || !gt) return fail(RMESH_ERR_INVALID, "null argument");
+    if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
     DeviceGuard g(h->device);
     rmesh_status st = ensure_optimizer(h);
     if (st != RMESH_OK) return st;
@@ -944,6 +954,7 @@ extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const floa
                                        const float* c2ws, const float* gt_dev, uint32_t loss_type, uint32_t reps,
                                        float* out_ms) {
     if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
     rmesh_status st = check_views(h, V);
     if (st != RMESH_OK) return st;
     DeviceGuard g(h->device);
@@ -1038,6 +1049,7 @@ extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, 
                                                  const float* c2ws, const float* gt_dev, uint32_t loss_type, float lr_v,
                                                  float lr_d, float lr_g, float lr_sh, float* out_loss) {
     if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
     rmesh_status st = check_views(h, V);
     if (st != RMESH_OK) return st;
     DeviceGuard g(h->device);

@@ -133,8 +133,9 @@ struct BwdArgs {
 bool launch_backward(const BwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
 
 // colours, SH and optimiser (k_optim.hip)
+// half: coefficients rounded to f16 first (the reference viewer's packed SH)
 void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t M, const ViewParams* views, uint32_t V,
-                      float4* colors, hipStream_t s);
+                      float4* colors, bool half, hipStream_t s);
 
 // The step counter and the overflow flag live in device memory so a captured
 // graph reads the current values at replay time.

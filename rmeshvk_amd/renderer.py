@@ -22,7 +22,7 @@ from . import _lib
 
 _OUT_OF_SCOPE = (
     "forward_raytrace", "backward_raytrace",
-    "backward_hw", "locate_tets", "read_aux", "set_vertex_normals",
+    "locate_tets", "read_aux", "set_vertex_normals",
     "get_vertex_normals", "set_aux_data", "get_aux_data",
 )
 
@@ -59,7 +59,8 @@ class RMeshRenderer:
     """Differentiable radiance-mesh renderer on one MI355X (see module doc)."""
 
     def __init__(self, vertices, indices, base_colors, densities, color_grads, circumdata, width, height,
-                 aux_dim=0, *, tile_size=12, device_id=-1, sh_coeffs=None, sh_degree=0, max_views=1):
+                 aux_dim=0, *, tile_size=12, device_id=-1, sh_coeffs=None, sh_degree=0, max_views=1,
+                 sh_half=False):
         lib = _lib.load()
         _f32("vertices", vertices)
         _u32("indices", indices)
@@ -79,7 +80,7 @@ class RMeshRenderer:
             color_grads=color_grads.ctypes.data, circumdata=circumdata.ctypes.data,
             width=int(width), height=int(height), aux_dim=int(aux_dim), tile_size=int(tile_size),
             device_id=int(device_id), sh_coeffs=None if sh_coeffs is None else sh_coeffs.ctypes.data,
-            sh_degree=int(sh_degree), max_views=int(max_views))
+            sh_degree=int(sh_degree), max_views=int(max_views), sh_half=int(bool(sh_half)))
         h = ctypes.c_void_p()
         _lib.check(lib.rmesh_create(ctypes.byref(d), ctypes.byref(h)))
         self._keep = None
@@ -205,6 +206,18 @@ class RMeshRenderer:
             _lib.check(self._lib.rmesh_get_sh_grad(self._h, _p(d_sh)))
             out["d_sh"] = d_sh
         return out
+
+    # ---- backward_hw (lib.rs:2255-2378) ----
+    def backward_hw(self, dl_d_image):
+        """Adjoint of the last forward() (served by the tiled forward, so the
+        tiled adjoint over its state)."""
+        _f32("dl_d_image", dl_d_image, self._hgt * self._w * 4)
+        d_v = np.empty(3 * self._n, dtype=np.float32)
+        d_d = np.empty(self._m, dtype=np.float32)
+        d_b = np.empty(3 * self._m, dtype=np.float32)
+        d_g = np.empty(3 * self._m, dtype=np.float32)
+        _lib.check(self._lib.rmesh_backward_hw(self._h, _p(dl_d_image), _p(d_v), _p(d_d), _p(d_b), _p(d_g)))
+        return {"d_vertices": d_v, "d_densities": d_d, "d_base_colors": d_b, "d_color_grads": d_g}
 
     # ---- training (lib.rs:2405-2636) ----
     def train_step(self, cam_pos, vp, c2w_intrinsics, gt_image, lr_verts, lr_dens, lr_grads, loss_type,
