@@ -26,9 +26,9 @@ orc_forward       an independent float64 quadrature renderer
                   (tests/npref.py) on the regular tet, two tets and a
                   280-tet Delaunay mesh; the regular-tet closed form
                   RGB = c * alpha; tile-size invariance; min_t; aux normals
-                  and depth of a single tet.
-                  PARITY UNPINNED: the aux entropy channels (aux[4..8]) have
-                  no independent check (they are outside the train step).
+                  and depth of a single tet; the aux entropy channels
+                  (aux[4..8]) of two overlapping tets against float64
+                  chords in depth order.
 orc_loss          KAT of kernel_tests.rs:176-305 (+-1/4096 at 64x64); the L2
                   closed form.
 orc_backward      float64 central finite differences (eps 1e-6) of the
@@ -49,11 +49,10 @@ orc_interval_forward
                   tet against float64 quadrature of the weight
                   sigma e^-sigma(t - t0) (m0, m1, self term) and of two tets
                   (the cross term).  All with constant colour per tet.
-                  PARITY UNPINNED: the screen-space (not perspective-
-                  correct) barycentric interpolation of the colour offsets
-                  when g != 0 (interval_generate.wgsl:290-330,
-                  interval_tiled_rasterize.wgsl:327-330) has no independent
-                  check; only GPU vs this restatement.
+                  g != 0 (screen-space interpolation of the colour offsets):
+                  in the affine limit (a 0.1 tet at 100 units, float64) the
+                  image equals the quadrature of the linear colour field
+                  to < 5e-4 while g moves it by 3e-2.
 orc_interval_backward (image terms), orc_interval_composite
                   float64 central differences of the compositing over frozen
                   decompositions (slot front/back NDC z, colour offsets, base

@@ -6,10 +6,8 @@
  * binning/sort/ranges and the tolerance checker for images and gradients) and
  * once with R = double (used for finite-difference gradient checks).
  *
- * What pins each function is listed in oracle.py.  Parity unpinned: the aux
- * entropy channels of orc_forward (aux[4..8]); the screen-space colour
- * offset interpolation of orc_interval_forward for g != 0; the depth and
- * distortion terms of orc_interval_backward.
+ * What pins each function is listed in oracle.py.  Parity unpinned: the
+ * depth and distortion terms of orc_interval_backward.
  *
  * Every function names the reference kernel and lines it follows.  Float
  * semantics chosen where WGSL leaves them to the driver (documented in
@@ -517,10 +515,9 @@ void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const
 /* per tet id (the reference stores it per compact visible index but     */
 /* reads it back through the tile lists' tet ids: the two agree when the */
 /* compaction is the identity).                                          */
-/* Parity unpinned: the screen-space interpolation of the colour offsets */
-/* for g != 0 (forward), and the adjoint's depth and distortion terms,   */
-/* which the reference does not form as derivatives of its forward       */
-/* (only the GPU is compared with them).                                 */
+/* Parity unpinned: the adjoint's depth and distortion terms, which the  */
+/* reference does not form as derivatives of its forward (only the GPU   */
+/* is compared with them).                                               */
 /* ------------------------------------------------------------------ */
 typedef struct {
     R px[5], py[5], zf[5], zb[5], of[5], ob[5]; /* slots 0-3 silhouette, 4 centre */

@@ -8,8 +8,9 @@ built with the same near/far and planar faces, the linearized chord is the
 ray chord, so with a constant colour per tet (g = 0) the image equals the float64
 quadrature renderer (tests/npref.py) and depth x ray scale equals the
 chord-weighted depth.  With g != 0 the reference interpolates the colour
-offsets with screen-space (not perspective-correct) barycentrics; that rule
-has no independent reference here (parity unpinned beyond GPU vs oracle).
+offsets with screen-space (not perspective-correct) barycentrics; as the
+projection becomes affine that rule tends to the linear colour field the
+quadrature integrates, which pins it (test_colour_offsets_in_the_affine_limit).
 Distortion (Mip-NeRF 360 state: x, y, z = sum w, w = sum w t, self term)
 is checked against float64 quadrature of the weight T sigma e^-sigma (t - t0)
 over the chords.  Bars: float32 NDC z at a 1e5 near/far ratio loses ~1e-4,
@@ -226,3 +227,38 @@ def test_interval_chain_back_matches_finite_differences():
         fdn[i] = (lp - S(s, p)) / (2 * eps)
     assert np.linalg.norm(g["d_vertex_normals"] - fdn) / np.linalg.norm(fdn) < 1e-6
     np.testing.assert_array_equal(g["d_densities"], d_ivl[:, 63])
+
+
+def test_colour_offsets_in_the_affine_limit():
+    """g != 0: the reference interpolates the colour offsets with screen-space
+    barycentrics; they equal the perspective-correct (linear-in-space) colour
+    as the projection becomes affine.  A 0.1-sized tet seen from 100 units
+    (depth spread / depth ~ 1e-3) in float64: the image equals the float64
+    quadrature of the linear colour field to ~|g| size 1e-3, while g moves
+    the image by ~5e-2."""
+    from rmeshvk_amd.scenes import regular_tet_scene
+
+    s = regular_tet_scene(10.0)
+    s["vertices"] = (s["vertices"] * np.float32(0.1)).astype(np.float32)
+    s["base_colors"] = np.array([0.6, 0.5, 0.7], np.float32)
+    s["color_grads"] = np.array([0.4, -0.3, 0.5], np.float32)
+    from rmeshvk_amd.scenes import circumspheres
+
+    s["circumdata"] = circumspheres(s["vertices"], s["indices"])
+    w = h = 40
+    fov = 2.0 * math.atan(0.1 * w / (2.0 * 30.0 * 100.0))  # the tet spans ~30 px
+    eye = np.array([35.0, -90.0, 30.0])
+    eye = eye / np.linalg.norm(eye) * 100.0
+    cam = _cam(tuple(eye), (0.0, 0.0, 0.0), fov, w, h)
+    assert npref.endpoint_colours_positive(s, cam, w, h)
+    ref = orc.render(s, cam, w, h)
+    f64 = {k: (np.asarray(v, np.float64) if k in ("vertices", "base_colors", "densities", "color_grads") else v)
+           for k, v in s.items()}
+    iv = orc.interval_forward(f64, cam, w, h, ref["sorted_vals"], ref["ranges"], dtype=np.float64)
+    q = npref.render(s, cam, w, h)
+    assert q[..., 3].max() > 0.3
+    err = np.abs(iv["image"] - q).max()
+    assert err < 5e-4, err
+    g0 = dict(f64, color_grads=np.zeros(3))
+    iv0 = orc.interval_forward(g0, cam, w, h, ref["sorted_vals"], ref["ranges"], dtype=np.float64)
+    assert np.abs(iv0["image"] - iv["image"]).max() > 2e-2

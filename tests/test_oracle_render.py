@@ -190,3 +190,38 @@ def test_f32_f64_oracle_agree():
     a = orc.render(s, cam, 48, 48)["rgba"]
     b = orc.render(s, cam, 48, 48, dtype=np.float64)["rgba"]
     assert np.abs(a - b).max() < IMG_TOL
+
+
+def test_aux_entropy_channels_two_tets():
+    """aux[4..8] (rasterize_compute.wgsl:447-456): per covered pair
+    w = T alpha and m = (t_min + t_max) / 2, summed as (w, w log w, w m,
+    (w m) log(w m)); checked against float64 chords of both tets in depth
+    order (the front tet's alpha gives the back tet's T)."""
+    s = two_tet_scene(np.random.default_rng(3))
+    w = h = 48
+    cam = make_camera((0.9, -1.6, 1.9), (0.5, 0.35, 0.0), math.radians(45), w, h)
+    out = orc.render(s, cam, w, h, want_aux=True)
+    aux = out["aux"].reshape(-1, 8)
+    o, d = npref.pixel_rays(cam, w, h)
+    tv = np.asarray(s["vertices"], np.float64).reshape(-1, 3)[np.asarray(s["indices"]).reshape(-1, 4)]
+    t0, t1, hit = npref.tet_chords(tv, o, d)
+    t0, t1 = np.where(hit, t0, 0.0), np.where(hit, t1, 0.0)
+    order = npref.depth_order(s, cam[0])
+    dens = np.asarray(s["densities"], np.float64)
+    e = np.zeros((len(d), 4))
+    T = np.ones(len(d))
+    for k in order:
+        L = np.where(hit[:, k], t1[:, k] - t0[:, k], 0.0)
+        a = 1.0 - np.exp(-dens[k] * L)
+        wt = T * a
+        m = 0.5 * (t0[:, k] + t1[:, k])
+        wc = wt * m
+        e[:, 0] += np.where(hit[:, k], wt, 0.0)
+        e[:, 1] += np.where(hit[:, k] & (wt > 1e-20), wt * np.log(np.maximum(wt, 1e-300)), 0.0)
+        e[:, 2] += np.where(hit[:, k], wc, 0.0)
+        e[:, 3] += np.where(hit[:, k] & (wc > 1e-20), wc * np.log(np.maximum(wc, 1e-300)), 0.0)
+        T = T * np.where(hit[:, k], 1.0 - a, 1.0)
+    # both tets behind one another somewhere, and pixels away from grazing chords
+    assert np.count_nonzero(hit.all(axis=1)) > 50
+    sel = np.abs(e[:, 0]) > 1e-3
+    np.testing.assert_allclose(aux[sel, 4:8], e[sel], atol=2e-4, rtol=1e-4)
