@@ -2521,12 +2521,14 @@ __device__ __forceinline__ void add_at(V3* a, int i, V3 v) {
 // Per-tet chain back (interval_chain_back.wgsl:160-545): the decomposition's
 // gradients through each vertex's NDC z and colour offset, with the screen
 // geometry (silhouette positions, barycentrics, crossing parameters) fixed.
+// Per tet: d vertex (4 x 3) then d vertex normal (4 x 3), in vertex slot order.
+constexpr int kTetVertTerms = 24;
+
 __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict__ rec,
-                                                        const uint32_t* __restrict__ idx,
                                                         const ViewParams* __restrict__ views, uint32_t M,
-                                                        const float* __restrict__ d_ivr, float* __restrict__ d_verts,
+                                                        const float* __restrict__ d_ivr, float* __restrict__ tv,
                                                         float* __restrict__ d_dens, float* __restrict__ d_base,
-                                                        float* __restrict__ d_grads, float* __restrict__ d_vn) {
+                                                        float* __restrict__ d_grads) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= M) return;
     const ViewParams& vw = views[0];
@@ -2537,7 +2539,12 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
                      mk3(r[3].x, r[3].y, r[3].z)};
     const V3 g = mk3(r[1].w, r[2].w, r[3].w);
     IvlGeo G;
-    if (!ivl_geometry(v, g, r[0].w, vw, G)) return;
+    if (!ivl_geometry(v, g, r[0].w, vw, G)) {
+        float4* tv4 = reinterpret_cast<float4*>(tv + (uint64_t)kTetVertTerms * t);
+#pragma unroll
+        for (int i = 0; i < kTetVertTerms / 4; ++i) tv4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
     float D[kIvlGradStride];
     const float4* D4 = reinterpret_cast<const float4*>(d_ivr + (uint64_t)kIvlGradStride * t);
@@ -2621,20 +2628,21 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
         add_at(dn, G.b1, scl3(dnb, G.tb));
     }
     d_dens[t] = D[23];
-    const uint4 q = reinterpret_cast<const uint4*>(idx)[t];
-    const uint32_t vid[4] = {q.x, q.y, q.z, q.w};
+    // the tet's per-vertex terms, gathered per vertex by k_vertex_gather
+    float o[kTetVertTerms];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const V3 a = safe_clip3(dv[i], kMinVal, kMaxVal), b = safe_clip3(dn[i], kMinVal, kMaxVal);
-        unsafeAtomicAdd(d_verts + 3ull * vid[i], a.x);
-        unsafeAtomicAdd(d_verts + 3ull * vid[i] + 1, a.y);
-        unsafeAtomicAdd(d_verts + 3ull * vid[i] + 2, a.z);
-        if (b.x != 0.0f || b.y != 0.0f || b.z != 0.0f) {
-            unsafeAtomicAdd(d_vn + 3ull * vid[i], b.x);
-            unsafeAtomicAdd(d_vn + 3ull * vid[i] + 1, b.y);
-            unsafeAtomicAdd(d_vn + 3ull * vid[i] + 2, b.z);
-        }
+        o[3 * i] = a.x;
+        o[3 * i + 1] = a.y;
+        o[3 * i + 2] = a.z;
+        o[12 + 3 * i] = b.x;
+        o[13 + 3 * i] = b.y;
+        o[14 + 3 * i] = b.z;
     }
+    float4* tv4 = reinterpret_cast<float4*>(tv + (uint64_t)kTetVertTerms * t);
+#pragma unroll
+    for (int i = 0; i < kTetVertTerms / 4; ++i) tv4[i] = make_float4(o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]);
     const V3 a = safe_clip3(dg, kMinVal, kMaxVal), b = safe_clip3(mk3(D[20], D[21], D[22]), kMinVal, kMaxVal);
     d_grads[3ull * t] = a.x;
     d_grads[3ull * t + 1] = a.y;
@@ -2642,6 +2650,33 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
     d_base[3ull * t] = b.x;
     d_base[3ull * t + 1] = b.y;
     d_base[3ull * t + 2] = b.z;
+}
+
+// One thread per vertex: the sum of its tets' terms over the vertex -> (tet,
+// slot) incidence lists (no atomics; the order of the sum is fixed).
+__global__ __launch_bounds__(256) void k_vertex_gather(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ ent, const float* __restrict__ tv,
+                                                       uint32_t N, float* __restrict__ d_verts,
+                                                       float* __restrict__ d_vn) {
+    const uint32_t vtx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vtx >= N) return;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, n0 = 0.f, n1 = 0.f, n2 = 0.f;
+    for (uint32_t e = off[vtx], end = off[vtx + 1]; e < end; ++e) {
+        const uint32_t te = ent[e];
+        const float* p = tv + (uint64_t)kTetVertTerms * (te >> 2) + 3u * (te & 3u);
+        a0 += p[0];
+        a1 += p[1];
+        a2 += p[2];
+        n0 += p[12];
+        n1 += p[13];
+        n2 += p[14];
+    }
+    d_verts[3ull * vtx] = a0;
+    d_verts[3ull * vtx + 1] = a1;
+    d_verts[3ull * vtx + 2] = a2;
+    d_vn[3ull * vtx] = n0;
+    d_vn[3ull * vtx + 1] = n1;
+    d_vn[3ull * vtx + 2] = n2;
 }
 
 template <int TS>
@@ -2659,8 +2694,11 @@ bool launch_interval_backward(const IvlBwdArgs& a, uint32_t tile_size, hipStream
         default: ivl_bwd_launch<16>(a, s); break;
     }
     if (a.M)
-        hipLaunchKernelGGL(k_interval_chain, dim3((a.M + 255) / 256), dim3(256), 0, s, a.rec, a.idx, a.views, a.M,
-                           a.d_ivr, a.d_verts, a.d_dens, a.d_base, a.d_grads, a.d_vn);
+        hipLaunchKernelGGL(k_interval_chain, dim3((a.M + 255) / 256), dim3(256), 0, s, a.rec, a.views, a.M, a.d_ivr,
+                           a.tv, a.d_dens, a.d_base, a.d_grads);
+    if (a.N)
+        hipLaunchKernelGGL(k_vertex_gather, dim3((a.N + 255) / 256), dim3(256), 0, s, a.v2t_off, a.v2t, a.tv, a.N,
+                           a.d_verts, a.d_vn);
     return true;
 }
 

@@ -103,7 +103,9 @@ struct rmesh_renderer {
     float4* ivr = nullptr;
     float *ivl_xyzd = nullptr, *ivl_dist = nullptr;
     // interval adjoint: upstream gradients, per-tet decomposition gradients, vertex-normal gradients
-    float *ivl_dxyzd = nullptr, *ivl_ddist = nullptr, *ivl_dgrad = nullptr, *ivl_dvn = nullptr;
+    float *ivl_dxyzd = nullptr, *ivl_ddist = nullptr, *ivl_dgrad = nullptr, *ivl_dvn = nullptr, *ivl_tv = nullptr;
+    // vertex -> (tet, slot) incidence lists of the (fixed) tet indices, built on first use
+    uint32_t *v2t_off = nullptr, *v2t = nullptr;
     bool have_interval = false;  // the buffers above hold the last interval forward's state
     // last forward
     uint32_t last_views = 0;
@@ -441,7 +443,7 @@ void free_all(rmesh_renderer* h) {
                     h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a, h->vals_b, h->hist, h->ranges, h->rgba,
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
                     h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd,
-                    h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn};
+                    h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn, h->ivl_tv, h->v2t_off, h->v2t};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -715,16 +717,27 @@ extern "C" rmesh_status rmesh_backward_interval_tiled(rmesh_renderer* h, const f
         HIP_TRY(dalloc(&h->ivl_ddist, 5 * npx));
         HIP_TRY(dalloc(&h->ivl_dgrad, 48ull * h->M));
         HIP_TRY(dalloc(&h->ivl_dvn, 3ull * h->N));
+        HIP_TRY(dalloc(&h->ivl_tv, 24ull * h->M));
+        // incidence lists by counting sort of the indices (tet-major, so each
+        // vertex's list is in tet order)
+        std::vector<uint32_t> ix(4ull * h->M), off(h->N + 1ull, 0u), ent(4ull * h->M);
+        HIP_TRY(hipMemcpy(ix.data(), h->idx, 16ull * h->M, hipMemcpyDeviceToHost));
+        for (uint32_t v : ix) ++off[v + 1];
+        for (uint32_t v = 0; v < h->N; ++v) off[v + 1] += off[v];
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (uint64_t e = 0; e < ix.size(); ++e) ent[cur[ix[e]]++] = (uint32_t)e;  // e = tet << 2 | slot
+        HIP_TRY(dalloc(&h->v2t_off, h->N + 1ull));
+        HIP_TRY(dalloc(&h->v2t, 4ull * h->M));
+        HIP_TRY(hipMemcpy(h->v2t_off, off.data(), 4ull * (h->N + 1ull), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->v2t, ent.data(), 16ull * h->M, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMemcpyAsync(h->dl, dl_img, npx * 16, hipMemcpyHostToDevice, s));
     if (dl_xyzd) HIP_TRY(hipMemcpyAsync(h->ivl_dxyzd, dl_xyzd, npx * 16, hipMemcpyHostToDevice, s));
     if (dl_dist) HIP_TRY(hipMemcpyAsync(h->ivl_ddist, dl_dist, npx * 20, hipMemcpyHostToDevice, s));
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->ivl_dgrad), 0u, 48ull * h->M, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_verts), 0u, 3ull * h->N, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_dens), 0u, h->M, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_colors), 0u, 3ull * h->M, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_grads), 0u, 3ull * h->M, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(h->ivl_dvn), 0u, 3ull * h->N, s);
     IvlBwdArgs a;
     a.rec = h->rec;
     a.idx = h->idx;
@@ -741,6 +754,10 @@ extern "C" rmesh_status rmesh_backward_interval_tiled(rmesh_renderer* h, const f
     a.dl_xyzd = dl_xyzd ? h->ivl_dxyzd : nullptr;
     a.dl_dist = dl_dist ? h->ivl_ddist : nullptr;
     a.d_ivr = h->ivl_dgrad;
+    a.tv = h->ivl_tv;
+    a.v2t_off = h->v2t_off;
+    a.v2t = h->v2t;
+    a.N = h->N;
     a.d_verts = h->d_verts;
     a.d_dens = h->d_dens;
     a.d_base = h->d_colors;

@@ -100,11 +100,15 @@ struct IvlBwdArgs {
     const float* dl_xyzd;        // [H][W][4] or nullptr
     const float* dl_dist;        // [H][W][5] or nullptr
     float* d_ivr;                // [M][48] zeroed scratch
-    float* d_verts;              // [N*3] zeroed, accumulated
+    float* tv;                   // [M][24] per-tet vertex terms (scratch, written by the chain back)
+    const uint32_t* v2t_off;     // [N + 1] vertex -> incidence list offsets
+    const uint32_t* v2t;         // [4M] incidences: tet << 2 | vertex slot, per vertex
+    uint32_t N;
+    float* d_verts;              // [N*3] written
     float* d_dens;               // [M] zeroed; written per decomposed tet
     float* d_base;               // [M*3] zeroed; written per decomposed tet
     float* d_grads;              // [M*3] zeroed; written per decomposed tet
-    float* d_vn;                 // [N*3] zeroed, accumulated
+    float* d_vn;                 // [N*3] written
 };
 bool launch_interval_backward(const IvlBwdArgs& a, uint32_t tile_size, hipStream_t s);
 
