@@ -180,8 +180,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        print(f"bench.py: the train step runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding over GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     V, Wd = args.views, args.width

@@ -10,8 +10,7 @@
  * depth and distortion terms of orc_interval_backward.
  *
  * Every function names the reference kernel and lines it follows.  Float
- * semantics chosen where WGSL leaves them to the driver (documented in
- * DESIGN.md, "oracle conventions"):
+ * semantics chosen where WGSL leaves them to the driver:
  *   dot(a,b)      = (a.x*b.x + a.y*b.y) + a.z*b.z, no FMA contraction
  *   cross(a,b)    = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
  *   normalize(v)  = v / sqrt(dot(v,v))   (IEEE division and square root)
