@@ -2521,8 +2521,9 @@ __device__ __forceinline__ void add_at(V3* a, int i, V3 v) {
 // Per-tet chain back (interval_chain_back.wgsl:160-545): the decomposition's
 // gradients through each vertex's NDC z and colour offset, with the screen
 // geometry (silhouette positions, barycentrics, crossing parameters) fixed.
-// Per tet: d vertex (4 x 3) then d vertex normal (4 x 3), in vertex slot order.
-constexpr int kTetVertTerms = 24;
+// Per tet and vertex slot, one 32-byte sector: (d vertex xyz, 0, d vertex
+// normal xyz, 0), so a vertex's gather reads one sector per incident tet.
+constexpr int kTetVertTerms = 32;
 
 __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict__ rec,
                                                         const ViewParams* __restrict__ views, uint32_t M,
@@ -2629,20 +2630,13 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
     }
     d_dens[t] = D[23];
     // the tet's per-vertex terms, gathered per vertex by k_vertex_gather
-    float o[kTetVertTerms];
+    float4* tv4 = reinterpret_cast<float4*>(tv + (uint64_t)kTetVertTerms * t);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const V3 a = safe_clip3(dv[i], kMinVal, kMaxVal), b = safe_clip3(dn[i], kMinVal, kMaxVal);
-        o[3 * i] = a.x;
-        o[3 * i + 1] = a.y;
-        o[3 * i + 2] = a.z;
-        o[12 + 3 * i] = b.x;
-        o[13 + 3 * i] = b.y;
-        o[14 + 3 * i] = b.z;
+        tv4[2 * i] = make_float4(a.x, a.y, a.z, 0.0f);
+        tv4[2 * i + 1] = make_float4(b.x, b.y, b.z, 0.0f);
     }
-    float4* tv4 = reinterpret_cast<float4*>(tv + (uint64_t)kTetVertTerms * t);
-#pragma unroll
-    for (int i = 0; i < kTetVertTerms / 4; ++i) tv4[i] = make_float4(o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]);
     const V3 a = safe_clip3(dg, kMinVal, kMaxVal), b = safe_clip3(mk3(D[20], D[21], D[22]), kMinVal, kMaxVal);
     d_grads[3ull * t] = a.x;
     d_grads[3ull * t + 1] = a.y;
@@ -2663,13 +2657,14 @@ __global__ __launch_bounds__(256) void k_vertex_gather(const uint32_t* __restric
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, n0 = 0.f, n1 = 0.f, n2 = 0.f;
     for (uint32_t e = off[vtx], end = off[vtx + 1]; e < end; ++e) {
         const uint32_t te = ent[e];
-        const float* p = tv + (uint64_t)kTetVertTerms * (te >> 2) + 3u * (te & 3u);
-        a0 += p[0];
-        a1 += p[1];
-        a2 += p[2];
-        n0 += p[12];
-        n1 += p[13];
-        n2 += p[14];
+        const float4* p = reinterpret_cast<const float4*>(tv + (uint64_t)kTetVertTerms * (te >> 2) + 8u * (te & 3u));
+        const float4 a = p[0], n = p[1];
+        a0 += a.x;
+        a1 += a.y;
+        a2 += a.z;
+        n0 += n.x;
+        n1 += n.y;
+        n2 += n.z;
     }
     d_verts[3ull * vtx] = a0;
     d_verts[3ull * vtx + 1] = a1;

@@ -717,7 +717,7 @@ extern "C" rmesh_status rmesh_backward_interval_tiled(rmesh_renderer* h, const f
         HIP_TRY(dalloc(&h->ivl_ddist, 5 * npx));
         HIP_TRY(dalloc(&h->ivl_dgrad, 48ull * h->M));
         HIP_TRY(dalloc(&h->ivl_dvn, 3ull * h->N));
-        HIP_TRY(dalloc(&h->ivl_tv, 24ull * h->M));
+        HIP_TRY(dalloc(&h->ivl_tv, 32ull * h->M));
         // incidence lists by counting sort of the indices (tet-major, so each
         // vertex's list is in tet order)
         std::vector<uint32_t> ix(4ull * h->M), off(h->N + 1ull, 0u), ent(4ull * h->M);

@@ -100,7 +100,7 @@ struct IvlBwdArgs {
     const float* dl_xyzd;        // [H][W][4] or nullptr
     const float* dl_dist;        // [H][W][5] or nullptr
     float* d_ivr;                // [M][48] zeroed scratch
-    float* tv;                   // [M][24] per-tet vertex terms (scratch, written by the chain back)
+    float* tv;                   // [M][32] per-tet vertex terms (scratch, written by the chain back)
     const uint32_t* v2t_off;     // [N + 1] vertex -> incidence list offsets
     const uint32_t* v2t;         // [4M] incidences: tet << 2 | vertex slot, per vertex
     uint32_t N;
