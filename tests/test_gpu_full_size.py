@@ -1,0 +1,124 @@
+"""GPU: parity at the benchmark workload's full size (config C: 200,000-point
+8-cluster Delaunay mesh, 1,348,948 tets, SH degree 3, 800x800), one view,
+against the oracle on the same inputs:
+
+- binning: the pair count, sorted tile lists and ranges of the view, bit for bit;
+- forward_tiled: image within 1e-4 max-abs;
+- forward_interval_tiled: image within 1e-4 max-abs, depth and distortion
+  (first four channels) within 1e-4 of each channel's maximum;
+- both adjoints, with the L2 loss gradient against a grey image: at this
+  size the float32 adjoint is itself ~1e-3 (vertices, densities) to ~1e-2
+  (colour terms) from the float64 adjoint over the same tile lists — the
+  undo reconstructs log T from the image alpha, log(1 - a), which loses
+  digits where a -> 1 — so each GPU gradient is held to the float64 oracle
+  within twice the float32 oracle's own distance from it (or 1e-3).
+
+The scene build (Delaunay) and the CPU oracle in float32 and float64 take
+about two minutes.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+W = 800
+
+
+@pytest.fixture(scope="module")
+def config_c():
+    from rmeshvk_amd.scenes import config_scene
+
+    scene, cams = config_scene("C", seed=0)
+    return scene, cams[0]
+
+
+@pytest.fixture(scope="module")
+def renderer(config_c):
+    from rmeshvk_amd import RMeshRenderer
+
+    scene, _ = config_c
+    return RMeshRenderer(scene["vertices"], scene["indices"], scene["base_colors"], scene["densities"],
+                         scene["color_grads"], scene["circumdata"], W, W, device_id=0, sh_coeffs=scene["sh"],
+                         sh_degree=int(scene["sh_degree"]), tile_size=16)
+
+
+@pytest.fixture(scope="module")
+def reference(config_c):
+    scene, cam = config_c
+    return orc.render(scene, cam, W, W, ts=16)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _f64(scene):
+    return {k: (np.asarray(v, np.float64) if k in ("vertices", "base_colors", "densities", "color_grads", "sh")
+                else v) for k, v in scene.items()}
+
+
+def _hold(name, got, o32, o64):
+    """got within max(2 x |o32 - o64|, 1e-3) of o64 (relative L2)."""
+    e32 = _rel(o32, o64)
+    eg = _rel(got, o64)
+    print(f"{name}: gpu vs f64 {eg:.2e}, f32 oracle vs f64 {e32:.2e}")
+    assert eg <= max(2.0 * e32, 1e-3), (name, eg, e32)
+
+
+def test_full_size_tiled_forward_backward(config_c, renderer, reference):
+    scene, cam = config_c
+    ref = reference
+    assert len(scene["indices"]) // 4 == 1348948
+    img = renderer.forward_tiled(*cam)
+    assert renderer.pair_count() == int(ref["P"])
+    sk, sv, rg = renderer.debug_sorted()
+    np.testing.assert_array_equal(sk, ref["sorted_keys"])
+    np.testing.assert_array_equal(sv, ref["sorted_vals"])
+    np.testing.assert_array_equal(rg, ref["ranges"])
+    assert ref["rgba"][..., 3].mean() > 0.2
+    assert np.abs(img - ref["rgba"]).max() <= 1e-4
+    dl, _ = orc.loss_grad(ref["rgba"], np.full((W, W, 3), 0.5, np.float32), 1)
+    dl[..., 3] = np.float32(0.01)
+    dl = np.ascontiguousarray(dl, np.float32)
+    g = renderer.backward(dl)
+    o32 = orc.backward(scene, cam, W, W, ref["sorted_vals"], ref["ranges"], ref["rgba"], dl, 16, ref["colors"])
+    s64 = _f64(scene)
+    col64 = np.asarray(ref["colors"], np.float64)
+    f64 = orc.forward(s64, cam, W, W, ref["sorted_vals"], ref["ranges"], 16, 0.0, col64, np.float64)
+    o64 = orc.backward(s64, cam, W, W, ref["sorted_vals"], ref["ranges"], f64["rgba"], dl.astype(np.float64), 16,
+                       col64, np.float64)
+    # the renderer's backward includes the SH adjoint of the colour gradient
+    sh32 = orc.sh_backward(scene, cam[0], o32["d_base_colors"])
+    sh64 = orc.sh_backward(s64, cam[0], o64["d_base_colors"], np.float64)
+    for o, sh in ((o32, sh32), (o64, sh64)):
+        o["d_color_grads"] = o["d_color_grads"] + sh["d_color_grads"]
+        o["d_vertices"] = o["d_vertices"] + sh["d_vertices"]
+    _hold("d_sh", g["d_sh"], sh32["d_sh"], sh64["d_sh"])
+    for k in ("d_vertices", "d_densities", "d_color_grads", "d_base_colors"):
+        _hold(k, g[k], o32[k], o64[k])
+
+
+def test_full_size_interval(config_c, renderer, reference):
+    scene, cam = config_c
+    ref = reference
+    got = renderer.forward_interval_tiled(*cam)
+    iv = orc.interval_forward(scene, cam, W, W, ref["sorted_vals"], ref["ranges"], ts=16, colors=ref["colors"])
+    assert np.abs(got["image"] - iv["image"]).max() <= 1e-4
+    for a, b in ((got["xyzd"][..., 3:], iv["xyzd"][..., 3:]), (got["distortion"][..., :4], iv["distortion"][..., :4])):
+        scale = np.abs(b).reshape(-1, b.shape[-1]).max(axis=0)
+        err = np.abs(a - b).reshape(-1, b.shape[-1]).max(axis=0) / scale
+        assert np.all(err <= 1e-4), err
+    dl, _ = orc.loss_grad(iv["image"], np.full((W, W, 3), 0.5, np.float32), 1)
+    dl = np.ascontiguousarray(dl, np.float32)
+    g = renderer.backward_interval_tiled(dl)
+    o32 = orc.interval_backward_full(scene, cam, W, W, ref["sorted_vals"], ref["ranges"], dl, ts=16,
+                                     colors=ref["colors"])
+    o64 = orc.interval_backward_full(_f64(scene), cam, W, W, ref["sorted_vals"], ref["ranges"], dl.astype(np.float64),
+                                     ts=16, colors=np.asarray(ref["colors"], np.float64), dtype=np.float64)
+    for k in ("d_vertices", "d_densities", "d_base_colors", "d_color_grads"):
+        _hold("interval " + k, g[k], o32[k], o64[k])
