@@ -330,6 +330,8 @@ rmesh_status parse_ply(const uint8_t* data, size_t n, rmesh_scene_file& s) {
     }
     if (!le) return bad("PLY: only binary_little_endian is supported");
     if (N == 0 || M == 0) return bad("PLY: empty");
+    // counts and indices are 32-bit in the scene (and keep the size products below from wrapping)
+    if (N > UINT32_MAX || M > UINT32_MAX) return bad("PLY: element count above 2^32 - 1");
     const uint32_t deg = (uint32_t)std::sqrt((double)num_sh) - 1;
     const uint32_t nc = (deg + 1) * (deg + 1);
     if (num_sh == 0 || nc != num_sh) return bad("PLY: SH count " + std::to_string(num_sh) + " is not a perfect square");

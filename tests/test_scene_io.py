@@ -180,6 +180,13 @@ def test_ply_errors():
         io.load_ply(blob.replace(b"end_header", b"end_hexder"))
     with pytest.raises(ValueError, match="truncated"):
         io.load_ply(blob[:-7])
+    # counts whose byte sizes would wrap 64 bits (12 * 2^62 + 12) are refused, not wrapped
+    big = b"4611686018427387905"
+    M = len(s["indices"]) // 4
+    for old in (b"element vertex 60\n", b"element tetrahedron %d\n" % M):
+        assert old in blob
+        with pytest.raises(ValueError, match="2\\^32"):
+            io.load_ply(blob.replace(old, old.rsplit(b" ", 1)[0] + b" " + big + b"\n"))
 
 
 def test_circumspheres():
