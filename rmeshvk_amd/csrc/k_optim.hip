@@ -187,10 +187,10 @@ __device__ __forceinline__ float adam_update(float& p, float gr, float& m, float
 // ... for all three channels, so the view direction, the basis and the
 // direction adjoint (one vjp of the channel-summed weights: the vjp is linear)
 // are formed once per (tet, view).  The workgroup's coefficients (and Adam's
-// m, v) are one contiguous block in HBM, moved between HBM and LDS with
-// coalesced dword accesses; the four lanes' partial d_sh are summed with two
-// quad_perm butterflies, staged in LDS rows (padded, see SR) and
-// applied (Adam) or stored element-parallel.  One lane per tet scatters the
+// m, v) are one contiguous block in HBM; the coefficients are moved to LDS
+// with coalesced dword accesses; the four lanes' partial d_sh are summed with
+// two quad_perm butterflies, staged in LDS rows (padded, see SR) and applied
+// (Adam, m and v read and written in HBM directly) or stored element-parallel.  One lane per tet scatters the
 // direction/offset adjoint to the vertices.
 constexpr int kShTets = 32;
 constexpr int kShLanes = 4;
@@ -211,9 +211,10 @@ __global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a)
     // LDS row stride: whole float4s, and 16 tets' rows (3 SR floats apart) on
     // distinct banks for ds_read_b128
     constexpr int SR = (NC + 3) / 4 * 4 + 4;
-    constexpr int NIO = ADAM ? 3 : 1;  // sh (+ m, v)
     constexpr int NT = kShLanes * kShTets;
-    __shared__ __align__(16) float s_io[NIO][3 * kShTets * SR];
+    // only the coefficients are staged: Adam's m, v are read from HBM in the
+    // element-parallel update, which keeps LDS (15 KB) off the occupancy limit
+    __shared__ __align__(16) float s_io[1][3 * kShTets * SR];
     __shared__ float s_g[3 * kShTets * SR];  // summed d_sh rows
     const uint32_t lt = threadIdx.x / kShLanes, j = threadIdx.x % kShLanes;
     const uint32_t t0 = blockIdx.x * kShTets;
@@ -221,9 +222,7 @@ __global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a)
     const uint32_t nflt = min((uint32_t)kShTets, a.M - t0) * 3u * NC;  // floats of this workgroup
     float* io_g[3] = {a.sh + (uint64_t)t0 * 3 * NC, ADAM ? a.m_sh + (uint64_t)t0 * 3 * NC : nullptr,
                       ADAM ? a.v_sh + (uint64_t)t0 * 3 * NC : nullptr};
-#pragma unroll
-    for (int q = 0; q < NIO; ++q)
-        for (uint32_t i = threadIdx.x; i < nflt; i += NT) s_io[q][(i / NC) * SR + i % NC] = io_g[q][i];
+    for (uint32_t i = threadIdx.x; i < nflt; i += NT) s_io[0][(i / NC) * SR + i % NC] = io_g[0][i];
     __syncthreads();
     const bool live = t < a.M;
     const int row0 = (int)(lt * 3) * SR;
@@ -331,18 +330,18 @@ __global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a)
                 for (int k = 0; k < NC; ++k) s_g[row0 + ch * SR + k] = dsh[ch][k];
     }
     __syncthreads();
-    if (ADAM) {
+    if constexpr (ADAM) {
         if (*a.st.skip) return;  // uniform: no barrier follows on this path
         const float stepf = (float)*a.st.step;
         const float b1p = expf(stepf * logf(a.st.beta1));
         const float b2p = expf(stepf * logf(a.st.beta2));
         for (uint32_t i = threadIdx.x; i < nflt; i += NT) {
             const uint32_t r = (i / NC) * SR + i % NC;
-            float p = s_io[0][r], m = s_io[NIO > 1 ? 1 : 0][r], v2 = s_io[NIO > 2 ? 2 : 0][r];
+            float p = s_io[0][r], m = io_g[1][i], v2 = io_g[2][i];
             adam_update(p, s_g[r], m, v2, a.lr, a.st.beta1, a.st.beta2, a.st.eps, b1p, b2p);
             io_g[0][i] = p;
-            if (NIO > 1) io_g[1][i] = m;
-            if (NIO > 2) io_g[2][i] = v2;
+            io_g[1][i] = m;
+            io_g[2][i] = v2;
         }
     } else {
         float* o = a.d_sh + (uint64_t)t0 * 3 * NC;
