@@ -1043,12 +1043,18 @@ __device__ __forceinline__ void seg_scan_dpp19(float* v, const SegMasksDpp& s) {
 #undef RMESH_SCAN_LEVEL
 #undef RMESH_FMAC_DPP
 
-// Vertex slot of each face role (a, b, c) for face fi; the fourth slot is the
-// opposite vertex.  FACES = {0,2,1}, {1,2,3}, {0,3,2}, {3,0,1}.
-__device__ __forceinline__ int face_slot(int fi, int role) {
-    // 4 bits per (face, role): slots packed as a | b << 4 | c << 8, faces 12 bits apart
-    constexpr uint64_t T = (0x120ull) | (0x321ull << 12) | (0x230ull << 24) | (0x103ull << 36);
-    return (int)((T >> (fi * 12 + role * 4)) & 0xFull);
+// Hit barycentrics of face fi's roles (a, b, c) placed at the tet's vertex
+// slots, 0 at the opposite vertex.  FACES = {0,2,1}, {1,2,3}, {0,3,2},
+// {3,0,1}: slot s of face f takes role
+//   f0: a c b -   f1: - a b c   f2: a - c b   f3: b c - a
+// Three face tests shared by the four slots (three selects each), in place
+// of a per-slot search over the face's role slots.
+__device__ __forceinline__ void face_slot_weights(int fi, float la, float lb, float lc, float* w) {
+    const bool f0 = fi == 0, f1 = fi == 1, f2 = fi == 2;
+    w[0] = f0 ? la : (f1 ? 0.0f : (f2 ? la : lb));
+    w[1] = f0 ? lc : (f1 ? la : (f2 ? 0.0f : lc));
+    w[2] = f0 ? lb : (f1 ? lb : (f2 ? lc : 0.0f));
+    w[3] = f0 ? 0.0f : (f1 ? lc : (f2 ? lb : la));
 }
 
 // Gradient terms of one covered (pixel, tet) pair, accumulated into acc
@@ -1117,9 +1123,7 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
         const float lb = f1.w + dot3(gb, h);
         const float lc = (1.0f - la) - lb;
         G[side] = scl3(nn, dt * inv_den);
-        const int sa = face_slot(fi, 0), sb = face_slot(fi, 1), sc = face_slot(fi, 2);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) wv[side][s] = s == sa ? la : (s == sb ? lb : (s == sc ? lc : 0.0f));
+        face_slot_weights(fi, la, lb, lc, wv[side]);
     }
     acc[0] += med_clip(d_dens_l, kMinVal, kMaxVal);
 #pragma unroll
