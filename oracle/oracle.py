@@ -118,12 +118,14 @@ def lib():
             f.argtypes = [rp, _u32p, rp, _u32, _f32p, _f32p, _f32p, _u32, _u32, _u32, _u32p, _u64p, _u32p, _u64]
             f.restype = _u64
             getattr(L, "orc_forward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p, _f32p,
-                                                         _u32, _u32, _u32, _f32, _u32, _u32, _u32, rp, _vp, _vp]
+                                                         _u32, _u32, _u32, _f32, _u32, _u32, _u32, rp, _vp, _vp,
+                                                         _vp, _vp, _vp]
             f = getattr(L, "orc_loss" + sfx)
             f.argtypes = [rp, rp, _u32, _u32, _u32, rp]
             f.restype = ctypes.c_double
             getattr(L, "orc_backward" + sfx).argtypes = [rp, _u32p, rp, rp, rp, _u32p, _u32p, _f32p, _f32p, _f32p,
-                                                          _u32, _u32, _u32, rp, rp, _u32, _u32, _u32, rp, rp, rp, rp]
+                                                          _u32, _u32, _u32, rp, rp, _u32, _u32, _u32, rp, rp, rp, rp,
+                                                          _vp, _vp, _vp]
             getattr(L, "orc_sh_colors" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp]
             getattr(L, "orc_sh_backward" + sfx).argtypes = [rp, _u32p, rp, rp, _u32, _u32, _f32p, rp, rp, rp, rp]
             getattr(L, "orc_error" + sfx).argtypes = [rp, _u32p, rp, _u32p, _u32p, _f32p, _f32p, _f32p, _u32, _u32,
@@ -140,6 +142,8 @@ def lib():
                                                                    _u32, _u32, rp, rp, rp, _vp, _vp, rp]
             getattr(L, "orc_interval_chain" + sfx).argtypes = [rp, _u32p, rp, _f32p, _f32p, _f32p, _u32, _u32, _u32,
                                                                 rp, rp, rp, rp, rp, rp, rp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         L.orc_sorting_bits.argtypes = [_u32]
         L.orc_sorting_bits.restype = _u32
         L.orc_sort.argtypes = [_u64p, _u32p, _u64, _u32]
@@ -148,6 +152,15 @@ def lib():
         L.orc_circumspheres.argtypes = [_f32p, _u32p, _u64, _f32p]
         _LIB = L
     return _LIB
+
+
+def set_threads(n: int) -> None:
+    """Host threads of the tile-parallel loops (0: all cores)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def _sfx(dtype):
@@ -208,26 +221,43 @@ def ranges(sorted_keys, num_tiles):
     return r
 
 
+def checkpoint_slots(n_pairs, num_tiles):
+    """Slots of the checkpoint buffer (each TS*TS values) for n_pairs pairs."""
+    return (int(n_pairs) >> 5) + int(num_tiles) + 1
+
+
 def forward(scene, cam, w, h, sorted_vals, rng_, ts=12, min_t=0.0, colors=None, dtype=np.float32,
-            tiles=None, want_aux=False, want_stats=False, tile_step=1):
+            tiles=None, want_aux=False, want_stats=False, tile_step=1, want_checkpoints=False):
+    """Tiled forward.  want_checkpoints: also the per-pixel transmittance
+    checkpoints of the checkpointed adjoint ("ck" [slots][ts*ts], "fin_logt"
+    [H, W], "fin_stop" [H, W])."""
     cp, vp, ci = cam
     _, _, T = tile_grid(w, h, ts)
     t0, t1 = (0, T) if tiles is None else tiles
     rgba = np.zeros(h * w * 4, dtype=dtype)
     aux = np.zeros(h * w * 8, dtype=dtype) if want_aux else None
     stats = np.zeros(h * w * 4, dtype=np.uint32) if want_stats else None
+    ck = fin = stop = None
+    if want_checkpoints:
+        ck = np.zeros(checkpoint_slots(len(sorted_vals), T) * ts * ts, dtype=dtype)
+        fin = np.zeros(h * w, dtype=dtype)
+        stop = np.zeros(h * w, dtype=np.uint32)
     col = scene["base_colors"] if colors is None else colors
     getattr(lib(), "orc_forward" + _sfx(dtype))(
         _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
         _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
         _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
         w, h, ts, min_t, t0, t1, tile_step, rgba,
-        None if aux is None else aux.ctypes.data, None if stats is None else stats.ctypes.data)
+        None if aux is None else aux.ctypes.data, None if stats is None else stats.ctypes.data,
+        None if ck is None else ck.ctypes.data, None if fin is None else fin.ctypes.data,
+        None if stop is None else stop.ctypes.data)
     out = {"rgba": rgba.reshape(h, w, 4)}
     if want_aux:
         out["aux"] = aux.reshape(h, w, 8)
     if want_stats:
         out["stats"] = stats.reshape(h, w, 4)
+    if want_checkpoints:
+        out.update(ck=ck, fin_logt=fin.reshape(h, w), fin_stop=stop.reshape(h, w))
     return out
 
 
@@ -240,7 +270,10 @@ def loss_grad(rgba, gt, loss_type, dtype=np.float32):
 
 
 def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, dtype=np.float32, tiles=None,
-             tile_step=1):
+             tile_step=1, checkpoints=None):
+    """Tiled adjoint.  checkpoints=None: the reference's undo from the image
+    alpha; otherwise the forward's output dict (with want_checkpoints=True):
+    the checkpointed adjoint."""
     cp, vp, ci = cam
     _, _, T = tile_grid(w, h, ts)
     t0, t1 = (0, T) if tiles is None else tiles
@@ -251,11 +284,17 @@ def backward(scene, cam, w, h, sorted_vals, rng_, rgba, dl, ts=12, colors=None, 
     dG = np.zeros(M * 3, dtype=dtype)
     dC = np.zeros(M * 3, dtype=dtype)
     col = scene["base_colors"] if colors is None else colors
+    keep = []
+    ckp = [None, None, None]
+    if checkpoints is not None:
+        keep = [_a(checkpoints["ck"], dtype), _a(checkpoints["fin_logt"], dtype).ravel(),
+                _a(checkpoints["fin_stop"], np.uint32).ravel()]
+        ckp = [k.ctypes.data for k in keep]
     getattr(lib(), "orc_backward" + _sfx(dtype))(
         _a(scene["vertices"], dtype), _a(scene["indices"], np.uint32), _a(col, dtype),
         _a(scene["densities"], dtype), _a(scene["color_grads"], dtype), _a(sorted_vals, np.uint32),
         _a(rng_, np.uint32), _a(cp, np.float32), _a(vp, np.float32), _a(ci, np.float32),
-        w, h, ts, _a(rgba, dtype).ravel(), _a(dl, dtype).ravel(), t0, t1, tile_step, dV, dD, dG, dC)
+        w, h, ts, _a(rgba, dtype).ravel(), _a(dl, dtype).ravel(), t0, t1, tile_step, dV, dD, dG, dC, *ckp)
     return {"d_vertices": dV, "d_densities": dD, "d_color_grads": dG, "d_base_colors": dC}
 
 

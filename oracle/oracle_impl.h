@@ -223,7 +223,9 @@ void FN(orc_project)(const R* verts, const uint32_t* idx, const R* dens, uint32_
                      uint32_t* tiles_touched, uint8_t* visible) {
     CAM c;
     FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
-    for (uint32_t t = 0; t < M; ++t) {
+#pragma omp parallel for schedule(static) if (M >= 4096)
+    for (int64_t ti = 0; ti < (int64_t)M; ++ti) {
+        const uint32_t t = (uint32_t)ti;
         int vis = 0;
         tiles_touched[t] = FN(project_one)(verts, idx, dens, t, &c, &vis);
         visible[t] = (uint8_t)vis;
@@ -386,121 +388,164 @@ static V3 FN(pixel_ray)(const CAM* c, uint32_t px, uint32_t py) {
     return FN(normalize)(d);
 }
 
-/* colors: [M*3] per-tet colour (base_colors or SH-evaluated); aux: [H*W*8] or NULL. */
+/* Per-pixel transmittance checkpoints (the checkpointed adjoint; the
+ * reference has none): the forward stores each pixel's log T before every
+ * tile-list position 32 j (j >= 1, 32 j < the tile's pair count) at slot
+ * (range start >> 5) + tile + j - 1 of ck[slot][TS*TS] (pixel row-major in
+ * the tile), and per image pixel the final log T and the stop position
+ * (1 + the last tet position it composited, 0 if none).  Slots of distinct
+ * tiles never overlap: (P >> 5) + tiles + 1 slots hold every tile's. */
+#define CK_GROUP 32u
+static inline uint64_t FN(ck_slot)(uint32_t rs, uint32_t tile, uint32_t j) {
+    return (uint64_t)(rs >> 5) + tile + j - 1u;
+}
+
+/* One tile of the forward (rasterize_compute.wgsl:92-511); st: TS*TS*12,
+ * ss: TS*TS*4, stp: TS*TS scratch of the calling thread. */
+static void FN(fwd_tile)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                         const uint32_t* sorted_vals, const uint32_t* ranges, const CAM* cp, uint32_t w, uint32_t h,
+                         uint32_t TS, uint32_t tile, R* rgba, R* aux, uint32_t* stats, R* st, uint32_t* ss,
+                         uint32_t* stp, R* ck, R* fin_logt, uint32_t* fin_stop) {
+    const CAM c = *cp;
+    uint32_t tiles_x = (w + TS - 1) / TS;
+    V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
+    uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
+    R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
+    memset(st, 0, sizeof(R) * TS * TS * 12);
+    memset(ss, 0, sizeof(uint32_t) * TS * TS * 4);
+    memset(stp, 0, sizeof(uint32_t) * TS * TS);
+    uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+    uint32_t cursor = re > rs ? re : rs;
+    while (cursor > rs) {
+        cursor -= 1;
+        const uint32_t q = re - 1 - cursor; /* position in compositing order */
+        if (ck && q > 0 && q % CK_GROUP == 0) {
+            R* dst = ck + FN(ck_slot)(rs, tile, q / CK_GROUP) * TS * TS;
+            for (uint32_t i = 0; i < TS * TS; ++i) dst[i] = st[12 * i + 3];
+        }
+        uint32_t t = sorted_vals[cursor];
+        V3 v[4];
+        for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
+        R proj[4][2];
+        int any_behind = FN(tile_proj)(&c, v, tile_ox, tile_oy, proj);
+        R density = dens[t];
+        V3 col = FN(load3)(colors, t), g = FN(load3)(grads, t);
+        V3 fn[4];
+        for (int fi = 0; fi < 4; ++fi) {
+            const int* f = FN(FACES)[fi];
+            V3 n = FN(cross)(FN(sub)(v[f[2]], v[f[0]]), FN(sub)(v[f[1]], v[f[0]]));
+            if (FN(dot)(n, FN(sub)(v[f[3]], v[f[0]])) < 0) n = FN(neg)(n);
+            fn[fi] = n;
+        }
+        for (uint32_t row = 0; row < TS; ++row) {
+            int32_t xl_i = 0, xr_i = -1;
+            if (!any_behind) FN(row_span)(proj, row, TS, 1, &xl_i, &xr_i);
+            for (int32_t col_i = xl_i; col_i <= xr_i; ++col_i) {
+                uint32_t colu = (uint32_t)col_i;
+                uint32_t pl = row * TS + colu;
+                uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+                if (px >= w || py >= h) continue;
+                R* s = st + 12 * pl;
+                if (s[3] < (R)-13.8) { ss[4 * pl + 2]++; continue; }
+                V3 rd = FN(pixel_ray)(&c, px, py);
+                V3 ro = FN(add)(cam, FN(scl)(rd, c.min_t));
+                R t_min = (R)-3.402823e38f, t_max = (R)3.402823e38f;
+                int entry = 4, valid = 1;
+                for (int fi = 0; fi < 4; ++fi) {
+                    const int* f = FN(FACES)[fi];
+                    V3 n = fn[fi];
+                    R num = FN(dot)(n, FN(sub)(v[f[0]], ro));
+                    R den = FN(dot)(n, rd);
+                    if (RABS(den) < (R)1e-20f) { if (num > 0) valid = 0; continue; }
+                    R tt = num / den;
+                    if (den > 0) { if (tt > t_min) { t_min = tt; entry = fi; } }
+                    else { if (tt < t_max) t_max = tt; }
+                }
+                if (!(valid && t_min < t_max)) { ss[4 * pl + 0]++; continue; }
+                R base_offset = FN(dot)(g, FN(sub)(ro, v[0]));
+                V3 bc = FN(mk)(col.x + base_offset, col.y + base_offset, col.z + base_offset);
+                R dc_dt = FN(dot)(g, rd);
+                R a0 = dc_dt * t_min, a1 = dc_dt * t_max;
+                V3 cs = FN(mk)(FN(rmax)(bc.x + a0, 0), FN(rmax)(bc.y + a0, 0), FN(rmax)(bc.z + a0, 0));
+                V3 ce = FN(mk)(FN(rmax)(bc.x + a1, 0), FN(rmax)(bc.y + a1, 0), FN(rmax)(bc.z + a1, 0));
+                R dist = t_max - t_min;
+                R od = FN(clampr)(density * dist, (R)1e-8f, (R)88.0);
+                R alpha_t = FN(safe_exp)(-od);
+                R phi_v = FN(phi)(od);
+                R w0 = phi_v - alpha_t, w1 = (R)1.0 - phi_v;
+                V3 cpm = FN(safe_clip3)(FN(mk)(ce.x * w0 + cs.x * w1, ce.y * w0 + cs.y * w1, ce.z * w0 + cs.z * w1),
+                                        (R)0.0, MAX_VAL_R);
+                R alpha = (R)1.0 - alpha_t;
+                R Tj = FN(safe_exp)(s[3]);
+                if (od < (R)0.01) ss[4 * pl + 1]++;
+                else if (Tj < (R)1e-6f) ss[4 * pl + 2]++;
+                else ss[4 * pl + 3]++;
+                if (Tj >= (R)1e-6f) {
+                    s[0] = s[0] + cpm.x * Tj; s[1] = s[1] + cpm.y * Tj; s[2] = s[2] + cpm.z * Tj;
+                    s[3] = s[3] - od;
+                    stp[pl] = q + 1;
+                    /* aux: normals + depth, entropy (rasterize_compute.wgsl:434-456) */
+                    V3 en = FN(mk)(0, 0, 0);
+                    if (entry < 4) en = FN(normalize)(FN(neg)(fn[entry]));
+                    R depth_premul = w0 * t_max + w1 * t_min;
+                    R ta = Tj * alpha;
+                    s[4] = s[4] + ta * en.x; s[5] = s[5] + ta * en.y; s[6] = s[6] + ta * en.z;
+                    s[7] = s[7] + Tj * depth_premul;
+                    R wt = alpha * Tj;
+                    R c_mid = (t_min + t_max) * (R)0.5;
+                    R wc = wt * c_mid;
+                    s[8] = s[8] + wt;
+                    s[9] = s[9] + (wt > (R)1e-20f ? wt * RLOG(wt) : (R)0.0);
+                    s[10] = s[10] + wt * c_mid;
+                    s[11] = s[11] + (wc > (R)1e-20f ? wc * RLOG(wc) : (R)0.0);
+                }
+            }
+        }
+    }
+    for (uint32_t i = 0; i < TS * TS; ++i) {
+        uint32_t row = i / TS, colu = i % TS;
+        uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
+        if (px < w && py < h) {
+            size_t pix = (size_t)py * w + px;
+            R* s = st + 12 * i;
+            rgba[4 * pix + 0] = s[0]; rgba[4 * pix + 1] = s[1]; rgba[4 * pix + 2] = s[2];
+            rgba[4 * pix + 3] = (R)1.0 - FN(safe_exp)(s[3]);
+            if (aux) for (int k = 0; k < 8; ++k) aux[8 * pix + k] = s[4 + k];
+            if (stats) for (int k = 0; k < 4; ++k) stats[4 * pix + k] = ss[4 * i + k];
+            if (fin_logt) fin_logt[pix] = s[3];
+            if (fin_stop) fin_stop[pix] = stp[i];
+        }
+    }
+}
+
+/* colors: [M*3] per-tet colour (base_colors or SH-evaluated); aux: [H*W*8] or NULL.
+ * Tiles are independent (each writes its own pixels and checkpoint slots), so
+ * they run in parallel on the host's cores; results do not depend on the
+ * thread count.  ck / fin_logt / fin_stop (NULL: not kept): the checkpoints
+ * of the checkpointed adjoint (layout above). */
 void FN(orc_forward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
                      const uint32_t* sorted_vals, const uint32_t* ranges,
                      const float* cam_pos, const float* vp, const float* c2w_intr,
                      uint32_t w, uint32_t h, uint32_t ts, float min_t,
-                     uint32_t tile_begin, uint32_t tile_end, uint32_t tile_step, R* rgba, R* aux, uint32_t* stats) {
+                     uint32_t tile_begin, uint32_t tile_end, uint32_t tile_step, R* rgba, R* aux, uint32_t* stats,
+                     R* ck, R* fin_logt, uint32_t* fin_stop) {
     CAM c;
     FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, min_t);
-    uint32_t tiles_x = (w + ts - 1) / ts;
-    uint32_t TS = ts;
-    R* st = (R*)calloc((size_t)TS * TS * 12, sizeof(R)); /* color3, log_t, nd4, ent4 */
-    uint32_t* ss = (uint32_t*)calloc((size_t)TS * TS * 4, sizeof(uint32_t));
-    V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
-    for (uint32_t tile = tile_begin; tile < tile_end; tile += (tile_step ? tile_step : 1)) {
-        uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
-        R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
-        memset(st, 0, sizeof(R) * TS * TS * 12);
-        memset(ss, 0, sizeof(uint32_t) * TS * TS * 4);
-        uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
-        uint32_t cursor = re;
-        while (cursor > rs) {
-            cursor -= 1;
-            uint32_t t = sorted_vals[cursor];
-            V3 v[4];
-            for (int i = 0; i < 4; ++i) v[i] = FN(load3)(verts, idx[4 * t + i]);
-            R proj[4][2];
-            int any_behind = FN(tile_proj)(&c, v, tile_ox, tile_oy, proj);
-            R density = dens[t];
-            V3 col = FN(load3)(colors, t), g = FN(load3)(grads, t);
-            V3 fn[4];
-            for (int fi = 0; fi < 4; ++fi) {
-                const int* f = FN(FACES)[fi];
-                V3 n = FN(cross)(FN(sub)(v[f[2]], v[f[0]]), FN(sub)(v[f[1]], v[f[0]]));
-                if (FN(dot)(n, FN(sub)(v[f[3]], v[f[0]])) < 0) n = FN(neg)(n);
-                fn[fi] = n;
-            }
-            for (uint32_t row = 0; row < TS; ++row) {
-                int32_t xl_i = 0, xr_i = -1;
-                if (!any_behind) FN(row_span)(proj, row, TS, 1, &xl_i, &xr_i);
-                for (int32_t col_i = xl_i; col_i <= xr_i; ++col_i) {
-                    uint32_t colu = (uint32_t)col_i;
-                    uint32_t pl = row * TS + colu;
-                    uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
-                    if (px >= w || py >= h) continue;
-                    R* s = st + 12 * pl;
-                    if (s[3] < (R)-13.8) { ss[4 * pl + 2]++; continue; }
-                    V3 rd = FN(pixel_ray)(&c, px, py);
-                    V3 ro = FN(add)(cam, FN(scl)(rd, c.min_t));
-                    R t_min = (R)-3.402823e38f, t_max = (R)3.402823e38f;
-                    int entry = 4, valid = 1;
-                    for (int fi = 0; fi < 4; ++fi) {
-                        const int* f = FN(FACES)[fi];
-                        V3 n = fn[fi];
-                        R num = FN(dot)(n, FN(sub)(v[f[0]], ro));
-                        R den = FN(dot)(n, rd);
-                        if (RABS(den) < (R)1e-20f) { if (num > 0) valid = 0; continue; }
-                        R tt = num / den;
-                        if (den > 0) { if (tt > t_min) { t_min = tt; entry = fi; } }
-                        else { if (tt < t_max) t_max = tt; }
-                    }
-                    if (!(valid && t_min < t_max)) { ss[4 * pl + 0]++; continue; }
-                    R base_offset = FN(dot)(g, FN(sub)(ro, v[0]));
-                    V3 bc = FN(mk)(col.x + base_offset, col.y + base_offset, col.z + base_offset);
-                    R dc_dt = FN(dot)(g, rd);
-                    R a0 = dc_dt * t_min, a1 = dc_dt * t_max;
-                    V3 cs = FN(mk)(FN(rmax)(bc.x + a0, 0), FN(rmax)(bc.y + a0, 0), FN(rmax)(bc.z + a0, 0));
-                    V3 ce = FN(mk)(FN(rmax)(bc.x + a1, 0), FN(rmax)(bc.y + a1, 0), FN(rmax)(bc.z + a1, 0));
-                    R dist = t_max - t_min;
-                    R od = FN(clampr)(density * dist, (R)1e-8f, (R)88.0);
-                    R alpha_t = FN(safe_exp)(-od);
-                    R phi_v = FN(phi)(od);
-                    R w0 = phi_v - alpha_t, w1 = (R)1.0 - phi_v;
-                    V3 cp = FN(safe_clip3)(FN(mk)(ce.x * w0 + cs.x * w1, ce.y * w0 + cs.y * w1, ce.z * w0 + cs.z * w1),
-                                           (R)0.0, MAX_VAL_R);
-                    R alpha = (R)1.0 - alpha_t;
-                    R Tj = FN(safe_exp)(s[3]);
-                    if (od < (R)0.01) ss[4 * pl + 1]++;
-                    else if (Tj < (R)1e-6f) ss[4 * pl + 2]++;
-                    else ss[4 * pl + 3]++;
-                    if (Tj >= (R)1e-6f) {
-                        s[0] = s[0] + cp.x * Tj; s[1] = s[1] + cp.y * Tj; s[2] = s[2] + cp.z * Tj;
-                        s[3] = s[3] - od;
-                        /* aux: normals + depth, entropy (rasterize_compute.wgsl:434-456) */
-                        V3 en = FN(mk)(0, 0, 0);
-                        if (entry < 4) en = FN(normalize)(FN(neg)(fn[entry]));
-                        R depth_premul = w0 * t_max + w1 * t_min;
-                        R ta = Tj * alpha;
-                        s[4] = s[4] + ta * en.x; s[5] = s[5] + ta * en.y; s[6] = s[6] + ta * en.z;
-                        s[7] = s[7] + Tj * depth_premul;
-                        R wt = alpha * Tj;
-                        R c_mid = (t_min + t_max) * (R)0.5;
-                        R wc = wt * c_mid;
-                        s[8] = s[8] + wt;
-                        s[9] = s[9] + (wt > (R)1e-20f ? wt * RLOG(wt) : (R)0.0);
-                        s[10] = s[10] + wt * c_mid;
-                        s[11] = s[11] + (wc > (R)1e-20f ? wc * RLOG(wc) : (R)0.0);
-                    }
-                }
-            }
-        }
-        for (uint32_t i = 0; i < TS * TS; ++i) {
-            uint32_t row = i / TS, colu = i % TS;
-            uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
-            if (px < w && py < h) {
-                size_t pix = (size_t)py * w + px;
-                R* s = st + 12 * i;
-                rgba[4 * pix + 0] = s[0]; rgba[4 * pix + 1] = s[1]; rgba[4 * pix + 2] = s[2];
-                rgba[4 * pix + 3] = (R)1.0 - FN(safe_exp)(s[3]);
-                if (aux) for (int k = 0; k < 8; ++k) aux[8 * pix + k] = s[4 + k];
-                if (stats) for (int k = 0; k < 4; ++k) stats[4 * pix + k] = ss[4 * i + k];
-            }
-        }
+    const uint32_t TS = ts, step = tile_step ? tile_step : 1;
+    const int64_t ntiles = tile_end > tile_begin ? ((int64_t)tile_end - tile_begin + step - 1) / step : 0;
+#pragma omp parallel if (ntiles >= 32)
+    {
+        R* st = (R*)malloc(sizeof(R) * TS * TS * 12);
+        uint32_t* ss = (uint32_t*)malloc(sizeof(uint32_t) * TS * TS * 4);
+        uint32_t* stp = (uint32_t*)malloc(sizeof(uint32_t) * TS * TS);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t k = 0; k < ntiles; ++k)
+            FN(fwd_tile)(verts, idx, colors, dens, grads, sorted_vals, ranges, &c, w, h, TS,
+                         tile_begin + (uint32_t)k * step, rgba, aux, stats, st, ss, stp, ck, fin_logt, fin_stop);
+        free(st);
+        free(ss);
+        free(stp);
     }
-    free(st);
-    free(ss);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1326,38 +1371,51 @@ double FN(orc_loss)(const R* rgba, const R* gt, uint32_t w, uint32_t h, uint32_t
 /* K13: reverse-order tiled adjoint                                      */
 /* crates/rmesh-backward/src/wgsl/backward_tiled_compute.wgsl:154-587   */
 /* Gradients accumulate (+=) into the output arrays.                    */
+/*                                                                      */
+/* Checkpointed mode (ck != NULL; no counterpart in the reference): the  */
+/* undo restarts at every 32-tet group of the tile list from the log T   */
+/* the forward stored at the group's end (or the pixel's final log T),   */
+/* instead of walking the whole list back from log(1 - alpha), and tets  */
+/* at or after the pixel's stop position (never composited) are skipped. */
+/* That is the exact adjoint of the forward as computed.                 */
 /* ------------------------------------------------------------------ */
-void FN(orc_backward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
-                      const uint32_t* sorted_vals, const uint32_t* ranges,
-                      const float* cam_pos, const float* vp, const float* c2w_intr,
-                      uint32_t w, uint32_t h, uint32_t ts,
-                      const R* rgba, const R* dl, uint32_t tile_begin, uint32_t tile_end, uint32_t tile_step,
-                      R* d_verts, R* d_dens, R* d_grads, R* d_colors) {
-    CAM c;
-    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
-    uint32_t tiles_x = (w + ts - 1) / ts;
-    uint32_t TS = ts;
-    R* st = (R*)calloc((size_t)TS * TS * 5, sizeof(R)); /* color3, log_t, d_log_t */
+#define BWD_TERMS 19
+/* One tile; acc[(cursor - rs) * 19]: the per-tet sums of this tile
+ * (density, 4 vertices, colour gradient, colour), st: TS*TS*5 and cur:
+ * TS*TS scratch of the calling thread. */
+static void FN(bwd_tile)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                         const uint32_t* sorted_vals, const uint32_t* ranges, const CAM* cp, uint32_t w, uint32_t h,
+                         uint32_t TS, uint32_t tile, const R* rgba, const R* dl, const R* ck, const R* fin_logt,
+                         const uint32_t* fin_stop, R* st, int64_t* cur, R* acc) {
+    const CAM c = *cp;
+    uint32_t tiles_x = (w + TS - 1) / TS;
     V3 cam = FN(mk)(c.cam[0], c.cam[1], c.cam[2]);
-    for (uint32_t tile = tile_begin; tile < tile_end; tile += (tile_step ? tile_step : 1)) {
+    {
         uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
         R tile_ox = (R)(tile_x * TS), tile_oy = (R)(tile_y * TS);
         for (uint32_t i = 0; i < TS * TS; ++i) {
             uint32_t row = i / TS, colu = i % TS;
             uint32_t px = tile_x * TS + colu, py = tile_y * TS + row;
             R* s = st + 5 * i;
+            cur[i] = -1;
             if (px < w && py < h) {
                 size_t pix = (size_t)py * w + px;
                 R a = rgba[4 * pix + 3];
                 s[0] = rgba[4 * pix]; s[1] = rgba[4 * pix + 1]; s[2] = rgba[4 * pix + 2];
-                s[3] = RLOG(FN(rmax)((R)1.0 - a, (R)1e-20f));
-                s[4] = -dl[4 * pix + 3] * ((R)1.0 - a);
+                if (ck) {
+                    s[3] = fin_logt[pix];
+                    s[4] = -dl[4 * pix + 3] * FN(safe_exp)(fin_logt[pix]);
+                } else {
+                    s[3] = RLOG(FN(rmax)((R)1.0 - a, (R)1e-20f));
+                    s[4] = -dl[4 * pix + 3] * ((R)1.0 - a);
+                }
             } else {
                 s[0] = s[1] = s[2] = s[3] = s[4] = 0;
             }
         }
         uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
         for (uint32_t cursor = rs; cursor < re; ++cursor) {
+            const uint32_t q = re - 1 - cursor; /* position in compositing order */
             uint32_t t = sorted_vals[cursor];
             uint32_t vi[4];
             V3 v[4];
@@ -1409,6 +1467,16 @@ void FN(orc_backward)(const R* verts, const uint32_t* idx, const R* colors, cons
                     V3 cp = FN(safe_clip3)(FN(mk)(ce.x * w0 + cs.x * w1, ce.y * w0 + cs.y * w1, ce.z * w0 + cs.z * w1),
                                            MIN_VAL_R, MAX_VAL_R);
                     R* s = st + 5 * pl;
+                    if (ck) {
+                        const uint32_t stop = fin_stop[pix];
+                        if (q >= stop) continue; /* never composited */
+                        const int64_t j = q / CK_GROUP;
+                        if (j != cur[pl]) {
+                            const uint32_t end = (uint32_t)(j + 1) * CK_GROUP;
+                            s[3] = end < stop ? ck[FN(ck_slot)(rs, tile, (uint32_t)j + 1) * TS * TS + pl] : fin_logt[pix];
+                            cur[pl] = j;
+                        }
+                    }
                     R log_t = s[3], d_log_t = s[4];
                     R prev_log_t = FN(rmin)(log_t + od, (R)0.0);
                     R t_prev = FN(safe_exp)(prev_log_t);
@@ -1466,15 +1534,72 @@ void FN(orc_backward)(const R* verts, const uint32_t* idx, const R* colors, cons
                     acc_g = FN(add)(acc_g, FN(safe_clip3)(dg_l, MIN_VAL_R, MAX_VAL_R));
                 }
             }
-            d_dens[t] += acc_d;
-            for (int k = 0; k < 4; ++k) {
-                d_verts[3 * vi[k]] += acc_v[k].x; d_verts[3 * vi[k] + 1] += acc_v[k].y; d_verts[3 * vi[k] + 2] += acc_v[k].z;
-            }
-            d_grads[3 * t] += acc_g.x; d_grads[3 * t + 1] += acc_g.y; d_grads[3 * t + 2] += acc_g.z;
-            d_colors[3 * t] += acc_c.x; d_colors[3 * t + 1] += acc_c.y; d_colors[3 * t + 2] += acc_c.z;
+            R* o = acc + (size_t)(cursor - rs) * BWD_TERMS;
+            o[0] = acc_d;
+            for (int k = 0; k < 4; ++k) { o[1 + 3 * k] = acc_v[k].x; o[2 + 3 * k] = acc_v[k].y; o[3 + 3 * k] = acc_v[k].z; }
+            o[13] = acc_g.x; o[14] = acc_g.y; o[15] = acc_g.z;
+            o[16] = acc_c.x; o[17] = acc_c.y; o[18] = acc_c.z;
         }
     }
-    free(st);
+}
+
+/* Tiles run in parallel in chunks; each tile's per-tet sums are then added
+ * into the gradients serially in tile and list order, so the result is
+ * that of the one-thread loop bit for bit, whatever the thread count. */
+void FN(orc_backward)(const R* verts, const uint32_t* idx, const R* colors, const R* dens, const R* grads,
+                      const uint32_t* sorted_vals, const uint32_t* ranges,
+                      const float* cam_pos, const float* vp, const float* c2w_intr,
+                      uint32_t w, uint32_t h, uint32_t ts,
+                      const R* rgba, const R* dl, uint32_t tile_begin, uint32_t tile_end, uint32_t tile_step,
+                      R* d_verts, R* d_dens, R* d_grads, R* d_colors,
+                      const R* ck, const R* fin_logt, const uint32_t* fin_stop) {
+    CAM c;
+    FN(cam_from_flat)(&c, cam_pos, vp, c2w_intr, w, h, ts, 0.0f);
+    const uint32_t TS = ts, step = tile_step ? tile_step : 1;
+    const int64_t ntiles = tile_end > tile_begin ? ((int64_t)tile_end - tile_begin + step - 1) / step : 0;
+    const uint64_t kChunk = 1u << 20; /* per-tet records held at once */
+    uint64_t* off = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(ntiles + 1));
+    for (int64_t k0 = 0; k0 < ntiles;) {
+        int64_t k1 = k0;
+        off[0] = 0;
+        while (k1 < ntiles && (k1 == k0 || off[k1 - k0] < kChunk)) {
+            const uint32_t tile = tile_begin + (uint32_t)k1 * step;
+            const uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+            off[k1 - k0 + 1] = off[k1 - k0] + (re > rs ? re - rs : 0);
+            ++k1;
+        }
+        R* acc = (R*)malloc(sizeof(R) * BWD_TERMS * (size_t)(off[k1 - k0] + 1));
+#pragma omp parallel if (k1 - k0 >= 32)
+        {
+            R* st = (R*)malloc(sizeof(R) * TS * TS * 5);
+            int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * TS * TS);
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t k = k0; k < k1; ++k)
+                FN(bwd_tile)(verts, idx, colors, dens, grads, sorted_vals, ranges, &c, w, h, TS,
+                             tile_begin + (uint32_t)k * step, rgba, dl, ck, fin_logt, fin_stop, st, cur,
+                             acc + (size_t)off[k - k0] * BWD_TERMS);
+            free(st);
+            free(cur);
+        }
+        for (int64_t k = k0; k < k1; ++k) {
+            const uint32_t tile = tile_begin + (uint32_t)k * step;
+            const uint32_t rs = ranges[2 * tile], re = ranges[2 * tile + 1];
+            for (uint32_t cursor = rs; cursor < re; ++cursor) {
+                const uint32_t t = sorted_vals[cursor];
+                const R* o = acc + (size_t)(off[k - k0] + (cursor - rs)) * BWD_TERMS;
+                d_dens[t] += o[0];
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t vi = idx[4 * t + i];
+                    d_verts[3 * vi] += o[1 + 3 * i]; d_verts[3 * vi + 1] += o[2 + 3 * i]; d_verts[3 * vi + 2] += o[3 + 3 * i];
+                }
+                d_grads[3 * t] += o[13]; d_grads[3 * t + 1] += o[14]; d_grads[3 * t + 2] += o[15];
+                d_colors[3 * t] += o[16]; d_colors[3 * t + 1] += o[17]; d_colors[3 * t + 2] += o[18];
+            }
+        }
+        free(acc);
+        k0 = k1;
+    }
+    free(off);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1555,7 +1680,9 @@ static void FN(sh_dir)(const R* verts, const uint32_t* idx, uint32_t t, const fl
 void FN(orc_sh_colors)(const R* verts, const uint32_t* idx, const R* sh, const R* grads, uint32_t M,
                        uint32_t deg, const float* cam_pos, R* colors) {
     uint32_t nc = (deg + 1) * (deg + 1);
-    for (uint32_t t = 0; t < M; ++t) {
+#pragma omp parallel for schedule(static) if (M >= 4096)
+    for (int64_t ti = 0; ti < (int64_t)M; ++ti) {
+        const uint32_t t = (uint32_t)ti;
         V3 v[4], centroid, raw, dir;
         R len;
         FN(sh_dir)(verts, idx, t, cam_pos, v, &centroid, &raw, &len, &dir);
@@ -1578,7 +1705,12 @@ void FN(orc_sh_backward)(const R* verts, const uint32_t* idx, const R* sh, const
                          uint32_t deg, const float* cam_pos, const R* d_colors,
                          R* d_sh, R* d_grads, R* d_verts) {
     uint32_t nc = (deg + 1) * (deg + 1);
-    for (uint32_t t = 0; t < M; ++t) {
+    /* per tet in parallel; the vertex terms are kept per tet and added in tet
+     * order afterwards (the one-thread result, bit for bit) */
+    V3* dvt = (V3*)malloc(sizeof(V3) * 2 * (size_t)(M ? M : 1));
+#pragma omp parallel for schedule(static) if (M >= 4096)
+    for (int64_t ti = 0; ti < (int64_t)M; ++ti) {
+        const uint32_t t = (uint32_t)ti;
         V3 v[4], centroid, raw, dir;
         R len;
         FN(sh_dir)(verts, idx, t, cam_pos, v, &centroid, &raw, &len, &dir);
@@ -1614,13 +1746,18 @@ void FN(orc_sh_backward)(const R* verts, const uint32_t* idx, const R* sh, const
             V3 draw = FN(scl)(FN(sub)(d_dir, FN(scl)(dir, dd)), (R)1.0 / len);
             dcen = FN(add)(dcen, draw);
         }
+        dvt[2 * t] = FN(scl)(dcen, (R)0.25);
+        dvt[2 * t + 1] = FN(scl)(g, s_sum);
+    }
+    for (uint32_t t = 0; t < M; ++t) {
         for (int i = 0; i < 4; ++i) {
             uint32_t vi = idx[4 * t + i];
-            V3 d = FN(scl)(dcen, (R)0.25);
-            if (i == 0) d = FN(add)(d, FN(scl)(g, s_sum));
+            V3 d = dvt[2 * t];
+            if (i == 0) d = FN(add)(d, dvt[2 * t + 1]);
             d_verts[3 * vi] += d.x; d_verts[3 * vi + 1] += d.y; d_verts[3 * vi + 2] += d.z;
         }
     }
+    free(dvt);
 }
 
 #undef V3

@@ -179,3 +179,8 @@ void orc_circumspheres(const float* verts, const uint32_t* idx, uint64_t M, floa
         out[4 * t + 3] = (float)(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     }
 }
+
+/* Host threads used by the tile-parallel loops (0: the OpenMP default). */
+#include <omp.h>
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
