@@ -121,6 +121,18 @@ rmesh_status rmesh_backward_hw(rmesh_renderer* h, const float* dl_d_image, float
                                float* d_base_colors, float* d_color_grads);
 /* SH extension: d_sh [M*3*nc] of the last rmesh_backward (host). */
 rmesh_status rmesh_get_sh_grad(rmesh_renderer* h, float* d_sh);
+/* Device-pointer variants of update_params / forward_tiled / backward for
+ * callers whose arrays live on the GPU (torch tensors: RMeshForward), same
+ * layouts: copies are device to device on the renderer's stream (set it to
+ * the caller's with rmesh_set_stream).  update_params_device and
+ * backward_device only enqueue work; forward_tiled_device waits for its pair
+ * count check. */
+rmesh_status rmesh_update_params_device(rmesh_renderer* h, const float* vertices, const float* base_colors,
+                                        const float* densities, const float* color_grads);
+rmesh_status rmesh_forward_tiled_device(rmesh_renderer* h, const float* cam, const float* vp,
+                                        const float* c2w_intrinsics, float* d_rgba);
+rmesh_status rmesh_backward_device(rmesh_renderer* h, const float* d_dl_d_image, float* d_vertices,
+                                   float* d_densities, float* d_base_colors, float* d_color_grads);
 
 /* RMeshRenderer.train_step (lib.rs:2405-2636): forward + loss + backward +
  * Adam (vertices, densities, color_grads; step += 1).  gt [H*W*3] host,
@@ -133,6 +145,23 @@ rmesh_status rmesh_train_step(rmesh_renderer* h, const float* cam, const float* 
 /* Extension learning rates for train steps: SH coefficients and base_colors
  * (0 disables; the reference never optimises base_colors). */
 rmesh_status rmesh_set_extra_lrs(rmesh_renderer* h, float lr_sh, float lr_base);
+
+/* Adjoint mode of backward and the train steps (no counterpart in the
+ * reference, whose adjoint is the undo).  RMESH_ADJOINT_UNDO (default): the
+ * reference's adjoint (backward_tiled_compute.wgsl:186-204,419-437), which
+ * rebuilds log T from the image alpha as log(1 - a) and walks the whole tile
+ * list back.  RMESH_ADJOINT_CHECKPOINT: the forward also stores each pixel's
+ * log T every 32 tets of the tile list (and its final log T and stop
+ * position); the adjoint restarts the undo from those and skips what the
+ * forward never composited: the exact adjoint of the forward, at
+ * 4 (P / 32 + tiles) TS^2 + 8 H W V bytes of device memory.  Switching modes
+ * requires a new forward before backward. */
+#define RMESH_ADJOINT_UNDO 0
+#define RMESH_ADJOINT_CHECKPOINT 1
+rmesh_status rmesh_set_adjoint_mode(rmesh_renderer* h, int32_t mode);
+int32_t rmesh_get_adjoint_mode(const rmesh_renderer* h);
+/* Device memory held by the renderer (bytes, every buffer it allocated). */
+rmesh_status rmesh_device_bytes(rmesh_renderer* h, uint64_t* out);
 
 /* RMeshRenderer.update_params / get_params (lib.rs:2641-2686), host arrays. */
 rmesh_status rmesh_update_params(rmesh_renderer* h, const float* vertices, const float* base_colors,
@@ -169,17 +198,24 @@ rmesh_status rmesh_set_graph_mode(rmesh_renderer* h, int enable);
 /* ---- tuning (no counterpart in the reference, which fixes the tile size at
  * 12 in its Python path, lib.rs:442, and has one kernel shape) ---- */
 
-/* Tile size (8, 12, 16) and batch-geometry variants of the forward and
- * adjoint compositing kernels (0 .. rmesh_num_variants(0|1) - 1).  Changing
- * them drops a captured graph and the last forward's state. */
+/* Tile size (8, 12, 16, 32) and batch-geometry variants of the forward and
+ * adjoint compositing kernels (0 .. rmesh_num_variants(0|1) - 1; tile 32
+ * has variants 0 and 1 only).  Changing them drops a captured graph and the
+ * last forward's state. */
 rmesh_status rmesh_set_tuning(rmesh_renderer* h, uint32_t tile_size, int32_t fwd_variant, int32_t bwd_variant);
 /* out3 = {tile_size, fwd_variant, bwd_variant}. */
 rmesh_status rmesh_get_tuning(const rmesh_renderer* h, int32_t* out3);
-/* Number of kernel variants: which = 0 forward, 1 adjoint. */
+/* Number of kernel variants: which = 0 forward, 1 adjoint (tiles 8-16). */
 int32_t rmesh_num_variants(int32_t which);
-/* Autotuner: times binning + sort, every forward variant and every adjoint
- * variant for each tile size on these views (best of `reps` runs, HIP
- * events) and keeps the fastest combination.  Parameters and optimiser state
+/* Radix-sort digit width: 8 (default, 256 buckets) or 4 bits (16 buckets,
+ * twice the passes).  The reference sorts 8-bit digits
+ * (crates/rmesh-sort/src/lib.rs:85-90); both give the same stable order. */
+rmesh_status rmesh_set_sort_digit_bits(rmesh_renderer* h, uint32_t bits);
+uint32_t rmesh_get_sort_digit_bits(const rmesh_renderer* h);
+/* Autotuner: times binning + sort at both digit widths, every forward
+ * variant and every adjoint variant for each tile size (8, 12, 16, 32) on
+ * these views (best of `reps` runs, HIP events) and keeps the fastest
+ * combination.  Parameters and optimiser state
  * are untouched; gradient buffers are cleared.  gt_dev as in
  * rmesh_train_step_batched.  out_ms (nullable, 4 floats): best total,
  * binning + sort, forward, adjoint. */
@@ -196,6 +232,8 @@ rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t n_views, const float* ca
 #define RMESH_BUF_D_VERTICES 7
 #define RMESH_BUF_D_DENSITIES 8
 #define RMESH_BUF_D_COLOR_GRADS 9
+#define RMESH_BUF_D_SH 10          /* float [SH floats of all scenes] (written without an SH learning rate) */
+#define RMESH_BUF_D_BASE_COLORS 11 /* float [views][M*3] per-view colour gradients (SH), or [M*3] */
 /* Device pointer of an internal buffer (zero-copy for torch callers). */
 void* rmesh_device_buffer(rmesh_renderer* h, int which);
 /* Upload n_views target images (host [n][H][W][3]) into RMESH_BUF_GT. */
@@ -215,6 +253,11 @@ rmesh_status rmesh_read_buffer(rmesh_renderer* h, int which, void* host, uint64_
  * times; reports the average ms per copy.  Used to calibrate HBM byte
  * counters against a kernel of known traffic. */
 rmesh_status rmesh_copy_probe(int32_t device_id, uint64_t nbytes, uint32_t iters, float* ms_per_copy);
+/* n_ops float atomic adds (no return), every lane on its own 64-byte line,
+ * a wave's lanes scattered over `lines` lines (977 .. 2^24); *ms = the
+ * launch time.  The atomic-rate ceiling the adjoint's flush is priced
+ * against. */
+rmesh_status rmesh_atomic_probe(int32_t device_id, uint32_t lines, uint64_t n_ops, float* ms);
 /* Synchronise the renderer's stream. */
 rmesh_status rmesh_synchronize(rmesh_renderer* h);
 
@@ -235,6 +278,47 @@ rmesh_status rmesh_debug_sorted(rmesh_renderer* h, uint64_t* keys, uint32_t* val
 rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t sorting_bits, int32_t device_id);
 /* Bits sorted for a given tile count (crates/rmesh-sort/src/drs.rs:23-26). */
 uint32_t rmesh_sorting_bits(uint32_t num_tiles);
+
+/* ---- multi-scene batches (no counterpart in the reference, which holds one
+ * scene per RMeshRenderer, crates/rmesh-python/src/lib.rs:265-277) ----
+ * One renderer over n scenes of uneven tet count, resolution (descs[i].width
+ * / height) and SH degree (0..3; every scene with SH coefficients, or none),
+ * each with up to descs[i].max_views views per step; every scene shares the
+ * tile size, device and sh_half of descs[0].  A step runs one launch per
+ * stage over all scenes' views (the SH colour and SH adjoint stages one per
+ * SH degree present): binning items per (tet, view) of each scene, one sort,
+ * compositing grids over all views' tiles, one Adam update over all
+ * parameters.  views_per_scene[i] views of scene i this step (0..max_views),
+ * cameras (cams [V][3], vps [V][16], c2ws [V][16]) listed scene by scene in
+ * scene order; images and targets are concatenated per view in that order,
+ * each view at its scene's resolution.  The single-scene entry points refuse
+ * a multi-scene renderer (RMESH_ERR_UNSUPPORTED). */
+rmesh_status rmesh_create_multi(const rmesh_scene_desc* descs, uint32_t n_scenes, rmesh_renderer** out);
+uint32_t rmesh_scene_count(const rmesh_renderer* h);
+/* out_rgba (nullable): the views' images, concatenated. */
+rmesh_status rmesh_forward_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* cams,
+                                 const float* vps, const float* c2ws, float* out_rgba);
+/* Targets for the views of a layout (host, [view][H][W][3] concatenated). */
+rmesh_status rmesh_set_targets_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* gt);
+/* One forward + loss + adjoint + SH adjoint + Adam step over the layout,
+ * graph-captured in graph mode (recaptured when the layout changes).
+ * out_view_losses (nullable): per-view loss; out_loss: their mean. */
+rmesh_status rmesh_train_step_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* cams,
+                                    const float* vps, const float* c2ws, const float* gt_dev, uint32_t loss_type,
+                                    float lr_verts, float lr_dens, float lr_grads, float lr_sh,
+                                    float* out_view_losses, float* out_loss);
+/* rmesh_autotune over a multi-scene layout (one tile size and variant for all). */
+rmesh_status rmesh_autotune_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* cams,
+                                  const float* vps, const float* c2ws, const float* gt_dev, uint32_t loss_type,
+                                  uint32_t reps, float* out_ms);
+/* Scene `scene`'s slice of a parameter or gradient buffer (RMESH_BUF_VERTICES,
+ * _DENSITIES, _COLOR_GRADS, _BASE_COLORS, _SH and their gradients). */
+rmesh_status rmesh_read_scene(rmesh_renderer* h, uint32_t scene, int which, void* host, uint64_t nbytes);
+/* Per-scene timing of the steps that follow: the wall clock at each step's
+ * start and when each scene's last adjoint tile retired. */
+rmesh_status rmesh_set_scene_timing(rmesh_renderer* h, int enable);
+/* ms[scene]: from the last step's start to the scene's last adjoint tile. */
+rmesh_status rmesh_scene_latency(rmesh_renderer* h, float* ms);
 
 /* ---- scene files (crates/rmesh-data/src/lib.rs) ----
  * load_rmesh / load_rmesh_raw / load_ply (lib.rs:140-148, 243-394, 711-853):

@@ -21,6 +21,8 @@ RMESH_ERR_CAPACITY = 4
 
 BUF_RGBA, BUF_GT, BUF_VERTICES, BUF_DENSITIES, BUF_COLOR_GRADS, BUF_BASE_COLORS, BUF_SH = range(7)
 BUF_D_VERTICES, BUF_D_DENSITIES, BUF_D_COLOR_GRADS = 7, 8, 9
+ADJOINT_UNDO, ADJOINT_CHECKPOINT = 0, 1
+BUF_D_SH, BUF_D_BASE_COLORS = 10, 11
 
 # Every symbol include/rmesh_hip.h declares, with (restype, argtypes).
 _vp = ctypes.c_void_p
@@ -93,10 +95,29 @@ SIGNATURES = {
     "rmesh_pair_count": (_st, [_vp, ctypes.POINTER(_u64)]),
     "rmesh_read_buffer": (_st, [_vp, ctypes.c_int, _vp, _u64]),
     "rmesh_copy_probe": (_st, [_i32, _u64, _u32, ctypes.POINTER(_f32)]),
+    "rmesh_atomic_probe": (_st, [_i32, _u32, _u64, ctypes.POINTER(_f32)]),
     "rmesh_debug_tiles_touched": (_st, [_vp, _u32, _vp]),
     "rmesh_debug_pairs": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]),
     "rmesh_debug_sorted": (_st, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp, _u64]),
     "rmesh_sort_pairs": (_st, [_vp, _vp, _u64, _u32, _i32]),
+    "rmesh_set_adjoint_mode": (_st, [_vp, _i32]),
+    "rmesh_update_params_device": (_st, [_vp, _vp, _vp, _vp, _vp]),
+    "rmesh_forward_tiled_device": (_st, [_vp, _vp, _vp, _vp, _vp]),
+    "rmesh_backward_device": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rmesh_create_multi": (_st, [ctypes.POINTER(SceneDesc), _u32, ctypes.POINTER(_vp)]),
+    "rmesh_scene_count": (_u32, [_vp]),
+    "rmesh_forward_multi": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rmesh_set_targets_multi": (_st, [_vp, _vp, _vp]),
+    "rmesh_train_step_multi": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32, _f32, _vp,
+                                     ctypes.POINTER(_f32)]),
+    "rmesh_autotune_multi": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
+    "rmesh_read_scene": (_st, [_vp, _u32, ctypes.c_int, _vp, _u64]),
+    "rmesh_set_scene_timing": (_st, [_vp, ctypes.c_int]),
+    "rmesh_scene_latency": (_st, [_vp, _vp]),
+    "rmesh_set_sort_digit_bits": (_st, [_vp, _u32]),
+    "rmesh_get_sort_digit_bits": (_u32, [_vp]),
+    "rmesh_get_adjoint_mode": (_i32, [_vp]),
+    "rmesh_device_bytes": (_st, [_vp, ctypes.POINTER(_u64)]),
     "rmesh_sorting_bits": (_u32, [_u32]),
 }
 
@@ -116,6 +137,14 @@ def load() -> ctypes.CDLL:
         raise ImportError(
             f"rmeshvk_amd HIP library not found at {LIB_PATH}; build it with "
             "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+    # One HIP runtime per process: the library's libamdhip64.so.7 binds to a
+    # copy already loaded, so torch (which loads its own build under that
+    # soname) goes first when it is installed; otherwise torch, imported later,
+    # would start a second runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -6,7 +6,10 @@ uploads the parameters and renders one view through the tiled pipeline
 (``render_mode="tiled"``) or the interval-tiled one
 (``render_mode="interval_tiled"``); backward hands dL/d(image) to the
 matching hand-written adjoint and returns gradients for (vertices,
-base_colors, densities, color_grads).  The reference's interval branch passes
+base_colors, densities, color_grads).  With parameters on the GPU the tiled
+mode runs zero-copy: the renderer works on torch's current stream, parameters,
+image, image gradient and parameter gradients move device to device (the
+reference copies every one through host arrays, autograd.py:26-64).  The reference's interval branch passes
 forward_interval_tiled's dict where an array is expected (autograd.py:40,49);
 here the branch uses the dict's image, so it runs.
 """
@@ -30,18 +33,36 @@ class RMeshForward(torch.autograd.Function):
     @staticmethod
     def forward(ctx, renderer, cam_pos, vp, c2w_intrinsics, vertices, base_colors, densities, color_grads,
                 render_mode="tiled"):
+        ctx.render_mode = render_mode
+        ctx.renderer = renderer
+        ctx.device_path = vertices.is_cuda and render_mode != "interval_tiled"
+        if ctx.device_path:
+            renderer.set_stream(torch.cuda.current_stream(vertices.device).cuda_stream)
+            params = [t.detach().reshape(-1).contiguous().float() for t in (vertices, base_colors, densities,
+                                                                          color_grads)]
+            renderer.update_params_device(*(t.data_ptr() for t in params))
+            image = torch.empty((renderer.height, renderer.width, 4), dtype=torch.float32, device=vertices.device)
+            renderer.forward_tiled_device(_np(cam_pos), _np(vp), _np(c2w_intrinsics), image.data_ptr())
+            ctx.save_for_backward(vertices, base_colors, densities, color_grads)
+            return image
         renderer.update_params(_np(vertices), _np(base_colors), _np(densities), _np(color_grads))
         if render_mode == "interval_tiled":
             image_np = renderer.forward_interval_tiled(_np(cam_pos), _np(vp), _np(c2w_intrinsics))["image"]
         else:
             image_np = renderer.forward_tiled(_np(cam_pos), _np(vp), _np(c2w_intrinsics))
-        ctx.render_mode = render_mode
-        ctx.renderer = renderer
         ctx.save_for_backward(vertices, base_colors, densities, color_grads)
         return torch.from_numpy(image_np.copy()).to(vertices.device)
 
     @staticmethod
     def backward(ctx, grad_output):
+        if ctx.device_path:
+            v, b, d, g = ctx.saved_tensors
+            dl = grad_output.detach().contiguous().float()
+            out = [torch.empty(t.numel(), dtype=torch.float32, device=t.device) for t in (v, d, b, g)]
+            ctx.renderer.backward_device(dl.data_ptr(), *(t.data_ptr() for t in out))
+            dv, dd, db, dg = out
+            return (None, None, None, None, dv.view(v.shape), db.view(b.shape), dd.view(d.shape), dg.view(g.shape),
+                    None)
         dl = np.ascontiguousarray(grad_output.detach().cpu().numpy().astype(np.float32))
         if ctx.render_mode == "interval_tiled":
             grads = ctx.renderer.backward_interval_tiled(dl)

@@ -45,6 +45,23 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
 }
 
+// Float atomic adds of known count for the atomic-rate ceiling: every lane
+// adds 1 to its own 64-byte line, the lines of a wave's lanes scattered over
+// `lines` lines (the adjoint flush's pattern: consecutive lanes on terms of
+// different tets), no return value.
+__global__ __launch_bounds__(256) void k_atomic_probe(float* __restrict__ p, uint32_t lines, uint32_t per_thread) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t line = (i * 7919u) % lines;
+    for (uint32_t k = 0; k < per_thread; ++k) {
+        unsafeAtomicAdd(p + 16ull * line, 1.0f);
+        line += 977u;
+        if (line >= lines) line -= lines;
+    }
+}
+void launch_atomic_probe(float* p, uint32_t lines, uint32_t blocks, uint32_t per_thread, hipStream_t s) {
+    hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, s, p, lines, per_thread);
+}
+
 // Plain streaming copy used to calibrate HBM byte counters (FETCH_SIZE /
 // WRITE_SIZE) against a kernel of known traffic.
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
@@ -185,23 +202,36 @@ __device__ __forceinline__ uint32_t project_count(const V3 v[4], float dens, con
     return count;
 }
 
-// K1, batched: one thread per (tet, view) item, item = tet * V + view (the
-// pair order of the reference within every view: tet-major).  counts[item];
-// per-block pair totals for the scan; visible tets of view 0 (the
-// reference's indirect_args.instance_count, project_compute.wgsl:181).
-__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec, uint32_t M,
-                                                 const ViewParams* __restrict__ views, uint32_t V,
+// (tet, view) of a binning item: scene by binary search, then tet-major
+// inside the scene.
+__device__ __forceinline__ void item_tet_view(const SceneRange* __restrict__ sc, uint32_t n_scenes, uint64_t item,
+                                              uint32_t& t, uint32_t& vv) {
+    const SceneRange& r = sc[scene_of_item(sc, n_scenes, item)];
+    const uint64_t local = item - r.item_base;
+    t = r.tet_base + (uint32_t)(local / r.V);
+    vv = r.view_base + (uint32_t)(local % r.V);
+}
+
+// K1, batched: one thread per (tet, view) item of every scene (item =
+// scene's item base + local tet * V + local view: the pair order of the
+// reference within every view is tet-major).  counts[item]; per-block pair
+// totals for the scan; visible tets of view 0 (the reference's
+// indirect_args.instance_count, project_compute.wgsl:181).
+__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec, const SceneRange* __restrict__ scenes,
+                                                 uint32_t n_scenes, uint64_t n_items,
+                                                 const ViewParams* __restrict__ views,
                                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ block_sums,
                                                  uint32_t* __restrict__ vis_count) {
     __shared__ uint32_t red[256 / kWave];
     __shared__ uint32_t s_vis;
     if (threadIdx.x == 0) s_vis = 0;
     const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = item < (uint64_t)M * V;
+    const bool live = item < n_items;
     uint32_t c = 0;
     bool vis0 = false;
     if (live) {
-        const uint32_t t = (uint32_t)(item / V), vv = (uint32_t)(item % V);
+        uint32_t t, vv;
+        item_tet_view(scenes, n_scenes, item, t, vv);
         V3 v[4];
         float dens;
         V3 g;
@@ -224,42 +254,57 @@ __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec,
 
 // Exclusive scan of block sums in place (single workgroup of 1024 threads,
 // 4096 sums per round, each thread 4 consecutive ones: coalesced); writes the
-// grand total (number of pairs P) to *total.
-__global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums, uint32_t n, uint32_t* __restrict__ total) {
+// grand total (number of pairs P) to *total.  The running total is kept in 64
+// bits: past 2^32 - 16 pairs the offsets would wrap, so *total saturates at
+// 0xFFFFFFFF (the host refuses that count) and the overflow flag is raised
+// (a captured step then leaves the parameters untouched).
+__global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums, uint32_t n, uint32_t* __restrict__ total,
+                                                     uint32_t* __restrict__ overflow) {
     __shared__ uint32_t red[1024 / kWave];
-    uint32_t carry = 0;
+    __shared__ unsigned long long s_round;  // the round's total in 64 bits
+    uint64_t carry = 0;
     for (uint32_t base = 0; base < n; base += 4096) {
         const uint32_t i0 = base + 4u * threadIdx.x;
         uint32_t x[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) x[k] = i0 + k < n ? sums[i0 + k] : 0u;
+        if (threadIdx.x == 0) s_round = 0ull;
+        __syncthreads();
+        atomicAdd(&s_round, ((unsigned long long)x[0] + x[1]) + ((unsigned long long)x[2] + x[3]));
         uint32_t tot;
-        uint32_t ex = block_excl_scan((x[0] + x[1]) + (x[2] + x[3]), red, tot) + carry;
+        uint32_t ex = block_excl_scan((x[0] + x[1]) + (x[2] + x[3]), red, tot) + (uint32_t)carry;  // ends with a barrier
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (i0 + k < n) sums[i0 + k] = ex;
             ex += x[k];
         }
-        carry += tot;
+        carry += s_round;
+        __syncthreads();  // every thread has read s_round before the next round clears it
     }
-    if (threadIdx.x == 0) *total = carry;
+    if (threadIdx.x == 0) {
+        const bool wrap = carry > 0xFFFFFFF0ull;
+        *total = wrap ? 0xFFFFFFFFu : (uint32_t)carry;
+        if (wrap) *overflow = 1u;
+    }
 }
 
 // K7 (+ K6 sentinel padding): thread per tet, loop over views, pairs written
 // at the scanned offset.  Keys: (global_tile << 32) | ~bits(depth).
-__global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, const float4* __restrict__ circ, uint32_t M,
-                                               const ViewParams* __restrict__ views, uint32_t V,
+__global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, const float4* __restrict__ circ,
+                                               const SceneRange* __restrict__ scenes, uint32_t n_scenes, uint64_t n_items,
+                                               const ViewParams* __restrict__ views,
                                                const uint32_t* __restrict__ counts, const uint32_t* __restrict__ block_off,
                                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t cap,
                                                uint32_t* __restrict__ overflow) {
     __shared__ uint32_t red[256 / kWave];
     const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = item < (uint64_t)M * V;
+    const bool live = item < n_items;
     const uint32_t max_write = live ? counts[item] : 0u;
     uint32_t tot;
     const uint32_t start = block_excl_scan(max_write, red, tot) + block_off[blockIdx.x];
     if (max_write == 0) return;
-    const uint32_t t = (uint32_t)(item / V), vv = (uint32_t)(item % V);
+    uint32_t t, vv;
+    item_tet_view(scenes, n_scenes, item, t, vv);
     V3 v[4];
     float dens;
     V3 g;
@@ -289,7 +334,7 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, c
             uint32_t tx0, tx1;
             row_tiles(xl, xr, T, b, tx0, tx1);
             for (uint32_t tx = tx0; tx <= tx1 && local < max_write; ++tx) {
-                uint32_t gt = vw.tile_base + ty * vw.tiles_x + tx;
+                uint32_t gt = vw.key_base + ty * vw.tiles_x + tx;
                 uint32_t wi = start + local;
                 if (wi < cap) {
                     keys[wi] = ((uint64_t)gt << 32) | inv_depth;
@@ -303,10 +348,11 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, c
     }
     // Slots counted by K1 but not produced by the division-based
     // re-projection keep the tile_fill sentinel (tile_gen_scan_compute.wgsl:202).
+    const uint64_t sentinel = ((uint64_t)vw.sentinel_tile << 32) | 0xFFFFFFFFull;
     for (; local < max_write; ++local) {
         uint32_t wi = start + local;
         if (wi < cap) {
-            keys[wi] = ~0ull;
+            keys[wi] = sentinel;
             vals[wi] = 0u;
         } else {
             *overflow = 1u;
@@ -315,18 +361,26 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, c
 }
 
 // K11: per-tile [start, end) in the sorted pair list, including the
-// reference's behaviour for trailing sentinels (tile_ranges_compute.wgsl:10-45).
+// reference's behaviour for trailing sentinels (tile_ranges_compute.wgsl:10-45):
+// a tile whose pairs are followed by sentinels keeps end = 0.  Key tiles map
+// to global tiles through ktile_gtile (0xFFFFFFFF: a view's sentinel tile;
+// the key tile 0xFFFFFFFF is the reference's own sentinel), so each view's
+// ranges are those of the reference's per-view sort.
+__device__ __forceinline__ uint32_t range_tile(uint64_t key, const uint32_t* __restrict__ ktile_gtile, uint32_t n_ktiles) {
+    const uint32_t kt = (uint32_t)(key >> 32);
+    return kt < n_ktiles ? ktile_gtile[kt] : 0xFFFFFFFFu;
+}
 __global__ void k_ranges(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr, uint32_t cap,
-                         uint32_t* __restrict__ ranges) {
+                         uint32_t* __restrict__ ranges, const uint32_t* __restrict__ ktile_gtile, uint32_t n_ktiles) {
     uint32_t n = min(*n_ptr, cap);
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t tile = (uint32_t)(keys[i] >> 32);
+        uint32_t tile = range_tile(keys[i], ktile_gtile, n_ktiles);
         if (tile == 0xFFFFFFFFu) continue;
         if (i == 0) {
             ranges[2ull * tile] = i;
         } else {
-            uint32_t prev = (uint32_t)(keys[i - 1] >> 32);
+            uint32_t prev = range_tile(keys[i - 1], ktile_gtile, n_ktiles);
             if (tile != prev) {
                 ranges[2ull * tile] = i;
                 if (prev != 0xFFFFFFFFu) ranges[2ull * prev + 1] = i;
@@ -357,22 +411,22 @@ void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
 void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, src, dst, n16);
 }
-uint32_t project_blocks(uint32_t M, uint32_t V) { return (uint32_t)(((uint64_t)M * V + 255) / 256); }
-void launch_project(const float4* rec, uint32_t M, const ViewParams* views, uint32_t V, uint32_t* counts,
-                    uint32_t* block_sums, uint32_t* vis_count, hipStream_t s) {
-    if (!M || !V) return;
-    hipLaunchKernelGGL(k_project, dim3(project_blocks(M, V)), dim3(256), 0, s, rec, M, views, V, counts, block_sums,
-                       vis_count);
+uint32_t item_blocks(uint64_t n_items) { return (uint32_t)((n_items + 255) / 256); }
+void launch_project(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
+                    const ViewParams* views, uint32_t* counts, uint32_t* block_sums, uint32_t* vis_count, hipStream_t s) {
+    if (!n_items) return;
+    hipLaunchKernelGGL(k_project, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, scenes, n_scenes, n_items, views,
+                       counts, block_sums, vis_count);
 }
-void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, hipStream_t s) {
-    hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total);
+void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s) {
+    hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total, overflow);
 }
-void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewParams* views, uint32_t V,
-                  const uint32_t* counts, const uint32_t* block_off, uint64_t* keys, uint32_t* vals, uint32_t cap,
-                  uint32_t* overflow, hipStream_t s) {
-    if (!M || !V) return;
-    hipLaunchKernelGGL(k_pairs, dim3(project_blocks(M, V)), dim3(256), 0, s, rec, circ, M, views, V, counts, block_off,
-                       keys, vals, cap, overflow);
+void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
+                  const ViewParams* views, const uint32_t* counts, const uint32_t* block_off, uint64_t* keys,
+                  uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s) {
+    if (!n_items) return;
+    hipLaunchKernelGGL(k_pairs, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes, n_items, views,
+                       counts, block_off, keys, vals, cap, overflow);
 }
 // Tiles heaviest first: the compositing grids take their tiles in this order,
 // so the few very dense tiles (tens of times the median pair count) start
@@ -389,6 +443,7 @@ __device__ __forceinline__ uint32_t tile_bucket(const uint32_t* __restrict__ ran
     const uint32_t len = 32u - (uint32_t)__clz((int)c);  // >= 3
     return len * 4u + ((c >> (len - 3u)) & 3u);           // 3 leading bits: 1xx
 }
+
 
 __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict__ ranges, uint32_t n,
                                                       uint32_t* __restrict__ order) {
@@ -410,15 +465,24 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     for (uint32_t t = tid; t < n; t += blockDim.x) order[atomicAdd(&off[tile_bucket(ranges, t)], 1u)] = t;
 }
 
+__global__ void k_stamp_start(uint64_t* __restrict__ p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+    if (threadIdx.x == 0) p[n] = wall_clock64();
+}
+void launch_stamp_start(uint64_t* p, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_stamp_start, dim3(1), dim3(64), 0, s, p, n);
+}
+
 void launch_tile_order(const uint32_t* ranges, uint32_t n_tiles, uint32_t* order, hipStream_t s) {
     if (n_tiles) hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, ranges, n_tiles, order);
 }
 
-void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges, hipStream_t s) {
+void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
+                   const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s) {
     uint32_t blocks = (cap + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, s, keys, n_ptr, cap, ranges);
+    hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, s, keys, n_ptr, cap, ranges, ktile_gtile, n_ktiles);
 }
 
 }  // namespace rmesh

@@ -52,10 +52,24 @@ __device__ __forceinline__ TileCtx tile_ctx(const ViewParams& vw, uint32_t gtile
     return c;
 }
 
+// Row span packing: tiles up to 16 pixels wide keep a span in a byte
+// (xl | xr << 4), wider tiles (32) in 16 bits (xl | xr << 8); the empty span
+// has xl = all ones > xr = 0.
+template <int TS>
+struct SpanFmt {
+    using T = typename std::conditional<(TS <= 16), uint8_t, uint16_t>::type;
+    static constexpr uint32_t SH = TS <= 16 ? 4u : 8u;
+    static constexpr uint32_t MASK = (1u << SH) - 1u;
+    static constexpr uint32_t EMPTY = MASK;
+};
+template <int TS>
+__device__ __forceinline__ int32_t span_xl(uint32_t sp) { return (int32_t)(sp & SpanFmt<TS>::MASK); }
+template <int TS>
+__device__ __forceinline__ int32_t span_xr(uint32_t sp) { return (int32_t)(sp >> SpanFmt<TS>::SH); }
+
 // Scanline spans of one projected tet (rasterize_compute.wgsl:170-275; FORWARD
 // adds the vertex-in-row rule, the adjoint omits it:
-// backward_tiled_compute.wgsl:242-299), packed in a byte as xl | xr << 4
-// (0x0F, xl > xr: empty).
+// backward_tiled_compute.wgsl:242-299), packed per SpanFmt.
 //
 // The reference tests each of the six edges, min(yi, yj) <= yc < max(yi, yj),
 // and takes the min / max of the crossing edges' x.  With the vertices sorted
@@ -101,8 +115,8 @@ struct TetRows {
         s[4] = slope(1, 3);
         s[5] = slope(2, 3);
     }
-    template <bool FORWARD>
-    __device__ __forceinline__ uint32_t span(uint32_t row, int32_t TS) const {
+    template <bool FORWARD, int TS>
+    __device__ __forceinline__ uint32_t span(uint32_t row) const {
         const float yc = (float)row + 0.5f;
         const float d0 = yc - y[0], d1 = yc - y[1], d2 = yc - y[2];
         const float x01 = x[0] + d0 * s[0], x02 = x[0] + d0 * s[1], x03 = x[0] + d0 * s[2];
@@ -131,14 +145,11 @@ struct TetRows {
             const float eps = 0.001f;
             int32_t l = max(sat_i32(ceilf(xl_f - 0.5f - eps)), 0);
             int32_t r = min(sat_i32(floorf(xr_f - 0.5f + eps)), TS - 1);
-            if (l <= r) return (uint32_t)l | ((uint32_t)r << 4);
+            if (l <= r) return (uint32_t)l | ((uint32_t)r << SpanFmt<TS>::SH);
         }
-        return 0x0Fu;
+        return SpanFmt<TS>::EMPTY;
     }
 };
-
-__device__ __forceinline__ int32_t span_xl(uint32_t sp) { return (int32_t)(sp & 15u); }
-__device__ __forceinline__ int32_t span_xr(uint32_t sp) { return (int32_t)(sp >> 4); }
 
 // LDS image of one batch of tets.  Row arrays use an odd stride (TS + 1) so
 // the pixel-sequential phase (fixed tet, lanes on different rows) is free of
@@ -154,9 +165,9 @@ struct BatchLDS {
     float pxy[8][B];        // tile-local projected vertices (x0..x3, y0..y3)
     uint32_t behind[B];
     uint32_t rows[B];       // tile rows the tet's projected y-range touches: r0 | r1 << 8 (r0 > r1: none)
-    uint8_t span[B * SP];    // [tet][row] span: first column | last column << 4 (0x0F: empty)
+    typename SpanFmt<TS>::T span[B * SP];  // [tet][row] span (SpanFmt)
     uint16_t rpref[B * SP];  // [tet][row] record base of the row - its first column (mod 2^16)
-    uint32_t wsum[4];       // layout scan carry per wave
+    uint32_t wsum[(TS * TS + 63) / 64];  // layout scan carry per wave (one wave per 64 pixels of the tile)
     uint32_t nbp;           // chunk count << 8 | tets processed this batch (capacity cut)
     // adjoint only: per face (grad lambda_a, lambda_a(v0), grad lambda_b,
     // lambda_b(v0)) in face order, then per face the sign s of the stored
@@ -485,6 +496,7 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ch
     constexpr int LPT = NT >= 8 * B ? 8 : NT >= 4 * B ? 4 : NT >= 2 * B ? 2 : 1;
     constexpr int QMAX = (TS + LPT - 1) / LPT;
     static_assert(LPT * B <= NT && B <= 64, "one pass over the batch");
+    static_assert(NT / 64 <= (int)(sizeof(L.wsum) / sizeof(L.wsum[0])), "a scan carry per wave");
     static_assert(B * TS * TS < 65536, "packed scan fields");
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int k = t / LPT, j = t % LPT;
@@ -507,9 +519,9 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ch
 #pragma unroll
             for (int i = 0; i < QMAX; ++i) {
                 if (i < nr) {
-                    sp[i] = tr.span<FORWARD_RULE>((uint32_t)(ra + i), TS);
-                    L.span[k * SP + ra + i] = (uint8_t)sp[i];
-                    s += (uint32_t)max(span_xr(sp[i]) - span_xl(sp[i]) + 1, 0);
+                    sp[i] = tr.span<FORWARD_RULE, TS>((uint32_t)(ra + i));
+                    L.span[k * SP + ra + i] = (typename SpanFmt<TS>::T)sp[i];
+                    s += (uint32_t)max(span_xr<TS>(sp[i]) - span_xl<TS>(sp[i]) + 1, 0);
                 }
             }
         }
@@ -546,7 +558,7 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ch
         for (int i = 0; i < QMAX; ++i) {
             if (i < nr) {
                 const uint32_t row = (uint32_t)(ra + i);
-                const int xl = span_xl(sp[i]), w = span_xr(sp[i]) - xl + 1;
+                const int xl = span_xl<TS>(sp[i]), w = span_xr<TS>(sp[i]) - xl + 1;
                 if (w > 0) {
                     if constexpr (R == 1) {
                         const uint32_t b = texc + base;  // the row segment's first record slot
@@ -600,7 +612,7 @@ struct PixWalk {
     __device__ __forceinline__ void init(const BatchLDS<TS, B, BWD>& L, const ChunkRef& c) {
         row = c.row;
         col = c.col;
-        xr = span_xr(L.span[c.k * (TS + 1) + c.row]);
+        xr = span_xr<TS>(L.span[c.k * (TS + 1) + c.row]);
         r1 = (int)(L.rows[c.k] >> 8);
     }
     template <int B, bool BWD>
@@ -609,10 +621,10 @@ struct PixWalk {
         if (++col > xr) {
             for (int r = row + 1; r <= r1; ++r) {
                 uint32_t sp = L.span[k * SP + r];
-                if (span_xr(sp) >= span_xl(sp)) {
+                if (span_xr<TS>(sp) >= span_xl<TS>(sp)) {
                     row = r;
-                    col = span_xl(sp);
-                    xr = span_xr(sp);
+                    col = span_xl<TS>(sp);
+                    xr = span_xr<TS>(sp);
                     return;
                 }
             }
@@ -729,12 +741,17 @@ struct FwdCfg {
     static constexpr int WPE = WPE_ > 0 ? WPE_ : 1;
 };
 
-template <class C, bool AUX>
+// CK: also store the transmittance checkpoints of the checkpointed adjoint:
+// a pixel still compositing writes its log T before every tile-list position
+// 32 j it reaches (coalesced: the tile's pixels write one 4 TS^2-byte slot),
+// and at the end its final log T and stop position (1 + the last position it
+// composited).
+template <class C, bool AUX, bool CK>
 __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_forward(
-    const float4* __restrict__ rec, const float4* __restrict__ colors, uint64_t color_view_stride,
+    const float4* __restrict__ rec, const float4* __restrict__ colors,
     const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges, const ViewParams* __restrict__ views,
-    uint32_t tiles_per_view, const uint32_t* __restrict__ tile_order, float* __restrict__ rgba,
-    float* __restrict__ aux) {
+    const uint32_t* __restrict__ tile_view, const uint32_t* __restrict__ tile_order, float* __restrict__ rgba,
+    float* __restrict__ aux, float* __restrict__ ck, float2* __restrict__ fin) {
     constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1;
     constexpr int NA = AUX ? CAP : 1;
     static_assert(B <= 64, "pixel masks hold 64 tets; the layout scan is one wave");
@@ -749,14 +766,14 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     __shared__ unsigned long long s_mask[TS * TS];  // covered tets of the batch per pixel
     __shared__ chunk_t<R> s_chunk[chunk_capacity(CAP, R, B)];
     __shared__ uint32_t s_start[R == 1 ? CAP / 32 : 1];  // R = 1: record slots that start a row segment
-    static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
+    static_assert(CAP <= 2048 && TS <= 32 && R <= 31, "chunk table packing");
     static_assert(R != 1 || (CAP % 32 == 0 && CAP / 32 <= NT), "segment start mask");
 
     const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
-    const uint32_t view = gtile / tiles_per_view;
+    const uint32_t view = tile_view[gtile];
     const ViewParams& vw = views[view];
     const TileCtx tc = tile_ctx(vw, gtile);
-    const float4* col_v = colors + color_view_stride * view;
+    const float4* col_v = colors + vw.color_off;
     const int tid = threadIdx.x;
     const int row = tid / TS, colp = tid % TS;
     const uint32_t px = tc.tile_x * TS + colp, py = tc.tile_y * TS + row;
@@ -778,6 +795,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     const uint32_t count = re > rs ? re - rs : 0u;
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
+    // checkpoints: next position to store, stop position, slot base
+    uint32_t next_ck = kCkGroup, stop = 0u;
+    const uint64_t ck_base = ((uint64_t)(rs >> 5) + gtile - 1u) * (uint64_t)(TS * TS) + (uint64_t)tid;
     TetFetch f;
     static_assert(C::PF != 2 || B <= 64, "LDS-DMA staging: one wave, next batch inside a 128-entry id window");
     __shared__ Staging<C::PF == 2 ? B : 1> S;
@@ -873,6 +893,10 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                 const uint32_t k = (uint32_t)__ffsll((long long)m) - 1u;
                 m &= m - 1ull;
                 if (log_t < -13.8f) break;
+                if constexpr (CK) {
+                    const uint32_t q = done + k;
+                    for (; next_ck <= q; next_ck += kCkGroup) ck[ck_base + (uint64_t)(next_ck / kCkGroup) * (TS * TS)] = log_t;
+                }
                 const uint32_t slot = (L.rpref[k * SP + row] + (uint32_t)colp) & 0xFFFFu;
                 const float od = s_od[slot];
                 const float Tj = fast_safe_exp(log_t);
@@ -881,6 +905,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                     cg = cg + s_cp[1][slot] * Tj;
                     cb = cb + s_cp[2][slot] * Tj;
                     log_t = log_t - od;
+                    if constexpr (CK) stop = done + k + 1u;
                     if (AUX) {
                         // rasterize_compute.wgsl:434-456
                         float alpha_t = fast_safe_exp(-od);
@@ -911,13 +936,17 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                 }
             }
             if (log_t < -13.8f) s_live[tid] = 0u;
+            else if constexpr (CK)  // positions up to the batch's end that this pixel passed uncovered
+                for (; next_ck <= next && next_ck < count; next_ck += kCkGroup)
+                    ck[ck_base + (uint64_t)(next_ck / kCkGroup) * (TS * TS)] = log_t;
         }
         done = next;
         nb = nb_next;
     }
     if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // no LDS DMA outlives the workgroup
     if (active) {
-        uint64_t pix = ((uint64_t)view * vw.height + py) * vw.width + px;
+        const uint64_t pix = vw.pix_base + (uint64_t)py * vw.width + px;
+        if constexpr (CK) fin[pix] = make_float2(log_t, __uint_as_float(stop));
         reinterpret_cast<float4*>(rgba)[pix] = make_float4(cr, cg, cb, 1.0f - safe_exp(log_t));
         if (AUX) {
             reinterpret_cast<float4*>(aux)[2 * pix] = make_float4(nx, ny, nz, dep);
@@ -1145,14 +1174,20 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
     acc[18] += dbcc.z;
 }
 
-template <class C>
+// CK: the checkpointed adjoint.  The undo restarts at every 32-position group
+// of the tile list from the log T the forward stored at the group's end (the
+// pixel's final log T for its last group), and pairs at or after the pixel's
+// stop position — never composited — are skipped: the exact adjoint of the
+// forward.  Without CK: the reference's undo from log(1 - alpha).
+template <class C, bool CK>
 __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE))) void k_backward(
     const float4* __restrict__ rec, const uint32_t* __restrict__ idx, const float4* __restrict__ colors,
-    uint64_t color_view_stride, const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges,
-    const ViewParams* __restrict__ views, uint32_t tiles_per_view, const uint32_t* __restrict__ tile_order,
+    const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges,
+    const ViewParams* __restrict__ views, const uint32_t* __restrict__ tile_view, const uint32_t* __restrict__ tile_order,
     const float* __restrict__ rgba, const float* __restrict__ dl_img, const float* __restrict__ gt, uint32_t loss_type,
     float* __restrict__ loss_partial, float* __restrict__ d_verts, float* __restrict__ d_dens,
-    float* __restrict__ d_grads, float* __restrict__ d_colors, uint64_t dcolor_view_stride) {
+    float* __restrict__ d_grads, float* __restrict__ d_colors,
+    const float* __restrict__ ck, const float2* __restrict__ fin, uint64_t* __restrict__ scene_done) {
     constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1, NW = NT / 64;
     __shared__ BatchLDS<TS, B, true> L;
     // per-pair record: od -> t_prev, dc . cp -> d_log_t before, t_min, t_max
@@ -1165,7 +1200,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     __shared__ float s_acc[B][kTerms];
     __shared__ unsigned int s_mask[TS * TS];  // covered tets of the batch per pixel (B <= 32)
     __shared__ chunk_t<R> s_chunk[chunk_capacity(CAP, R, B)];
-    static_assert(CAP <= 2048 && TS <= 16 && R <= 31, "chunk table packing");
+    static_assert(CAP <= 2048 && TS <= 32 && R <= 31, "chunk table packing");
     __shared__ float s_loss[NW];
     static_assert(B <= 32, "pixel masks hold 32 tets");
     __shared__ uint32_t s_start[R == 1 ? CAP / 32 : 1];  // R = 1: record slots that start a row segment
@@ -1173,29 +1208,38 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     static_assert(CAP >= TS * TS, "one tet's pixels of a tile must fit a batch");
 
     const uint32_t gtile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
-    const uint32_t view = gtile / tiles_per_view;
+    const uint32_t view = tile_view[gtile];
     const ViewParams& vw = views[view];
     const TileCtx tc = tile_ctx(vw, gtile);
-    const float4* col_v = colors + color_view_stride * view;
-    float* dcol_v = d_colors + 3ull * dcolor_view_stride * view;
+    const float4* col_v = colors + vw.color_off;
+    float* dcol_v = d_colors + 3 * vw.color_off;
     const int tid = threadIdx.x, wid = tid / 64, lane = (int)__lane_id();
     const int row = tid / TS, colp = tid % TS;
     const uint32_t px = tc.tile_x * TS + colp, py = tc.tile_y * TS + row;
     const bool active = tid < TS * TS && px < vw.width && py < vw.height;
-    const uint64_t pix = ((uint64_t)view * vw.height + py) * vw.width + px;
+    const uint64_t pix = vw.pix_base + (uint64_t)py * vw.width + px;
     const V3 cam = mk3(vw.cam[0], vw.cam[1], vw.cam[2]);
 
     // Initial state from the rendered image (backward_tiled_compute.wgsl:186-204);
     // with a ground-truth image the loss gradient is formed here
     // (loss_compute.wgsl:24-72): dL/dalpha = 0.
     float log_t = 0.f, d_log_t = 0.f, lsum = 0.f;
+    float fin_lt = 0.f;   // CK: the pixel's final log T
+    uint32_t stop = 0u;   // CK: pairs at positions >= stop were never composited
     if (tid < TS * TS) {
         float dcv[3] = {0.f, 0.f, 0.f};
         V3 rd = mk3(0.f, 0.f, 1.f);
         if (active) {
             rd = pixel_ray(vw, px, py);
             float4 im = reinterpret_cast<const float4*>(rgba)[pix];
-            log_t = logf(fmaxf(1.0f - im.w, 1e-20f));
+            if constexpr (CK) {
+                const float2 fs = fin[pix];
+                fin_lt = fs.x;
+                stop = __float_as_uint(fs.y);
+                log_t = fin_lt;
+            } else {
+                log_t = logf(fmaxf(1.0f - im.w, 1e-20f));
+            }
             if (gt) {
                 float npx = (float)(vw.width * vw.height);
                 float df[3] = {im.x - gt[3 * pix], im.y - gt[3 * pix + 1], im.z - gt[3 * pix + 2]};
@@ -1215,7 +1259,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                 dcv[0] = d.x;
                 dcv[1] = d.y;
                 dcv[2] = d.z;
-                d_log_t = -d.w * (1.0f - im.w);
+                d_log_t = -d.w * (CK ? safe_exp(fin_lt) : 1.0f - im.w);
             }
         }
         s_ray[0][tid] = rd.x;
@@ -1240,8 +1284,12 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     const uint32_t count = re > rs ? re - rs : 0u;
     uint32_t done = 0;
     int nb = (int)min<uint32_t>(B, count);
+    // CK: slot of checkpoint j + 1 (the end of group j) is ck_base + j
+    const uint64_t ck_base = ((uint64_t)(rs >> 5) + gtile) * (uint64_t)(TS * TS) + (uint64_t)tid;
+    int32_t cur = -1;  // CK: group the pixel's undo is in
     TetFetch f;
     static_assert(C::PF != 2 || B <= 64, "LDS-DMA staging: one wave, next batch inside a 128-entry id window");
+    static_assert(!CK || B <= (int)kCkGroup, "a batch spans at most two checkpoint groups");
     __shared__ Staging<C::PF == 2 ? B : 1> S;
     QuadFetch qf{};
     if (C::PF == 1 && tid < nb) fetch_tet<true>(f, sorted_vals[rs + tid], rec, idx, col_v);
@@ -1272,6 +1320,16 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         const uint32_t nchunks = L.nbp >> 8;
         const uint32_t next = done + nbp;
         const int nb_next = (int)min<uint32_t>(B, count - next);
+        // CK: log T at the end of the (at most two) groups this batch spans,
+        // positions [count - next, count - 1 - done] in compositing order
+        float ck_hi = 0.f, ck_lo = 0.f;
+        int32_t j_hi = 0;
+        if (CK && active) {
+            j_hi = (int32_t)((count - 1u - done) / kCkGroup);
+            const int32_t j_lo = (int32_t)((count - next) / kCkGroup);
+            ck_hi = (uint32_t)(j_hi + 1) * kCkGroup < stop ? ck[ck_base + (uint64_t)j_hi * (TS * TS)] : fin_lt;
+            ck_lo = (uint32_t)(j_lo + 1) * kCkGroup < stop ? ck[ck_base + (uint64_t)j_lo * (TS * TS)] : fin_lt;
+        }
         if (C::PF == 1 && tid < nb_next) fetch_tet<true>(f, sorted_vals[rs + next + tid], rec, idx, col_v);
         if (C::PF == 3 && tid < 4 * nb_next)
             fetch_quad<true>(qf, sorted_vals[rs + next + (tid >> 2)], tid & 3, rec, idx, col_v);
@@ -1329,6 +1387,15 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
             while (m) {
                 const uint32_t k = (uint32_t)__ffs((int)m) - 1u;
                 m &= m - 1u;
+                if constexpr (CK) {
+                    const uint32_t q = count - 1u - done - k;  // position in compositing order
+                    if (q >= stop) continue;
+                    const int32_t j = (int32_t)(q / kCkGroup);
+                    if (j != cur) {
+                        log_t = j == j_hi ? ck_hi : ck_lo;
+                        cur = j;
+                    }
+                }
                 const uint32_t slot = (L.rpref[k * SP + row] + (uint32_t)colp) & 0xFFFFu;
                 const float od = s_r[0][slot];
                 const float prev_log_t = fminf(log_t + od, 0.0f);
@@ -1429,6 +1496,8 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         nb = nb_next;
     }
     if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // no LDS DMA outlives the workgroup
+    if (scene_done && tid == 0) atomicMax(reinterpret_cast<unsigned long long*>(scene_done) + vw.scene,
+                                          (unsigned long long)wall_clock64());
 }
 
 // ---------------------------------------------------------------------------
@@ -1437,14 +1506,18 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
 // ---------------------------------------------------------------------------
 template <class C>
 static void fwd_launch(const FwdArgs& a, hipStream_t s) {
-    if (a.aux)
-        hipLaunchKernelGGL((k_forward<C, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
-                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.tile_order,
-                           a.rgba, a.aux);
+    if (a.aux)  // the aux outputs serve the forward-only API: no checkpoints with them
+        hipLaunchKernelGGL((k_forward<C, true, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
+                           a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
+                           a.rgba, a.aux, nullptr, nullptr);
+    else if (a.ck)
+        hipLaunchKernelGGL((k_forward<C, false, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
+                           a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
+                           a.rgba, nullptr, a.ck, a.fin);
     else
-        hipLaunchKernelGGL((k_forward<C, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
-                           a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.tile_order,
-                           a.rgba, a.aux);
+        hipLaunchKernelGGL((k_forward<C, false, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.colors,
+                           a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
+                           a.rgba, nullptr, nullptr, nullptr);
 }
 
 template <int TS>
@@ -1462,20 +1535,38 @@ static bool fwd_variant(const FwdArgs& a, int v, hipStream_t s) {
     }
 }
 
+// Tile 32: 1024-thread workgroups; a batch must hold one tet's 1024 pixels
+// (CAP >= TS^2), so two geometries only.
+static bool fwd_variant32(const FwdArgs& a, int v, hipStream_t s) {
+    switch (v) {
+        case 0: fwd_launch<FwdCfg<32, 32, 1024, 1, 3>>(a, s); return true;
+        case 1: fwd_launch<FwdCfg<32, 48, 1536, 1, 3>>(a, s); return true;
+        default: return false;
+    }
+}
+
 bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream_t s) {
     switch (tile_size) {
         case 8: return fwd_variant<8>(a, variant, s);
         case 12: return fwd_variant<12>(a, variant, s);
         case 16: return fwd_variant<16>(a, variant, s);
+        case 32: return fwd_variant32(a, variant, s);
         default: return false;
     }
 }
 
 template <class C>
 static void bwd_launch(const BwdArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_backward<C>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
-                       a.color_view_stride, a.sorted_vals, a.ranges, a.views, a.tiles_per_view, a.tile_order, a.rgba,
-                       a.dl_img, a.gt, a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads, a.d_colors, a.dcolor_view_stride);
+    if (a.ck)
+        hipLaunchKernelGGL((k_backward<C, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
+                           a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
+                           a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads,
+                           a.d_colors, a.ck, a.fin, a.scene_done);
+    else
+        hipLaunchKernelGGL((k_backward<C, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
+                           a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
+                           a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads,
+                           a.d_colors, nullptr, nullptr, a.scene_done);
 }
 
 template <int TS>
@@ -1491,11 +1582,20 @@ static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
     }
 }
 
+static bool bwd_variant32(const BwdArgs& a, int v, hipStream_t s) {
+    switch (v) {
+        case 0: bwd_launch<BwdCfg<32, 32, 1024, 1, 3, 3, 4>>(a, s); return true;
+        case 1: bwd_launch<BwdCfg<32, 16, 1024, 1, 3, 3, 4>>(a, s); return true;
+        default: return false;
+    }
+}
+
 bool launch_backward(const BwdArgs& a, uint32_t tile_size, int variant, hipStream_t s) {
     switch (tile_size) {
         case 8: return bwd_variant<8>(a, variant, s);
         case 12: return bwd_variant<12>(a, variant, s);
         case 16: return bwd_variant<16>(a, variant, s);
+        case 32: return bwd_variant32(a, variant, s);
         default: return false;
     }
 }
@@ -1712,7 +1812,7 @@ __global__ __launch_bounds__(C::NT) void k_error(const float4* __restrict__ rec,
 
 template <int TS>
 static void err_launch(const ErrArgs& a, hipStream_t s) {
-    using C = FwdCfg<TS, 32, 512, 1, 3>;
+    using C = FwdCfg<TS, 32, (TS * TS > 512 ? TS * TS : 512), 1, 3>;
     hipLaunchKernelGGL((k_error<C>), dim3(a.num_tiles), dim3(C::NT), 0, s, a.rec, a.sorted_vals, a.ranges, a.views,
                        a.pixel_err, a.ssim_err, a.tet_err, a.tet_count);
 }
@@ -1722,6 +1822,7 @@ bool launch_error(const ErrArgs& a, uint32_t tile_size, hipStream_t s) {
         case 8: err_launch<8>(a, s); return true;
         case 12: err_launch<12>(a, s); return true;
         case 16: err_launch<16>(a, s); return true;
+        case 32: err_launch<32>(a, s); return true;
         default: return false;
     }
 }
@@ -2680,17 +2781,18 @@ __global__ __launch_bounds__(256) void k_vertex_gather(const uint32_t* __restric
 
 template <int TS>
 static void ivl_bwd_launch(const IvlBwdArgs& a, hipStream_t s) {
-    using C = FwdCfg<TS, 32, 512, 1, 1>;
+    using C = FwdCfg<TS, (TS > 16 ? 16 : 32), (TS * TS > 512 ? TS * TS : 512), 1, 1>;
     hipLaunchKernelGGL((k_interval_backward<C>), dim3(a.num_tiles), dim3(C::NT), 0, s, a.ivr, a.sorted_vals, a.ranges,
                        a.views, a.tile_order, a.rgba, a.dist, a.dl_img, a.dl_xyzd, a.dl_dist, a.d_ivr);
 }
 
 bool launch_interval_backward(const IvlBwdArgs& a, uint32_t tile_size, hipStream_t s) {
-    if (tile_size != 8 && tile_size != 12 && tile_size != 16) return false;
+    if (tile_size != 8 && tile_size != 12 && tile_size != 16 && tile_size != 32) return false;
     switch (tile_size) {
         case 8: ivl_bwd_launch<8>(a, s); break;
         case 12: ivl_bwd_launch<12>(a, s); break;
-        default: ivl_bwd_launch<16>(a, s); break;
+        case 16: ivl_bwd_launch<16>(a, s); break;
+        default: ivl_bwd_launch<32>(a, s); break;
     }
     if (a.M)
         hipLaunchKernelGGL(k_interval_chain, dim3((a.M + 255) / 256), dim3(256), 0, s, a.rec, a.views, a.M, a.d_ivr,
@@ -2703,19 +2805,20 @@ bool launch_interval_backward(const IvlBwdArgs& a, uint32_t tile_size, hipStream
 
 template <int TS>
 static void ivl_launch(const IvlArgs& a, hipStream_t s) {
-    using C = FwdCfg<TS, 32, 512, 1, 1>;
+    using C = FwdCfg<TS, (TS > 16 ? 16 : 32), (TS * TS > 512 ? TS * TS : 512), 1, 1>;
     hipLaunchKernelGGL((k_interval_forward<C>), dim3(a.num_tiles), dim3(C::NT), 0, s, a.ivr, a.sorted_vals, a.ranges,
                        a.views, a.tile_order, a.rgba, a.xyzd, a.dist);
 }
 
 bool launch_interval(const IvlArgs& a, uint32_t tile_size, hipStream_t s) {
-    if (tile_size != 8 && tile_size != 12 && tile_size != 16) return false;
+    if (tile_size != 8 && tile_size != 12 && tile_size != 16 && tile_size != 32) return false;
     if (a.M) hipLaunchKernelGGL(k_interval_generate, dim3((a.M + 255) / 256), dim3(256), 0, s, a.rec, a.colors, a.views,
                                 a.M, a.ivr);
     switch (tile_size) {
         case 8: ivl_launch<8>(a, s); break;
         case 12: ivl_launch<12>(a, s); break;
-        default: ivl_launch<16>(a, s); break;
+        case 16: ivl_launch<16>(a, s); break;
+        default: ivl_launch<32>(a, s); break;
     }
     return true;
 }

@@ -130,17 +130,22 @@ __device__ __forceinline__ V3 view_dir(V3 centroid, const ViewParams& vw, float&
     return d;
 }
 
+// SH colours of the tets [t0, t0 + n) (one SH degree; their coefficients
+// start at sh) for every view of each tet's scene.
 template <int DEG>
-__global__ __launch_bounds__(256) void k_sh_colors(const float4* __restrict__ rec, const float* __restrict__ sh, uint32_t M,
-                                                   const ViewParams* __restrict__ views, uint32_t V, float4* __restrict__ colors,
+__global__ __launch_bounds__(256) void k_sh_colors(const float4* __restrict__ rec, const float* __restrict__ sh, uint32_t t0,
+                                                   uint32_t n, const SceneRange* __restrict__ scenes, uint32_t n_scenes,
+                                                   const ViewParams* __restrict__ views, float4* __restrict__ colors,
                                                    bool half) {
     constexpr int NC = (DEG + 1) * (DEG + 1);
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = t0 + i;
+    const SceneRange& sc = scenes[scene_of_tet(scenes, n_scenes, t)];
     V3 v[4], g, cen;
     tet_geom(rec, t, v, g, cen);
     float c[3 * NC];
-    const float* s = sh + (uint64_t)t * 3 * NC;
+    const float* s = sh + (uint64_t)i * 3 * NC;
 #pragma unroll
     for (int i = 0; i < 3 * NC; ++i) c[i] = s[i];
     // the viewer path reads the coefficients as packed f16 (project_compute.wgsl:71-75,
@@ -150,7 +155,7 @@ __global__ __launch_bounds__(256) void k_sh_colors(const float4* __restrict__ re
         for (int i = 0; i < 3 * NC; ++i) c[i] = (float)(_Float16)c[i];
     }
     float offset = dot3(g, sub3(v[0], cen));
-    for (uint32_t vv = 0; vv < V; ++vv) {
+    for (uint32_t vv = sc.view_base; vv < sc.view_base + sc.V; ++vv) {
         float len;
         V3 d = view_dir(cen, views[vv], len);
         float Y[NC];
@@ -163,7 +168,7 @@ __global__ __launch_bounds__(256) void k_sh_colors(const float4* __restrict__ re
             for (int k = 1; k < NC; ++k) val = val + Y[k] * c[ch * NC + k];
             rgb[ch] = softplus_f((val + 0.5f) + offset);
         }
-        colors[(uint64_t)vv * M + t] = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
+        colors[views[vv].color_off + t] = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
     }
 }
 
@@ -217,14 +222,14 @@ __global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a)
     __shared__ __align__(16) float s_io[1][3 * kShTets * SR];
     __shared__ float s_g[3 * kShTets * SR];  // summed d_sh rows
     const uint32_t lt = threadIdx.x / kShLanes, j = threadIdx.x % kShLanes;
-    const uint32_t t0 = blockIdx.x * kShTets;
-    const uint32_t t = t0 + lt;
-    const uint32_t nflt = min((uint32_t)kShTets, a.M - t0) * 3u * NC;  // floats of this workgroup
-    float* io_g[3] = {a.sh + (uint64_t)t0 * 3 * NC, ADAM ? a.m_sh + (uint64_t)t0 * 3 * NC : nullptr,
-                      ADAM ? a.v_sh + (uint64_t)t0 * 3 * NC : nullptr};
+    const uint32_t i0 = blockIdx.x * kShTets;  // first tet of the workgroup within the class
+    const uint32_t t = a.t0 + i0 + lt;         // global tet id
+    const uint32_t nflt = min((uint32_t)kShTets, a.M - i0) * 3u * NC;  // floats of this workgroup
+    float* io_g[3] = {a.sh + (uint64_t)i0 * 3 * NC, ADAM ? a.m_sh + (uint64_t)i0 * 3 * NC : nullptr,
+                      ADAM ? a.v_sh + (uint64_t)i0 * 3 * NC : nullptr};
     for (uint32_t i = threadIdx.x; i < nflt; i += NT) s_io[0][(i / NC) * SR + i % NC] = io_g[0][i];
     __syncthreads();
-    const bool live = t < a.M;
+    const bool live = i0 + lt < a.M;
     const int row0 = (int)(lt * 3) * SR;
     V3 v[4], g = mk3(0.f, 0.f, 0.f), cen, off = mk3(0.f, 0.f, 0.f);
     float dsh[3][NC];
@@ -238,25 +243,27 @@ __global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a)
         tet_geom(a.rec, t, v, g, cen);
         off = sub3(v[0], cen);
         const float offset = dot3(g, off);
+        const SceneRange& sc = a.scenes[scene_of_tet(a.scenes, a.n_scenes, t)];
+        const uint32_t vb = sc.view_base, V = sc.V;
         // colour gradients one view ahead (their load latency overlaps a view's work)
         float dn[3] = {0.f, 0.f, 0.f};
-        if (j < a.V) {
-            const float* dcp = a.d_colors + ((uint64_t)j * a.M + t) * 3;
+        if (j < V) {
+            const float* dcp = a.d_colors + ((uint64_t)(a.views[vb + j].color_off + t)) * 3;
             dn[0] = dcp[0];
             dn[1] = dcp[1];
             dn[2] = dcp[2];
         }
-        for (uint32_t vv = j; vv < a.V; vv += kShLanes) {
+        for (uint32_t vv = j; vv < V; vv += kShLanes) {
             const float dc[3] = {dn[0], dn[1], dn[2]};
-            if (vv + kShLanes < a.V) {
-                const float* dcp = a.d_colors + ((uint64_t)(vv + kShLanes) * a.M + t) * 3;
+            if (vv + kShLanes < V) {
+                const float* dcp = a.d_colors + ((uint64_t)(a.views[vb + vv + kShLanes].color_off + t)) * 3;
                 dn[0] = dcp[0];
                 dn[1] = dcp[1];
                 dn[2] = dcp[2];
             }
             if (dc[0] == 0.0f && dc[1] == 0.0f && dc[2] == 0.0f) continue;
             float len;
-            const V3 d = view_dir(cen, a.views[vv], len);
+            const V3 d = view_dir(cen, a.views[vb + vv], len);
             float Y[NC], w[NC];
             sh_basis<DEG>(d, Y);
 #pragma unroll
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a)
             io_g[2][i] = v2;
         }
     } else {
-        float* o = a.d_sh + (uint64_t)t0 * 3 * NC;
+        float* o = a.d_sh + (uint64_t)i0 * 3 * NC;
         for (uint32_t i = threadIdx.x; i < nflt; i += NT) o[i] = s_g[(i / NC) * SR + i % NC];
     }
 }
@@ -378,27 +385,28 @@ __global__ void k_step_inc(uint32_t* step, const uint32_t* skip) {
 }
 
 // Per-view loss: sum of per-tile partials in tile order (deterministic).
-__global__ __launch_bounds__(256) void k_reduce_loss(const float* __restrict__ partial, uint32_t tiles_per_view,
-                                                     float* __restrict__ out, float inv_npx) {
+__global__ __launch_bounds__(256) void k_reduce_loss(const float* __restrict__ partial, const ViewParams* __restrict__ views,
+                                                     float* __restrict__ out) {
     __shared__ float red[4];
-    uint32_t v = blockIdx.x;
+    const ViewParams& vw = views[blockIdx.x];
+    const uint32_t nt = vw.tiles_x * vw.tiles_y;
     float s = 0.0f;
-    for (uint32_t i = threadIdx.x; i < tiles_per_view; i += blockDim.x) s += partial[(uint64_t)v * tiles_per_view + i];
+    for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) s += partial[(uint64_t)vw.tile_base + i];
     s = wave_sum(s);
     if (__lane_id() == 0) red[threadIdx.x / 64] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[v] = ((red[0] + red[1]) + (red[2] + red[3])) * inv_npx;
+    if (threadIdx.x == 0) out[blockIdx.x] = ((red[0] + red[1]) + (red[2] + red[3])) * (1.0f / (float)(vw.width * vw.height));
 }
 
-void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t M, const ViewParams* views, uint32_t V,
-                      float4* colors, bool half, hipStream_t s) {
-    if (!M) return;
-    dim3 gr((M + 255) / 256), bl(256);
+void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t t0, uint32_t n, const SceneRange* scenes,
+                      uint32_t n_scenes, const ViewParams* views, float4* colors, bool half, hipStream_t s) {
+    if (!n) return;
+    dim3 gr((n + 255) / 256), bl(256);
     switch (deg) {
-        case 0: hipLaunchKernelGGL(k_sh_colors<0>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
-        case 1: hipLaunchKernelGGL(k_sh_colors<1>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
-        case 2: hipLaunchKernelGGL(k_sh_colors<2>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
-        default: hipLaunchKernelGGL(k_sh_colors<3>, gr, bl, 0, s, rec, sh, M, views, V, colors, half); break;
+        case 0: hipLaunchKernelGGL(k_sh_colors<0>, gr, bl, 0, s, rec, sh, t0, n, scenes, n_scenes, views, colors, half); break;
+        case 1: hipLaunchKernelGGL(k_sh_colors<1>, gr, bl, 0, s, rec, sh, t0, n, scenes, n_scenes, views, colors, half); break;
+        case 2: hipLaunchKernelGGL(k_sh_colors<2>, gr, bl, 0, s, rec, sh, t0, n, scenes, n_scenes, views, colors, half); break;
+        default: hipLaunchKernelGGL(k_sh_colors<3>, gr, bl, 0, s, rec, sh, t0, n, scenes, n_scenes, views, colors, half); break;
     }
 }
 
@@ -432,9 +440,8 @@ void launch_step_inc(uint32_t* step, const uint32_t* skip, hipStream_t s) {
     hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, s, step, skip);
 }
 
-void launch_reduce_loss(const float* partial, uint32_t tiles_per_view, uint32_t V, float* out, float inv_npx,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_loss, dim3(V), dim3(256), 0, s, partial, tiles_per_view, out, inv_npx);
+void launch_reduce_loss(const float* partial, const ViewParams* views, uint32_t V, float* out, hipStream_t s) {
+    if (V) hipLaunchKernelGGL(k_reduce_loss, dim3(V), dim3(256), 0, s, partial, views, out);
 }
 
 }  // namespace rmesh

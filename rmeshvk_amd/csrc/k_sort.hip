@@ -8,9 +8,13 @@
 //    whose length is derived on the device from the pair count — the launch
 //    shape never depends on a host readback, so a whole step can be captured
 //    in a hipGraph, and no block ever waits on another (no look-back spins);
-//  * stable in-tile ranking with wave64 ballots: each wave matches the 8 digit
-//    bits of its 64 keys (8 ballots), ranks by popcount, and keeps a private
-//    256-entry running count in LDS — no barriers inside the ranking rounds;
+//  * stable in-tile ranking with wave64 ballots: each wave matches the digit
+//    bits of its 64 keys (one ballot per bit), ranks by popcount, and keeps a
+//    private per-digit running count in LDS — no barriers inside the ranking
+//    rounds;
+//  * 8-bit digits (256 buckets) or 4-bit digits (16 buckets, twice the passes,
+//    half the ballots per key and far smaller histograms), picked per renderer
+//    by the autotuner;
 //  * the tile (2048 keys) is re-ordered by digit in LDS before the scatter,
 //    so the global writes are runs per digit instead of 8-byte scatters.
 #include "rmesh_common.hpp"
@@ -32,26 +36,33 @@ __device__ __forceinline__ void chunk_of(uint32_t n, uint32_t G, uint32_t b, uin
     hi = (uint32_t)min<uint64_t>(l + per, n);
 }
 
+// DB: digit width (8: 256 buckets, 6 passes over 48 key bits; 4: 16 buckets,
+// twice the passes with a quarter of the ballots per key each); chosen by the
+// autotuner.
+template <int DB>
 __global__ __launch_bounds__(kSortThreads) void k_rs_upsweep(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr,
                                                              uint32_t cap, uint32_t shift, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kSortWaves][256];
+    constexpr uint32_t RADIX = 1u << DB, DMASK = RADIX - 1u;
+    __shared__ uint32_t h[kSortWaves][RADIX];
     uint32_t G = gridDim.x, b = blockIdx.x;
     uint32_t n = min(*n_ptr, cap);
     uint32_t lo, hi;
     chunk_of(n, G, b, lo, hi);
-    for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads) (&h[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < kSortWaves * (int)RADIX; i += kSortThreads) (&h[0][0])[i] = 0;
     __syncthreads();
     uint32_t w = threadIdx.x / kWave;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
-        uint32_t d = (uint32_t)(keys[i] >> shift) & 255u;
+        uint32_t d = (uint32_t)(keys[i] >> shift) & DMASK;
         atomicAdd(&h[w][d], 1u);
     }
     __syncthreads();
     uint32_t d = threadIdx.x;
-    uint32_t s = 0;
+    if (d < RADIX) {
+        uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kSortWaves; ++k) s += h[k][d];
-    hist[(uint64_t)d * G + b] = s;
+        for (int k = 0; k < kSortWaves; ++k) s += h[k][d];
+        hist[(uint64_t)d * G + b] = s;
+    }
 }
 
 // Exclusive scan of hist[d][0..G) for one digit d per workgroup (256
@@ -74,16 +85,19 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, ui
     if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
+template <int DB>
 __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                                uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                                const uint32_t* __restrict__ n_ptr, uint32_t cap, uint32_t shift,
                                                                const uint32_t* __restrict__ hist, const uint32_t* __restrict__ totals) {
+    constexpr uint32_t RADIX = 1u << DB, DMASK = RADIX - 1u;
+    static_assert(RADIX <= (uint32_t)kSortThreads, "one thread per digit");
     __shared__ uint64_t s_keys[kTileKeys];
     __shared__ uint32_t s_vals[kTileKeys];
-    __shared__ uint32_t s_wcnt[kSortWaves][256];
-    __shared__ uint32_t s_loc[256];
-    __shared__ uint32_t s_glob[256];
-    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_wcnt[kSortWaves][RADIX];
+    __shared__ uint32_t s_loc[RADIX];
+    __shared__ uint32_t s_glob[RADIX];
+    __shared__ uint32_t s_run[RADIX];
     __shared__ uint32_t s_red[kSortWaves];
 
     uint32_t G = gridDim.x, b = blockIdx.x;
@@ -94,13 +108,13 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
     {
         // global start of digit `tid` = exclusive scan of the digit totals
         uint32_t all;
-        uint32_t dstart = block_excl_scan(totals[tid], s_red, all);
-        s_run[tid] = dstart + hist[(uint64_t)tid * G + b];
+        uint32_t dstart = block_excl_scan(tid < RADIX ? totals[tid] : 0u, s_red, all);
+        if (tid < RADIX) s_run[tid] = dstart + hist[(uint64_t)tid * G + b];
     }
     const uint64_t lt = lanemask_lt();
 
     for (uint32_t base = lo; base < hi; base += kTileKeys) {
-        for (int i = tid; i < kSortWaves * 256; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+        for (int i = tid; i < kSortWaves * (int)RADIX; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
         uint64_t key[kRounds];
         uint32_t val[kRounds];
@@ -112,10 +126,10 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
             bool valid = i < hi;
             key[j] = valid ? kin[i] : 0ull;
             val[j] = valid ? vin[i] : 0u;
-            uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+            uint32_t d = (uint32_t)(key[j] >> shift) & DMASK;
             uint64_t peers = __ballot(valid);
 #pragma unroll
-            for (int bit = 0; bit < 8; ++bit) {
+            for (int bit = 0; bit < DB; ++bit) {
                 uint64_t bb = __ballot((d >> bit) & 1u);
                 peers &= ((d >> bit) & 1u) ? bb : ~bb;
             }
@@ -130,20 +144,23 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
         __syncthreads();
         // Phase 2: per digit — wave prefixes, tile counts, local starts, global bases.
         {
-            uint32_t d = tid;
-            uint32_t c[kSortWaves];
+            const uint32_t d = tid;
             uint32_t tot = 0;
+            if (d < RADIX) {
 #pragma unroll
-            for (int k = 0; k < kSortWaves; ++k) {
-                c[k] = s_wcnt[k][d];
-                s_wcnt[k][d] = tot;
-                tot += c[k];
+                for (int k = 0; k < kSortWaves; ++k) {
+                    const uint32_t c = s_wcnt[k][d];
+                    s_wcnt[k][d] = tot;
+                    tot += c;
+                }
             }
             uint32_t all;
             uint32_t loc = block_excl_scan(tot, s_red, all);
-            s_loc[d] = loc;
-            s_glob[d] = s_run[d];
-            s_run[d] += tot;
+            if (d < RADIX) {
+                s_loc[d] = loc;
+                s_glob[d] = s_run[d];
+                s_run[d] += tot;
+            }
         }
         __syncthreads();
         // Phase 3: stage in digit order.
@@ -161,7 +178,7 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
         uint32_t tile_n = min((uint32_t)kTileKeys, hi - base);
         for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
             uint64_t k = s_keys[i];
-            uint32_t d = (uint32_t)(k >> shift) & 255u;
+            uint32_t d = (uint32_t)(k >> shift) & DMASK;
             uint32_t gp = s_glob[d] + (i - s_loc[d]);
             kout[gp] = k;
             vout[gp] = s_vals[i];
@@ -178,22 +195,30 @@ uint32_t sort_grid(uint32_t cap) {
 }
 
 // Sorts the pairs in (ka, va); returns true when the result ended in (kb, vb).
-bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
-                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s) {
+template <int DB>
+static bool radix_passes(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
+                         uint32_t sorting_bits, uint32_t* hist, hipStream_t s) {
+    constexpr uint32_t RADIX = 1u << DB;
     uint32_t G = sort_grid(cap);
     bool in_b = false;
-    for (uint32_t shift = 0; shift < sorting_bits; shift += 8) {
+    for (uint32_t shift = 0; shift < sorting_bits; shift += DB) {
         const uint64_t* kin = in_b ? kb : ka;
         const uint32_t* vin = in_b ? vb : va;
         uint64_t* kout = in_b ? ka : kb;
         uint32_t* vout = in_b ? va : vb;
-        hipLaunchKernelGGL(k_rs_upsweep, dim3(G), dim3(kSortThreads), 0, s, kin, n_ptr, cap, shift, hist);
-        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, s, hist, G, hist + 256ull * G);
-        hipLaunchKernelGGL(k_rs_downsweep, dim3(G), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_ptr, cap, shift, hist,
-                           hist + 256ull * G);
+        hipLaunchKernelGGL(k_rs_upsweep<DB>, dim3(G), dim3(kSortThreads), 0, s, kin, n_ptr, cap, shift, hist);
+        hipLaunchKernelGGL(k_rs_scan, dim3(RADIX), dim3(256), 0, s, hist, G, hist + (uint64_t)RADIX * G);
+        hipLaunchKernelGGL(k_rs_downsweep<DB>, dim3(G), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_ptr, cap, shift,
+                           hist, hist + (uint64_t)RADIX * G);
         in_b = !in_b;
     }
     return in_b;
+}
+
+bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
+                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s, uint32_t digit_bits) {
+    if (digit_bits == 4) return radix_passes<4>(ka, va, kb, vb, n_ptr, cap, sorting_bits, hist, s);
+    return radix_passes<8>(ka, va, kb, vb, n_ptr, cap, sorting_bits, hist, s);
 }
 
 size_t radix_hist_words(uint32_t cap) { return 256ull * sort_grid(cap) + 256; }

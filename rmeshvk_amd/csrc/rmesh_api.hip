@@ -56,13 +56,49 @@ hipError_t dalloc(T** p, size_t n) {
 
 void rmesh::set_last_error(const std::string& msg) { g_err = msg; }
 
+// One scene of a renderer (a multi-scene renderer holds several, each with its
+// own tets, resolution, SH degree and view capacity; a plain renderer one).
+struct SceneHost {
+    uint32_t N = 0, M = 0;           // vertices, tets
+    uint32_t vert_base = 0, tet_base = 0;  // offsets in the concatenated arrays
+    uint32_t W = 0, H = 0, max_views = 1;
+    uint32_t deg = 0, nc = 1;        // SH degree (0 without SH) and coefficients per channel
+    uint64_t sh_base = 0;            // first SH float
+    uint32_t user = 0;               // the caller's scene index
+};
+// Tets of one SH degree: a contiguous run of scenes (scenes are kept sorted by degree).
+struct ShClass {
+    uint32_t deg, t0, n;
+    uint64_t sh_off;
+};
+
 struct rmesh_renderer {
     int device = 0;
+    // scenes in tet order (sorted by SH degree), the caller's index of each, the SH classes
+    std::vector<SceneHost> sc;
+    std::vector<uint32_t> sc_of_user;
+    std::vector<ShClass> sh_classes;
+    uint64_t sh_total = 0;        // SH floats of all scenes
+    uint64_t pix_cap = 0;         // pixels of all view slots (sum of max_views W H)
+    uint64_t color_cap = 0;       // float4 colours of all view slots
+    uint64_t item_cap = 0;        // binning items at full view capacity
+    uint32_t tile_cap = 0;        // global tiles at full view capacity, tile size 8
+    uint32_t view_cap = 0;        // views at full capacity
+    // the current view layout: views per scene (caller order) and what follows from it
+    std::vector<uint32_t> step_views;
+    std::vector<SceneRange> hscenes;
+    std::vector<uint32_t> htile_view, hktile;
+    uint64_t n_items = 0, color_len = 0, pix_len = 0;
+    uint32_t n_gt = 0, n_kt = 0, n_views = 0;
+    SceneRange* d_scenes = nullptr;
+    uint32_t* d_tile_view = nullptr;   // global tile -> view
+    uint32_t* d_ktile = nullptr;       // key tile -> global tile (0xFFFFFFFF: sentinel)
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     uint32_t N = 0, M = 0, W = 0, H = 0, TS = 12, tiles_x = 0, tiles_y = 0, T = 0;
     uint32_t T_alloc = 0;          // per-tile buffers are sized for the smallest tile (8)
     int fwd_var = 0, bwd_var = 0;  // batch-geometry variants of the compositing kernels
+    uint32_t sort_db = 8;          // radix-sort digit width (8 or 4)
     uint32_t max_views = 1;
     bool has_sh = false;
     bool sh_half = false;  // forward colours from f16-rounded SH (the viewer path)
@@ -93,6 +129,13 @@ struct rmesh_renderer {
     uint32_t* ranges = nullptr;
     uint32_t* tile_order = nullptr;  // compositing workgroup -> global tile, heaviest first
     bool sorted_in_b = false;
+    // adjoint mode: false = the reference's undo from the image alpha, true =
+    // the checkpointed adjoint (transmittance checkpoints stored by the forward)
+    bool adj_ck = false;
+    bool have_ck = false;        // the last forward stored checkpoints
+    float* ck = nullptr;         // [slots][TS*TS]
+    uint64_t ck_floats = 0;
+    float2* fin = nullptr;       // [max_views][H][W]: final log T, stop position
     // images
     float *rgba = nullptr, *aux = nullptr, *dl = nullptr, *gt = nullptr, *loss_partial = nullptr, *loss_out = nullptr;
     bool aux_alloc = false;
@@ -116,14 +159,20 @@ struct rmesh_renderer {
     hipGraphExec_t graph_exec[8] = {};
     int graph_parts = 0;
     uint32_t graph_views = 0, graph_loss = 0;
-    float graph_lrs[4] = {0, 0, 0, 0};
+    std::vector<uint32_t> graph_layout;  // views per scene the graph was captured for
+    // per-scene timing: wall clock at the step start and when each scene's last adjoint tile retired
+    bool scene_timing = false;
+    uint64_t* scene_clock = nullptr;  // [scenes + 1]
+    float graph_lrs[5] = {0, 0, 0, 0, 0};  // lr_v, lr_d, lr_g, lr_sh, lr_base baked into the graph
     std::vector<ViewParams> hviews;
     // optional per-stage timing (HIP events recorded between stages)
     bool timing = false;
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
-    uint32_t gtiles(uint32_t V) const { return V * T; }
-    uint32_t sorting_bits(uint32_t V) const { return rmesh_sorting_bits(gtiles(V)); }
+    // global tiles and sort bits of the current view layout (the V arguments name the layout's view count)
+    uint32_t gtiles(uint32_t) const { return n_gt; }
+    uint32_t sorting_bits(uint32_t) const { return rmesh_sorting_bits(n_kt); }
+    bool multi() const { return sc.size() > 1; }
 };
 
 extern "C" uint32_t rmesh_sorting_bits(uint32_t num_tiles) {
@@ -146,17 +195,18 @@ namespace {
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev) {
-        hipGetDevice(&prev);
-        if (prev != dev) hipSetDevice(dev);
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
     }
     ~DeviceGuard() {
         int cur = -1;
-        hipGetDevice(&cur);
-        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+        (void)hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) (void)hipSetDevice(prev);
     }
 };
 
-void make_view(const rmesh_renderer* h, uint32_t v, const float* cam, const float* vp, const float* ci, ViewParams& o) {
+void make_view(const rmesh_renderer* h, const SceneHost& sh, const float* cam, const float* vp, const float* ci,
+               ViewParams& o) {
     std::memset(&o, 0, sizeof(o));
     for (int i = 0; i < 16; ++i) o.vp[i] = vp[i];
     for (int j = 0; j < 3; ++j)
@@ -164,22 +214,92 @@ void make_view(const rmesh_renderer* h, uint32_t v, const float* cam, const floa
     for (int i = 0; i < 4; ++i) o.intr[i] = ci[12 + i];
     for (int i = 0; i < 3; ++i) o.cam[i] = cam[i];
     o.min_t = h->min_t;
-    o.Wf = (float)h->W;
-    o.Hf = (float)h->H;
-    o.width = h->W;
-    o.height = h->H;
-    o.tiles_x = h->tiles_x;
-    o.tiles_y = h->tiles_y;
-    o.tile_base = v * h->T;
+    o.Wf = (float)sh.W;
+    o.Hf = (float)sh.H;
+    o.width = sh.W;
+    o.height = sh.H;
+    o.tiles_x = (sh.W + h->TS - 1) / h->TS;
+    o.tiles_y = (sh.H + h->TS - 1) / h->TS;
     o.tile_size = h->TS;
 }
 
-rmesh_status upload_views(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps, const float* c2ws) {
+// The view layout of a step: per_user[u] views of the caller's scene u, in the
+// caller's scene order (cams/vps/c2ws in that order).  Views get their global
+// tiles, sort-key tiles (one sentinel tile per view when several views share
+// the sort, none with one view: the reference's own keys), colour and pixel
+// offsets; scenes (tet order) get their binning items.  Uploads the view
+// table, the scene table and the tile tables.
+rmesh_status upload_layout(rmesh_renderer* h, const uint32_t* per_user, const float* cams, const float* vps,
+                           const float* c2ws) {
+    const uint32_t ns = (uint32_t)h->sc.size();
+    uint32_t V = 0;
+    for (uint32_t u = 0; u < ns; ++u) {
+        const SceneHost& sh = h->sc[h->sc_of_user[u]];
+        if (per_user[u] > sh.max_views)
+            return fail(RMESH_ERR_INVALID, "scene " + std::to_string(u) + ": n_views must be at most max_views=" +
+                                               std::to_string(sh.max_views));
+        V += per_user[u];
+    }
+    if (V == 0) return fail(RMESH_ERR_INVALID, "no views");
     h->have_interval = false;
+    h->step_views.assign(per_user, per_user + ns);
     h->hviews.resize(V);
-    for (uint32_t v = 0; v < V; ++v) make_view(h, v, cams + 3 * v, vps + 16 * v, c2ws + 16 * v, h->hviews[v]);
+    h->htile_view.clear();
+    h->hktile.clear();
+    std::vector<uint32_t> first_view(ns, 0);
+    uint64_t color = 0, pix = 0;
+    uint32_t gt = 0, v = 0;
+    for (uint32_t u = 0; u < ns; ++u) {
+        const uint32_t si = h->sc_of_user[u];
+        const SceneHost& sh = h->sc[si];
+        first_view[si] = v;
+        for (uint32_t j = 0; j < per_user[u]; ++j, ++v) {
+            ViewParams& o = h->hviews[v];
+            make_view(h, sh, cams + 3ull * v, vps + 16ull * v, c2ws + 16ull * v, o);
+            const uint32_t T = o.tiles_x * o.tiles_y;
+            o.tile_base = gt;
+            o.key_base = (uint32_t)h->hktile.size();
+            o.sentinel_tile = V == 1 ? 0xFFFFFFFFu : o.key_base + T;
+            o.color_off = h->has_sh ? (int64_t)color - (int64_t)sh.tet_base : 0;
+            o.pix_base = pix;
+            o.scene = u;
+            for (uint32_t t = 0; t < T; ++t) {
+                h->htile_view.push_back(v);
+                h->hktile.push_back(gt + t);
+            }
+            if (V > 1) h->hktile.push_back(0xFFFFFFFFu);
+            gt += T;
+            color += sh.M;
+            pix += (uint64_t)sh.W * sh.H;
+        }
+    }
+    h->n_views = V;
+    h->n_gt = gt;
+    h->n_kt = (uint32_t)h->hktile.size();
+    h->color_len = h->has_sh ? color : h->M;
+    h->pix_len = pix;
+    h->hscenes.resize(ns);
+    uint64_t item = 0;
+    for (uint32_t si = 0; si < ns; ++si) {
+        const SceneHost& sh = h->sc[si];
+        const uint32_t nv = per_user[sh.user];
+        h->hscenes[si] = SceneRange{item, sh.tet_base, sh.M, first_view[si], nv};
+        item += (uint64_t)sh.M * nv;
+    }
+    h->n_items = item;
     HIP_TRY(hipMemcpyAsync(h->d_views, h->hviews.data(), sizeof(ViewParams) * V, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_scenes, h->hscenes.data(), sizeof(SceneRange) * ns, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_tile_view, h->htile_view.data(), 4ull * h->n_gt, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_ktile, h->hktile.data(), 4ull * h->n_kt, hipMemcpyHostToDevice, h->stream));
+    // the host vectors stay alive (and unchanged) until the next layout: the
+    // asynchronous copies read them in stream order
     return RMESH_OK;
+}
+
+// V views of a renderer's single scene.
+rmesh_status upload_views(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps, const float* c2ws) {
+    if (h->multi()) return fail(RMESH_ERR_UNSUPPORTED, "a multi-scene renderer takes its views per scene");
+    return upload_layout(h, &V, cams, vps, c2ws);
 }
 
 void drop_graph(rmesh_renderer* h) {
@@ -209,11 +329,11 @@ rmesh_status ensure_capacity(rmesh_renderer* h, uint64_t need) {
     uint64_t c = std::max<uint64_t>(need + need / 4 + (1u << 16), 1u << 20);
     if (c > 0xFFFFFFF0ull) c = 0xFFFFFFF0ull;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    hipFree(h->keys_a);
-    hipFree(h->keys_b);
-    hipFree(h->vals_a);
-    hipFree(h->vals_b);
-    hipFree(h->hist);
+    (void)hipFree(h->keys_a);
+    (void)hipFree(h->keys_b);
+    (void)hipFree(h->vals_a);
+    (void)hipFree(h->vals_b);
+    (void)hipFree(h->hist);
     h->keys_a = h->keys_b = nullptr;
     h->vals_a = h->vals_b = nullptr;
     h->hist = nullptr;
@@ -228,6 +348,25 @@ rmesh_status ensure_capacity(rmesh_renderer* h, uint64_t need) {
     return RMESH_OK;
 }
 
+// Checkpoint buffers for V views at the current capacity and tile size
+// (allocated on demand in the checkpointed mode; growing them drops the graph).
+rmesh_status ensure_checkpoints(rmesh_renderer* h, uint32_t V) {
+    if (!h->adj_ck) return RMESH_OK;
+    const uint64_t need = checkpoint_slots(h->cap, h->n_gt) * h->TS * h->TS;
+    if (need > h->ck_floats || !h->ck) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        (void)hipFree(h->ck);
+        h->ck = nullptr;
+        h->ck_floats = 0;
+        const uint64_t c = need + need / 8;
+        HIP_TRY(dalloc(&h->ck, c));
+        h->ck_floats = c;
+        drop_graph(h);
+    }
+    if (!h->fin) HIP_TRY(dalloc(&h->fin, (size_t)h->pix_cap));
+    return RMESH_OK;
+}
+
 void mark(rmesh_renderer* h, int i) {
     if (h->timing) (void)hipEventRecord(h->ev[i], h->stream);
 }
@@ -236,53 +375,73 @@ void mark(rmesh_renderer* h, int i) {
 // to size the pair buffers (skipped under graph capture).
 rmesh_status stage_binning_front(rmesh_renderer* h, uint32_t V, bool capture) {
     hipStream_t s = h->stream;
+    const uint32_t ns = (uint32_t)h->sc.size();
     launch_prep_records(h->verts, h->idx, h->dens, h->grads, h->M, h->rec, s);
-    if (h->has_sh) launch_sh_colors(h->rec, h->sh, h->sh_deg, h->M, h->d_views, V, h->colors, h->sh_half, s);
-    else launch_copy_colors(h->base, h->M, h->colors, s);
-    launch_fill_u32(h->vis_count, 0u, h->max_views, s);
+    if (h->has_sh) {
+        for (const ShClass& c : h->sh_classes)  // one launch per SH degree present
+            launch_sh_colors(h->rec, h->sh + c.sh_off, c.deg, c.t0, c.n, h->d_scenes, ns, h->d_views, h->colors,
+                             h->sh_half, s);
+    } else {
+        launch_copy_colors(h->base, h->M, h->colors, s);
+    }
+    launch_fill_u32(h->vis_count, 0u, 1, s);
     launch_fill_u32(h->overflow, 0u, 1, s);
-    launch_project(h->rec, h->M, h->d_views, V, h->counts, h->block_sums, h->vis_count, s);
-    launch_spine_scan(h->block_sums, project_blocks(h->M, V), h->n_pairs, s);
+    launch_project(h->rec, h->d_scenes, ns, h->n_items, h->d_views, h->counts, h->block_sums, h->vis_count, s);
+    launch_spine_scan(h->block_sums, item_blocks(h->n_items), h->n_pairs, h->overflow, s);
     if (!capture) {
         uint32_t P = 0;
         HIP_TRY(hipMemcpyAsync(&P, h->n_pairs, 4, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         rmesh_status st = ensure_capacity(h, P);
         if (st != RMESH_OK) return st;
+        st = ensure_checkpoints(h, V);
+        if (st != RMESH_OK) return st;
     }
     return RMESH_OK;
 }
 
+void launch_pair_gen(rmesh_renderer* h, hipStream_t s) {
+    launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->d_scenes, (uint32_t)h->sc.size(), h->n_items,
+                 h->d_views, h->counts, h->block_sums, h->keys_a, h->vals_a, h->cap, h->overflow, s);
+}
+void launch_sort_stage(rmesh_renderer* h, hipStream_t s) {
+    h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
+                                       h->sorting_bits(0), h->hist, s, h->sort_db);
+}
+void launch_ranges_stage(rmesh_renderer* h, hipStream_t s) {
+    launch_fill_u32(h->ranges, 0u, 2ull * h->n_gt, s);
+    launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, h->d_ktile, h->n_kt, s);
+    launch_tile_order(h->ranges, h->n_gt, h->tile_order, s);
+}
+
 rmesh_status stage_binning_back(rmesh_renderer* h, uint32_t V) {
     hipStream_t s = h->stream;
-    launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, V, h->counts, h->block_sums,
-                 h->keys_a, h->vals_a, h->cap, h->overflow, s);
-    h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
-                                       h->sorting_bits(V), h->hist, s);
-    launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
-    launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
-    launch_tile_order(h->ranges, h->gtiles(V), h->tile_order, s);
+    launch_pair_gen(h, s);
+    launch_sort_stage(h, s);
+    launch_ranges_stage(h, s);
     return RMESH_OK;
 }
 
 const uint32_t* sorted_vals(const rmesh_renderer* h) { return h->sorted_in_b ? h->vals_b : h->vals_a; }
 
-uint64_t color_stride(const rmesh_renderer* h) { return h->has_sh ? h->M : 0; }
 
 rmesh_status stage_forward(rmesh_renderer* h, uint32_t V, bool want_aux) {
     FwdArgs a;
     a.rec = h->rec;
     a.colors = h->colors;
-    a.color_view_stride = color_stride(h);
     a.sorted_vals = sorted_vals(h);
     a.ranges = h->ranges;
     a.views = h->d_views;
-    a.tiles_per_view = h->T;
-    a.num_gtiles = h->gtiles(V);
+    a.tile_view = h->d_tile_view;
+    a.num_gtiles = h->n_gt;
     a.tile_order = h->tile_order;
     a.rgba = h->rgba;
     a.aux = want_aux ? h->aux : nullptr;
+    const bool ck = h->adj_ck && !want_aux;  // the aux forward keeps no checkpoints
+    a.ck = ck ? h->ck : nullptr;
+    a.fin = ck ? h->fin : nullptr;
     if (!launch_forward(a, h->TS, h->fwd_var, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
+    h->have_ck = ck;
     return RMESH_OK;
 }
 
@@ -291,7 +450,7 @@ void zero_grads(rmesh_renderer* h, uint32_t V) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_verts), 0u, 3ull * h->N, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_dens), 0u, h->M, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_grads), 0u, 3ull * h->M, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_colors), 0u, 3ull * h->M * (h->has_sh ? V : 1), s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_colors), 0u, 3ull * h->color_len, s);
 }
 
 rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, const float* gt, uint32_t loss_type) {
@@ -299,12 +458,11 @@ rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, cons
     a.rec = h->rec;
     a.idx = h->idx;
     a.colors = h->colors;
-    a.color_view_stride = color_stride(h);
     a.sorted_vals = sorted_vals(h);
     a.ranges = h->ranges;
     a.views = h->d_views;
-    a.tiles_per_view = h->T;
-    a.num_gtiles = h->gtiles(V);
+    a.tile_view = h->d_tile_view;
+    a.num_gtiles = h->n_gt;
     a.tile_order = h->tile_order;
     a.rgba = h->rgba;
     a.dl_img = dl;
@@ -315,7 +473,17 @@ rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, cons
     a.d_dens = h->d_dens;
     a.d_grads = h->d_grads;
     a.d_colors = h->d_colors;
-    a.dcolor_view_stride = h->has_sh ? h->M : 0;
+    a.ck = nullptr;
+    a.fin = nullptr;
+    a.scene_done = h->scene_timing ? h->scene_clock : nullptr;
+    if (h->adj_ck) {
+        if (!h->have_ck)
+            return fail(RMESH_ERR_INVALID,
+                        "checkpointed adjoint: the last forward kept no checkpoints (a render_aux forward, or the "
+                        "mode was switched after it); run forward_tiled again");
+        a.ck = h->ck;
+        a.fin = h->fin;
+    }
     if (!launch_backward(a, h->TS, h->bwd_var, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
     return RMESH_OK;
 }
@@ -340,7 +508,7 @@ rmesh_status ensure_optimizer(rmesh_renderer* h) {
     HIP_TRY(hipMemset(h->m_b, 0, ng * 4));
     HIP_TRY(hipMemset(h->v_b, 0, ng * 4));
     if (h->has_sh) {
-        size_t ns = 3ull * h->M * h->nc;
+        size_t ns = h->sh_total;
         HIP_TRY(dalloc(&h->m_sh, ns));
         HIP_TRY(dalloc(&h->v_sh, ns));
         HIP_TRY(hipMemset(h->m_sh, 0, ns * 4));
@@ -349,32 +517,41 @@ rmesh_status ensure_optimizer(rmesh_renderer* h) {
     return RMESH_OK;
 }
 
+// SH adjoint (+ fused Adam when adam) of every SH degree class: one launch per
+// degree present.
+void launch_sh_classes(rmesh_renderer* h, bool adam, float lr, AdamStep st, hipStream_t s) {
+    for (const ShClass& c : h->sh_classes) {
+        ShBwdArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.rec = h->rec;
+        a.idx = h->idx;
+        a.sh = h->sh + c.sh_off;
+        a.deg = c.deg;
+        a.t0 = c.t0;
+        a.M = c.n;
+        a.scenes = h->d_scenes;
+        a.n_scenes = (uint32_t)h->sc.size();
+        a.views = h->d_views;
+        a.d_colors = h->d_colors;
+        a.d_sh = h->d_sh + c.sh_off;
+        a.d_grads = h->d_grads;
+        a.d_verts = h->d_verts;
+        a.adam = adam;
+        a.m_sh = adam ? h->m_sh + c.sh_off : nullptr;
+        a.v_sh = adam ? h->v_sh + c.sh_off : nullptr;
+        a.lr = lr;
+        a.st = st;
+        launch_sh_backward(a, s);
+    }
+}
+
 // Adam stage (adam_compute.wgsl; groups and order of lib.rs:2590-2626):
 // vertices, densities, color_grads (+ base_colors / SH when enabled).
 rmesh_status stage_adam(rmesh_renderer* h, uint32_t V, float lr_v, float lr_d, float lr_g, float lr_sh) {
     hipStream_t s = h->stream;
     launch_step_inc(h->d_step, h->overflow, s);
     AdamStep st{h->d_step, h->overflow, 0.9f, 0.999f, 1e-8f};
-    if (h->has_sh) {
-        ShBwdArgs a;
-        a.rec = h->rec;
-        a.idx = h->idx;
-        a.sh = h->sh;
-        a.deg = h->sh_deg;
-        a.M = h->M;
-        a.views = h->d_views;
-        a.V = V;
-        a.d_colors = h->d_colors;
-        a.d_sh = h->d_sh;
-        a.d_grads = h->d_grads;
-        a.d_verts = h->d_verts;
-        a.adam = lr_sh != 0.0f;
-        a.m_sh = h->m_sh;
-        a.v_sh = h->v_sh;
-        a.lr = lr_sh;
-        a.st = st;
-        launch_sh_backward(a, s);
-    }
+    if (h->has_sh) launch_sh_classes(h, lr_sh != 0.0f, lr_sh, st, s);
     AdamArgs aa;
     std::memset(&aa, 0, sizeof(aa));
     aa.st = st;
@@ -385,7 +562,7 @@ rmesh_status stage_adam(rmesh_renderer* h, uint32_t V, float lr_v, float lr_d, f
     if (!h->has_sh && h->lr_base != 0.0f) aa.grp[n++] = AdamGroup{h->base, h->d_colors, h->m_b, h->v_b, 3ull * h->M, h->lr_base};
     aa.ngroups = n;
     launch_adam(aa, s);
-    launch_reduce_loss(h->loss_partial, h->T, V, h->loss_out, 1.0f / (float)(h->W * h->H), s);
+    launch_reduce_loss(h->loss_partial, h->d_views, h->n_views, h->loss_out, s);
     return RMESH_OK;
 }
 
@@ -399,23 +576,20 @@ rmesh_status record_stage(rmesh_renderer* h, int stage, uint32_t V, const float*
     hipStream_t s = h->stream;
     switch (stage) {
         case 0: {
+            if (h->scene_timing) launch_stamp_start(h->scene_clock, (uint32_t)h->sc.size(), s);
             rmesh_status st = stage_binning_front(h, V, capture);
             if (st != RMESH_OK) return st;
             zero_grads(h, V);
             return RMESH_OK;
         }
         case 1:
-            launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, V, h->counts,
-                         h->block_sums, h->keys_a, h->vals_a, h->cap, h->overflow, s);
+            launch_pair_gen(h, s);
             return RMESH_OK;
         case 2:
-            h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
-                                               h->sorting_bits(V), h->hist, s);
+            launch_sort_stage(h, s);
             return RMESH_OK;
         case 3:
-            launch_fill_u32(h->ranges, 0u, 2ull * h->gtiles(V), s);
-            launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, s);
-            launch_tile_order(h->ranges, h->gtiles(V), h->tile_order, s);
+            launch_ranges_stage(h, s);
             return RMESH_OK;
         case 4: return stage_forward(h, V, false);
         case 5: return stage_backward(h, V, nullptr, gt, loss_type);
@@ -443,12 +617,13 @@ void free_all(rmesh_renderer* h) {
                     h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a, h->vals_b, h->hist, h->ranges, h->rgba,
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
                     h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd,
-                    h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn, h->ivl_tv, h->v2t_off, h->v2t};
+                    h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn, h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin,
+                    h->d_scenes, h->d_tile_view, h->d_ktile, h->scene_clock};
     for (void* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev)
         if (e) (void)hipEventDestroy(e);
-    if (h->own_stream) hipStreamDestroy(h->own_stream);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
 }
 
 rmesh_status check_views(rmesh_renderer* h, uint32_t V) {
@@ -462,40 +637,99 @@ rmesh_status check_views(rmesh_renderer* h, uint32_t V) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer** out) {
-    if (!d || !out) return fail(RMESH_ERR_INVALID, "null argument");
+// Scenes from n descriptors (one: a plain renderer).  Tets, vertices and SH
+// are concatenated with the scenes sorted by SH degree (stable), so each
+// degree's tets are one contiguous run; the caller's scene order is kept for
+// views and images.
+static rmesh_status create_impl(const rmesh_scene_desc* descs, uint32_t n_scenes, rmesh_renderer** out) {
+    if (!descs || !out || n_scenes == 0) return fail(RMESH_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (!d->vertices || !d->indices || !d->base_colors || !d->densities || !d->color_grads || !d->circumdata)
-        return fail(RMESH_ERR_INVALID, "scene arrays must be non-null");
-    if (d->aux_dim != 0) return fail(RMESH_ERR_UNSUPPORTED, "custom aux channels (aux_dim > 0) are not supported");
-    if (d->width == 0 || d->height == 0) return fail(RMESH_ERR_INVALID, "width and height must be positive");
-    if (d->n_tets > 0xFFFFFFFFull || d->n_vertices > 0x55555555ull) return fail(RMESH_ERR_INVALID, "scene too large");
-    uint32_t ts = d->tile_size ? d->tile_size : 12;
-    if (ts != 8 && ts != 12 && ts != 16) return fail(RMESH_ERR_INVALID, "tile_size must be 8, 12 or 16");
-    if (d->sh_coeffs && d->sh_degree > 3) return fail(RMESH_ERR_INVALID, "sh_degree must be 0..3");
-    for (uint64_t i = 0; i < 4 * d->n_tets; ++i)
-        if (d->indices[i] >= d->n_vertices) return fail(RMESH_ERR_INVALID, "tet index out of range");
+    uint64_t NT = 0, MT = 0;
+    for (uint32_t u = 0; u < n_scenes; ++u) {
+        const rmesh_scene_desc* d = descs + u;
+        const std::string who = n_scenes > 1 ? "scene " + std::to_string(u) + ": " : "";
+        if (!d->vertices || !d->indices || !d->base_colors || !d->densities || !d->color_grads || !d->circumdata)
+            return fail(RMESH_ERR_INVALID, who + "scene arrays must be non-null");
+        if (d->aux_dim != 0) return fail(RMESH_ERR_UNSUPPORTED, "custom aux channels (aux_dim > 0) are not supported");
+        if (d->width == 0 || d->height == 0) return fail(RMESH_ERR_INVALID, who + "width and height must be positive");
+        if (d->n_tets > 0xFFFFFFFFull || d->n_vertices > 0x55555555ull) return fail(RMESH_ERR_INVALID, "scene too large");
+        if (d->sh_coeffs && d->sh_degree > 3) return fail(RMESH_ERR_INVALID, who + "sh_degree must be 0..3");
+        if ((d->sh_coeffs != nullptr) != (descs[0].sh_coeffs != nullptr))
+            return fail(RMESH_ERR_INVALID, "either every scene carries SH coefficients or none does");
+        if ((d->tile_size ? d->tile_size : 12) != (descs[0].tile_size ? descs[0].tile_size : 12))
+            return fail(RMESH_ERR_INVALID, "every scene of a renderer shares one tile size");
+        if (d->sh_half != descs[0].sh_half) return fail(RMESH_ERR_INVALID, "every scene shares sh_half");
+        for (uint64_t i = 0; i < 4 * d->n_tets; ++i)
+            if (d->indices[i] >= d->n_vertices) return fail(RMESH_ERR_INVALID, who + "tet index out of range");
+        NT += d->n_vertices;
+        MT += d->n_tets;
+    }
+    if (MT > 0xFFFFFFFFull || NT > 0x55555555ull) return fail(RMESH_ERR_INVALID, "scenes too large together");
+    const rmesh_scene_desc* d0 = descs;
+    uint32_t ts = d0->tile_size ? d0->tile_size : 12;
+    if (ts != 8 && ts != 12 && ts != 16 && ts != 32) return fail(RMESH_ERR_INVALID, "tile_size must be 8, 12, 16 or 32");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RMESH_ERR_DEVICE, "no HIP device available");
-    int dev = d->device_id;
-    if (dev < 0) hipGetDevice(&dev);
+    int dev = d0->device_id;
+    if (dev < 0) (void)hipGetDevice(&dev);
     if (dev >= ndev) return fail(RMESH_ERR_INVALID, "device_id out of range");
 
     rmesh_renderer* h = new rmesh_renderer();
     h->device = dev;
     DeviceGuard g(dev);
-    h->N = (uint32_t)d->n_vertices;
-    h->M = (uint32_t)d->n_tets;
-    h->W = d->width;
-    h->H = d->height;
+    h->has_sh = d0->sh_coeffs != nullptr;
+    h->sh_half = h->has_sh && d0->sh_half != 0;
+    // scene order: by SH degree, then the caller's order
+    std::vector<uint32_t> order(n_scenes);
+    for (uint32_t u = 0; u < n_scenes; ++u) order[u] = u;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return (h->has_sh ? descs[a].sh_degree : 0) < (h->has_sh ? descs[b].sh_degree : 0);
+    });
+    h->sc.resize(n_scenes);
+    h->sc_of_user.resize(n_scenes);
+    uint32_t vb = 0, tb = 0;
+    uint64_t shb = 0;
+    for (uint32_t i = 0; i < n_scenes; ++i) {
+        const rmesh_scene_desc* d = descs + order[i];
+        SceneHost& c = h->sc[i];
+        c.N = (uint32_t)d->n_vertices;
+        c.M = (uint32_t)d->n_tets;
+        c.vert_base = vb;
+        c.tet_base = tb;
+        c.W = d->width;
+        c.H = d->height;
+        c.max_views = d->max_views ? d->max_views : 1;
+        c.deg = h->has_sh ? d->sh_degree : 0;
+        c.nc = (c.deg + 1) * (c.deg + 1);
+        c.sh_base = shb;
+        c.user = order[i];
+        h->sc_of_user[order[i]] = i;
+        vb += c.N;
+        tb += c.M;
+        if (h->has_sh) shb += 3ull * c.M * c.nc;
+        if (h->has_sh && c.M) {
+            if (h->sh_classes.empty() || h->sh_classes.back().deg != c.deg)
+                h->sh_classes.push_back(ShClass{c.deg, c.tet_base, 0, c.sh_base});
+            h->sh_classes.back().n += c.M;
+        }
+        h->pix_cap += (uint64_t)c.max_views * c.W * c.H;
+        h->color_cap += h->has_sh ? (uint64_t)c.max_views * c.M : 0;
+        h->item_cap += (uint64_t)c.max_views * c.M;
+        h->tile_cap += c.max_views * (((c.W + 7) / 8) * ((c.H + 7) / 8));
+        h->view_cap += c.max_views;
+    }
+    if (!h->has_sh) h->color_cap = tb;
+    h->sh_total = shb;
+    h->N = vb;
+    h->M = tb;
+    // the single-scene entry points work on the first scene's image size
+    h->W = h->sc[h->sc_of_user[0]].W;
+    h->H = h->sc[h->sc_of_user[0]].H;
     set_tiles(h, ts);
     h->T_alloc = ((h->W + 7) / 8) * ((h->H + 7) / 8);
-    h->max_views = d->max_views ? d->max_views : 1;
-    h->has_sh = d->sh_coeffs != nullptr;
-    h->sh_half = h->has_sh && d->sh_half != 0;
-    h->sh_deg = h->has_sh ? d->sh_degree : 0;
+    h->max_views = h->sc[h->sc_of_user[0]].max_views;
+    h->sh_deg = h->sc[0].deg;
     h->nc = (h->sh_deg + 1) * (h->sh_deg + 1);
-    const uint32_t V = h->max_views;
     auto bail = [&](rmesh_status s) {
         free_all(h);
         delete h;
@@ -515,49 +749,66 @@ extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer**
     CREATE_TRY(dalloc(&h->grads, M3));
     CREATE_TRY(dalloc(&h->circ, 4 * M));
     CREATE_TRY(dalloc(&h->idx, 4 * M));
-    CREATE_TRY(hipMemcpy(h->verts, d->vertices, N3 * 4, hipMemcpyHostToDevice));
-    CREATE_TRY(hipMemcpy(h->base, d->base_colors, M3 * 4, hipMemcpyHostToDevice));
-    CREATE_TRY(hipMemcpy(h->dens, d->densities, M * 4, hipMemcpyHostToDevice));
-    CREATE_TRY(hipMemcpy(h->grads, d->color_grads, M3 * 4, hipMemcpyHostToDevice));
-    CREATE_TRY(hipMemcpy(h->circ, d->circumdata, 4 * M * 4, hipMemcpyHostToDevice));
-    CREATE_TRY(hipMemcpy(h->idx, d->indices, 4 * M * 4, hipMemcpyHostToDevice));
     if (h->has_sh) {
-        size_t ns = M3 * h->nc;
-        CREATE_TRY(dalloc(&h->sh, ns));
-        CREATE_TRY(dalloc(&h->d_sh, ns));
-        CREATE_TRY(hipMemcpy(h->sh, d->sh_coeffs, ns * 4, hipMemcpyHostToDevice));
+        CREATE_TRY(dalloc(&h->sh, h->sh_total));
+        CREATE_TRY(dalloc(&h->d_sh, h->sh_total));
+    }
+    for (const SceneHost& c : h->sc) {
+        const rmesh_scene_desc* d = descs + c.user;
+        CREATE_TRY(hipMemcpy(h->verts + 3ull * c.vert_base, d->vertices, 12ull * c.N, hipMemcpyHostToDevice));
+        CREATE_TRY(hipMemcpy(h->base + 3ull * c.tet_base, d->base_colors, 12ull * c.M, hipMemcpyHostToDevice));
+        CREATE_TRY(hipMemcpy(h->dens + c.tet_base, d->densities, 4ull * c.M, hipMemcpyHostToDevice));
+        CREATE_TRY(hipMemcpy(h->grads + 3ull * c.tet_base, d->color_grads, 12ull * c.M, hipMemcpyHostToDevice));
+        CREATE_TRY(hipMemcpy(h->circ + 4ull * c.tet_base, d->circumdata, 16ull * c.M, hipMemcpyHostToDevice));
+        if (h->has_sh)
+            CREATE_TRY(hipMemcpy(h->sh + c.sh_base, d->sh_coeffs, 12ull * c.M * c.nc, hipMemcpyHostToDevice));
+        // tet indices refer to the concatenated vertex array
+        std::vector<uint32_t> idx(d->indices, d->indices + 4ull * c.M);
+        if (c.vert_base)
+            for (uint32_t& x : idx) x += c.vert_base;
+        CREATE_TRY(hipMemcpy(h->idx + 4ull * c.tet_base, idx.data(), 16ull * c.M, hipMemcpyHostToDevice));
     }
     CREATE_TRY(dalloc(&h->d_verts, N3));
     CREATE_TRY(dalloc(&h->d_dens, M));
     CREATE_TRY(dalloc(&h->d_grads, M3));
-    CREATE_TRY(dalloc(&h->d_colors, M3 * (h->has_sh ? V : 1)));
+    CREATE_TRY(dalloc(&h->d_colors, 3 * h->color_cap));
     CREATE_TRY(dalloc(&h->rec, 4 * M));
-    CREATE_TRY(dalloc(&h->colors, M * (h->has_sh ? V : 1)));
-    CREATE_TRY(dalloc(&h->d_views, V));
-    CREATE_TRY(dalloc(&h->counts, M * V));
-    CREATE_TRY(dalloc(&h->block_sums, project_blocks(h->M, V) + 1));
-    CREATE_TRY(dalloc(&h->vis_count, V));
+    CREATE_TRY(dalloc(&h->colors, h->color_cap));
+    CREATE_TRY(dalloc(&h->d_views, h->view_cap));
+    CREATE_TRY(dalloc(&h->d_scenes, n_scenes));
+    CREATE_TRY(dalloc(&h->d_tile_view, h->tile_cap));
+    CREATE_TRY(dalloc(&h->d_ktile, h->tile_cap + h->view_cap));
+    CREATE_TRY(dalloc(&h->counts, h->item_cap));
+    CREATE_TRY(dalloc(&h->block_sums, item_blocks(h->item_cap) + 1));
+    CREATE_TRY(dalloc(&h->vis_count, 1));
     CREATE_TRY(dalloc(&h->n_pairs, 1));
     CREATE_TRY(dalloc(&h->overflow, 1));
     CREATE_TRY(dalloc(&h->d_step, 1));
     CREATE_TRY(hipMemset(h->d_step, 0, 4));
-    CREATE_TRY(dalloc(&h->ranges, 2ull * h->T_alloc * V));
-    CREATE_TRY(dalloc(&h->tile_order, (size_t)h->T_alloc * V));
-    size_t npx = (size_t)h->W * h->H;
-    CREATE_TRY(dalloc(&h->rgba, 4 * npx * V));
-    CREATE_TRY(dalloc(&h->dl, 4 * npx));
-    CREATE_TRY(dalloc(&h->gt, 3 * npx * V));
-    CREATE_TRY(dalloc(&h->loss_partial, (size_t)h->T_alloc * V));
-    CREATE_TRY(dalloc(&h->loss_out, V));
+    CREATE_TRY(dalloc(&h->ranges, 2ull * h->tile_cap));
+    CREATE_TRY(dalloc(&h->tile_order, (size_t)h->tile_cap));
+    CREATE_TRY(dalloc(&h->rgba, 4 * h->pix_cap));
+    CREATE_TRY(dalloc(&h->dl, 4ull * h->W * h->H));
+    CREATE_TRY(dalloc(&h->gt, 3 * h->pix_cap));
+    CREATE_TRY(dalloc(&h->loss_partial, (size_t)h->tile_cap));
+    CREATE_TRY(dalloc(&h->loss_out, h->view_cap));
 #undef CREATE_TRY
     *out = h;
     return RMESH_OK;
 }
 
+extern "C" rmesh_status rmesh_create(const rmesh_scene_desc* d, rmesh_renderer** out) {
+    return create_impl(d, 1, out);
+}
+
+extern "C" rmesh_status rmesh_create_multi(const rmesh_scene_desc* descs, uint32_t n_scenes, rmesh_renderer** out) {
+    return create_impl(descs, n_scenes, out);
+}
+
 extern "C" void rmesh_destroy(rmesh_renderer* h) {
     if (!h) return;
     DeviceGuard g(h->device);
-    hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->stream);
     free_all(h);
     delete h;
 }
@@ -795,27 +1046,59 @@ extern "C" rmesh_status rmesh_backward(rmesh_renderer* h, const float* dl, float
     if (st != RMESH_OK) return st;
     // colour gradient returned as d_base_colors (the per-tet colours of this view)
     HIP_TRY(hipMemcpyAsync(dB, h->d_colors, 3ull * h->M * 4, hipMemcpyDeviceToHost, s));
-    if (h->has_sh) {
-        ShBwdArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.rec = h->rec;
-        a.idx = h->idx;
-        a.sh = h->sh;
-        a.deg = h->sh_deg;
-        a.M = h->M;
-        a.views = h->d_views;
-        a.V = 1;
-        a.d_colors = h->d_colors;
-        a.d_sh = h->d_sh;
-        a.d_grads = h->d_grads;
-        a.d_verts = h->d_verts;
-        a.adam = false;
-        launch_sh_backward(a, s);
-    }
+    if (h->has_sh) launch_sh_classes(h, false, 0.0f, AdamStep{h->d_step, h->overflow, 0.9f, 0.999f, 1e-8f}, s);
     HIP_TRY(hipMemcpyAsync(dV, h->d_verts, 3ull * h->N * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(dD, h->d_dens, 1ull * h->M * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(dG, h->d_grads, 3ull * h->M * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return RMESH_OK;
+}
+
+// Device-pointer variants for callers whose tensors already live on the GPU
+// (torch): copies are device to device on the renderer's stream (set it to
+// the caller's with rmesh_set_stream), no host round trip.
+extern "C" rmesh_status rmesh_update_params_device(rmesh_renderer* h, const float* v, const float* b, const float* d,
+                                                   const float* gr) {
+    if (!h || !v || !b || !d || !gr) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemcpyAsync(h->verts, v, 12ull * h->N, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(h->base, b, 12ull * h->M, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(h->dens, d, 4ull * h->M, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(h->grads, gr, 12ull * h->M, hipMemcpyDeviceToDevice, s));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_forward_tiled_device(rmesh_renderer* h, const float* cam, const float* vp,
+                                                   const float* ci, float* d_rgba) {
+    if (!h || !cam || !vp || !ci || !d_rgba) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    rmesh_status st = forward_views(h, 1, cam, vp, ci, false);
+    if (st != RMESH_OK) return st;
+    HIP_TRY(hipMemcpyAsync(d_rgba, h->rgba, 16ull * h->W * h->H, hipMemcpyDeviceToDevice, h->stream));
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpyAsync(&ovf, h->overflow, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (ovf) return fail(RMESH_ERR_CAPACITY, "pair buffer overflow");
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_backward_device(rmesh_renderer* h, const float* d_dl, float* dV, float* dD, float* dB,
+                                              float* dG) {
+    if (!h || !d_dl || !dV || !dD || !dB || !dG) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->have_forward || h->last_views != 1) return fail(RMESH_ERR_INVALID, "backward requires a preceding forward_tiled");
+    if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
+    DeviceGuard g(h->device);
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemcpyAsync(h->dl, d_dl, 16ull * h->W * h->H, hipMemcpyDeviceToDevice, s));
+    zero_grads(h, 1);
+    rmesh_status st = stage_backward(h, 1, h->dl, nullptr, 0);
+    if (st != RMESH_OK) return st;
+    HIP_TRY(hipMemcpyAsync(dB, h->d_colors, 12ull * h->M, hipMemcpyDeviceToDevice, s));
+    if (h->has_sh) launch_sh_classes(h, false, 0.0f, AdamStep{h->d_step, h->overflow, 0.9f, 0.999f, 1e-8f}, s);
+    HIP_TRY(hipMemcpyAsync(dV, h->d_verts, 12ull * h->N, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dD, h->d_dens, 4ull * h->M, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dG, h->d_grads, 12ull * h->M, hipMemcpyDeviceToDevice, s));
     return RMESH_OK;
 }
 
@@ -830,11 +1113,11 @@ extern "C" rmesh_status rmesh_get_sh_grad(rmesh_renderer* h, float* d_sh) {
     if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
     DeviceGuard g(h->device);
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipMemcpy(d_sh, h->d_sh, 3ull * h->M * h->nc * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(d_sh, h->d_sh, 4ull * h->sh_total, hipMemcpyDeviceToHost));
     return RMESH_OK;
 }
 
-static rmesh_status finish_train(rmesh_renderer* h, uint32_t V, float* out_loss) {
+static rmesh_status finish_train(rmesh_renderer* h, uint32_t V, float* out_loss, float* out_view_losses = nullptr) {
     std::vector<float> lv(V);
     uint32_t ovf = 0;
     HIP_TRY(hipMemcpyAsync(lv.data(), h->loss_out, 4ull * V, hipMemcpyDeviceToHost, h->stream));
@@ -849,6 +1132,7 @@ static rmesh_status finish_train(rmesh_renderer* h, uint32_t V, float* out_loss)
         for (float x : lv) s += x;
         *out_loss = (float)(s / V);
     }
+    if (out_view_losses) std::memcpy(out_view_losses, lv.data(), 4ull * V);
     return RMESH_OK;
 }
 
@@ -872,6 +1156,56 @@ extern "C" rmesh_status rmesh_set_extra_lrs(rmesh_renderer* h, float lr_sh, floa
     if (!h) return fail(RMESH_ERR_INVALID, "null handle");
     h->lr_sh = lr_sh;
     h->lr_base = lr_base;
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_set_adjoint_mode(rmesh_renderer* h, int32_t mode) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    if (mode != RMESH_ADJOINT_UNDO && mode != RMESH_ADJOINT_CHECKPOINT)
+        return fail(RMESH_ERR_INVALID, "adjoint mode must be RMESH_ADJOINT_UNDO or RMESH_ADJOINT_CHECKPOINT");
+    const bool ck = mode == RMESH_ADJOINT_CHECKPOINT;
+    if (ck != h->adj_ck) {
+        DeviceGuard g(h->device);
+        drop_graph(h);
+        h->adj_ck = ck;
+        h->have_ck = false;
+        h->have_forward = false;  // the next backward needs a forward in this mode
+        if (!ck) {
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            (void)hipFree(h->ck);
+            (void)hipFree(h->fin);
+            h->ck = nullptr;
+            h->fin = nullptr;
+            h->ck_floats = 0;
+        }
+    }
+    return RMESH_OK;
+}
+
+extern "C" int32_t rmesh_get_adjoint_mode(const rmesh_renderer* h) {
+    if (!h) return -1;
+    return h->adj_ck ? RMESH_ADJOINT_CHECKPOINT : RMESH_ADJOINT_UNDO;
+}
+
+extern "C" rmesh_status rmesh_device_bytes(rmesh_renderer* h, uint64_t* out) {
+    if (!h || !out) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    const void* ptrs[] = {h->verts, h->base, h->dens, h->grads, h->sh, h->circ, h->idx, h->m_v, h->v_v, h->m_d,
+                          h->v_d, h->m_g, h->v_g, h->m_sh, h->v_sh, h->m_b, h->v_b, h->d_verts, h->d_dens,
+                          h->d_grads, h->d_colors, h->d_sh, h->rec, h->colors, h->d_views, h->counts, h->block_sums,
+                          h->vis_count, h->n_pairs, h->overflow, h->d_step, h->keys_a, h->keys_b, h->vals_a,
+                          h->vals_b, h->hist, h->ranges, h->rgba, h->aux, h->dl, h->gt, h->loss_partial,
+                          h->loss_out, h->err_pe, h->err_se, h->tet_err, h->tet_cnt, h->tile_order, h->ivr,
+                          h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd, h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn,
+                          h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin, h->d_scenes, h->d_tile_view, h->d_ktile};
+    uint64_t total = 0;
+    for (const void* p : ptrs) {
+        if (!p) continue;
+        size_t n = 0;
+        HIP_TRY(hipMemPtrGetInfo(const_cast<void*>(p), &n));
+        total += n;
+    }
+    *out = total;
     return RMESH_OK;
 }
 
@@ -903,7 +1237,7 @@ extern "C" rmesh_status rmesh_update_sh(rmesh_renderer* h, const float* sh) {
     if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
     DeviceGuard g(h->device);
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipMemcpy(h->sh, sh, 12ull * h->M * h->nc, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->sh, sh, 4ull * h->sh_total, hipMemcpyHostToDevice));
     return RMESH_OK;
 }
 
@@ -912,7 +1246,7 @@ extern "C" rmesh_status rmesh_get_sh(rmesh_renderer* h, float* sh) {
     if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
     DeviceGuard g(h->device);
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipMemcpy(sh, h->sh, 12ull * h->M * h->nc, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sh, h->sh, 4ull * h->sh_total, hipMemcpyDeviceToHost));
     return RMESH_OK;
 }
 
@@ -943,9 +1277,11 @@ extern "C" rmesh_status rmesh_set_graph_mode(rmesh_renderer* h, int enable) {
 extern "C" rmesh_status rmesh_set_tuning(rmesh_renderer* h, uint32_t tile_size, int32_t fwd_variant,
                                          int32_t bwd_variant) {
     if (!h) return fail(RMESH_ERR_INVALID, "null handle");
-    if (tile_size != 8 && tile_size != 12 && tile_size != 16) return fail(RMESH_ERR_INVALID, "tile_size must be 8, 12 or 16");
-    if (fwd_variant < 0 || fwd_variant >= kFwdVariants || bwd_variant < 0 || bwd_variant >= kBwdVariants)
-        return fail(RMESH_ERR_INVALID, "kernel variant out of range");
+    if (tile_size != 8 && tile_size != 12 && tile_size != 16 && tile_size != 32)
+        return fail(RMESH_ERR_INVALID, "tile_size must be 8, 12, 16 or 32");
+    if (fwd_variant < 0 || fwd_variant >= fwd_variants(tile_size) || bwd_variant < 0 ||
+        bwd_variant >= bwd_variants(tile_size))
+        return fail(RMESH_ERR_INVALID, "kernel variant out of range for this tile size");
     if (tile_size != h->TS) set_tiles(h, tile_size);
     if (fwd_variant != h->fwd_var || bwd_variant != h->bwd_var) drop_graph(h);
     h->fwd_var = fwd_variant;
@@ -963,20 +1299,31 @@ extern "C" rmesh_status rmesh_get_tuning(const rmesh_renderer* h, int32_t* out3)
 
 extern "C" int32_t rmesh_num_variants(int32_t which) { return which == 0 ? kFwdVariants : kBwdVariants; }
 
-// Times the binning + sort, every forward variant and every adjoint variant
-// for each tile size on the given views (eager, HIP events, best of `reps`),
-// then keeps the fastest combination.  Parameters and optimiser state are
+extern "C" rmesh_status rmesh_set_sort_digit_bits(rmesh_renderer* h, uint32_t bits) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    if (bits != 4 && bits != 8) return fail(RMESH_ERR_INVALID, "sort digit width must be 4 or 8 bits");
+    if (bits != h->sort_db) drop_graph(h);
+    h->sort_db = bits;
+    return RMESH_OK;
+}
+
+extern "C" uint32_t rmesh_get_sort_digit_bits(const rmesh_renderer* h) { return h ? h->sort_db : 0u; }
+
+// Times the binning + sort at both sort digit widths (8 and 4 bits), every
+// forward variant and every adjoint variant for each tile size (8, 12, 16,
+// 32) on the given views (eager, HIP events, best of `reps`), then keeps the
+// fastest combination.  Parameters and optimiser state are
 // untouched; the gradient buffers are cleared afterwards.
-extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps,
-                                       const float* c2ws, const float* gt_dev, uint32_t loss_type, uint32_t reps,
-                                       float* out_ms) {
-    if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+static rmesh_status autotune_layout(rmesh_renderer* h, const uint32_t* per_user, const float* cams, const float* vps,
+                                    const float* c2ws, const float* gt_dev, uint32_t loss_type, uint32_t reps,
+                                    float* out_ms) {
     if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
-    rmesh_status st = check_views(h, V);
-    if (st != RMESH_OK) return st;
     DeviceGuard g(h->device);
+    rmesh_status st = upload_layout(h, per_user, cams, vps, c2ws);
+    if (st != RMESH_OK) return st;
+    const uint32_t V = h->n_views;
     if (gt_dev && gt_dev != h->gt)
-        HIP_TRY(hipMemcpyAsync(h->gt, gt_dev, 12ull * h->W * h->H * V, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->gt, gt_dev, 12ull * h->pix_len, hipMemcpyDeviceToDevice, h->stream));
     if (reps == 0) reps = 3;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -995,24 +1342,36 @@ extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const floa
         }
         return RMESH_OK;
     };
-    const uint32_t sizes[3] = {8, 12, 16};
+    const uint32_t sizes[4] = {8, 12, 16, 32};
     float best_total = 3.0e38f, best_parts[3] = {0, 0, 0};
-    uint32_t best_ts = h->TS;
+    uint32_t best_ts = h->TS, best_db = h->sort_db;
     int best_f = 0, best_b = 0;
     rmesh_status result = RMESH_OK;
     for (uint32_t ts : sizes) {
         set_tiles(h, ts);
-        if ((result = upload_views(h, V, cams, vps, c2ws)) != RMESH_OK) break;
+        if ((result = upload_layout(h, per_user, cams, vps, c2ws)) != RMESH_OK) break;
         if ((result = stage_binning_front(h, V, false)) != RMESH_OK) break;  // sizes the pair buffers
-        float t_bin = 0.0f, t_fwd = 3.0e38f, t_bwd = 3.0e38f;
+        float t_bin = 3.0e38f, t_fwd = 3.0e38f, t_bwd = 3.0e38f;
         int bf = 0, bb = 0;
-        result = timed([&]() {
-            rmesh_status s2 = stage_binning_front(h, V, true);
-            if (s2 != RMESH_OK) return s2;
-            return stage_binning_back(h, V);
-        }, t_bin);
+        uint32_t bdb = 8;
+        for (uint32_t db : {8u, 4u}) {  // sort digit width
+            h->sort_db = db;
+            float t = 0.0f;
+            result = timed([&]() {
+                rmesh_status s2 = stage_binning_front(h, V, true);
+                if (s2 != RMESH_OK) return s2;
+                return stage_binning_back(h, V);
+            }, t);
+            if (result != RMESH_OK) break;
+            if (t < t_bin) {
+                t_bin = t;
+                bdb = db;
+            }
+        }
         if (result != RMESH_OK) break;
-        for (int f = 0; f < kFwdVariants && result == RMESH_OK; ++f) {
+        h->sort_db = bdb;
+        if ((result = stage_binning_back(h, V)) != RMESH_OK) break;  // lists of the chosen width
+        for (int f = 0; f < fwd_variants(ts) && result == RMESH_OK; ++f) {
             h->fwd_var = f;
             float t = 0.0f;
             result = timed([&]() { return stage_forward(h, V, false); }, t);
@@ -1024,7 +1383,7 @@ extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const floa
         if (result != RMESH_OK) break;
         h->fwd_var = bf;
         if ((result = stage_forward(h, V, false)) != RMESH_OK) break;  // image of the chosen variant
-        for (int b = 0; b < kBwdVariants && result == RMESH_OK; ++b) {
+        for (int b = 0; b < bwd_variants(ts) && result == RMESH_OK; ++b) {
             h->bwd_var = b;
             float t = 0.0f;
             result = timed([&]() { return stage_backward(h, V, nullptr, h->gt, loss_type); }, t);
@@ -1038,6 +1397,7 @@ extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const floa
         if (total < best_total) {
             best_total = total;
             best_ts = ts;
+            best_db = bdb;
             best_f = bf;
             best_b = bb;
             best_parts[0] = t_bin;
@@ -1048,6 +1408,7 @@ extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const floa
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     set_tiles(h, best_ts);
+    h->sort_db = best_db;
     h->fwd_var = best_f;
     h->bwd_var = best_b;
     zero_grads(h, V);
@@ -1062,26 +1423,49 @@ extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const floa
     return RMESH_OK;
 }
 
-extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps,
-                                                 const float* c2ws, const float* gt_dev, uint32_t loss_type, float lr_v,
-                                                 float lr_d, float lr_g, float lr_sh, float* out_loss) {
+extern "C" rmesh_status rmesh_autotune(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps,
+                                       const float* c2ws, const float* gt_dev, uint32_t loss_type, uint32_t reps,
+                                       float* out_ms) {
     if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
-    if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
+    if (h->multi()) return fail(RMESH_ERR_UNSUPPORTED, "a multi-scene renderer tunes with rmesh_autotune_multi");
     rmesh_status st = check_views(h, V);
     if (st != RMESH_OK) return st;
+    return autotune_layout(h, &V, cams, vps, c2ws, gt_dev, loss_type, reps, out_ms);
+}
+
+extern "C" rmesh_status rmesh_autotune_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* cams,
+                                             const float* vps, const float* c2ws, const float* gt_dev,
+                                             uint32_t loss_type, uint32_t reps, float* out_ms) {
+    if (!h || !views_per_scene || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    return autotune_layout(h, views_per_scene, cams, vps, c2ws, gt_dev, loss_type, reps, out_ms);
+}
+
+// One train step over a view layout (per_user[u] views of the caller's scene
+// u): eager, or a captured hipGraph replayed while the layout, loss type and
+// learning rates stay the same.
+static rmesh_status train_layout(rmesh_renderer* h, const uint32_t* per_user, const float* cams, const float* vps,
+                                 const float* c2ws, const float* gt_dev, uint32_t loss_type, float lr_v, float lr_d,
+                                 float lr_g, float lr_sh, float* out_loss, float* out_view_losses) {
+    if (h->sh_half) return fail(RMESH_ERR_UNSUPPORTED, "f16-rounded SH (sh_half) serves the forward only");
     DeviceGuard g(h->device);
-    st = ensure_optimizer(h);
+    rmesh_status st = ensure_optimizer(h);
     if (st != RMESH_OK) return st;
-    st = upload_views(h, V, cams, vps, c2ws);
+    st = upload_layout(h, per_user, cams, vps, c2ws);
     if (st != RMESH_OK) return st;
+    const uint32_t V = h->n_views;
     if (gt_dev && gt_dev != h->gt)
-        HIP_TRY(hipMemcpyAsync(h->gt, gt_dev, 12ull * h->W * h->H * V, hipMemcpyDeviceToDevice, h->stream));
-    float lrs[4] = {lr_v, lr_d, lr_g, lr_sh};
+        HIP_TRY(hipMemcpyAsync(h->gt, gt_dev, 12ull * h->pix_len, hipMemcpyDeviceToDevice, h->stream));
+    float lrs[5] = {lr_v, lr_d, lr_g, lr_sh, h->lr_base};
     // With timing on, every stage is its own graph and the timing events are
     // recorded between graph launches (never inside a capture).
     const int parts = h->timing ? kStages : 1;
-    bool graph_ok = h->graph_parts == parts && h->graph_views == V && h->graph_loss == loss_type &&
+    bool graph_ok = h->graph_parts == parts && h->graph_layout == h->step_views && h->graph_loss == loss_type &&
                     std::memcmp(lrs, h->graph_lrs, sizeof(lrs)) == 0;
+    if (h->graph_mode && h->cap > 0) {
+        st = ensure_checkpoints(h, V);  // host allocation, never inside a capture (drops a stale graph)
+        if (st != RMESH_OK) return st;
+        graph_ok = graph_ok && h->graph_parts == parts;
+    }
     if (h->graph_mode && h->cap > 0 && !graph_ok) {
         drop_graph(h);
         for (int p = 0; p < parts; ++p) {
@@ -1092,7 +1476,7 @@ extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, 
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(h->stream, &graph);
             if (cst != RMESH_OK || e != hipSuccess) {
-                if (graph) hipGraphDestroy(graph);
+                if (graph) (void)hipGraphDestroy(graph);
                 drop_graph(h);
                 if (cst != RMESH_OK) return cst;
                 return fail(RMESH_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -1106,6 +1490,7 @@ extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, 
         }
         h->graph_parts = parts;
         h->graph_views = V;
+        h->graph_layout = h->step_views;
         h->graph_loss = loss_type;
         std::memcpy(h->graph_lrs, lrs, sizeof(lrs));
         graph_ok = true;
@@ -1116,23 +1501,140 @@ extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, 
             HIP_TRY(hipGraphLaunch(h->graph_exec[p], h->stream));
         }
         if (h->graph_parts > 1) mark(h, kStages);
-        st = finish_train(h, V, out_loss);
+        st = finish_train(h, V, out_loss, out_view_losses);
         if (st == RMESH_ERR_CAPACITY) {
             // Parameters were left untouched on overflow: grow and redo eagerly.
             uint32_t P = 0;
             HIP_TRY(hipMemcpy(&P, h->n_pairs, 4, hipMemcpyDeviceToHost));
             st = ensure_capacity(h, P);
             if (st != RMESH_OK) return st;
+            st = ensure_checkpoints(h, V);
+            if (st != RMESH_OK) return st;
             st = record_train(h, V, h->gt, loss_type, lr_v, lr_d, lr_g, lr_sh, false);
             if (st != RMESH_OK) return st;
-            return finish_train(h, V, out_loss);
+            return finish_train(h, V, out_loss, out_view_losses);
         }
         return st;
     }
     st = record_train(h, V, h->gt, loss_type, lr_v, lr_d, lr_g, lr_sh, false);
     if (st != RMESH_OK) return st;
-    return finish_train(h, V, out_loss);
+    return finish_train(h, V, out_loss, out_view_losses);
 }
+
+extern "C" rmesh_status rmesh_train_step_batched(rmesh_renderer* h, uint32_t V, const float* cams, const float* vps,
+                                                 const float* c2ws, const float* gt_dev, uint32_t loss_type, float lr_v,
+                                                 float lr_d, float lr_g, float lr_sh, float* out_loss) {
+    if (!h || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    if (h->multi()) return fail(RMESH_ERR_UNSUPPORTED, "a multi-scene renderer trains with rmesh_train_step_multi");
+    rmesh_status st = check_views(h, V);
+    if (st != RMESH_OK) return st;
+    return train_layout(h, &V, cams, vps, c2ws, gt_dev, loss_type, lr_v, lr_d, lr_g, lr_sh, out_loss, nullptr);
+}
+
+extern "C" rmesh_status rmesh_train_step_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* cams,
+                                               const float* vps, const float* c2ws, const float* gt_dev,
+                                               uint32_t loss_type, float lr_v, float lr_d, float lr_g, float lr_sh,
+                                               float* out_view_losses, float* out_loss) {
+    if (!h || !views_per_scene || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    return train_layout(h, views_per_scene, cams, vps, c2ws, gt_dev, loss_type, lr_v, lr_d, lr_g, lr_sh, out_loss,
+                        out_view_losses);
+}
+
+extern "C" rmesh_status rmesh_forward_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* cams,
+                                            const float* vps, const float* c2ws, float* out_rgba) {
+    if (!h || !views_per_scene || !cams || !vps || !c2ws) return fail(RMESH_ERR_INVALID, "null argument");
+    DeviceGuard g(h->device);
+    rmesh_status st = upload_layout(h, views_per_scene, cams, vps, c2ws);
+    if (st != RMESH_OK) return st;
+    const uint32_t V = h->n_views;
+    if ((st = stage_binning_front(h, V, false)) != RMESH_OK) return st;
+    if ((st = stage_binning_back(h, V)) != RMESH_OK) return st;
+    if ((st = stage_forward(h, V, false)) != RMESH_OK) return st;
+    h->last_views = V;
+    h->have_forward = !h->multi();
+    if (out_rgba) HIP_TRY(hipMemcpyAsync(out_rgba, h->rgba, 16ull * h->pix_len, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpy(&ovf, h->overflow, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(RMESH_ERR_CAPACITY, "pair buffer overflow");
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_set_targets_multi(rmesh_renderer* h, const uint32_t* views_per_scene, const float* gt) {
+    if (!h || !views_per_scene || !gt) return fail(RMESH_ERR_INVALID, "null argument");
+    uint64_t px = 0;
+    for (uint32_t u = 0; u < (uint32_t)h->sc.size(); ++u) {
+        const SceneHost& sh = h->sc[h->sc_of_user[u]];
+        if (views_per_scene[u] > sh.max_views) return fail(RMESH_ERR_INVALID, "more views than max_views");
+        px += (uint64_t)views_per_scene[u] * sh.W * sh.H;
+    }
+    DeviceGuard g(h->device);
+    HIP_TRY(hipMemcpyAsync(h->gt, gt, 12ull * px, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return RMESH_OK;
+}
+
+extern "C" uint32_t rmesh_scene_count(const rmesh_renderer* h) { return h ? (uint32_t)h->sc.size() : 0u; }
+
+// Per-scene slice of a parameter or gradient buffer (RMESH_BUF_*), in the
+// scene's own indexing.
+extern "C" rmesh_status rmesh_read_scene(rmesh_renderer* h, uint32_t scene, int which, void* host, uint64_t nbytes) {
+    if (!h || !host) return fail(RMESH_ERR_INVALID, "null argument");
+    if (scene >= h->sc.size()) return fail(RMESH_ERR_INVALID, "no such scene");
+    const SceneHost& c = h->sc[h->sc_of_user[scene]];
+    uint64_t off = 0, cap = 0;  // bytes
+    switch (which) {
+        case RMESH_BUF_VERTICES: case RMESH_BUF_D_VERTICES: off = 12ull * c.vert_base; cap = 12ull * c.N; break;
+        case RMESH_BUF_DENSITIES: case RMESH_BUF_D_DENSITIES: off = 4ull * c.tet_base; cap = 4ull * c.M; break;
+        case RMESH_BUF_COLOR_GRADS: case RMESH_BUF_BASE_COLORS: case RMESH_BUF_D_COLOR_GRADS:
+            off = 12ull * c.tet_base;
+            cap = 12ull * c.M;
+            break;
+        case RMESH_BUF_SH: case RMESH_BUF_D_SH:
+            if (!h->has_sh) return fail(RMESH_ERR_INVALID, "renderer has no SH coefficients");
+            off = 4ull * c.sh_base;
+            cap = 12ull * c.M * c.nc;
+            break;
+        default: return fail(RMESH_ERR_INVALID, "buffer has no per-scene slice");
+    }
+    if (nbytes > cap) return fail(RMESH_ERR_INVALID, "read beyond the scene's slice");
+    char* src = static_cast<char*>(rmesh_device_buffer(h, which));
+    if (!src) return fail(RMESH_ERR_INVALID, "buffer not allocated");
+    DeviceGuard g(h->device);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (nbytes) HIP_TRY(hipMemcpy(host, src + off, nbytes, hipMemcpyDeviceToHost));
+    return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_set_scene_timing(rmesh_renderer* h, int enable) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    DeviceGuard g(h->device);
+    if (enable && !h->scene_clock) HIP_TRY(dalloc(&h->scene_clock, h->sc.size() + 1));
+    if ((enable != 0) != h->scene_timing) {
+        drop_graph(h);
+        h->scene_timing = enable != 0;
+    }
+    return RMESH_OK;
+}
+
+// Per scene (the caller's order): ms from the last step's start to the end of
+// the scene's last adjoint tile.
+extern "C" rmesh_status rmesh_scene_latency(rmesh_renderer* h, float* ms) {
+    if (!h || !ms) return fail(RMESH_ERR_INVALID, "null argument");
+    if (!h->scene_timing) return fail(RMESH_ERR_INVALID, "scene timing is off");
+    DeviceGuard g(h->device);
+    const size_t n = h->sc.size();
+    std::vector<uint64_t> c(n + 1);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(c.data(), h->scene_clock, 8 * (n + 1), hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
+    if (khz <= 0) return fail(RMESH_ERR_DEVICE, "no wall clock rate");
+    for (size_t u = 0; u < n; ++u) ms[u] = c[u] > c[n] ? (float)((double)(c[u] - c[n]) / (double)khz) : 0.0f;
+    return RMESH_OK;
+}
+
+
 
 extern "C" void* rmesh_device_buffer(rmesh_renderer* h, int which) {
     if (!h) return nullptr;
@@ -1147,6 +1649,8 @@ extern "C" void* rmesh_device_buffer(rmesh_renderer* h, int which) {
         case RMESH_BUF_D_VERTICES: return h->d_verts;
         case RMESH_BUF_D_DENSITIES: return h->d_dens;
         case RMESH_BUF_D_COLOR_GRADS: return h->d_grads;
+        case RMESH_BUF_D_SH: return h->d_sh;
+        case RMESH_BUF_D_BASE_COLORS: return h->d_colors;
         default: return nullptr;
     }
 }
@@ -1177,8 +1681,7 @@ extern "C" rmesh_status rmesh_debug_pairs(rmesh_renderer* h, const float* cam, c
     if (st != RMESH_OK) return st;
     st = stage_binning_front(h, 1, false);
     if (st != RMESH_OK) return st;
-    launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->M, h->d_views, 1, h->counts, h->block_sums,
-                 h->keys_a, h->vals_a, h->cap, h->overflow, h->stream);
+    launch_pair_gen(h, h->stream);
     uint32_t P = 0;
     HIP_TRY(hipMemcpyAsync(&P, h->n_pairs, 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1213,7 +1716,7 @@ extern "C" rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_
     if (n > 0xFFFFFFF0ull) return fail(RMESH_ERR_INVALID, "too many pairs");
     if (n == 0) return RMESH_OK;
     int dev = device_id;
-    if (dev < 0) hipGetDevice(&dev);
+    if (dev < 0) (void)hipGetDevice(&dev);
     DeviceGuard g(dev);
     uint32_t cap = (uint32_t)n;
     uint64_t *ka = nullptr, *kb = nullptr;
@@ -1228,25 +1731,29 @@ extern "C" rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_
             st = fail(RMESH_ERR_DEVICE, "allocation failed");
             break;
         }
-        hipMemcpy(ka, keys, n * 8, hipMemcpyHostToDevice);
-        hipMemcpy(va, vals, n * 4, hipMemcpyHostToDevice);
-        hipMemcpy(np, &cap, 4, hipMemcpyHostToDevice);
+        if (hipMemcpy(ka, keys, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(va, vals, n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(np, &cap, 4, hipMemcpyHostToDevice) != hipSuccess) {
+            st = fail(RMESH_ERR_DEVICE, "sort: upload failed");
+            break;
+        }
         bool in_b = launch_radix_sort(ka, va, kb, vb, np, cap, bits, hist, s);
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
             st = fail(RMESH_ERR_DEVICE, std::string("sort: ") + hipGetErrorString(e));
             break;
         }
-        hipMemcpy(keys, in_b ? kb : ka, n * 8, hipMemcpyDeviceToHost);
-        hipMemcpy(vals, in_b ? vb : va, n * 4, hipMemcpyDeviceToHost);
+        if (hipMemcpy(keys, in_b ? kb : ka, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(vals, in_b ? vb : va, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            st = fail(RMESH_ERR_DEVICE, "sort: download failed");
     } while (0);
-    hipFree(ka);
-    hipFree(kb);
-    hipFree(va);
-    hipFree(vb);
-    hipFree(hist);
-    hipFree(np);
-    if (s) hipStreamDestroy(s);
+    (void)hipFree(ka);
+    (void)hipFree(kb);
+    (void)hipFree(va);
+    (void)hipFree(vb);
+    (void)hipFree(hist);
+    (void)hipFree(np);
+    if (s) (void)hipStreamDestroy(s);
     return st;
 }
 
@@ -1294,14 +1801,14 @@ extern "C" rmesh_status rmesh_pair_count(rmesh_renderer* h, uint64_t* out) {
 extern "C" rmesh_status rmesh_read_buffer(rmesh_renderer* h, int which, void* host, uint64_t nbytes) {
     if (!h || !host) return fail(RMESH_ERR_INVALID, "null argument");
     uint64_t cap = 0;
-    size_t npx = (size_t)h->W * h->H;
     switch (which) {
-        case RMESH_BUF_RGBA: cap = 16ull * npx * h->max_views; break;
-        case RMESH_BUF_GT: cap = 12ull * npx * h->max_views; break;
+        case RMESH_BUF_RGBA: cap = 16ull * h->pix_cap; break;
+        case RMESH_BUF_GT: cap = 12ull * h->pix_cap; break;
         case RMESH_BUF_VERTICES: case RMESH_BUF_D_VERTICES: cap = 12ull * h->N; break;
         case RMESH_BUF_DENSITIES: case RMESH_BUF_D_DENSITIES: cap = 4ull * h->M; break;
         case RMESH_BUF_COLOR_GRADS: case RMESH_BUF_BASE_COLORS: case RMESH_BUF_D_COLOR_GRADS: cap = 12ull * h->M; break;
-        case RMESH_BUF_SH: cap = h->has_sh ? 12ull * h->M * h->nc : 0; break;
+        case RMESH_BUF_SH: case RMESH_BUF_D_SH: cap = h->has_sh ? 4ull * h->sh_total : 0; break;
+        case RMESH_BUF_D_BASE_COLORS: cap = 12ull * h->color_len; break;
         default: return fail(RMESH_ERR_INVALID, "unknown buffer");
     }
     if (nbytes > cap) return fail(RMESH_ERR_INVALID, "read beyond the buffer");
@@ -1311,6 +1818,39 @@ extern "C" rmesh_status rmesh_read_buffer(rmesh_renderer* h, int which, void* ho
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy(host, src, nbytes, hipMemcpyDeviceToHost));
     return RMESH_OK;
+}
+
+extern "C" rmesh_status rmesh_atomic_probe(int32_t device_id, uint32_t lines, uint64_t n_ops, float* ms) {
+    if (lines < 977 || lines > (1u << 24)) return fail(RMESH_ERR_INVALID, "lines must be in [977, 2^24]");
+    const uint32_t per_thread = 64;
+    const uint64_t threads = n_ops / per_thread;
+    if (threads < 256 || threads / 256 > 0xFFFFFFFFull) return fail(RMESH_ERR_INVALID, "n_ops out of range");
+    const uint32_t blocks = (uint32_t)(threads / 256);
+    int dev = device_id;
+    if (dev < 0) (void)hipGetDevice(&dev);
+    DeviceGuard g(dev);
+    float* p = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rmesh_status st = RMESH_OK;
+    if (dalloc(&p, 16ull * lines) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        st = fail(RMESH_ERR_DEVICE, "atomic probe allocation failed");
+    } else {
+        launch_fill_u32(reinterpret_cast<uint32_t*>(p), 0u, 16ull * lines, s);
+        launch_atomic_probe(p, lines, blocks, per_thread, s);  // warm
+        (void)hipEventRecord(e0, s);
+        launch_atomic_probe(p, lines, blocks, per_thread, s);
+        (void)hipEventRecord(e1, s);
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(RMESH_ERR_DEVICE, std::string("atomic probe: ") + hipGetErrorString(e));
+        else if (ms) (void)hipEventElapsedTime(ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(p);
+    return st;
 }
 
 extern "C" rmesh_status rmesh_copy_probe(int32_t device_id, uint64_t nbytes, uint32_t iters, float* ms_per_copy) {

@@ -31,10 +31,52 @@ struct alignas(16) ViewParams {
     float Wf, Hf;
     uint32_t width, height;
     uint32_t tiles_x, tiles_y;
-    uint32_t tile_base;   // first global tile id of this view
+    uint32_t tile_base;   // first global tile id of this view (ranges, compositing grids)
     uint32_t tile_size;
-    uint32_t pad[2];
+    // Sort-key tile ids.  One view: the tile itself and the reference's
+    // sentinel key ~0.  Several views in one sort: each view owns tiles_x *
+    // tiles_y + 1 key tiles, the last one its sentinel tile, so a view's
+    // sentinel pairs sort between its own pairs and the next view's, as the
+    // reference's per-view sort leaves them after its pairs.
+    uint32_t key_base;       // first key tile of this view
+    uint32_t sentinel_tile;  // key tile of this view's sentinels (0xFFFFFFFF: the reference's ~0 key)
+    uint32_t scene;          // the caller's index of this view's scene
+    // Per-view buffer offsets (several views, several scenes of uneven size and
+    // resolution in one step): this view's colours (and colour gradients) are
+    // colors[color_off + t] for the global tet ids t of its scene, and its
+    // pixels start at pix_base in the image buffers (rgba, aux, targets,
+    // image gradients, transmittance finals).
+    int64_t color_off;
+    uint64_t pix_base;
 };
+
+// One scene of a step: tets [tet_base, tet_base + M), views [view_base,
+// view_base + V); its binning items are item_base + local tet * V + local
+// view (tet-major: the reference's pair order within every view).  Scenes
+// are listed in tet order; item_base and tet_base never decrease.
+struct SceneRange {
+    uint64_t item_base;
+    uint32_t tet_base, M;
+    uint32_t view_base, V;
+};
+
+// Last scene s with key(s) <= x (binary search over the n scenes).
+template <typename F>
+__device__ __forceinline__ uint32_t scene_search(uint32_t n, F key, uint64_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key(mid) <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t scene_of_item(const SceneRange* __restrict__ sc, uint32_t n, uint64_t item) {
+    return scene_search(n, [&](uint32_t s) { return sc[s].item_base; }, item);
+}
+__device__ __forceinline__ uint32_t scene_of_tet(const SceneRange* __restrict__ sc, uint32_t n, uint32_t t) {
+    return scene_search(n, [&](uint32_t s) { return (uint64_t)sc[s].tet_base; }, t);
+}
 
 struct V3 {
     float x, y, z;

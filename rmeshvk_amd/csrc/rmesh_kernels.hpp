@@ -14,40 +14,55 @@ void launch_prep_records(const float* verts, const uint32_t* idx, const float* d
                          float4* rec, hipStream_t s);
 void launch_copy_colors(const float* base, uint32_t M, float4* col, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+// blocks x 256 lanes x per_thread float atomic adds over `lines` 64-byte lines (lines >= 977)
+void launch_atomic_probe(float* p, uint32_t lines, uint32_t blocks, uint32_t per_thread, hipStream_t s);
 void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t s);
-uint32_t project_blocks(uint32_t M, uint32_t V);  // workgroups of the (tet, view) item kernels
-void launch_project(const float4* rec, uint32_t M, const ViewParams* views, uint32_t V, uint32_t* counts,
-                    uint32_t* block_sums, uint32_t* vis_count, hipStream_t s);
-void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, hipStream_t s);
-void launch_pairs(const float4* rec, const float4* circ, uint32_t M, const ViewParams* views, uint32_t V,
-                  const uint32_t* counts, const uint32_t* block_off, uint64_t* keys, uint32_t* vals, uint32_t cap,
-                  uint32_t* overflow, hipStream_t s);
-void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges, hipStream_t s);
+uint32_t item_blocks(uint64_t n_items);  // workgroups of the (tet, view) item kernels
+void launch_project(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
+                    const ViewParams* views, uint32_t* counts, uint32_t* block_sums, uint32_t* vis_count, hipStream_t s);
+void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s);
+void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
+                  const ViewParams* views, const uint32_t* counts, const uint32_t* block_off, uint64_t* keys,
+                  uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s);
+// ranges indexed by global tile; ktile_gtile maps every key tile to its global tile (0xFFFFFFFF: sentinel)
+void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
+                   const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s);
 // Permutation of the global tiles, heaviest (most pairs) first.
 void launch_tile_order(const uint32_t* ranges, uint32_t n_tiles, uint32_t* order, hipStream_t s);
+// p[0..n) = 0, p[n] = the wall clock now (start of a step's per-scene timing)
+void launch_stamp_start(uint64_t* p, uint32_t n, hipStream_t s);
 
 // sort (k_sort.hip)
+// digit_bits: 8 (default) or 4; sorting_bits must be a multiple of it
 bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
-                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s);
+                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s, uint32_t digit_bits = 8);
 size_t radix_hist_words(uint32_t cap);
 
 // compositing (k_composite.hip)
 struct FwdArgs {
     const float4* rec;
-    const float4* colors;
-    uint64_t color_view_stride;  // in float4 units; 0 = view-independent colours
+    const float4* colors;        // view v's colours at colors[views[v].color_off + t]
     const uint32_t* sorted_vals;
     const uint32_t* ranges;
     const ViewParams* views;
-    uint32_t tiles_per_view;
+    const uint32_t* tile_view;   // global tile -> view
     uint32_t num_gtiles;
     const uint32_t* tile_order;  // nullable: workgroup -> global tile (identity when null)
     float* rgba;
     float* aux;  // nullable: [V][H][W][8]
+    // nullable: transmittance checkpoints for the checkpointed adjoint.
+    // ck[slot][TS*TS]: a pixel's log T before tile-list position 32 j (j >= 1,
+    // 32 j < the tile's pair count) at slot (range start >> 5) + global tile + j - 1;
+    // fin[V][H][W]: (final log T, stop position as the float's bits).
+    float* ck;
+    float2* fin;
 };
 // Batch-geometry variants of the compositing kernels (rmesh_autotune picks one).
 constexpr int kFwdVariants = 8;
 constexpr int kBwdVariants = 6;
+// tile 32 compiles fewer geometries (a batch must hold one tet's 1024 pixels)
+inline int fwd_variants(uint32_t tile_size) { return tile_size == 32 ? 2 : kFwdVariants; }
+inline int bwd_variants(uint32_t tile_size) { return tile_size == 32 ? 2 : kBwdVariants; }
 bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
 
 // Per-tet error statistics (forward_error) over one view's tile lists.
@@ -116,11 +131,10 @@ struct BwdArgs {
     const float4* rec;
     const uint32_t* idx;
     const float4* colors;
-    uint64_t color_view_stride;
     const uint32_t* sorted_vals;
     const uint32_t* ranges;
     const ViewParams* views;
-    uint32_t tiles_per_view;
+    const uint32_t* tile_view;
     uint32_t num_gtiles;
     const uint32_t* tile_order;  // nullable, as FwdArgs::tile_order
     const float* rgba;
@@ -131,15 +145,27 @@ struct BwdArgs {
     float* d_verts;
     float* d_dens;
     float* d_grads;
-    float* d_colors;          // [V][M*3] (SH) or [M*3] (stride 0)
-    uint64_t dcolor_view_stride;  // in tets
+    float* d_colors;          // view v's colour gradients at d_colors[3 (views[v].color_off + t)]
+    // nullable: the forward's checkpoints (FwdArgs::ck / fin); set: the
+    // checkpointed adjoint, null: the undo from the image alpha
+    const float* ck;
+    const float2* fin;
+    // nullable: per scene, the wall clock (s_memrealtime) when its last tile
+    // of this adjoint retired (atomic max over the scene's workgroups)
+    uint64_t* scene_done;
 };
+// Tile-list positions per transmittance checkpoint.
+constexpr uint32_t kCkGroup = 32;
+// Checkpoint slots for P pairs over G global tiles (each slot TS*TS floats).
+inline uint64_t checkpoint_slots(uint64_t P, uint64_t G) { return (P >> 5) + G + 1; }
 bool launch_backward(const BwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
 
 // colours, SH and optimiser (k_optim.hip)
 // half: coefficients rounded to f16 first (the reference viewer's packed SH)
-void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t M, const ViewParams* views, uint32_t V,
-                      float4* colors, bool half, hipStream_t s);
+// colours of the tets [t0, t0 + n) of one SH degree (coefficients from sh) in
+// every view of each tet's scene
+void launch_sh_colors(const float4* rec, const float* sh, uint32_t deg, uint32_t t0, uint32_t n, const SceneRange* scenes,
+                      uint32_t n_scenes, const ViewParams* views, float4* colors, bool half, hipStream_t s);
 
 // The step counter and the overflow flag live in device memory so a captured
 // graph reads the current values at replay time.
@@ -148,15 +174,19 @@ struct AdamStep {
     const uint32_t* skip;   // device: non-zero -> leave parameters untouched
     float beta1, beta2, eps;
 };
+// SH adjoint of the tets [t0, t0 + M) of one degree; sh, d_sh, m_sh, v_sh
+// point at the first of them.
 struct ShBwdArgs {
     const float4* rec;
     const uint32_t* idx;
     float* sh;               // parameters (updated in place when adam)
     uint32_t deg;
+    uint32_t t0;
     uint32_t M;
+    const SceneRange* scenes;
+    uint32_t n_scenes;
     const ViewParams* views;
-    uint32_t V;
-    const float* d_colors;   // [V][M*3]
+    const float* d_colors;   // view v's at d_colors[3 (views[v].color_off + t)]
     float* d_sh;             // output gradient (when !adam)
     float* d_grads;          // += (per tet, plain read-modify-write)
     float* d_verts;          // += (atomic)
@@ -184,7 +214,8 @@ struct AdamArgs {
 };
 void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_step_inc(uint32_t* step, const uint32_t* skip, hipStream_t s);
-void launch_reduce_loss(const float* partial, uint32_t tiles_per_view, uint32_t V, float* out, float inv_npx,
+// per view: the sum of its tiles' partial losses over its pixel count
+void launch_reduce_loss(const float* partial, const ViewParams* views, uint32_t V, float* out,
                         hipStream_t s);
 
 }  // namespace rmesh

@@ -207,6 +207,24 @@ class RMeshRenderer:
             out["d_sh"] = d_sh
         return out
 
+    # ---- device-pointer variants (torch tensors on the GPU) ----
+    def update_params_device(self, vertices, base_colors, densities, color_grads):
+        """Parameters from device pointers (integers, e.g. tensor.data_ptr()),
+        copied on the renderer's stream."""
+        _lib.check(self._lib.rmesh_update_params_device(self._h, ctypes.c_void_p(vertices), ctypes.c_void_p(base_colors),
+                                                        ctypes.c_void_p(densities), ctypes.c_void_p(color_grads)))
+
+    def forward_tiled_device(self, cam_pos, vp, c2w_intrinsics, rgba_ptr):
+        """forward_tiled into a device buffer of H*W*4 floats."""
+        c, v, ci = self._cam(cam_pos, vp, c2w_intrinsics)
+        _lib.check(self._lib.rmesh_forward_tiled_device(self._h, _p(c), _p(v), _p(ci), ctypes.c_void_p(rgba_ptr)))
+
+    def backward_device(self, dl_ptr, d_vertices, d_densities, d_base_colors, d_color_grads):
+        """backward from/to device buffers (dl: H*W*4; outputs as backward's)."""
+        _lib.check(self._lib.rmesh_backward_device(self._h, ctypes.c_void_p(dl_ptr), ctypes.c_void_p(d_vertices),
+                                                   ctypes.c_void_p(d_densities), ctypes.c_void_p(d_base_colors),
+                                                   ctypes.c_void_p(d_color_grads)))
+
     # ---- backward_hw (lib.rs:2255-2378) ----
     def backward_hw(self, dl_d_image):
         """Adjoint of the last forward() (served by the tiled forward, so the
@@ -276,24 +294,54 @@ class RMeshRenderer:
         _lib.check(self._lib.rmesh_forward_batched(self._h, len(cams), _p(cp), _p(vp), _p(ci), _p(out)))
         return out
 
-    def train_step_batched(self, cams, loss_type, lr_verts, lr_dens, lr_grads, lr_sh=0.0, gt_device_ptr=None):
+    def train_step_batched(self, cams, loss_type, lr_verts, lr_dens, lr_grads, lr_sh=0.0, gt_device_ptr=None,
+                           lr_base=0.0):
         """One step over all views.  Targets come from ``gt_device_ptr`` (device
         [V][H][W][3]) or, when None, from the renderer's own target buffer
-        (``device_buffer(BUF_GT)`` / ``set_targets``)."""
+        (``device_buffer(BUF_GT)`` / ``set_targets``).  ``lr_base`` is the
+        base-colour group's learning rate (scenes without SH; 0 leaves the
+        group out), set on every call."""
         cp, vp, ci = self._stack_cams(cams)
+        _lib.check(self._lib.rmesh_set_extra_lrs(self._h, float(lr_sh), float(lr_base)))
         loss = ctypes.c_float()
         _lib.check(self._lib.rmesh_train_step_batched(
             self._h, len(cams), _p(cp), _p(vp), _p(ci), ctypes.c_void_p(gt_device_ptr or 0), int(loss_type),
             float(lr_verts), float(lr_dens), float(lr_grads), float(lr_sh), ctypes.byref(loss)))
         return float(loss.value)
 
+    def set_adjoint_mode(self, mode: str):
+        """"undo" (default): the reference's adjoint, log T rebuilt from the
+        image alpha; "checkpoint": the forward stores log T every 32 tets per
+        pixel and the adjoint restarts from it (the exact adjoint of the
+        forward).  A backward needs a forward run in the same mode."""
+        m = {"undo": _lib.ADJOINT_UNDO, "checkpoint": _lib.ADJOINT_CHECKPOINT}.get(mode)
+        if m is None:
+            raise ValueError("adjoint mode must be 'undo' or 'checkpoint'")
+        _lib.check(self._lib.rmesh_set_adjoint_mode(self._h, m))
+
+    def get_adjoint_mode(self) -> str:
+        return "checkpoint" if self._lib.rmesh_get_adjoint_mode(self._h) == _lib.ADJOINT_CHECKPOINT else "undo"
+
+    def device_bytes(self) -> int:
+        """Device memory held by the renderer, in bytes."""
+        n = ctypes.c_uint64()
+        _lib.check(self._lib.rmesh_device_bytes(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     def set_graph_mode(self, enable: bool):
         _lib.check(self._lib.rmesh_set_graph_mode(self._h, 1 if enable else 0))
 
     # ---- tuning extension ----
     def set_tuning(self, tile_size: int, fwd_variant: int = 0, bwd_variant: int = 0):
-        """Tile size (8/12/16) and batch-geometry variants of the compositing kernels."""
+        """Tile size (8/12/16/32) and batch-geometry variants of the compositing kernels."""
         _lib.check(self._lib.rmesh_set_tuning(self._h, int(tile_size), int(fwd_variant), int(bwd_variant)))
+
+    def set_sort_digit_bits(self, bits: int):
+        """Radix-sort digit width, 8 (default) or 4 bits."""
+        _lib.check(self._lib.rmesh_set_sort_digit_bits(self._h, int(bits)))
+
+    def get_sort_digit_bits(self) -> int:
+        return int(self._lib.rmesh_get_sort_digit_bits(self._h))
 
     def get_tuning(self):
         """(tile_size, fwd_variant, bwd_variant) in use."""
@@ -314,7 +362,7 @@ class RMeshRenderer:
         ms = np.zeros(4, dtype=np.float32)
         _lib.check(self._lib.rmesh_autotune(self._h, len(cams), _p(cp), _p(vp), _p(ci),
                                             ctypes.c_void_p(gt_device_ptr or 0), int(loss_type), int(reps), _p(ms)))
-        return {"tuning": self.get_tuning(),
+        return {"tuning": self.get_tuning(), "sort_digit_bits": self.get_sort_digit_bits(),
                 "ms": dict(zip(("total", "binning_sort", "forward", "backward"), ms.tolist()))}
 
     def device_buffer(self, which: int) -> int:
@@ -352,6 +400,16 @@ class RMeshRenderer:
         return {"d_vertices": self.read_buffer(_lib.BUF_D_VERTICES, 3 * self._n),
                 "d_densities": self.read_buffer(_lib.BUF_D_DENSITIES, self._m),
                 "d_color_grads": self.read_buffer(_lib.BUF_D_COLOR_GRADS, 3 * self._m)}
+
+    def sh_gradient(self):
+        """SH-coefficient gradient [M*3*nc] left by the last backward or by a
+        train step run without an SH learning rate (with one, the fused Adam
+        consumes the gradient in registers and this buffer is not written)."""
+        if self._sh_degree is None:
+            raise ValueError("renderer has no SH coefficients")
+        out = np.empty(3 * self._m * (self._sh_degree + 1) ** 2, dtype=np.float32)
+        _lib.check(self._lib.rmesh_get_sh_grad(self._h, _p(out)))
+        return out
 
     def pair_count(self) -> int:
         n = ctypes.c_uint64()

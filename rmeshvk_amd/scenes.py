@@ -159,7 +159,7 @@ def single_tet_scene(seed: int = 42) -> dict:
 
 
 def config_scene(name: str, seed: int = 0) -> tuple[dict, list]:
-    """Scenes and cameras of the benchmark configurations A, B, C (sizes per name)."""
+    """Scenes and cameras of the benchmark configurations A, B, C, E (sizes per name)."""
     from .camera import fibonacci_cameras, make_camera
     import math
 
@@ -174,4 +174,30 @@ def config_scene(name: str, seed: int = 0) -> tuple[dict, list]:
     if name == "C":
         scene = make_scene(cluster_points(200000, rng, 8, 0.15), rng, sh_degree=3)
         return scene, fibonacci_cameras(16, 3.0, 50.0, 800, 800)
+    if name == "E":
+        # 2,000,000 points: 16 Gaussian clusters + 20 % uniform background; 4 views at 1920x1080
+        scene = make_scene(cluster_points(2000000, rng, 16, 0.15, background=0.2), rng, sh_degree=3)
+        return scene, fibonacci_cameras(4, 3.0, 50.0, 1920, 1080)
     raise ValueError(f"unknown config {name!r}")
+
+
+def config_d_scenes(seed: int = 0, n_scenes: int = 32) -> list:
+    """The uneven multi-scene batch: per scene a vertex count log-uniform in
+    [5k, 100k] (uniform in the cube or 4-8 clusters, by coin), SH degree from
+    {0, 1, 3} and a square resolution from {256, 512, 1024}, 4 Fibonacci
+    views.  Returns [(scene, cams, resolution)]."""
+    from .camera import fibonacci_cameras
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_scenes):
+        n = int(round(np.exp(rng.uniform(np.log(5000), np.log(100000)))))
+        deg = int(rng.choice([0, 1, 3]))
+        res = int(rng.choice([256, 512, 1024]))
+        if rng.random() < 0.5:
+            pts = uniform_points(n, rng)
+        else:
+            pts = cluster_points(n, rng, int(rng.integers(4, 9)), 0.15)
+        scene = make_scene(pts, rng, sh_degree=deg)
+        out.append((scene, fibonacci_cameras(4, 3.0, 50.0, res, res), res))
+    return out
