@@ -6,12 +6,17 @@ against the oracle on the same inputs:
 - forward_tiled: image within 1e-4 max-abs;
 - forward_interval_tiled: image within 1e-4 max-abs, depth and distortion
   (first four channels) within 1e-4 of each channel's maximum;
-- both adjoints, with the L2 loss gradient against a grey image: at this
-  size the float32 adjoint is itself ~1e-3 (vertices, densities) to ~1e-2
-  (colour terms) from the float64 adjoint over the same tile lists — the
-  undo reconstructs log T from the image alpha, log(1 - a), which loses
-  digits where a -> 1 — so each GPU gradient is held to the float64 oracle
-  within twice the float32 oracle's own distance from it (or 1e-3).
+- both adjoints, with the L2 loss gradient against a grey image: every GPU
+  gradient within 1e-3 relative L2 of the float32 oracle (the same
+  algorithm, the bar of the contract).  The float64 oracle over the same
+  tile lists is reported beside it: at this size the float32 algorithm is
+  itself ~1e-3 (vertices, densities) to ~1e-2 (colour terms) from float64 —
+  the undo reconstructs log T from the image alpha, log(1 - a), which loses
+  digits where a -> 1 — and the GPU is held within twice that distance.
+- the batched, graph-replayed 16-view train step (zero learning rates): its
+  images equal the per-view forward bit for bit, and its gradients equal the
+  sum of the 16 per-view train steps' gradients (each of which is the
+  per-view path checked against the oracle above) within 1e-5 relative L2.
 
 The scene build (Delaunay) and the CPU oracle in float32 and float64 take
 about two minutes.
@@ -63,10 +68,13 @@ def _f64(scene):
 
 
 def _hold(name, got, o32, o64):
-    """got within max(2 x |o32 - o64|, 1e-3) of o64 (relative L2)."""
+    """got within 1e-3 of the float32 oracle, and within max(2 x |o32 - o64|,
+    1e-3) of the float64 oracle (relative L2)."""
     e32 = _rel(o32, o64)
     eg = _rel(got, o64)
-    print(f"{name}: gpu vs f64 {eg:.2e}, f32 oracle vs f64 {e32:.2e}")
+    eo = _rel(got, o32)
+    print(f"{name}: gpu vs f32 oracle {eo:.2e}, gpu vs f64 {eg:.2e}, f32 oracle vs f64 {e32:.2e}")
+    assert eo <= 1e-3, (name, eo)
     assert eg <= max(2.0 * e32, 1e-3), (name, eg, e32)
 
 
@@ -122,3 +130,42 @@ def test_full_size_interval(config_c, renderer, reference):
                                      ts=16, colors=np.asarray(ref["colors"], np.float64), dtype=np.float64)
     for k in ("d_vertices", "d_densities", "d_base_colors", "d_color_grads"):
         _hold("interval " + k, g[k], o32[k], o64[k])
+
+
+def test_full_size_batched_graph_step(config_c):
+    """The benchmarked step itself: 16 views in one graph-replayed step equal
+    16 per-view steps (images bit for bit, gradients summed)."""
+    from rmeshvk_amd import RMeshRenderer
+    from rmeshvk_amd.camera import fibonacci_cameras
+
+    scene, _ = config_c
+    cams = fibonacci_cameras(16, 3.0, 50.0, W, W)
+    V = len(cams)
+    kw = dict(device_id=0, sh_coeffs=scene["sh"], sh_degree=int(scene["sh_degree"]), tile_size=16)
+    args = (scene["vertices"], scene["indices"], scene["base_colors"], scene["densities"], scene["color_grads"],
+            scene["circumdata"], W, W)
+    rng = np.random.default_rng(11)
+    gts = np.ascontiguousarray(rng.random((V, W, W, 3)).astype(np.float32))
+    rb = RMeshRenderer(*args, max_views=V, **kw)
+    rb.set_targets(gts)
+    rb.set_graph_mode(True)
+    loss_b = rb.train_step_batched(cams, 0, 0.0, 0.0, 0.0, 0.0)  # capture + replay
+    imgs = rb.read_buffer(0, V * W * W * 4).reshape(V, W, W, 4)
+    gb = rb.gradients()
+    gb["d_sh"] = rb.sh_gradient()
+    assert rb.step == 1
+    r1 = RMeshRenderer(*args, **kw)
+    acc = {k: np.zeros(len(v), np.float64) for k, v in gb.items()}
+    loss_sum = 0.0
+    for v in range(V):
+        loss_sum += r1.train_step(*cams[v], gts[v], 0.0, 0.0, 0.0, 0)
+        np.testing.assert_array_equal(r1.read_buffer(0, W * W * 4).reshape(W, W, 4), imgs[v])
+        g = r1.gradients()
+        g["d_sh"] = r1.sh_gradient()
+        for k in acc:
+            acc[k] += g[k]
+    assert abs(loss_b - loss_sum / V) <= 1e-5 * abs(loss_sum / V), (loss_b, loss_sum / V)  # the mean over views
+    for k in acc:
+        e = _rel(gb[k], acc[k])
+        print(f"batched {k}: vs sum of per-view steps {e:.2e}")
+        assert e <= 1e-5, (k, e)

@@ -82,7 +82,7 @@ def test_binning_and_image(name):
     assert err < IMG_TOL
 
 
-@pytest.mark.parametrize("ts", [8, 16])
+@pytest.mark.parametrize("ts", [8, 16, 32])
 def test_tile_sizes(ts, config_a):
     scene, cam, w, h = config_a
     r = _renderer(scene, w, h, tile_size=ts)
@@ -169,7 +169,8 @@ def _variant_grid():
     from rmeshvk_amd import RMeshRenderer
 
     nf, nb = RMeshRenderer.num_variants()
-    return [(ts, v, v % nb) for ts in (8, 12, 16) for v in range(max(nf, nb)) if v < nf]
+    grid = [(ts, v, v % nb) for ts in (8, 12, 16) for v in range(max(nf, nb)) if v < nf]
+    return grid + [(32, 0, 0), (32, 1, 1)]  # tile 32: two geometries each
 
 
 @pytest.mark.parametrize("ts,fv,bv", _variant_grid())
@@ -201,7 +202,8 @@ def test_autotune_keeps_parameters(config_a):
     before = r.get_params()
     res = r.autotune(cams, loss_type=0, reps=1)
     ts, fv, bv = res["tuning"]
-    assert ts in (8, 12, 16) and res["ms"]["total"] > 0
+    assert ts in (8, 12, 16, 32) and res["ms"]["total"] > 0
+    assert res["sort_digit_bits"] in (4, 8)
     after = r.get_params()
     for k in before:
         np.testing.assert_array_equal(before[k], after[k])
@@ -214,7 +216,11 @@ def test_autotune_keeps_parameters(config_a):
 def test_batched_binning_at_scale():
     """~300K tets x 4 views at 512^2 (> 4096 scan blocks of (tet, view) items):
     the batched sort holds, view by view, exactly the oracle's sorted pairs
-    (global tile = view * T + tile), and the ranges match their closed form."""
+    (key tile = view * (T + 1) + tile; tile T of each view is its sentinel
+    tile), every view's sentinels sort between its pairs and the next view's,
+    and each view's ranges are the oracle's per-view ranges shifted by the
+    view's offset — including a tile followed by its view's sentinels, which
+    keeps end = 0 as in the reference's per-view sort."""
     from rmeshvk_amd import RMeshRenderer
 
     rng = np.random.default_rng(21)
@@ -226,19 +232,58 @@ def test_batched_binning_at_scale():
     r.forward_batched(cams, copy_out=False)
     sk, sv, rg = r.debug_sorted(views=len(cams))
     _, _, T = orc.tile_grid(w, h, 12)
+    KT = T + 1
     M = len(scene["indices"]) // 4
     assert (M * len(cams) + 255) // 256 > 4096
-    tile = (sk >> np.uint64(32)).astype(np.int64)
-    real = tile != 0xFFFFFFFF
+    kt = (sk >> np.uint64(32)).astype(np.int64)
+    v_of, t_of = kt // KT, kt % KT
+    assert np.all(np.diff(kt) >= 0)
     for v, cam in enumerate(cams):
         tt, _ = orc.project(scene, cam, w, h, 12)
         keys, vals, P = orc.pairs(scene, cam, w, h, tt, 12)
         ok, ov = orc.sort(keys, vals, orc.sorting_bits(T))
-        sel = real & (tile >= v * T) & (tile < (v + 1) * T)
-        np.testing.assert_array_equal(sk[sel] - (np.uint64(v * T) << np.uint64(32)), ok[(ok >> np.uint64(32)) != 0xFFFFFFFF])
-        np.testing.assert_array_equal(sv[sel], ov[(ok >> np.uint64(32)) != 0xFFFFFFFF])
-    # ranges: closed form over the real pairs
-    t_real = tile[real]
-    for gtile in np.unique(t_real)[::97]:
-        lo = np.searchsorted(t_real, gtile, "left")
-        assert rg[2 * gtile] == lo
+        o_real = (ok >> np.uint64(32)) != 0xFFFFFFFF
+        mine = v_of == v
+        sel = mine & (t_of < T)
+        np.testing.assert_array_equal(sk[sel] - (np.uint64(v * KT) << np.uint64(32)), ok[o_real])
+        np.testing.assert_array_equal(sv[sel], ov[o_real])
+        assert int((mine & (t_of == T)).sum()) == int((~o_real).sum())  # this view's sentinels
+        base = int(np.argmax(mine))
+        rv = orc.ranges(ok, T).reshape(T, 2).astype(np.int64)
+        counts = np.bincount((ok[o_real] >> np.uint64(32)).astype(np.int64), minlength=T)
+        exp = np.zeros((T, 2), np.int64)
+        ne = counts > 0
+        exp[ne, 0] = rv[ne, 0] + base
+        exp[ne, 1] = np.where(rv[ne, 1] != 0, rv[ne, 1] + base, 0)
+        np.testing.assert_array_equal(rg[2 * v * T:2 * (v + 1) * T].reshape(T, 2), exp)
+
+
+@pytest.mark.parametrize("ts", [12, 32])
+def test_sort_digit_width(ts):
+    """4-bit digits (twice the passes) give the 8-bit sort's lists bit for bit,
+    one view (the oracle's) and four views in one sort."""
+    rng = np.random.default_rng(77)
+    scene = make_scene(cluster_points(6000, rng, 4, 0.2), rng)
+    w, h = 160, 128
+    cams = fibonacci_cameras(4, 3.0, 50.0, w, h)
+    out = {}
+    for bits in (8, 4):
+        r = _renderer(scene, w, h, tile_size=ts, max_views=4)
+        r.set_sort_digit_bits(bits)
+        assert r.get_sort_digit_bits() == bits
+        img = r.forward_tiled(*cams[0])
+        one = r.debug_sorted()
+        imgs = r.forward_batched(cams)
+        out[bits] = (img, one, r.debug_sorted(views=4), imgs)
+    ref = orc.render(scene, cams[0], w, h, ts=ts)
+    for bits in (8, 4):
+        img, (sk, sv, rg), _, _ = out[bits]
+        np.testing.assert_array_equal(sk, ref["sorted_keys"])
+        np.testing.assert_array_equal(sv, ref["sorted_vals"])
+        np.testing.assert_array_equal(rg, ref["ranges"])
+        assert np.abs(img - ref["rgba"]).max() < IMG_TOL
+    for a, b in zip(out[8][2], out[4][2]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(out[8][3], out[4][3])
+    with pytest.raises(ValueError):
+        _renderer(scene, w, h).set_sort_digit_bits(5)

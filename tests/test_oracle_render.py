@@ -9,6 +9,7 @@ itself is exercised by the GPU-vs-oracle parity tests.
 
 from __future__ import annotations
 
+import itertools
 import math
 
 import numpy as np
@@ -163,6 +164,68 @@ def test_binning_covers_every_hit_pixel():
         tiles = (pix // w // ts) * tx + (pix % w) // ts
         missing = [(a, b) for a, b in zip(tiles.tolist(), tet.tolist()) if (a, b) not in binned]
         assert not missing, missing[:10]
+
+
+def _hull_meets_rect(pts, x0, y0, x1, y1, eps):
+    """Separating-axis test between the convex hull of 2-D points and the
+    rectangle [x0, x1] x [y0, y1] grown by eps (float64)."""
+    if pts[:, 0].max() < x0 - eps or pts[:, 0].min() > x1 + eps:
+        return False
+    if pts[:, 1].max() < y0 - eps or pts[:, 1].min() > y1 + eps:
+        return False
+    corners = np.array([[x0 - eps, y0 - eps], [x1 + eps, y0 - eps], [x1 + eps, y1 + eps], [x0 - eps, y1 + eps]])
+    for i, j in itertools.combinations(range(len(pts)), 2):
+        e = pts[j] - pts[i]
+        n = np.array([-e[1], e[0]])
+        if not n.any():
+            continue
+        side = (pts - pts[i]) @ n
+        if side.max() <= 0.0 or side.min() >= 0.0:  # a hull edge: every point on one side
+            s_out = 1.0 if side.max() <= 0.0 else -1.0
+            if np.all(s_out * ((corners - pts[i]) @ n) > 0.0):
+                return False
+    return True
+
+
+def test_binning_is_tight():
+    """Every binned (tile, tet) pair's tile meets the tet's projected hull: the
+    hull of its four vertices' float64 projections, tested against the tile's
+    rectangle by separating axes (independent of the scanline), within 1e-3
+    px for float32 rounding.  Over-binning (a tile past the hull's extent in
+    its row band) fails it.  The one exception is the reference's own: a row
+    band whose hull extent lies left of the image (or right of it) is clamped
+    into the edge column (tile_gen_scan_compute.wgsl:184-185,
+    clamp(max(floor(x / T), 0), tile_min_x, tile_max_x)), so the edge
+    columns' rectangles reach past the image on that side."""
+    for seed, (w, h, ts) in ((3, (72, 60, 12)), (4, (64, 64, 8)), (5, (50, 70, 16))):
+        s = _positive_colour_scene(80, seed)
+        V = np.asarray(s["vertices"], np.float64).reshape(-1, 3)
+        tets = np.asarray(s["indices"], np.int64).reshape(-1, 4)
+        tx = (w + ts - 1) // ts
+        for cam in fibonacci_cameras(3, 3.0, 50.0, w, h):
+            tt, _ = orc.project(s, cam, w, h, ts)
+            keys, vals, P = orc.pairs(s, cam, w, h, tt, ts)
+            vp = np.asarray(cam[1], np.float64).reshape(4, 4).T  # column-major
+            clip = np.c_[V, np.ones(len(V))] @ vp.T
+            ndc = clip[:, :3] / (clip[:, 3:4] + 1e-6)
+            proj = np.c_[(ndc[:, 0] + 1.0) * 0.5 * w, (1.0 - ndc[:, 1]) * 0.5 * h]
+            tiles = (keys >> np.uint64(32)).astype(np.int64)
+            real = keys != np.uint64(0xFFFFFFFFFFFFFFFF)
+            assert real.sum() > 50
+            for tile, t in zip(tiles[real].tolist(), vals[real].astype(np.int64).tolist()):
+                c, y0 = tile % tx, (tile // tx) * ts
+                x0 = -np.inf if c == 0 else c * ts
+                x1 = np.inf if c == tx - 1 else (c + 1) * ts
+                assert _hull_meets_rect(proj[tets[t]], x0, y0, x1, y0 + ts, 1e-3), (seed, tile, t)
+
+
+def test_hull_rect_test_detects_misses():
+    """The separating-axis check above rejects a rectangle beside a triangle's
+    hypotenuse (inside its bounding box) and accepts one across it."""
+    tri = np.array([[0.0, 0.0], [10.0, 0.0], [0.0, 10.0], [0.0, 0.0]])
+    assert not _hull_meets_rect(tri, 7.0, 7.0, 9.0, 9.0, 1e-3)
+    assert _hull_meets_rect(tri, 4.0, 4.0, 6.0, 6.0, 1e-3)
+    assert _hull_meets_rect(tri, 5.0, 5.0 - 1e-4, 6.0, 6.0, 1e-3)  # touches (5, 5) on the hypotenuse
 
 
 def test_keys_decode_to_tile_and_depth():
