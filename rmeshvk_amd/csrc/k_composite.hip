@@ -697,6 +697,9 @@ struct Integral {
     float cs[3], ce[3], csr[3], cer[3];
     float cp[3];
 };
+// EXTRA = false: bo_extra is 0 and not added (x + 0 only differs from x in
+// the sign of a zero, which no use of b sees).
+template <bool EXTRA = true>
 __device__ __forceinline__ void integral(const TetRegs& T, V3 rd, float t_min, float t_max, float bo_extra, float clip_lo,
                                          Integral& I) {
 #pragma clang fp contract(fast)
@@ -704,7 +707,7 @@ __device__ __forceinline__ void integral(const TetRegs& T, V3 rd, float t_min, f
     float a0 = dc_dt * t_min, a1 = dc_dt * t_max;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        float b = T.bc[c] + bo_extra;
+        float b = EXTRA ? T.bc[c] + bo_extra : T.bc[c];
         I.csr[c] = b + a0;
         I.cer[c] = b + a1;
         I.cs[c] = relu(I.csr[c]);
@@ -1086,6 +1089,9 @@ __device__ __forceinline__ void face_slot_weights(int fi, float la, float lb, fl
     w[3] = f0 ? 0.0f : (f1 ? lc : (f2 ? lb : la));
 }
 
+// fused dot product (the adjoint's own terms; the forward keeps dot3)
+__device__ __forceinline__ float fdot3(V3 a, V3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+
 // Gradient terms of one covered (pixel, tet) pair, accumulated into acc
 // (backward_tiled_compute.wgsl:446-543, per-pixel clipping as there).  The
 // intersection derivatives use dt/dva = ((vc - hit) x (vb - hit)) / den (and
@@ -1096,24 +1102,33 @@ __device__ __forceinline__ void face_slot_weights(int fi, float la, float lb, fl
 // carries their gradients and their values at v0 (store_tet) and a pair
 // evaluates them with three dot products; they are scattered to the face's
 // vertex slots.
-template <int TS, int B>
+//
+// The reference clips every vertex term, each side's term and the colour-
+// gradient term to +-1e20 per pixel.  A pair whose terms are bounded below
+// 1e19 in magnitude (|G| max-norm x the largest weight per side, plus
+// |g| |dbo|; |cam - v0| |dbo| + |rd| |d_dcdt| for dg) cannot reach a clip,
+// so it accumulates n (k w) with fused multiply-adds and no clips; any other
+// pair (and NaN) takes the clipped path, term by term as the reference.
+// FIRST: acc holds nothing yet (one pair per lane): terms are assigned, not
+// added to zeros.
+template <int TS, int B, bool FIRST>
 __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k, const TetRegs& T, V3 v0, V3 cam,
                                           V3 rd, float dc0, float dc1, float dc2, float t_prev, float d_log_t_b,
                                           float t_min, float t_max, int min_face, int max_face, float* acc) {
 #pragma clang fp contract(fast)
     Integral I;
-    integral(T, rd, t_min, t_max, 0.0f, kMinVal, I);
+    integral<false>(T, rd, t_min, t_max, 0.0f, kMinVal, I);
     const float dist = t_max - t_min;
     const float dens = T.dens;
-    const float dc_dt = dot3(T.g, rd);
+    const float dc_dt = fdot3(T.g, rd);
     float dcp0 = med_clip(dc0 * t_prev, kMinVal, kMaxVal);
     float dcp1 = med_clip(dc1 * t_prev, kMinVal, kMaxVal);
     float dcp2 = med_clip(dc2 * t_prev, kMinVal, kMaxVal);
     float d_od_state = -d_log_t_b;
     float dphi = fast_dphi(I.od, I.alpha_t);
     float dw0 = dphi + I.alpha_t, dw1 = -dphi;
-    float d_od_int = (dcp0 * (I.ce[0] * dw0 + I.cs[0] * dw1) + dcp1 * (I.ce[1] * dw0 + I.cs[1] * dw1)) +
-                     dcp2 * (I.ce[2] * dw0 + I.cs[2] * dw1);
+    float d_od_int = fmaf(dcp0, fmaf(I.ce[0], dw0, I.cs[0] * dw1),
+                          fmaf(dcp1, fmaf(I.ce[1], dw0, I.cs[1] * dw1), dcp2 * fmaf(I.ce[2], dw0, I.cs[2] * dw1)));
     float d_od = med_clip(d_od_state + d_od_int, kMinVal, kMaxVal);
     float od_raw = dens * dist;
     bool act = (od_raw >= 1e-8f) && (od_raw <= 88.0f);
@@ -1127,14 +1142,15 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
     float dce2 = I.cer[2] > 0.f ? dcp2 * I.w0 : 0.f;
     float dbc0 = dcs0 + dce0, dbc1 = dcs1 + dce1, dbc2 = dcs2 + dce2;
     float sum_s = (dcs0 + dcs1) + dcs2, sum_e = (dce0 + dce1) + dce2;
-    float d_dcdt = sum_s * t_min + sum_e * t_max;
-    d_tmin = d_tmin + sum_s * dc_dt;
-    d_tmax = d_tmax + sum_e * dc_dt;
+    float d_dcdt = fmaf(sum_s, t_min, sum_e * t_max);
+    d_tmin = fmaf(sum_s, dc_dt, d_tmin);
+    d_tmax = fmaf(sum_e, dc_dt, d_tmax);
     float dbo = (dbc0 + dbc1) + dbc2;
     const V3 cam_v0 = sub3(cam, v0);
-    V3 dg = add3(scl3(cam_v0, dbo), scl3(rd, d_dcdt));
     float wv[2][4];   // per side: barycentric weight of each vertex slot
-    V3 G[2];          // per side: n * (dt / den)
+    V3 nn[2];         // per side: raw face normal
+    float kd[2];      // per side: dt / den
+    float bound = 0.0f;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const int fi = side == 0 ? min_face : max_face;
@@ -1145,33 +1161,63 @@ __device__ __forceinline__ void pair_grad(const BatchLDS<TS, B, true>& L, int k,
         const float4 nq = L.tr[k * L.TR + fi];
         const V3 ga = mk3(f0.x, f0.y, f0.z), gb = mk3(f1.x, f1.y, f1.z);
         const float sg = L.face[k * L.FS + 32 + fi];
-        const V3 nn = mk3(nq.x * sg, nq.y * sg, nq.z * sg);  // raw normal
-        const float inv_den = fast_safe_div(1.0f, dot3(nn, rd));
-        const V3 h = add3(cam_v0, scl3(rd, tt));  // hit - v0
-        const float la = f0.w + dot3(ga, h);
-        const float lb = f1.w + dot3(gb, h);
+        nn[side] = mk3(nq.x * sg, nq.y * sg, nq.z * sg);
+        const float inv_den = fast_safe_div(1.0f, fdot3(nn[side], rd));
+        const V3 h = mk3(fmaf(rd.x, tt, cam_v0.x), fmaf(rd.y, tt, cam_v0.y), fmaf(rd.z, tt, cam_v0.z));  // hit - v0
+        const float la = f0.w + fdot3(ga, h);
+        const float lb = f1.w + fdot3(gb, h);
         const float lc = (1.0f - la) - lb;
-        G[side] = scl3(nn, dt * inv_den);
+        kd[side] = dt * inv_den;
         face_slot_weights(fi, la, lb, lc, wv[side]);
+        const float nmax = fmaxf(fmaxf(fabsf(nq.x), fabsf(nq.y)), fabsf(nq.z));
+        const float wmax = fmaxf(fmaxf(fabsf(la), fabsf(lb)), fabsf(lc));
+        bound = fmaf(fabsf(kd[side]) * nmax, wmax, bound);
     }
-    acc[0] += med_clip(d_dens_l, kMinVal, kMaxVal);
+    const float gmax = fmaxf(fmaxf(fabsf(T.g.x), fabsf(T.g.y)), fabsf(T.g.z));
+    const float cmax = fmaxf(fmaxf(fabsf(cam_v0.x), fabsf(cam_v0.y)), fabsf(cam_v0.z));
+    const float rmax = fmaxf(fmaxf(fabsf(rd.x), fabsf(rd.y)), fabsf(rd.z));
+    bound = fmaf(gmax, fabsf(dbo), bound);
+    const float bound_g = fmaf(cmax, fabsf(dbo), rmax * fabsf(d_dcdt));
+    auto put = [&](int q, float x) { acc[q] = FIRST ? x : acc[q] + x; };
+    // x1 k1 + x0 k0 (+ acc)
+    auto put2 = [&](int q, float x0, float k0, float x1, float k1) {
+        acc[q] = FIRST ? fmaf(x1, k1, x0 * k0) : fmaf(x1, k1, fmaf(x0, k0, acc[q]));
+    };
+    put(0, med_clip(d_dens_l, kMinVal, kMaxVal));
+    if (bound <= 1e19f && bound_g <= 1e19f) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        V3 d = add3(med_clip3(scl3(G[0], wv[0][s]), kMinVal, kMaxVal), med_clip3(scl3(G[1], wv[1][s]), kMinVal, kMaxVal));
-        if (s == 0) d = add3(d, scl3(neg3(T.g), dbo));
-        d = med_clip3(d, kMinVal, kMaxVal);
-        acc[1 + 3 * s] += d.x;
-        acc[2 + 3 * s] += d.y;
-        acc[3 + 3 * s] += d.z;
+        for (int s = 0; s < 4; ++s) {
+            const float k0 = kd[0] * wv[0][s], k1 = kd[1] * wv[1][s];
+            put2(1 + 3 * s, nn[0].x, k0, nn[1].x, k1);
+            put2(2 + 3 * s, nn[0].y, k0, nn[1].y, k1);
+            put2(3 + 3 * s, nn[0].z, k0, nn[1].z, k1);
+        }
+        acc[1] = fmaf(-T.g.x, dbo, acc[1]);
+        acc[2] = fmaf(-T.g.y, dbo, acc[2]);
+        acc[3] = fmaf(-T.g.z, dbo, acc[3]);
+        put2(13, rd.x, d_dcdt, cam_v0.x, dbo);
+        put2(14, rd.y, d_dcdt, cam_v0.y, dbo);
+        put2(15, rd.z, d_dcdt, cam_v0.z, dbo);
+    } else {
+        const V3 G0 = scl3(nn[0], kd[0]), G1 = scl3(nn[1], kd[1]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            V3 d = add3(med_clip3(scl3(G0, wv[0][s]), kMinVal, kMaxVal), med_clip3(scl3(G1, wv[1][s]), kMinVal, kMaxVal));
+            if (s == 0) d = add3(d, scl3(neg3(T.g), dbo));
+            d = med_clip3(d, kMinVal, kMaxVal);
+            put(1 + 3 * s, d.x);
+            put(2 + 3 * s, d.y);
+            put(3 + 3 * s, d.z);
+        }
+        V3 dgc = med_clip3(add3(scl3(cam_v0, dbo), scl3(rd, d_dcdt)), kMinVal, kMaxVal);
+        put(13, dgc.x);
+        put(14, dgc.y);
+        put(15, dgc.z);
     }
-    V3 dgc = med_clip3(dg, kMinVal, kMaxVal);
-    acc[13] += dgc.x;
-    acc[14] += dgc.y;
-    acc[15] += dgc.z;
     V3 dbcc = med_clip3(mk3(dbc0, dbc1, dbc2), kMinVal, kMaxVal);
-    acc[16] += dbcc.x;
-    acc[17] += dbcc.y;
-    acc[18] += dbcc.z;
+    put(16, dbcc.x);
+    put(17, dbcc.y);
+    put(18, dbcc.z);
 }
 
 // CK: the checkpointed adjoint.  The undo restarts at every 32-position group
@@ -1357,7 +1403,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                         // the reference's adjoint defaults both faces to 0 (backward_tiled_compute.wgsl:365-366)
                         if (fmin == 4) fmin = 0;
                         Integral I;
-                        integral(T, rd, t_min, t_max, 0.0f, kMinVal, I);
+                        integral<false>(T, rd, t_min, t_max, 0.0f, kMinVal, I);
                         s_r[0][slot] = I.od;
                         s_r[1][slot] = (s_dc[0][p] * I.cp[0] + s_dc[1][p] * I.cp[1]) + s_dc[2][p] * I.cp[2];
                         s_r[2][slot] = t_min;
@@ -1415,8 +1461,11 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         for (uint32_t base = (uint32_t)wid * 64; base < nchunks; base += NT) {
             const uint32_t j = base + lane;
             float acc[kTerms];
+            if constexpr (R != 1) {
 #pragma unroll
-            for (int q = 0; q < kTerms; ++q) acc[q] = 0.0f;
+                for (int q = 0; q < kTerms; ++q) acc[q] = 0.0f;
+            }
+            bool have = false;  // R = 1: acc assigned by the lane's one pair
             int key = 0x7fffffff;
             if (j < nchunks) {
                 ChunkRef c;
@@ -1432,9 +1481,10 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                     const uint32_t meta = s_meta[slot];
                     if (meta & 32u) {
                         const V3 rd = mk3(s_ray[0][p], s_ray[1][p], s_ray[2][p]);
-                        pair_grad<TS, B>(L, k, T, v0, cam, rd, s_dc[0][p], s_dc[1][p], s_dc[2][p], s_r[0][slot],
-                                         s_r[1][slot], s_r[2][slot], s_r[3][slot], (int)(meta & 3u),
-                                         (int)((meta >> 2) & 3u), acc);
+                        pair_grad<TS, B, R == 1>(L, k, T, v0, cam, rd, s_dc[0][p], s_dc[1][p], s_dc[2][p],
+                                                 s_r[0][slot], s_r[1][slot], s_r[2][slot], s_r[3][slot],
+                                                 (int)(meta & 3u), (int)((meta >> 2) & 3u), acc);
+                        have = true;
                     }
                 };
                 if constexpr (R == 1) {
@@ -1446,6 +1496,12 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                         one(c.slot0 + q, w.row * TS + w.col);
                         w.next(L, k);
                     }
+                }
+            }
+            if constexpr (R == 1) {
+                if (!have) {
+#pragma unroll
+                    for (int q = 0; q < kTerms; ++q) acc[q] = 0.0f;
                 }
             }
             if (C::SCAN >= 2) {
@@ -1460,9 +1516,12 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                 } else {
                     seg_scan_dpp<kTerms>(acc, sm);
                 }
+                // no zero test per term: a tail's adds are one LDS atomic per
+                // term for the few tail lanes, cheaper than 19 compares and branches
+                if (tail) {
 #pragma unroll
-                for (int q = 0; q < kTerms; ++q)
-                    if (tail && acc[q] != 0.0f) atomicAdd(&s_acc[key][q], acc[q]);
+                    for (int q = 0; q < kTerms; ++q) atomicAdd(&s_acc[key][q], acc[q]);
+                }
             } else if (C::SCAN == 1) {
                 const SegMasks sm = seg_masks(key);
                 const int next_key = __shfl_down(key, 1, 64);
