@@ -276,6 +276,18 @@ rmesh_status rmesh_debug_sorted(rmesh_renderer* h, uint64_t* keys, uint32_t* val
 /* Standalone sort of host (key, value) pairs on the GPU with the pipeline's
  * radix sort: stable on the low `sorting_bits` bits (multiple of 8). */
 rmesh_status rmesh_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t sorting_bits, int32_t device_id);
+/* rmesh_sort_pairs without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under hipGraph
+ * capture, serving loops, harnesses that forbid hipMalloc).  keys/vals are
+ * DEVICE pointers, sorted in place (stable on the low `sorting_bits` bits,
+ * multiple of 8) by the same kernels; the call only enqueues work on
+ * `hip_stream` (NULL: the null stream) of device `device_id` (-1: current)
+ * and returns.  Scratch is a fixed per-device array of the library (14 MB,
+ * held until the process ends): n above 2^20 pairs returns
+ * RMESH_ERR_INVALID.  Calls that overlap on one GPU share that array and
+ * race; serialise them (one stream, or events). */
+rmesh_status rmesh_sort_pairs_noalloc(uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t sorting_bits,
+                                      int32_t device_id, void* hip_stream);
 /* Bits sorted for a given tile count (crates/rmesh-sort/src/drs.rs:23-26). */
 uint32_t rmesh_sorting_bits(uint32_t num_tiles);
 

@@ -100,6 +100,7 @@ SIGNATURES = {
     "rmesh_debug_pairs": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]),
     "rmesh_debug_sorted": (_st, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp, _u64]),
     "rmesh_sort_pairs": (_st, [_vp, _vp, _u64, _u32, _i32]),
+    "rmesh_sort_pairs_noalloc": (_st, [_vp, _vp, _u64, _u32, _i32, _vp]),
     "rmesh_set_adjoint_mode": (_st, [_vp, _i32]),
     "rmesh_update_params_device": (_st, [_vp, _vp, _vp, _vp, _vp]),
     "rmesh_forward_tiled_device": (_st, [_vp, _vp, _vp, _vp, _vp]),
