@@ -154,18 +154,31 @@ def test_batched_gradients_sum_views(small_sh):
 
 
 def test_batched_sh_adam_matches_oracle(small_sh):
-    """The SH adjoint's fused Adam: a step with an SH learning rate moves every
-    coefficient exactly as the oracle's Adam step on the view-summed SH
-    gradient that the same step computes without one (checked against the
-    oracle's SH adjoint at 1e-3)."""
+    """The SH adjoint's fused Adam: a step with an SH learning rate moves the
+    coefficients as the oracle's Adam step on the view-summed SH gradient that
+    the same step computes without one (checked against the oracle's SH
+    adjoint at 1e-3).
+
+    The fused step cannot hand out its own gradient, and two runs of the
+    adjoint differ in the order of their float atomics, so the reference
+    gradient carries run-to-run noise.  Two gradient-only runs bound it per
+    coefficient (their difference, floored at 1e-6 of the gradient); the
+    tolerance of each move adds the Adam step's sensitivity to that noise,
+    lr eps 4 noise / (|g| - 4 noise + eps)^2 (step 1, zero moments: the move is
+    lr g / (|g| + eps)).  Coefficients whose gradient lies within 4 noise of 0
+    (sign not determined) only have to move by at most lr; they are asserted
+    to be under 1 % of the group."""
     scene, cams, w, h = small_sh
     V = len(cams)
     gts = np.ascontiguousarray(np.random.default_rng(4).random((V, h, w, 3)).astype(np.float32))
-    lr_sh = 2.5e-3
-    ra = _renderer(scene, w, h, max_views=V)
-    ra.set_targets(gts)
-    ra.train_step_batched(cams, 1, 0.0, 0.0, 0.0, 0.0)  # gradients only: d_sh written
-    d_sh_gpu = ra.sh_gradient()
+    lr_sh, eps = 2.5e-3, 1e-8
+    runs = []
+    for _ in range(2):
+        ra = _renderer(scene, w, h, max_views=V)
+        ra.set_targets(gts)
+        ra.train_step_batched(cams, 1, 0.0, 0.0, 0.0, 0.0)  # gradients only: d_sh written
+        runs.append(ra.sh_gradient().astype(np.float64))
+    d_sh_gpu = runs[0]
     d_sh = np.zeros(len(scene["sh"]), np.float64)
     for v in range(V):
         d_sh += orc.render_backward(scene, cams[v], w, h, gts[v], 1)[3]["d_sh"]
@@ -173,7 +186,21 @@ def test_batched_sh_adam_matches_oracle(small_sh):
     rb = _renderer(scene, w, h, max_views=V)
     rb.set_targets(gts)
     rb.train_step_batched(cams, 1, 0.0, 0.0, 0.0, lr_sh)
-    _adam_exact("sh_coeffs", rb.get_params()["sh_coeffs"], scene["sh"], d_sh_gpu, lr_sh)
+    got = rb.get_params()["sh_coeffs"].astype(np.float64)
+    p0 = scene["sh"]
+    z = np.zeros_like(p0)
+    p_ref, _, _ = orc.adam(p0, d_sh_gpu.astype(np.float32), z, z, 1, lr_sh)
+    ag = np.abs(d_sh_gpu)
+    noise = np.abs(runs[0] - runs[1]) + 1e-6 * ag
+    open_sign = ag <= 4.0 * noise
+    assert open_sign.mean() < 0.01, open_sign.mean()
+    lo = np.maximum(ag - 4.0 * noise, 0.0)
+    tol = (2.0 * np.spacing(np.abs(p_ref).astype(np.float32)) + 1e-6 * lr_sh
+           + lr_sh * eps * 4.0 * noise / (lo + eps) ** 2)
+    bad = ~open_sign & (np.abs(got - p_ref) > tol)
+    assert not bad.any(), (int(bad.sum()), np.flatnonzero(bad)[:8])
+    assert np.all(np.abs(got - p0)[open_sign] <= lr_sh * (1 + 1e-6) + 2.0 * np.spacing(np.abs(p0[open_sign])))
+    assert np.count_nonzero(got != p0) > 0.5 * len(p0)  # the step moved the group
 
 
 def test_batched_base_color_lr_graph(small):
