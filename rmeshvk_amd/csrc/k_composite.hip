@@ -172,10 +172,9 @@ struct BatchLDS {
     // adjoint only: per face (grad lambda_a, lambda_a(v0), grad lambda_b,
     // lambda_b(v0)) in face order, then per face the sign s of the stored
     // inward normal n = s * raw normal; 36 floats per tet so that lanes reading
-    // different tets start on different banks; vertex ids; tet id
+    // different tets start on different banks; tet id
     static constexpr int FS = 4 * 8 + 4;
     alignas(16) float face[BWD ? B * FS : 1];
-    uint32_t vi[BWD ? 4 : 1][B];
     uint32_t tid[BWD ? B : 1];
 };
 
@@ -183,7 +182,6 @@ struct BatchLDS {
 struct TetFetch {
     float4 r[4];
     float4 col;
-    uint4 q;
     uint32_t t;
 };
 
@@ -194,7 +192,6 @@ __device__ __forceinline__ void fetch_tet(TetFetch& f, uint32_t t, const float4*
 #pragma unroll
     for (int i = 0; i < 4; ++i) f.r[i] = rec[4ull * t + i];
     f.col = colors[t];
-    if (BWD) f.q = reinterpret_cast<const uint4*>(idx)[t];
 }
 
 // LDS staging of a batch's tet data, filled by direct global -> LDS loads
@@ -204,7 +201,6 @@ template <int B>
 struct Staging {
     float4 rec[4][B];
     float4 col[B];
-    uint4 q[B];
     uint32_t t[B];
 };
 
@@ -218,7 +214,6 @@ __device__ __forceinline__ void stage_async(Staging<B>& S, uint32_t t, bool on, 
         for (int i = 0; i < 4; ++i)
             __builtin_amdgcn_global_load_lds((const void*)(rec + 4ull * t + i), (lds_ptr)&S.rec[i][0], 16, 0, 0);
         __builtin_amdgcn_global_load_lds((const void*)(colors + t), (lds_ptr)&S.col[0], 16, 0, 0);
-        if (BWD) __builtin_amdgcn_global_load_lds((const void*)(idx + 4ull * t), (lds_ptr)&S.q[0], 16, 0, 0);
         S.t[__lane_id()] = t;
     }
 }
@@ -228,7 +223,6 @@ __device__ __forceinline__ void staged_tet(const Staging<B>& S, int k, TetFetch&
 #pragma unroll
     for (int i = 0; i < 4; ++i) f.r[i] = S.rec[i][k];
     f.col = S.col[k];
-    if (BWD) f.q = S.q[k];
     f.t = S.t[k];
 }
 
@@ -291,10 +285,6 @@ __device__ __forceinline__ void store_rest(BatchLDS<TS, B, BWD>& L, int k, const
     }
     L.rows[k] = rr;
     if (BWD) {
-        L.vi[0][k] = f.q.x;
-        L.vi[1][k] = f.q.y;
-        L.vi[2][k] = f.q.z;
-        L.vi[3][k] = f.q.w;
         L.tid[k] = f.t;
     }
 }
@@ -317,11 +307,10 @@ __device__ __forceinline__ void store_tet(BatchLDS<TS, B, BWD>& L, int k, const 
 
 // Quad prefetch: lane j of the four lanes 4k..4k+3 of tet slot k holds the
 // tet's vertex record j (vertex j; w = density for j = 0, else g component
-// j - 1), its colour, vertex id j (adjoint) and the tet id.
+// j - 1), its colour and the tet id.
 struct QuadFetch {
     float4 r;
     float4 col;
-    uint32_t vid;
     uint32_t t;
 };
 
@@ -331,7 +320,6 @@ __device__ __forceinline__ void fetch_quad(QuadFetch& f, uint32_t t, int j, cons
     f.t = t;
     f.r = rec[4ull * t + j];
     f.col = colors[t];
-    if (BWD) f.vid = idx[4ull * t + j];
 }
 
 // quad_perm: lane j of each quad reads lane P_j of the quad (CTRL = P0 | P1 << 2 | P2 << 4 | P3 << 6).
@@ -397,7 +385,6 @@ __device__ __forceinline__ void store_quad(BatchLDS<TS, B, BWD>& L, const QuadFe
     } else if (j == 2) {
         L.tr[k * L.TR + 6] = make_float4(v0.x, v0.y, v0.z, 0.0f);
     }
-    if (BWD) L.vi[j][k] = f.vid;
 }
 
 // Five lanes per tet stage the batch from the LDS staging area: lanes 0-3 a
@@ -1231,8 +1218,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     const uint32_t* __restrict__ sorted_vals, const uint32_t* __restrict__ ranges,
     const ViewParams* __restrict__ views, const uint32_t* __restrict__ tile_view, const uint32_t* __restrict__ tile_order,
     const float* __restrict__ rgba, const float* __restrict__ dl_img, const float* __restrict__ gt, uint32_t loss_type,
-    float* __restrict__ loss_partial, float* __restrict__ d_verts, float* __restrict__ d_dens,
-    float* __restrict__ d_grads, float* __restrict__ d_colors,
+    float* __restrict__ loss_partial, float* __restrict__ tet_g, float* __restrict__ d_colors,
     const float* __restrict__ ck, const float2* __restrict__ fin, uint64_t* __restrict__ scene_done) {
     constexpr int TS = C::TS, NT = C::NT, B = C::B, CAP = C::CAP, R = C::R, SP = TS + 1, NW = NT / 64;
     __shared__ BatchLDS<TS, B, true> L;
@@ -1538,17 +1524,15 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
             }
         }
         __syncthreads();
-        // flush: one atomic per non-zero term per (tile, tet)
+        // flush: one atomic per non-zero term per (tile, tet), the first 16
+        // into the tet's 64-byte staging record (one L2 request for the lanes
+        // of a tet), the colour terms into the view's colour gradients
         for (int it = tid; it < (int)nbp * kTerms; it += NT) {
             const int k = it / kTerms, q = it % kTerms;
             const float s = s_acc[k][q];
             if (s == 0.0f) continue;
             const uint32_t t = L.tid[k];
-            float* dst;
-            if (q == 0) dst = d_dens + t;
-            else if (q <= 12) dst = d_verts + 3ull * L.vi[(q - 1) / 3][k] + (q - 1) % 3;
-            else if (q <= 15) dst = d_grads + 3ull * t + (q - 13);
-            else dst = dcol_v + 3ull * t + (q - 16);
+            float* dst = q < 16 ? tet_g + 16ull * t + q : dcol_v + 3ull * t + (q - 16);
             unsafeAtomicAdd(dst, s);
         }
         done = next;
@@ -1619,13 +1603,13 @@ static void bwd_launch(const BwdArgs& a, hipStream_t s) {
     if (a.ck)
         hipLaunchKernelGGL((k_backward<C, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
                            a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
-                           a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads,
-                           a.d_colors, a.ck, a.fin, a.scene_done);
+                           a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.tet_g, a.d_colors, a.ck, a.fin,
+                           a.scene_done);
     else
         hipLaunchKernelGGL((k_backward<C, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
                            a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
-                           a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.d_verts, a.d_dens, a.d_grads,
-                           a.d_colors, nullptr, nullptr, a.scene_done);
+                           a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.tet_g, a.d_colors, nullptr,
+                           nullptr, a.scene_done);
 }
 
 template <int TS>
@@ -2808,6 +2792,46 @@ __global__ __launch_bounds__(256) void k_interval_chain(const float4* __restrict
     d_base[3ull * t] = b.x;
     d_base[3ull * t + 1] = b.y;
     d_base[3ull * t + 2] = b.z;
+}
+
+// The tiled adjoint's per-tet staging records (16 floats: density, vertex
+// slots 0..3 xyz, colour gradient xyz) out to the parameter gradients: thread
+// i < M writes tet i's density and colour-gradient terms, thread i < N sums
+// vertex i's slots over its (tet << 2 | slot) incidence list in tet order (no
+// atomics).
+__global__ __launch_bounds__(256) void k_tet_gather(const uint32_t* __restrict__ off, const uint32_t* __restrict__ ent,
+                                                    const float* __restrict__ tet_g, uint32_t N, uint32_t M,
+                                                    float* __restrict__ d_verts, float* __restrict__ d_dens,
+                                                    float* __restrict__ d_grads) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) {
+        const float4* g4 = reinterpret_cast<const float4*>(tet_g + 16ull * i);
+        const float4 a = g4[0], d = g4[3];
+        d_dens[i] = a.x;
+        d_grads[3ull * i] = d.y;
+        d_grads[3ull * i + 1] = d.z;
+        d_grads[3ull * i + 2] = d.w;
+    }
+    if (i < N) {
+        float x = 0.f, y = 0.f, z = 0.f;
+        for (uint32_t e = off[i], end = off[i + 1]; e < end; ++e) {
+            const uint32_t te = ent[e];
+            const float* p = tet_g + 16ull * (te >> 2) + 1u + 3u * (te & 3u);
+            x += p[0];
+            y += p[1];
+            z += p[2];
+        }
+        d_verts[3ull * i] = x;
+        d_verts[3ull * i + 1] = y;
+        d_verts[3ull * i + 2] = z;
+    }
+}
+
+void launch_tet_gather(const uint32_t* off, const uint32_t* ent, const float* tet_g, uint32_t N, uint32_t M,
+                       float* d_verts, float* d_dens, float* d_grads, hipStream_t s) {
+    const uint32_t n = N > M ? N : M;
+    if (n) hipLaunchKernelGGL(k_tet_gather, dim3((n + 255) / 256), dim3(256), 0, s, off, ent, tet_g, N, M, d_verts,
+                              d_dens, d_grads);
 }
 
 // One thread per vertex: the sum of its tets' terms over the vertex -> (tet,

@@ -115,6 +115,8 @@ struct rmesh_renderer {
     float *m_sh = nullptr, *v_sh = nullptr, *m_b = nullptr, *v_b = nullptr;
     // gradients
     float *d_verts = nullptr, *d_dens = nullptr, *d_grads = nullptr, *d_colors = nullptr, *d_sh = nullptr;
+    // the tiled adjoint's per-tet staging: [M][16] = density, 4 vertex slots x xyz, colour gradient xyz
+    float* tet_g = nullptr;
     // derived per step
     float4* rec = nullptr;
     float4* colors = nullptr;  // [max_views][M] (SH) or [M]
@@ -447,6 +449,7 @@ rmesh_status stage_forward(rmesh_renderer* h, uint32_t V, bool want_aux) {
 
 void zero_grads(rmesh_renderer* h, uint32_t V) {
     hipStream_t s = h->stream;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(h->tet_g), 0u, 16ull * h->M, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_verts), 0u, 3ull * h->N, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_dens), 0u, h->M, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(h->d_grads), 0u, 3ull * h->M, s);
@@ -469,9 +472,7 @@ rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, cons
     a.gt = gt;
     a.loss_type = loss_type;
     a.loss_partial = gt ? h->loss_partial : nullptr;
-    a.d_verts = h->d_verts;
-    a.d_dens = h->d_dens;
-    a.d_grads = h->d_grads;
+    a.tet_g = h->tet_g;
     a.d_colors = h->d_colors;
     a.ck = nullptr;
     a.fin = nullptr;
@@ -485,6 +486,9 @@ rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, cons
         a.fin = h->fin;
     }
     if (!launch_backward(a, h->TS, h->bwd_var, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
+    // per-tet sums out to the density / colour-gradient arrays; per vertex, the
+    // sum over its (tet, slot) incidence list
+    launch_tet_gather(h->v2t_off, h->v2t, h->tet_g, h->N, h->M, h->d_verts, h->d_dens, h->d_grads, h->stream);
     return RMESH_OK;
 }
 
@@ -618,7 +622,7 @@ void free_all(rmesh_renderer* h) {
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
                     h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd,
                     h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn, h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin,
-                    h->d_scenes, h->d_tile_view, h->d_ktile, h->scene_clock};
+                    h->d_scenes, h->d_tile_view, h->d_ktile, h->scene_clock, h->tet_g};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -768,6 +772,21 @@ static rmesh_status create_impl(const rmesh_scene_desc* descs, uint32_t n_scenes
             for (uint32_t& x : idx) x += c.vert_base;
         CREATE_TRY(hipMemcpy(h->idx + 4ull * c.tet_base, idx.data(), 16ull * c.M, hipMemcpyHostToDevice));
     }
+    {
+        // vertex -> (tet << 2 | slot) incidence lists over all scenes' tets
+        // (counting sort of the global indices: each list in tet order)
+        std::vector<uint32_t> ix(4ull * h->M), off(h->N + 1ull, 0u), ent(4ull * h->M);
+        CREATE_TRY(hipMemcpy(ix.data(), h->idx, 16ull * h->M, hipMemcpyDeviceToHost));
+        for (uint32_t v : ix) ++off[v + 1];
+        for (uint32_t v = 0; v < h->N; ++v) off[v + 1] += off[v];
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (uint64_t e = 0; e < ix.size(); ++e) ent[cur[ix[e]]++] = (uint32_t)e;
+        CREATE_TRY(dalloc(&h->v2t_off, h->N + 1ull));
+        CREATE_TRY(dalloc(&h->v2t, 4ull * h->M));
+        CREATE_TRY(hipMemcpy(h->v2t_off, off.data(), 4ull * (h->N + 1ull), hipMemcpyHostToDevice));
+        CREATE_TRY(hipMemcpy(h->v2t, ent.data(), 16ull * h->M, hipMemcpyHostToDevice));
+    }
+    CREATE_TRY(dalloc(&h->tet_g, 16ull * h->M));
     CREATE_TRY(dalloc(&h->d_verts, N3));
     CREATE_TRY(dalloc(&h->d_dens, M));
     CREATE_TRY(dalloc(&h->d_grads, M3));
@@ -969,18 +988,7 @@ extern "C" rmesh_status rmesh_backward_interval_tiled(rmesh_renderer* h, const f
         HIP_TRY(dalloc(&h->ivl_dgrad, 48ull * h->M));
         HIP_TRY(dalloc(&h->ivl_dvn, 3ull * h->N));
         HIP_TRY(dalloc(&h->ivl_tv, 32ull * h->M));
-        // incidence lists by counting sort of the indices (tet-major, so each
-        // vertex's list is in tet order)
-        std::vector<uint32_t> ix(4ull * h->M), off(h->N + 1ull, 0u), ent(4ull * h->M);
-        HIP_TRY(hipMemcpy(ix.data(), h->idx, 16ull * h->M, hipMemcpyDeviceToHost));
-        for (uint32_t v : ix) ++off[v + 1];
-        for (uint32_t v = 0; v < h->N; ++v) off[v + 1] += off[v];
-        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
-        for (uint64_t e = 0; e < ix.size(); ++e) ent[cur[ix[e]]++] = (uint32_t)e;  // e = tet << 2 | slot
-        HIP_TRY(dalloc(&h->v2t_off, h->N + 1ull));
-        HIP_TRY(dalloc(&h->v2t, 4ull * h->M));
-        HIP_TRY(hipMemcpy(h->v2t_off, off.data(), 4ull * (h->N + 1ull), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(h->v2t, ent.data(), 16ull * h->M, hipMemcpyHostToDevice));
+        // (the vertex incidence lists are built at create)
     }
     HIP_TRY(hipMemcpyAsync(h->dl, dl_img, npx * 16, hipMemcpyHostToDevice, s));
     if (dl_xyzd) HIP_TRY(hipMemcpyAsync(h->ivl_dxyzd, dl_xyzd, npx * 16, hipMemcpyHostToDevice, s));
@@ -1197,7 +1205,8 @@ extern "C" rmesh_status rmesh_device_bytes(rmesh_renderer* h, uint64_t* out) {
                           h->vals_b, h->hist, h->ranges, h->rgba, h->aux, h->dl, h->gt, h->loss_partial,
                           h->loss_out, h->err_pe, h->err_se, h->tet_err, h->tet_cnt, h->tile_order, h->ivr,
                           h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd, h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn,
-                          h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin, h->d_scenes, h->d_tile_view, h->d_ktile};
+                          h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin, h->d_scenes, h->d_tile_view, h->d_ktile,
+                          h->tet_g};
     uint64_t total = 0;
     for (const void* p : ptrs) {
         if (!p) continue;

@@ -142,9 +142,10 @@ struct BwdArgs {
     const float* gt;          // [V][H][W][3] or nullptr
     uint32_t loss_type;
     float* loss_partial;      // [num_gtiles] or nullptr
-    float* d_verts;
-    float* d_dens;
-    float* d_grads;
+    // per tet t, 16 floats at tet_g[16 t]: density, vertex slots 0..3 (xyz),
+    // colour gradient (xyz); launch_tet_gather moves them to the parameter
+    // gradients
+    float* tet_g;
     float* d_colors;          // view v's colour gradients at d_colors[3 (views[v].color_off + t)]
     // nullable: the forward's checkpoints (FwdArgs::ck / fin); set: the
     // checkpointed adjoint, null: the undo from the image alpha
@@ -159,6 +160,10 @@ constexpr uint32_t kCkGroup = 32;
 // Checkpoint slots for P pairs over G global tiles (each slot TS*TS floats).
 inline uint64_t checkpoint_slots(uint64_t P, uint64_t G) { return (P >> 5) + G + 1; }
 bool launch_backward(const BwdArgs& a, uint32_t tile_size, int variant, hipStream_t s);
+// d_dens / d_grads per tet and d_verts per vertex (sum over the vertex's
+// (tet << 2 | slot) incidence list off / ent) out of the adjoint's tet_g.
+void launch_tet_gather(const uint32_t* off, const uint32_t* ent, const float* tet_g, uint32_t N, uint32_t M,
+                       float* d_verts, float* d_dens, float* d_grads, hipStream_t s);
 
 // colours, SH and optimiser (k_optim.hip)
 // half: coefficients rounded to f16 first (the reference viewer's packed SH)
