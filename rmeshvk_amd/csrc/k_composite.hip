@@ -645,6 +645,11 @@ __device__ __forceinline__ void tet_regs(const BatchLDS<TS, B, BWD>& L, int k, T
     T.g = mk3(gg.x, gg.y, gg.z);
 }
 
+// Fused dot product for the adjoint's own terms (the slab intersection and
+// the volume integral keep the unfused order of the reference's arithmetic:
+// vertex gradients magnify a one-ulp change of t through grazing faces).
+__device__ __forceinline__ float fdot3(V3 a, V3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+
 // Ray–tet slab intersection against inward planes (rasterize_compute.wgsl:354-390).
 __device__ __forceinline__ bool intersect(const TetRegs& T, V3 rd, float min_t, float& t_min, float& t_max, int& fmin,
                                           int& fmax) {
@@ -799,24 +804,44 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         const uint32_t t = lane < nb ? sorted_vals[re - 1 - lane] : 0u;
         stage_async<false>(S, t, lane < nb, rec, nullptr, col_v);
     }
+    // MERGE: a batch's LDS image is built right after the previous batch's
+    // pixel-sequential phase, without a barrier between them (that phase
+    // reads only the row bases and the pair records, which the image does not
+    // touch; each lane clears its own pixel's mask after reading it), so one
+    // barrier phase per batch goes.  Not with the aux outputs (their phase
+    // reads the face planes) or LDS-DMA staging (its loads need a barrier).
+    constexpr bool MERGE = !AUX && C::PF != 2;
+    auto stage = [&](int nbs, uint32_t d) {
+        if (C::PF == 0 && tid < nbs) fetch_tet<false>(f, sorted_vals[re - 1 - d - tid], rec, nullptr, col_v);
+        if constexpr (C::PF == 3) store_quad<TS, B, false>(L, qf, tid >> 2, tid & 3, tid < 4 * nbs, vw, tc);
+        else if (tid < nbs) store_tet<TS, B, false>(L, tid, f, vw, tc);
+        if (tid < TS * TS) s_mask[tid] = 0ull;
+        if (R == 1 && tid < CAP / 32) s_start[tid] = 0u;
+    };
+    if constexpr (MERGE) {
+        if (count) stage(nb, 0u);
+    }
     while (done < count) {
         // staged records of this batch have landed (and none is in flight at an early exit)
         if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);
         if (__syncthreads_and(!active || log_t < -13.8f)) break;
-        // wave 0: tet ids of this and the next batch (back to front of the sorted
-        // list), 64 per register
+        // wave 0: tet ids of this and the next batch (back to front of the
+        // sorted list), 64 per register
         uint32_t win = 0, win1 = 0;
         if (C::PF == 2 && wid == 0) {
             win = done + lane < count ? sorted_vals[re - 1 - done - lane] : 0u;
             if constexpr (B > 32) win1 = done + 64 + lane < count ? sorted_vals[re - 1 - done - 64 - lane] : 0u;
         }
-        if (C::PF == 0 && tid < nb) fetch_tet<false>(f, sorted_vals[re - 1 - done - tid], rec, nullptr, col_v);
-        if constexpr (C::PF == 2) store_batch_staged<TS, B, false, NT>(L, S, nb, vw, tc);
-        else if constexpr (C::PF == 3) store_quad<TS, B, false>(L, qf, tid >> 2, tid & 3, tid < 4 * nb, vw, tc);
-        else if (tid < nb) store_tet<TS, B, false>(L, tid, f, vw, tc);
-        if (tid < TS * TS) s_mask[tid] = 0ull;
-        if (R == 1 && tid < CAP / 32) s_start[tid] = 0u;
-        __syncthreads();
+        if constexpr (!MERGE) {
+            if constexpr (C::PF == 2) {
+                store_batch_staged<TS, B, false, NT>(L, S, nb, vw, tc);
+                if (tid < TS * TS) s_mask[tid] = 0ull;
+                if (R == 1 && tid < CAP / 32) s_start[tid] = 0u;
+            } else {
+                stage(nb, done);
+            }
+            __syncthreads();
+        }
         batch_layout<TS, B, false, true, CAP, R, NT>(L, nb, s_chunk, s_start);
         __syncthreads();
         const uint32_t nbp = L.nbp & 0xFFu;
@@ -929,6 +954,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
             else if constexpr (CK)  // positions up to the batch's end that this pixel passed uncovered
                 for (; next_ck <= next && next_ck < count; next_ck += kCkGroup)
                     ck[ck_base + (uint64_t)(next_ck / kCkGroup) * (TS * TS)] = log_t;
+        }
+        if constexpr (MERGE) {
+            if (nb_next > 0) stage(nb_next, next);
         }
         done = next;
         nb = nb_next;
@@ -1075,9 +1103,6 @@ __device__ __forceinline__ void face_slot_weights(int fi, float la, float lb, fl
     w[2] = f0 ? lb : (f1 ? lb : (f2 ? lc : 0.0f));
     w[3] = f0 ? 0.0f : (f1 ? lc : (f2 ? lb : la));
 }
-
-// fused dot product (the adjoint's own terms; the forward keeps dot3)
-__device__ __forceinline__ float fdot3(V3 a, V3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
 
 // Gradient terms of one covered (pixel, tet) pair, accumulated into acc
 // (backward_tiled_compute.wgsl:446-543, per-pixel clipping as there).  The
@@ -1230,6 +1255,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     __shared__ float s_ray[3][TS * TS];
     __shared__ float s_dc[3][TS * TS];
     __shared__ float s_acc[B][kTerms];
+    __shared__ uint32_t s_ftid[B];  // tet ids of the batch whose terms s_acc holds
     __shared__ unsigned int s_mask[TS * TS];  // covered tets of the batch per pixel (B <= 32)
     __shared__ chunk_t<R> s_chunk[chunk_capacity(CAP, R, B)];
     static_assert(CAP <= 2048 && TS <= 32 && R <= 31, "chunk table packing");
@@ -1330,9 +1356,27 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         const uint32_t t = lane < nb ? sorted_vals[rs + lane] : 0u;
         stage_async<true>(S, t, lane < nb, rec, idx, col_v);
     }
+    // The flush of a batch's terms runs at the top of the next iteration, in
+    // the same phase as the next batch's LDS image (they touch disjoint LDS):
+    // one barrier phase fewer per batch, and the atomics overlap the staging.
+    uint32_t flush_n = 0;  // tets of the previous batch to flush
+    auto flush = [&]() {
+        // one atomic per non-zero term per (tile, tet), the first 16 into the
+        // tet's 64-byte staging record (one L2 request for the lanes of a
+        // tet), the colour terms into the view's colour gradients
+        for (int it = tid; it < (int)flush_n * kTerms; it += NT) {
+            const int k = it / kTerms, q = it % kTerms;
+            const float s = s_acc[k][q];
+            if (s == 0.0f) continue;
+            const uint32_t t = s_ftid[k];
+            float* dst = q < 16 ? tet_g + 16ull * t + q : dcol_v + 3ull * t + (q - 16);
+            unsafeAtomicAdd(dst, s);
+        }
+    };
     while (done < count) {
         if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // staged records of this batch have landed
         __syncthreads();
+        flush();
         uint32_t win = 0, win1 = 0;  // wave 0: tet ids of this and the next batch, 64 per register
         if (C::PF == 2 && wid == 0) {
             win = done + lane < count ? sorted_vals[rs + done + lane] : 0u;
@@ -1342,7 +1386,6 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         if constexpr (C::PF == 2) store_batch_staged<TS, B, true, NT>(L, S, nb, vw, tc);
         else if constexpr (C::PF == 3) store_quad<TS, B, true>(L, qf, tid >> 2, tid & 3, tid < 4 * nb, vw, tc);
         else if (tid < nb) store_tet<TS, B, true>(L, tid, f, vw, tc);
-        for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
         if (tid < TS * TS) s_mask[tid] = 0u;
         if (R == 1 && tid < CAP / 32) s_start[tid] = 0u;
         __syncthreads();
@@ -1369,6 +1412,9 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
             const uint32_t tn = window_id<B>(win, win1, (int)nbp + lane);
             stage_async<true>(S, tn, lane < nb_next, rec, idx, col_v);
         }
+        // the flush above has read s_acc and s_ftid (two barriers ago)
+        for (int i = tid; i < B * kTerms; i += NT) (&s_acc[0][0])[i] = 0.0f;
+        if (tid < (int)nbp) s_ftid[tid] = L.tid[tid];
         // phase A (record- or chunk-parallel): intersection + integral
         for (uint32_t j = tid; j < nchunks; j += NT) {
             ChunkRef c;
@@ -1523,21 +1569,12 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                     if (acc[q] != 0.0f) atomicAdd(&s_acc[key][q], acc[q]);
             }
         }
-        __syncthreads();
-        // flush: one atomic per non-zero term per (tile, tet), the first 16
-        // into the tet's 64-byte staging record (one L2 request for the lanes
-        // of a tet), the colour terms into the view's colour gradients
-        for (int it = tid; it < (int)nbp * kTerms; it += NT) {
-            const int k = it / kTerms, q = it % kTerms;
-            const float s = s_acc[k][q];
-            if (s == 0.0f) continue;
-            const uint32_t t = L.tid[k];
-            float* dst = q < 16 ? tet_g + 16ull * t + q : dcol_v + 3ull * t + (q - 16);
-            unsafeAtomicAdd(dst, s);
-        }
+        flush_n = nbp;
         done = next;
         nb = nb_next;
     }
+    __syncthreads();
+    flush();
     if (C::PF == 2) __builtin_amdgcn_s_waitcnt(0);  // no LDS DMA outlives the workgroup
     if (scene_done && tid == 0) atomicMax(reinterpret_cast<unsigned long long*>(scene_done) + vw.scene,
                                           (unsigned long long)wall_clock64());
@@ -1621,6 +1658,8 @@ static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
         case 3: bwd_launch<BwdCfg<TS, 32, 576, 1, 3, 3, 4>>(a, s); return true;
         case 4: bwd_launch<BwdCfg<TS, 32, 640, 1, 2, 3, 4>>(a, s); return true;
         case 5: bwd_launch<BwdCfg<TS, 32, 640, 1, 3, 3, 4>>(a, s); return true;
+        // 31.5 KB of LDS and 96 VGPRs at tile 16: 5 workgroups (20 waves) per CU
+        case 6: bwd_launch<BwdCfg<TS, 32, 544, 1, 3, 3, 5>>(a, s); return true;
         default: return false;
     }
 }
