@@ -122,10 +122,12 @@ struct TetRows {
         const float x01 = x[0] + d0 * s[0], x02 = x[0] + d0 * s[1], x03 = x[0] + d0 * s[2];
         const float x12 = x[1] + d1 * s[3], x13 = x[1] + d1 * s[4], x23 = x[2] + d2 * s[5];
         const bool lo = yc < y[1], mid = yc < y[2];
-        float xl_f = lo ? fminf(fminf(x01, x02), x03) : mid ? fminf(fminf(fminf(x02, x03), x12), x13)
-                                                            : fminf(fminf(x03, x13), x23);
-        float xr_f = lo ? fmaxf(fmaxf(x01, x02), x03) : mid ? fmaxf(fmaxf(fmaxf(x02, x03), x12), x13)
-                                                            : fmaxf(fmaxf(x03, x13), x23);
+        // edge 03 crosses every band; the other two operands per band:
+        // lo {01, 02}, mid {02 with 12, 13}, hi {13, 23}
+        const float pl = lo ? x01 : (mid ? fminf(x02, x12) : x13), ql = lo ? x02 : (mid ? x13 : x23);
+        const float pr = lo ? x01 : (mid ? fmaxf(x02, x12) : x13);
+        float xl_f = fminf(fminf(x03, pl), ql);
+        float xr_f = fmaxf(fmaxf(x03, pr), ql);
         if (!(y[0] <= yc && yc < y[3])) {
             xl_f = 1e10f;
             xr_f = -1e10f;
@@ -142,9 +144,10 @@ struct TetRows {
             }
         }
         if (xl_f <= xr_f) {
+            // clamped in float: in [0, TS - 1] when the span is not empty
             const float eps = 0.001f;
-            int32_t l = max(sat_i32(ceilf(xl_f - 0.5f - eps)), 0);
-            int32_t r = min(sat_i32(floorf(xr_f - 0.5f + eps)), TS - 1);
+            const float l = fmaxf(ceilf(xl_f - 0.5f - eps), 0.0f);
+            const float r = fminf(floorf(xr_f - 0.5f + eps), (float)(TS - 1));
             if (l <= r) return (uint32_t)l | ((uint32_t)r << SpanFmt<TS>::SH);
         }
         return SpanFmt<TS>::EMPTY;
@@ -2853,7 +2856,29 @@ __global__ __launch_bounds__(256) void k_tet_gather(const uint32_t* __restrict__
     }
     if (i < N) {
         float x = 0.f, y = 0.f, z = 0.f;
-        for (uint32_t e = off[i], end = off[i + 1]; e < end; ++e) {
+        uint32_t e = off[i];
+        const uint32_t end = off[i + 1];
+        // four incidences in flight at a time (the sum keeps the list order)
+        for (; e + 4 <= end; e += 4) {
+            uint32_t te[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) te[u] = ent[e + u];
+            float q[4][3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float* p = tet_g + 16ull * (te[u] >> 2) + 1u + 3u * (te[u] & 3u);
+                q[u][0] = p[0];
+                q[u][1] = p[1];
+                q[u][2] = p[2];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                x += q[u][0];
+                y += q[u][1];
+                z += q[u][2];
+            }
+        }
+        for (; e < end; ++e) {
             const uint32_t te = ent[e];
             const float* p = tet_g + 16ull * (te >> 2) + 1u + 3u * (te & 3u);
             x += p[0];

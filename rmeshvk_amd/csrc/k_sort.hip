@@ -17,6 +17,8 @@
 //    by the autotuner;
 //  * the tile (2048 keys) is re-ordered by digit in LDS before the scatter,
 //    so the global writes are runs per digit instead of 8-byte scatters.
+#include <mutex>
+
 #include "rmesh_common.hpp"
 #include "rmesh_kernels.hpp"
 
@@ -51,10 +53,17 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_upsweep(const uint64_t* __r
     for (int i = threadIdx.x; i < kSortWaves * (int)RADIX; i += kSortThreads) (&h[0][0])[i] = 0;
     __syncthreads();
     uint32_t w = threadIdx.x / kWave;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortThreads) {
-        uint32_t d = (uint32_t)(keys[i] >> shift) & DMASK;
-        atomicAdd(&h[w][d], 1u);
+    // eight keys in flight per thread before their histogram adds
+    constexpr int U = 8;
+    uint32_t i0 = lo + threadIdx.x;
+    for (; i0 + (U - 1) * kSortThreads < hi; i0 += U * kSortThreads) {
+        uint64_t kk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) kk[u] = keys[i0 + u * kSortThreads];
+#pragma unroll
+        for (int u = 0; u < U; ++u) atomicAdd(&h[w][(uint32_t)(kk[u] >> shift) & DMASK], 1u);
     }
+    for (; i0 < hi; i0 += kSortThreads) atomicAdd(&h[w][(uint32_t)(keys[i0] >> shift) & DMASK], 1u);
     __syncthreads();
     uint32_t d = threadIdx.x;
     if (d < RADIX) {
@@ -119,13 +128,20 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
         uint64_t key[kRounds];
         uint32_t val[kRounds];
         uint32_t dr[kRounds];  // digit | rank << 9 ; digit 256 marks an empty slot
-        // Phase 1: per-wave stable ranking, 64 keys per round.
+        // all of the tile's loads first (one memory latency per tile, not one
+        // per round: the rounds' wave barriers would otherwise serialise them)
 #pragma unroll
         for (int j = 0; j < kRounds; ++j) {
             uint32_t i = base + w * kWaveKeys + j * kWave + lane;
             bool valid = i < hi;
             key[j] = valid ? kin[i] : 0ull;
             val[j] = valid ? vin[i] : 0u;
+        }
+        // Phase 1: per-wave stable ranking, 64 keys per round.
+#pragma unroll
+        for (int j = 0; j < kRounds; ++j) {
+            uint32_t i = base + w * kWaveKeys + j * kWave + lane;
+            bool valid = i < hi;
             uint32_t d = (uint32_t)(key[j] >> shift) & DMASK;
             uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -187,8 +203,36 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
     }
 }
 
+// Blocks of the downsweep that are resident at once on the current device
+// (CUs x blocks per CU at its LDS and register use), cached per device.  A
+// grid of at most that many blocks runs every block of a pass in one round:
+// no second, partly filled round of blocks at the end of each pass.
+static uint32_t resident_sort_blocks() {
+    static std::mutex mu;
+    static uint32_t cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2048u;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cached[dev]) {
+        int cus = 0, b8 = 0, b4 = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b8, reinterpret_cast<const void*>(k_rs_downsweep<8>),
+                                                         kSortThreads, 0) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b4, reinterpret_cast<const void*>(k_rs_downsweep<4>),
+                                                         kSortThreads, 0) != hipSuccess ||
+            cus <= 0 || b8 <= 0 || b4 <= 0) {
+            cached[dev] = 2048u;
+        } else {
+            cached[dev] = (uint32_t)cus * (uint32_t)(b8 < b4 ? b8 : b4);
+        }
+    }
+    return cached[dev];
+}
+
 uint32_t sort_grid(uint32_t cap) {
     uint32_t g = (cap + kTileKeys - 1) / kTileKeys;
+    const uint32_t r = resident_sort_blocks();
+    if (g > r) g = r;
     if (g > 2048) g = 2048;
     if (g < 1) g = 1;
     return g;
