@@ -304,30 +304,39 @@ def config_d(seed, steps):
     mr.set_targets(nv, targets)
     tune = mr.autotune(nv, flat, loss_type=0, reps=2)
     log(f"config D autotune: {tune}")
-    mr.set_graph_mode(True)
     mr.set_scene_timing(True)
     lr = (LR["lr_verts"], LR["lr_dens"], LR["lr_grads"], LR["lr_sh"])
-    for _ in range(2):
-        mr.train_step(nv, flat, 0, *lr)
-    lat = []
-    mr.synchronize()
-    tb = time.perf_counter()
-    for _ in range(steps):
-        mr.train_step(nv, flat, 0, *lr)
-        lat.append(mr.scene_latency())
-    mr.synchronize()
-    ms_batch = 1000.0 * (time.perf_counter() - tb) / steps
-    lat = np.concatenate(lat)
     V = sum(nv)
     out = {"workload": f"{len(scenes)} scenes ({sum(len(s['indices']) // 4 for s in scenes)} tets), SH degrees "
                        f"{sorted(set(int(s['sh_degree']) for s in scenes))}, resolutions "
-                       f"{sorted(set(int(s['width']) for s in scenes))}, {V} views per step",
-           "batched": {"ms_per_step": round(ms_batch, 3), "views_per_sec": round(V * 1000.0 / ms_batch, 2),
-                       "tuning": tune["tuning"], "sort_digit_bits": tune["sort_digit_bits"],
-                       "scene_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-                       "scene_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
-                       "latency_note": "per scene: step start to its last adjoint tile (device wall clock); the "
-                                       "shared SH adjoint + Adam stage follows"}}
+                       f"{sorted(set(int(s['width']) for s in scenes))}, {V} views per step"}
+    ms_batch = None
+    for order in ("interleaved", "by_scene"):
+        # the compositing grids' tile order across scenes: all scenes' tiles
+        # heaviest first, or scene after scene (fewest pairs first)
+        mr.set_scene_order(order)
+        mr.set_graph_mode(True)
+        for _ in range(2):
+            mr.train_step(nv, flat, 0, *lr)
+        lat = []
+        mr.synchronize()
+        tb = time.perf_counter()
+        for _ in range(steps):
+            mr.train_step(nv, flat, 0, *lr)
+            lat.append(mr.scene_latency())
+        mr.synchronize()
+        ms = 1000.0 * (time.perf_counter() - tb) / steps
+        lat = np.concatenate(lat)
+        key = "batched" if order == "interleaved" else "batched_by_scene"
+        out[key] = {"ms_per_step": round(ms, 3), "views_per_sec": round(V * 1000.0 / ms, 2),
+                    "tuning": tune["tuning"], "sort_digit_bits": tune["sort_digit_bits"], "scene_order": order,
+                    "scene_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                    "scene_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                    "latency_note": "per scene: step start to its last adjoint tile (device wall clock); the "
+                                    "shared SH adjoint + Adam stage follows"}
+        if order == "interleaved":
+            ms_batch = ms
+        log(f"config D {order}: {json.dumps(out[key])}")
     del mr
     # the same scenes one renderer at a time
     per = []

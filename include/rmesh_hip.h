@@ -331,6 +331,15 @@ rmesh_status rmesh_read_scene(rmesh_renderer* h, uint32_t scene, int which, void
 rmesh_status rmesh_set_scene_timing(rmesh_renderer* h, int enable);
 /* ms[scene]: from the last step's start to the scene's last adjoint tile. */
 rmesh_status rmesh_scene_latency(rmesh_renderer* h, float* ms);
+/* Order of the compositing grids' tiles across scenes.  INTERLEAVED (the
+ * default): all scenes' tiles together, heaviest first (best balance; every
+ * scene finishes near the end of the stage).  BY_SCENE: the scenes one after
+ * another, fewest pairs first, each heaviest first (early scenes finish
+ * early; at most 64 scenes).  Results are the same either way. */
+#define RMESH_SCENE_ORDER_INTERLEAVED 0u
+#define RMESH_SCENE_ORDER_BY_SCENE 1u
+rmesh_status rmesh_set_scene_order(rmesh_renderer* h, uint32_t order);
+uint32_t rmesh_get_scene_order(const rmesh_renderer* h);
 
 /* ---- scene files (crates/rmesh-data/src/lib.rs) ----
  * load_rmesh / load_rmesh_raw / load_ply (lib.rs:140-148, 243-394, 711-853):

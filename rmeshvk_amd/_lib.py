@@ -107,6 +107,8 @@ SIGNATURES = {
     "rmesh_backward_device": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rmesh_create_multi": (_st, [ctypes.POINTER(SceneDesc), _u32, ctypes.POINTER(_vp)]),
     "rmesh_scene_count": (_u32, [_vp]),
+    "rmesh_set_scene_order": (_st, [_vp, _u32]),
+    "rmesh_get_scene_order": (_u32, [_vp]),
     "rmesh_forward_multi": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rmesh_set_targets_multi": (_st, [_vp, _vp, _vp]),
     "rmesh_train_step_multi": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _f32, _f32, _f32, _f32, _vp,

@@ -465,6 +465,55 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* __restrict_
     for (uint32_t t = tid; t < n; t += blockDim.x) order[atomicAdd(&off[tile_bucket(ranges, t)], 1u)] = t;
 }
 
+// Scene-major order for batches of several scenes: the scenes one after
+// another, the one with the fewest pairs first (ties: the caller's order), and
+// the tiles of each scene heaviest first as above.  A scene's tiles then finish
+// together early in the step instead of all scenes finishing at its end; the
+// tile grid is the same, so the total work is too.  Counts per (scene, bucket)
+// in LDS, then offsets by one thread.
+constexpr uint32_t kOrderMaxScenes = 64;
+__global__ __launch_bounds__(1024) void k_tile_order_scenes(const uint32_t* __restrict__ ranges, uint32_t n,
+                                                            const uint32_t* __restrict__ tile_view,
+                                                            const ViewParams* __restrict__ views, uint32_t n_scenes,
+                                                            uint32_t* __restrict__ order) {
+    __shared__ uint32_t cnt[kOrderMaxScenes * kOrderBuckets];  // counts, then running offsets
+    __shared__ unsigned long long s_pairs[kOrderMaxScenes];
+    __shared__ uint32_t s_by_rank[kOrderMaxScenes];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < n_scenes * kOrderBuckets; i += blockDim.x) cnt[i] = 0u;
+    if (tid < n_scenes) s_pairs[tid] = 0ull;
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += blockDim.x) {
+        const uint32_t sc = views[tile_view[t]].scene;
+        atomicAdd(&cnt[sc * kOrderBuckets + tile_bucket(ranges, t)], 1u);
+        const uint32_t rs = ranges[2ull * t], re = ranges[2ull * t + 1];
+        atomicAdd(&s_pairs[sc], (unsigned long long)(re > rs ? re - rs : 0u));
+    }
+    __syncthreads();
+    if (tid < n_scenes) {
+        uint32_t r = 0;
+        for (uint32_t o = 0; o < n_scenes; ++o) r += (s_pairs[o] < s_pairs[tid] || (s_pairs[o] == s_pairs[tid] && o < tid)) ? 1u : 0u;
+        s_by_rank[r] = tid;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t r = 0; r < n_scenes; ++r) {
+            uint32_t* c = cnt + s_by_rank[r] * kOrderBuckets;
+            for (int b = kOrderBuckets - 1; b >= 0; --b) {
+                const uint32_t k = c[b];
+                c[b] = run;
+                run += k;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += blockDim.x) {
+        const uint32_t sc = views[tile_view[t]].scene;
+        order[atomicAdd(&cnt[sc * kOrderBuckets + tile_bucket(ranges, t)], 1u)] = t;
+    }
+}
+
 __global__ void k_stamp_start(uint64_t* __restrict__ p, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
     if (threadIdx.x == 0) p[n] = wall_clock64();
@@ -475,6 +524,15 @@ void launch_stamp_start(uint64_t* p, uint32_t n, hipStream_t s) {
 
 void launch_tile_order(const uint32_t* ranges, uint32_t n_tiles, uint32_t* order, hipStream_t s) {
     if (n_tiles) hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, ranges, n_tiles, order);
+}
+
+bool launch_tile_order_scenes(const uint32_t* ranges, uint32_t n_tiles, const uint32_t* tile_view,
+                              const ViewParams* views, uint32_t n_scenes, uint32_t* order, hipStream_t s) {
+    if (n_scenes == 0 || n_scenes > kOrderMaxScenes) return false;
+    if (n_tiles)
+        hipLaunchKernelGGL(k_tile_order_scenes, dim3(1), dim3(1024), 0, s, ranges, n_tiles, tile_view, views, n_scenes,
+                           order);
+    return true;
 }
 
 void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,

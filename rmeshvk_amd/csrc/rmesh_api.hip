@@ -164,6 +164,7 @@ struct rmesh_renderer {
     std::vector<uint32_t> graph_layout;  // views per scene the graph was captured for
     // per-scene timing: wall clock at the step start and when each scene's last adjoint tile retired
     bool scene_timing = false;
+    uint32_t scene_order = RMESH_SCENE_ORDER_INTERLEAVED;  // compositing grids' tile order across scenes
     uint64_t* scene_clock = nullptr;  // [scenes + 1]
     float graph_lrs[5] = {0, 0, 0, 0, 0};  // lr_v, lr_d, lr_g, lr_sh, lr_base baked into the graph
     std::vector<ViewParams> hviews;
@@ -413,7 +414,10 @@ void launch_sort_stage(rmesh_renderer* h, hipStream_t s) {
 void launch_ranges_stage(rmesh_renderer* h, hipStream_t s) {
     launch_fill_u32(h->ranges, 0u, 2ull * h->n_gt, s);
     launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, h->d_ktile, h->n_kt, s);
-    launch_tile_order(h->ranges, h->n_gt, h->tile_order, s);
+    if (h->scene_order != RMESH_SCENE_ORDER_BY_SCENE ||
+        !launch_tile_order_scenes(h->ranges, h->n_gt, h->d_tile_view, h->d_views, (uint32_t)h->sc.size(),
+                                  h->tile_order, s))
+        launch_tile_order(h->ranges, h->n_gt, h->tile_order, s);
 }
 
 rmesh_status stage_binning_back(rmesh_renderer* h, uint32_t V) {
@@ -1625,6 +1629,19 @@ extern "C" rmesh_status rmesh_set_scene_timing(rmesh_renderer* h, int enable) {
     }
     return RMESH_OK;
 }
+
+extern "C" rmesh_status rmesh_set_scene_order(rmesh_renderer* h, uint32_t order) {
+    if (!h) return fail(RMESH_ERR_INVALID, "null handle");
+    if (order != RMESH_SCENE_ORDER_INTERLEAVED && order != RMESH_SCENE_ORDER_BY_SCENE)
+        return fail(RMESH_ERR_INVALID, "scene order must be RMESH_SCENE_ORDER_INTERLEAVED or _BY_SCENE");
+    if (order == RMESH_SCENE_ORDER_BY_SCENE && h->sc.size() > 64)
+        return fail(RMESH_ERR_UNSUPPORTED, "scene-major order holds at most 64 scenes");
+    if (order != h->scene_order) drop_graph(h);
+    h->scene_order = order;
+    return RMESH_OK;
+}
+
+extern "C" uint32_t rmesh_get_scene_order(const rmesh_renderer* h) { return h ? h->scene_order : 0u; }
 
 // Per scene (the caller's order): ms from the last step's start to the end of
 // the scene's last adjoint tile.

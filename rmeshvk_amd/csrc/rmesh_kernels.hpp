@@ -29,6 +29,10 @@ void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, ui
                    const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s);
 // Permutation of the global tiles, heaviest (most pairs) first.
 void launch_tile_order(const uint32_t* ranges, uint32_t n_tiles, uint32_t* order, hipStream_t s);
+// Scene-major heaviest-first order (scenes by ascending pair count); false
+// (nothing launched) above 64 scenes.
+bool launch_tile_order_scenes(const uint32_t* ranges, uint32_t n_tiles, const uint32_t* tile_view,
+                              const ViewParams* views, uint32_t n_scenes, uint32_t* order, hipStream_t s);
 // p[0..n) = 0, p[n] = the wall clock now (start of a step's per-scene timing)
 void launch_stamp_start(uint64_t* p, uint32_t n, hipStream_t s);
 

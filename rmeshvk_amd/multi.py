@@ -144,6 +144,18 @@ class MultiSceneRenderer(RMeshRenderer):
     def set_scene_timing(self, enable: bool):
         _lib.check(self._lib.rmesh_set_scene_timing(self._h, 1 if enable else 0))
 
+    def set_scene_order(self, order: str):
+        """"interleaved" (default: every scene's tiles heaviest first, best
+        balance) or "by_scene" (scenes one after another, fewest pairs first:
+        early scenes finish early).  Results are the same either way."""
+        code = {"interleaved": 0, "by_scene": 1}.get(order)
+        if code is None:
+            raise ValueError("order must be 'interleaved' or 'by_scene'")
+        _lib.check(self._lib.rmesh_set_scene_order(self._h, code))
+
+    def get_scene_order(self) -> str:
+        return ("interleaved", "by_scene")[int(self._lib.rmesh_get_scene_order(self._h))]
+
     def scene_latency(self):
         """ms from the last step's start to each scene's last adjoint tile."""
         out = np.empty(len(self._info), dtype=np.float32)

@@ -6,7 +6,9 @@ gives: images bit for bit (same lists, same arithmetic), gradients of a
 zero-learning-rate step within 1e-5 relative L2 (float atomics in another
 order), parameters after two steps with learning rates within 1e-5; images
 of one scene per SH degree against the oracle (1e-4); graph replay equal to
-eager; per-scene latencies reported.
+eager; per-scene latencies reported; the scene-major tile order gives the
+same images and gradients (its effect on latency needs a batch larger than
+the GPU's resident workgroups: bench.py --config-d).
 """
 
 import math
@@ -151,3 +153,29 @@ def test_multi_uneven_view_counts_and_errors(batch):
         mr.forward([5] + nv[1:], _flat(cams) + [cams[0][0]] * 2)  # more views than max_views
     with pytest.raises(NotImplementedError):
         mr.forward_tiled(*cams[0][0])  # single-scene entry point
+
+
+def test_scene_major_order_same_results_earlier_scenes(batch):
+    scenes, cams = batch
+    nv = [len(c) for c in cams]
+    gts = _targets(scenes, cams, 13)
+    out = {}
+    for order in ("interleaved", "by_scene"):
+        mr = _multi(scenes)
+        mr.set_scene_order(order)
+        assert mr.get_scene_order() == order
+        mr.set_targets(nv, gts)
+        mr.set_graph_mode(True)
+        mr.set_scene_timing(True)
+        imgs = mr.forward(nv, _flat(cams))
+        for _ in range(2):
+            mr.train_step(nv, _flat(cams), 1, 0.0, 0.0, 0.0, 0.0)
+        out[order] = (imgs, {(i, k): mr.read_scene(i, k) for i in range(len(scenes))
+                             for k in ("d_vertices", "d_densities", "d_color_grads", "d_sh")}, mr.scene_latency())
+    for i in range(len(scenes)):
+        np.testing.assert_array_equal(out["by_scene"][0][i], out["interleaved"][0][i])
+    for key, g in out["interleaved"][1].items():
+        assert _rel(out["by_scene"][1][key], g) <= 1e-5, key
+    assert out["by_scene"][2].shape == (len(scenes),) and np.all(out["by_scene"][2] > 0.0)
+    with pytest.raises(ValueError):
+        _multi(scenes).set_scene_order("random")
