@@ -195,8 +195,11 @@ __device__ __forceinline__ float adam_update(float& p, float gr, float& m, float
 // m, v) are one contiguous block in HBM; the coefficients are moved to LDS
 // with coalesced dword accesses; the four lanes' partial d_sh are summed with
 // two quad_perm butterflies, staged in LDS rows (padded, see SR) and applied
-// (Adam, m and v read and written in HBM directly) or stored element-parallel.  One lane per tet scatters the
-// direction/offset adjoint to the vertices.
+// (Adam, m and v read and written in HBM directly) or stored element-parallel.
+// The direction/offset adjoint is added to the tet's staging record of vertex
+// slots and colour gradient (one coalesced float4 per lane), so it reaches the
+// vertices through the same fixed-order gather as the compositing adjoint's
+// terms: this kernel runs after that adjoint and before the gather.
 constexpr int kShTets = 32;
 constexpr int kShLanes = 4;
 
@@ -311,22 +314,39 @@ __global__ __launch_bounds__(kShLanes * kShTets) void k_sh_backward(ShBwdArgs a)
     for (int ch = 0; ch < 3; ++ch)
 #pragma unroll
         for (int k = 0; k < NC; ++k) dsh[ch][k] = quad_sum(dsh[ch][k]);
-    if (live && j == 0) {
-        // offset = g . (v0 - centroid)
-        a.d_grads[3ull * t] += s_all * off.x;
-        a.d_grads[3ull * t + 1] += s_all * off.y;
-        a.d_grads[3ull * t + 2] += s_all * off.z;
+    if (live) {
+        // offset = g . (v0 - centroid): its adjoint goes to the colour gradient
+        // and (with the centroid's) to the four vertex slots of the tet's
+        // staging record, which the vertex gather sums afterwards.  Lane j
+        // adds to the record's float4 j: (density, slot 0), (slot 1, slot 2.x),
+        // (slot 2.yz, slot 3.xy), (slot 3.z, colour gradient); no atomics.
+        const V3 dgr = scl3(off, s_all);
         dcen = add3(dcen, scl3(g, -s_all));
-        const uint4 q = reinterpret_cast<const uint4*>(a.idx)[t];
-        const uint32_t vi[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            V3 dvv = scl3(dcen, 0.25f);
-            if (i == 0) dvv = add3(dvv, scl3(g, s_all));
-            if (dvv.x != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i], dvv.x);
-            if (dvv.y != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 1, dvv.y);
-            if (dvv.z != 0.0f) unsafeAtomicAdd(a.d_verts + 3ull * vi[i] + 2, dvv.z);
+        const V3 dq = scl3(dcen, 0.25f);           // slots 1..3
+        const V3 d0 = add3(dq, scl3(g, s_all));   // slot 0 (v0 enters the offset too)
+        float4* rp = reinterpret_cast<float4*>(a.tet_g + 16ull * t) + j;
+        float4 x = *rp;
+        if (j == 0) {
+            x.y += d0.x;
+            x.z += d0.y;
+            x.w += d0.z;
+        } else if (j == 1) {
+            x.x += dq.x;
+            x.y += dq.y;
+            x.z += dq.z;
+            x.w += dq.x;
+        } else if (j == 2) {
+            x.x += dq.y;
+            x.y += dq.z;
+            x.z += dq.x;
+            x.w += dq.y;
+        } else {
+            x.x += dq.z;
+            x.y += dgr.x;
+            x.z += dgr.y;
+            x.w += dgr.z;
         }
+        *rp = x;
     }
     // lane j < 3 stages channel j's row
     if (live && j < 3) {

@@ -490,10 +490,14 @@ rmesh_status stage_backward(rmesh_renderer* h, uint32_t V, const float* dl, cons
         a.fin = h->fin;
     }
     if (!launch_backward(a, h->TS, h->bwd_var, h->stream)) return fail(RMESH_ERR_INVALID, "unsupported tile size");
-    // per-tet sums out to the density / colour-gradient arrays; per vertex, the
-    // sum over its (tet, slot) incidence list
-    launch_tet_gather(h->v2t_off, h->v2t, h->tet_g, h->N, h->M, h->d_verts, h->d_dens, h->d_grads, h->stream);
     return RMESH_OK;
+}
+
+// The staged per-tet terms (the compositing adjoint's, then the SH adjoint's)
+// out to the gradient arrays: per tet, density and colour gradient; per
+// vertex, the sum over its (tet, slot) incidence list.
+void launch_gather_grads(rmesh_renderer* h, hipStream_t s) {
+    launch_tet_gather(h->v2t_off, h->v2t, h->tet_g, h->N, h->M, h->d_verts, h->d_dens, h->d_grads, s);
 }
 
 rmesh_status ensure_optimizer(rmesh_renderer* h) {
@@ -532,7 +536,6 @@ void launch_sh_classes(rmesh_renderer* h, bool adam, float lr, AdamStep st, hipS
         ShBwdArgs a;
         std::memset(&a, 0, sizeof(a));
         a.rec = h->rec;
-        a.idx = h->idx;
         a.sh = h->sh + c.sh_off;
         a.deg = c.deg;
         a.t0 = c.t0;
@@ -542,8 +545,7 @@ void launch_sh_classes(rmesh_renderer* h, bool adam, float lr, AdamStep st, hipS
         a.views = h->d_views;
         a.d_colors = h->d_colors;
         a.d_sh = h->d_sh + c.sh_off;
-        a.d_grads = h->d_grads;
-        a.d_verts = h->d_verts;
+        a.tet_g = h->tet_g;
         a.adam = adam;
         a.m_sh = adam ? h->m_sh + c.sh_off : nullptr;
         a.v_sh = adam ? h->v_sh + c.sh_off : nullptr;
@@ -560,6 +562,7 @@ rmesh_status stage_adam(rmesh_renderer* h, uint32_t V, float lr_v, float lr_d, f
     launch_step_inc(h->d_step, h->overflow, s);
     AdamStep st{h->d_step, h->overflow, 0.9f, 0.999f, 1e-8f};
     if (h->has_sh) launch_sh_classes(h, lr_sh != 0.0f, lr_sh, st, s);
+    launch_gather_grads(h, s);
     AdamArgs aa;
     std::memset(&aa, 0, sizeof(aa));
     aa.st = st;
@@ -576,7 +579,7 @@ rmesh_status stage_adam(rmesh_renderer* h, uint32_t V, float lr_v, float lr_d, f
 
 // The train step as 7 stages; timing marks sit between them:
 // 0 colours + projection + scan | 1 pair generation | 2 sort | 3 ranges |
-// 4 forward | 5 adjoint | 6 SH adjoint + Adam + loss.
+// 4 forward | 5 adjoint | 6 SH adjoint + gradient gather + Adam + loss.
 constexpr int kStages = 7;
 
 rmesh_status record_stage(rmesh_renderer* h, int stage, uint32_t V, const float* gt, uint32_t loss_type, float lr_v,
@@ -1059,6 +1062,7 @@ extern "C" rmesh_status rmesh_backward(rmesh_renderer* h, const float* dl, float
     // colour gradient returned as d_base_colors (the per-tet colours of this view)
     HIP_TRY(hipMemcpyAsync(dB, h->d_colors, 3ull * h->M * 4, hipMemcpyDeviceToHost, s));
     if (h->has_sh) launch_sh_classes(h, false, 0.0f, AdamStep{h->d_step, h->overflow, 0.9f, 0.999f, 1e-8f}, s);
+    launch_gather_grads(h, s);
     HIP_TRY(hipMemcpyAsync(dV, h->d_verts, 3ull * h->N * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(dD, h->d_dens, 1ull * h->M * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(dG, h->d_grads, 3ull * h->M * 4, hipMemcpyDeviceToHost, s));
@@ -1108,6 +1112,7 @@ extern "C" rmesh_status rmesh_backward_device(rmesh_renderer* h, const float* d_
     if (st != RMESH_OK) return st;
     HIP_TRY(hipMemcpyAsync(dB, h->d_colors, 12ull * h->M, hipMemcpyDeviceToDevice, s));
     if (h->has_sh) launch_sh_classes(h, false, 0.0f, AdamStep{h->d_step, h->overflow, 0.9f, 0.999f, 1e-8f}, s);
+    launch_gather_grads(h, s);
     HIP_TRY(hipMemcpyAsync(dV, h->d_verts, 12ull * h->N, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(dD, h->d_dens, 4ull * h->M, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(dG, h->d_grads, 12ull * h->M, hipMemcpyDeviceToDevice, s));

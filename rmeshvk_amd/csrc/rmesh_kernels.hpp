@@ -187,7 +187,6 @@ struct AdamStep {
 // point at the first of them.
 struct ShBwdArgs {
     const float4* rec;
-    const uint32_t* idx;
     float* sh;               // parameters (updated in place when adam)
     uint32_t deg;
     uint32_t t0;
@@ -197,8 +196,7 @@ struct ShBwdArgs {
     const ViewParams* views;
     const float* d_colors;   // view v's at d_colors[3 (views[v].color_off + t)]
     float* d_sh;             // output gradient (when !adam)
-    float* d_grads;          // += (per tet, plain read-modify-write)
-    float* d_verts;          // += (atomic)
+    float* tet_g;            // += per-tet staging record (16 floats, the adjoint's); one quad owns a tet
     bool adam;
     float* m_sh;
     float* v_sh;
