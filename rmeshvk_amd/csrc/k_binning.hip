@@ -4,15 +4,17 @@
 //   project_compute.wgsl:88-262        -> k_project  (cull + conservative tile count)
 //   prepare_dispatch.wgsl + rts.wgsl   -> block sums in k_project + k_spine_scan
 //   tile_fill_compute.wgsl             -> sentinel padding inside k_pairs
-//   tile_gen_scan_compute.wgsl:61-203  -> k_pairs
+//   tile_gen_scan_compute.wgsl:61-203  -> k_pairs_sorted (k_pairs)
 //   tile_ranges_compute.wgsl:10-45     -> k_ranges
 //
-// MI355X design: one thread per tet looping over every view of the batch,
-// so a tet record (64 B, 4 x float4, coalesced across the wave) is fetched
-// once per step instead of once per view; pair offsets come from per-block
-// sums written by k_project, one spine scan, and a wave64 block scan redone
-// inside k_pairs — no offsets array and no atomics on the ordering path, so
-// the pre-sort pair order is deterministic (tet-id order within a view).
+// MI355X design: one thread per (tet, view) item of every view of the step,
+// tet-major, so the views of a tet share its record (64 B, 4 x float4) in one
+// wave.  Pair offsets come from per-block sums, one spine scan and a wave64
+// block scan redone in the writing kernel — no offsets array and no atomics
+// on the ordering path, so the pair order is deterministic.  The items are
+// sorted by depth key before pair generation (k_project writes the keys,
+// k_pairs_sorted emits in that order), so the pair sort needs only the tile
+// bits; k_pairs keeps the reference's item-order pairs with 64-bit keys.
 #include "rmesh_common.hpp"
 #include "rmesh_kernels.hpp"
 
@@ -80,64 +82,99 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ rec, uint32_
 }
 
 // Scanline tile-row extent at tile granularity (project_compute.wgsl:219-258,
-// tile_gen_scan_compute.wgsl:151-182).
-__device__ __forceinline__ bool tile_row_extent(const float proj[4][2], uint32_t ty, float T, float& xl, float& xr) {
-    xl = 1e10f;
-    xr = -1e10f;
-    const int EI[6] = {0, 0, 0, 1, 1, 2};
-    const int EJ[6] = {1, 2, 3, 2, 3, 3};
+// tile_gen_scan_compute.wgsl:151-182): a row's x range is the min / max over
+// the edge crossings of its top line y = ty T and its bottom line
+// y = (ty + 1) T, and over the vertices inside the band.  RowScan walks the
+// rows top to bottom and carries each bottom line's crossings over as the
+// next row's top line (the same expression, so the same values), which halves
+// the crossing divisions of every row after the first.  fminf / fmaxf give
+// the same extent in any order.
+struct RowScan {
+    float top_lo, top_hi;
+
+    // x range of the edge crossings of line y (1e10 / -1e10: none)
+    __device__ __forceinline__ static void line(const float proj[4][2], float y, float& lo, float& hi) {
+        constexpr int EI[6] = {0, 0, 0, 1, 1, 2};
+        constexpr int EJ[6] = {1, 2, 3, 2, 3, 3};
+        lo = 1e10f;
+        hi = -1e10f;
 #pragma unroll
-    for (int e = 0; e < 6; ++e) {
-        float yi = proj[EI[e]][1], yj = proj[EJ[e]][1];
-        float xi = proj[EI[e]][0], xj = proj[EJ[e]][0];
-#pragma unroll
-        for (uint32_t b = 0; b < 2; ++b) {
-            float ytest = (float)(ty + b) * T;
-            if ((yi <= ytest && yj > ytest) || (yj <= ytest && yi > ytest)) {
-                float tt = (ytest - yi) / (yj - yi);
-                float x = xi + tt * (xj - xi);
-                xl = fminf(xl, x);
-                xr = fmaxf(xr, x);
+        for (int e = 0; e < 6; ++e) {
+            const float yi = proj[EI[e]][1], yj = proj[EJ[e]][1];
+            const float xi = proj[EI[e]][0], xj = proj[EJ[e]][0];
+            if ((yi <= y && yj > y) || (yj <= y && yi > y)) {
+                const float tt = (y - yi) / (yj - yi);
+                const float x = xi + tt * (xj - xi);
+                lo = fminf(lo, x);
+                hi = fmaxf(hi, x);
             }
         }
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        float vy = proj[k][1];
-        if (vy >= (float)ty * T && vy <= (float)(ty + 1u) * T) {
-            xl = fminf(xl, proj[k][0]);
-            xr = fmaxf(xr, proj[k][0]);
-        }
+    __device__ __forceinline__ void start(const float proj[4][2], uint32_t ty0, float T) {
+        line(proj, (float)ty0 * T, top_lo, top_hi);
     }
-    return xl <= xr;
+    // extent of row ty (rows visited in increasing order from start's ty0)
+    __device__ __forceinline__ bool row(const float proj[4][2], uint32_t ty, float T, float& xl, float& xr) {
+        float bot_lo, bot_hi;
+        line(proj, (float)(ty + 1u) * T, bot_lo, bot_hi);
+        xl = fminf(top_lo, bot_lo);
+        xr = fmaxf(top_hi, bot_hi);
+        top_lo = bot_lo;
+        top_hi = bot_hi;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float vy = proj[k][1];
+            if (vy >= (float)ty * T && vy <= (float)(ty + 1u) * T) {
+                xl = fminf(xl, proj[k][0]);
+                xr = fmaxf(xr, proj[k][0]);
+            }
+        }
+        return xl <= xr;
+    }
+};
+
+// x / T.  For a power-of-two T the product with the exact reciprocal is the
+// same correctly rounded value, without the division sequence.
+template <bool POW2>
+__device__ __forceinline__ float div_tile(float x, float T, float invT) {
+    if constexpr (POW2) return x * invT;
+    else return x / T;
 }
 
 struct TileBox {
     uint32_t x0, x1, y0, y1;
 };
 
-// AABB in tiles (project_compute.wgsl:202-216)
-__device__ __forceinline__ bool tile_box(const float proj[4][2], float W, float H, float T, TileBox& b) {
+// AABB in tiles (project_compute.wgsl:202-216).  The tile counts per axis,
+// ceil(W / T) and ceil(H / T) there, are the view's tiles_x / tiles_y:
+// (W + T - 1) / T in integers, the same numbers for any image below 2^24
+// pixels a side (a non-integer W / T is at least 1 / T from an integer, far
+// more than its rounding error).
+template <bool POW2>
+__device__ __forceinline__ bool tile_box(const float proj[4][2], const ViewParams& vw, float T, float invT, TileBox& b) {
+    const float W = vw.Wf, H = vw.Hf;
     float pix_min_x = fmaxf(fminf(fminf(proj[0][0], proj[1][0]), fminf(proj[2][0], proj[3][0])), 0.0f);
     float pix_max_x = fminf(fmaxf(fmaxf(proj[0][0], proj[1][0]), fmaxf(proj[2][0], proj[3][0])), W - 1.0f);
     float pix_min_y = fmaxf(fminf(fminf(proj[0][1], proj[1][1]), fminf(proj[2][1], proj[3][1])), 0.0f);
     float pix_max_y = fminf(fmaxf(fmaxf(proj[0][1], proj[1][1]), fmaxf(proj[2][1], proj[3][1])), H - 1.0f);
     if (pix_min_x > pix_max_x || pix_min_y > pix_max_y) return false;
-    uint32_t tiles_x_total = sat_u32(ceilf(W / T));
-    b.x0 = sat_u32(pix_min_x / T);
-    b.x1 = min(sat_u32(pix_max_x / T), tiles_x_total - 1u);
-    b.y0 = sat_u32(pix_min_y / T);
-    b.y1 = min(sat_u32(pix_max_y / T), sat_u32(ceilf(H / T)) - 1u);
+    b.x0 = sat_u32(div_tile<POW2>(pix_min_x, T, invT));
+    b.x1 = min(sat_u32(div_tile<POW2>(pix_max_x, T, invT)), vw.tiles_x - 1u);
+    b.y0 = sat_u32(div_tile<POW2>(pix_min_y, T, invT));
+    b.y1 = min(sat_u32(div_tile<POW2>(pix_max_y, T, invT)), vw.tiles_y - 1u);
     return true;
 }
 
-__device__ __forceinline__ void row_tiles(float xl, float xr, float T, const TileBox& b, uint32_t& tx0, uint32_t& tx1) {
-    tx0 = uclamp(sat_u32(fmaxf(floorf(xl / T), 0.0f)), b.x0, b.x1);
-    tx1 = uclamp(sat_u32(fmaxf(floorf(xr / T), 0.0f)), b.x0, b.x1);
+template <bool POW2>
+__device__ __forceinline__ void row_tiles(float xl, float xr, float T, float invT, const TileBox& b, uint32_t& tx0,
+                                          uint32_t& tx1) {
+    tx0 = uclamp(sat_u32(fmaxf(floorf(div_tile<POW2>(xl, T, invT)), 0.0f)), b.x0, b.x1);
+    tx1 = uclamp(sat_u32(fmaxf(floorf(div_tile<POW2>(xr, T, invT)), 0.0f)), b.x0, b.x1);
 }
 
 // K1 for one (tet, view): tile count; `visible` is the reference's vis_idx
 // flag (project_compute.wgsl:111-262).
+template <bool POW2>
 __device__ __forceinline__ uint32_t project_count(const V3 v[4], float dens, const ViewParams& vw, bool& visible) {
     float p[4][4];
 #pragma unroll
@@ -187,15 +224,17 @@ __device__ __forceinline__ uint32_t project_count(const V3 v[4], float dens, con
         proj[i][0] = ((p[i][0] + 1.0f) * 0.5f) * W;
         proj[i][1] = ((1.0f - p[i][1]) * 0.5f) * H;
     }
-    float T = (float)vw.tile_size;
+    const float T = (float)vw.tile_size, invT = 1.0f / T;
     TileBox b;
-    if (!tile_box(proj, W, H, T, b)) return 0;
+    if (!tile_box<POW2>(proj, vw, T, invT, b)) return 0;
     uint32_t count = 0;
+    RowScan rs;
+    rs.start(proj, b.y0, T);
     for (uint32_t ty = b.y0; ty <= b.y1; ++ty) {
         float xl, xr;
-        if (tile_row_extent(proj, ty, T, xl, xr)) {
+        if (rs.row(proj, ty, T, xl, xr)) {
             uint32_t tx0, tx1;
-            row_tiles(xl, xr, T, b, tx0, tx1);
+            row_tiles<POW2>(xl, xr, T, invT, b, tx0, tx1);
             count += tx1 - tx0 + 1u;
         }
     }
@@ -212,16 +251,38 @@ __device__ __forceinline__ void item_tet_view(const SceneRange* __restrict__ sc,
     vv = r.view_base + (uint32_t)(local % r.V);
 }
 
+// Depth sort values: item << 5 | count, the count saturating at 31 (then read
+// from counts[item]).
+constexpr uint32_t kCountBits = 5, kCountEsc = (1u << kCountBits) - 1u;
+__device__ __forceinline__ uint32_t sorted_count(uint32_t sv, const uint32_t* __restrict__ counts) {
+    const uint32_t c = sv & kCountEsc;
+    return c == kCountEsc ? counts[sv >> kCountBits] : c;
+}
+
+// Sort key of a tet's depth in a view: ~bits(|c - cam|^2 - r^2) over its
+// circumsphere (c, r^2) (tile_gen_scan_compute.wgsl:135-143): ascending key =
+// descending power distance, the reference's back-to-front order.
+__device__ __forceinline__ uint32_t depth_key(const float4& cd, const ViewParams& vw) {
+    const V3 diff = sub3(mk3(cd.x, cd.y, cd.z), mk3(vw.cam[0], vw.cam[1], vw.cam[2]));
+    const float depth = dot3(diff, diff) - cd.w;
+    return ~__float_as_uint(depth);
+}
+
 // K1, batched: one thread per (tet, view) item of every scene (item =
 // scene's item base + local tet * V + local view: the pair order of the
 // reference within every view is tet-major).  counts[item]; per-block pair
 // totals for the scan; visible tets of view 0 (the reference's
-// indirect_args.instance_count, project_compute.wgsl:181).
-__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec, const SceneRange* __restrict__ scenes,
-                                                 uint32_t n_scenes, uint64_t n_items,
-                                                 const ViewParams* __restrict__ views,
+// indirect_args.instance_count, project_compute.wgsl:181).  With dkey set,
+// also the item's depth sort key (0xFFFFFFFF for items without pairs) and
+// its sort value item << 5 | min(count, 31) (31: look the count up; items
+// < 2^27), the input of the depth sort of items, and the item count for it.
+template <bool POW2>
+__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec, const float4* __restrict__ circ,
+                                                 const SceneRange* __restrict__ scenes, uint32_t n_scenes,
+                                                 uint64_t n_items, const ViewParams* __restrict__ views,
                                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ block_sums,
-                                                 uint32_t* __restrict__ vis_count) {
+                                                 uint32_t* __restrict__ vis_count, uint32_t* __restrict__ dkey,
+                                                 uint32_t* __restrict__ ditem, uint32_t* __restrict__ n_items_out) {
     __shared__ uint32_t red[256 / kWave];
     __shared__ uint32_t s_vis;
     if (threadIdx.x == 0) s_vis = 0;
@@ -237,10 +298,16 @@ __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec,
         V3 g;
         load_rec(rec, t, v, dens, g);
         bool vis = false;
-        c = project_count(v, dens, views[vv], vis);
+        const ViewParams& vw = views[vv];
+        c = project_count<POW2>(v, dens, vw, vis);
         counts[item] = c;
         vis0 = vis && vv == 0;
+        if (dkey) {
+            dkey[item] = c ? depth_key(circ[t], vw) : 0xFFFFFFFFu;
+            ditem[item] = ((uint32_t)item << kCountBits) | min(c, kCountEsc);
+        }
     }
+    if (dkey && blockIdx.x == 0 && threadIdx.x == 0) *n_items_out = (uint32_t)n_items;
     const uint64_t bal = __ballot(vis0);
     __syncthreads();
     if (__lane_id() == 0 && bal) atomicAdd(&s_vis, (uint32_t)__popcll(bal));
@@ -253,28 +320,44 @@ __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec,
 }
 
 // Exclusive scan of block sums in place (single workgroup of 1024 threads,
-// 4096 sums per round, each thread 4 consecutive ones: coalesced); writes the
-// grand total (number of pairs P) to *total.  The running total is kept in 64
-// bits: past 2^32 - 16 pairs the offsets would wrap, so *total saturates at
-// 0xFFFFFFFF (the host refuses that count) and the overflow flag is raised
-// (a captured step then leaves the parameters untouched).
+// 16384 sums per round, each thread 16 consecutive ones: four 16-byte loads);
+// writes the grand total (number of pairs P) to *total.  The running total is
+// kept in 64 bits: past 2^32 - 16 pairs the offsets would wrap, so *total
+// saturates at 0xFFFFFFFF (the host refuses that count) and the overflow flag
+// is raised (a captured step then leaves the parameters untouched).
+constexpr uint32_t kSpinePer = 16;
 __global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums, uint32_t n, uint32_t* __restrict__ total,
                                                      uint32_t* __restrict__ overflow) {
     __shared__ uint32_t red[1024 / kWave];
     __shared__ unsigned long long s_round;  // the round's total in 64 bits
     uint64_t carry = 0;
-    for (uint32_t base = 0; base < n; base += 4096) {
-        const uint32_t i0 = base + 4u * threadIdx.x;
-        uint32_t x[4];
+    for (uint32_t base = 0; base < n; base += kSpinePer * 1024u) {
+        const uint32_t i0 = base + kSpinePer * threadIdx.x;
+        uint32_t x[kSpinePer];
+        if (i0 + kSpinePer <= n) {
+            const uint4* p = reinterpret_cast<const uint4*>(sums + i0);  // i0 is a multiple of 16: aligned
 #pragma unroll
-        for (int k = 0; k < 4; ++k) x[k] = i0 + k < n ? sums[i0 + k] : 0u;
+            for (uint32_t q = 0; q < kSpinePer / 4; ++q) {
+                const uint4 v = p[q];
+                x[4 * q] = v.x;
+                x[4 * q + 1] = v.y;
+                x[4 * q + 2] = v.z;
+                x[4 * q + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < kSpinePer; ++k) x[k] = i0 + k < n ? sums[i0 + k] : 0u;
+        }
+        uint64_t loc = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kSpinePer; ++k) loc += x[k];
         if (threadIdx.x == 0) s_round = 0ull;
         __syncthreads();
-        atomicAdd(&s_round, ((unsigned long long)x[0] + x[1]) + ((unsigned long long)x[2] + x[3]));
+        atomicAdd(&s_round, (unsigned long long)loc);
         uint32_t tot;
-        uint32_t ex = block_excl_scan((x[0] + x[1]) + (x[2] + x[3]), red, tot) + (uint32_t)carry;  // ends with a barrier
+        uint32_t ex = block_excl_scan((uint32_t)loc, red, tot) + (uint32_t)carry;  // ends with a barrier
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (uint32_t k = 0; k < kSpinePer; ++k) {
             if (i0 + k < n) sums[i0 + k] = ex;
             ex += x[k];
         }
@@ -288,8 +371,80 @@ __global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums
     }
 }
 
-// K7 (+ K6 sentinel padding): thread per tet, loop over views, pairs written
-// at the scanned offset.  Keys: (global_tile << 32) | ~bits(depth).
+// K7 (+ K6 sentinel padding) for one item: its max_write pairs at
+// [start, start + max_write).  64-bit keys: (key tile << 32) | ~bits(depth);
+// 32-bit keys: the key tile alone (pairs already in depth order).
+template <bool POW2, typename KT>
+__device__ __forceinline__ void emit_item_pairs(uint64_t item, uint32_t start, uint32_t max_write,
+                                                const float4* __restrict__ rec, const float4* __restrict__ circ,
+                                                const SceneRange* __restrict__ scenes, uint32_t n_scenes,
+                                                const ViewParams* __restrict__ views, KT* __restrict__ keys,
+                                                uint32_t* __restrict__ vals, uint32_t cap,
+                                                uint32_t* __restrict__ overflow) {
+    constexpr bool FULL = sizeof(KT) == 8;
+    uint32_t t, vv;
+    item_tet_view(scenes, n_scenes, item, t, vv);
+    V3 v[4];
+    float dens;
+    V3 g;
+    load_rec(rec, t, v, dens, g);
+    const ViewParams& vw = views[vv];
+    const float W = vw.Wf, H = vw.Hf, T = (float)vw.tile_size, invT = 1.0f / T;
+    float proj[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float c[4];
+        mul_vp(vw.vp, v[i], c);
+        float nx = c[0] / (c[3] + 1e-6f);
+        float ny = c[1] / (c[3] + 1e-6f);
+        proj[i][0] = ((nx + 1.0f) * 0.5f) * W;
+        proj[i][1] = ((1.0f - ny) * 0.5f) * H;
+    }
+    uint32_t local = 0;
+    TileBox b;
+    if (tile_box<POW2>(proj, vw, T, invT, b)) {
+        uint32_t inv_depth = 0u;
+        if constexpr (FULL) inv_depth = depth_key(circ[t], vw);
+        RowScan rs;
+        rs.start(proj, b.y0, T);
+        for (uint32_t ty = b.y0; ty <= b.y1 && local < max_write; ++ty) {
+            float xl, xr;
+            if (!rs.row(proj, ty, T, xl, xr)) continue;
+            uint32_t tx0, tx1;
+            row_tiles<POW2>(xl, xr, T, invT, b, tx0, tx1);
+            for (uint32_t tx = tx0; tx <= tx1 && local < max_write; ++tx) {
+                uint32_t gt = vw.key_base + ty * vw.tiles_x + tx;
+                uint32_t wi = start + local;
+                if (wi < cap) {
+                    if constexpr (FULL) keys[wi] = ((uint64_t)gt << 32) | inv_depth;
+                    else keys[wi] = gt;
+                    vals[wi] = t;
+                } else {
+                    *overflow = 1u;
+                }
+                local++;
+            }
+        }
+    }
+    // Slots counted by K1 but not produced by the division-based
+    // re-projection keep the tile_fill sentinel (tile_gen_scan_compute.wgsl:202).
+    KT sentinel;
+    if constexpr (FULL) sentinel = ((uint64_t)vw.sentinel_tile << 32) | 0xFFFFFFFFull;
+    else sentinel = vw.sentinel_tile;
+    for (; local < max_write; ++local) {
+        uint32_t wi = start + local;
+        if (wi < cap) {
+            keys[wi] = sentinel;
+            vals[wi] = 0u;
+        } else {
+            *overflow = 1u;
+        }
+    }
+}
+
+// Pairs in item order (the reference's pre-sort order): thread per item,
+// offsets from the item-order block sums and a block scan redone here.
+template <bool POW2>
 __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, const float4* __restrict__ circ,
                                                const SceneRange* __restrict__ scenes, uint32_t n_scenes, uint64_t n_items,
                                                const ViewParams* __restrict__ views,
@@ -303,61 +458,66 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, c
     uint32_t tot;
     const uint32_t start = block_excl_scan(max_write, red, tot) + block_off[blockIdx.x];
     if (max_write == 0) return;
-    uint32_t t, vv;
-    item_tet_view(scenes, n_scenes, item, t, vv);
-    V3 v[4];
-    float dens;
-    V3 g;
-    load_rec(rec, t, v, dens, g);
-    const float4 cd = circ[t];
-    const ViewParams& vw = views[vv];
-    const float W = vw.Wf, H = vw.Hf, T = (float)vw.tile_size;
-    float proj[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float c[4];
-        mul_vp(vw.vp, v[i], c);
-        float nx = c[0] / (c[3] + 1e-6f);
-        float ny = c[1] / (c[3] + 1e-6f);
-        proj[i][0] = ((nx + 1.0f) * 0.5f) * W;
-        proj[i][1] = ((1.0f - ny) * 0.5f) * H;
-    }
-    uint32_t local = 0;
-    TileBox b;
-    if (tile_box(proj, W, H, T, b)) {
-        V3 diff = sub3(mk3(cd.x, cd.y, cd.z), mk3(vw.cam[0], vw.cam[1], vw.cam[2]));
-        float depth = dot3(diff, diff) - cd.w;
-        uint32_t inv_depth = ~__float_as_uint(depth);
-        for (uint32_t ty = b.y0; ty <= b.y1 && local < max_write; ++ty) {
-            float xl, xr;
-            if (!tile_row_extent(proj, ty, T, xl, xr)) continue;
-            uint32_t tx0, tx1;
-            row_tiles(xl, xr, T, b, tx0, tx1);
-            for (uint32_t tx = tx0; tx <= tx1 && local < max_write; ++tx) {
-                uint32_t gt = vw.key_base + ty * vw.tiles_x + tx;
-                uint32_t wi = start + local;
-                if (wi < cap) {
-                    keys[wi] = ((uint64_t)gt << 32) | inv_depth;
-                    vals[wi] = t;
-                } else {
-                    *overflow = 1u;
-                }
-                local++;
-            }
-        }
-    }
-    // Slots counted by K1 but not produced by the division-based
-    // re-projection keep the tile_fill sentinel (tile_gen_scan_compute.wgsl:202).
-    const uint64_t sentinel = ((uint64_t)vw.sentinel_tile << 32) | 0xFFFFFFFFull;
-    for (; local < max_write; ++local) {
-        uint32_t wi = start + local;
-        if (wi < cap) {
-            keys[wi] = sentinel;
-            vals[wi] = 0u;
-        } else {
-            *overflow = 1u;
-        }
-    }
+    emit_item_pairs<POW2, uint64_t>(item, start, max_write, rec, circ, scenes, n_scenes, views, keys, vals, cap,
+                                    overflow);
+}
+
+// Depth-ordered pair generation.  The items, sorted stably by depth key
+// (sorted values sval), emit their pairs in that order, so a stable sort of
+// the pairs on the tile bits alone gives, inside every tile, the order by
+// (depth key, item) — exactly the order of the stable (tile, depth) sort of
+// the pairs in item order.  Two passes over the sorted items: per-block pair
+// totals, then (after the spine scan) the pairs at their offsets.  The pair
+// writes are sequential; the tet records are read in depth order (gathers
+// from a table that stays in the last-level cache), which measured cheaper
+// than writing the pairs by item to scattered offsets.
+__global__ __launch_bounds__(256) void k_sorted_sums(const uint32_t* __restrict__ sval, uint32_t n_items,
+                                                     const uint32_t* __restrict__ counts,
+                                                     uint32_t* __restrict__ block_sums) {
+    __shared__ uint32_t red[256 / kWave];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = i < n_items ? sorted_count(sval[i], counts) : 0u;
+    uint32_t tot;
+    (void)block_excl_scan(c, red, tot);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+template <bool POW2>
+__global__ __launch_bounds__(256) void k_pairs_sorted(const float4* __restrict__ rec,
+                                                      const SceneRange* __restrict__ scenes, uint32_t n_scenes,
+                                                      const uint32_t* __restrict__ sval, uint32_t n_items,
+                                                      const ViewParams* __restrict__ views,
+                                                      const uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ block_off,
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                      uint32_t cap, uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t red[256 / kWave];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t sv = i < n_items ? sval[i] : 0u;
+    const uint32_t max_write = i < n_items ? sorted_count(sv, counts) : 0u;
+    uint32_t tot;
+    const uint32_t start = block_excl_scan(max_write, red, tot) + block_off[blockIdx.x];
+    if (max_write == 0) return;
+    emit_item_pairs<POW2, uint32_t>(sv >> kCountBits, start, max_write, rec, nullptr, scenes, n_scenes, views, keys,
+                                    vals, cap, overflow);
+}
+
+// The 64-bit keys of depth-ordered pairs (key tile, sorted values): the depth
+// key recomputed from the tet's circumsphere and its view's camera; the
+// sentinels' full key.  For inspection (the 64-bit key lists of the
+// reference's sort), not on the step path.
+__global__ __launch_bounds__(256) void k_full_keys(const uint32_t* __restrict__ ktile, const uint32_t* __restrict__ vals,
+                                                   const uint32_t* __restrict__ n_ptr, uint32_t cap,
+                                                   const float4* __restrict__ circ, const ViewParams* __restrict__ views,
+                                                   const uint32_t* __restrict__ ktile_gtile, uint32_t n_ktiles,
+                                                   const uint32_t* __restrict__ tile_view, uint64_t* __restrict__ out) {
+    const uint32_t n = min(*n_ptr, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t kt = ktile[i];
+    const uint32_t gt = kt < n_ktiles ? ktile_gtile[kt] : 0xFFFFFFFFu;
+    const uint32_t d = gt == 0xFFFFFFFFu ? 0xFFFFFFFFu : depth_key(circ[vals[i]], views[tile_view[gt]]);
+    out[i] = ((uint64_t)kt << 32) | d;
 }
 
 // K11: per-tile [start, end) in the sorted pair list, including the
@@ -366,11 +526,14 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ rec, c
 // to global tiles through ktile_gtile (0xFFFFFFFF: a view's sentinel tile;
 // the key tile 0xFFFFFFFF is the reference's own sentinel), so each view's
 // ranges are those of the reference's per-view sort.
-__device__ __forceinline__ uint32_t range_tile(uint64_t key, const uint32_t* __restrict__ ktile_gtile, uint32_t n_ktiles) {
-    const uint32_t kt = (uint32_t)(key >> 32);
+// The key tile: the high word of a 64-bit key, a 32-bit key itself.
+template <typename KT>
+__device__ __forceinline__ uint32_t range_tile(KT key, const uint32_t* __restrict__ ktile_gtile, uint32_t n_ktiles) {
+    const uint32_t kt = sizeof(KT) == 8 ? (uint32_t)((uint64_t)key >> 32) : (uint32_t)key;
     return kt < n_ktiles ? ktile_gtile[kt] : 0xFFFFFFFFu;
 }
-__global__ void k_ranges(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr, uint32_t cap,
+template <typename KT>
+__global__ void k_ranges(const KT* __restrict__ keys, const uint32_t* __restrict__ n_ptr, uint32_t cap,
                          uint32_t* __restrict__ ranges, const uint32_t* __restrict__ ktile_gtile, uint32_t n_ktiles) {
     uint32_t n = min(*n_ptr, cap);
     uint32_t stride = gridDim.x * blockDim.x;
@@ -412,21 +575,47 @@ void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t
     hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, src, dst, n16);
 }
 uint32_t item_blocks(uint64_t n_items) { return (uint32_t)((n_items + 255) / 256); }
-void launch_project(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
-                    const ViewParams* views, uint32_t* counts, uint32_t* block_sums, uint32_t* vis_count, hipStream_t s) {
+static bool pow2(uint32_t ts) { return ts && !(ts & (ts - 1u)); }
+void launch_project(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes,
+                    uint64_t n_items, const ViewParams* views, uint32_t tile_size, uint32_t* counts,
+                    uint32_t* block_sums, uint32_t* vis_count, uint32_t* dkey, uint32_t* ditem,
+                    uint32_t* n_items_out, hipStream_t s) {
     if (!n_items) return;
-    hipLaunchKernelGGL(k_project, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, scenes, n_scenes, n_items, views,
-                       counts, block_sums, vis_count);
+    if (pow2(tile_size))
+        hipLaunchKernelGGL(k_project<true>, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes,
+                           n_items, views, counts, block_sums, vis_count, dkey, ditem, n_items_out);
+    else
+        hipLaunchKernelGGL(k_project<false>, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes,
+                           n_items, views, counts, block_sums, vis_count, dkey, ditem, n_items_out);
 }
 void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s) {
     hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total, overflow);
 }
 void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
-                  const ViewParams* views, const uint32_t* counts, const uint32_t* block_off, uint64_t* keys,
-                  uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s) {
+                  const ViewParams* views, uint32_t tile_size, const uint32_t* counts, const uint32_t* block_off,
+                  uint64_t* keys, uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s) {
     if (!n_items) return;
-    hipLaunchKernelGGL(k_pairs, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes, n_items, views,
-                       counts, block_off, keys, vals, cap, overflow);
+    if (pow2(tile_size))
+        hipLaunchKernelGGL(k_pairs<true>, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes,
+                           n_items, views, counts, block_off, keys, vals, cap, overflow);
+    else
+        hipLaunchKernelGGL(k_pairs<false>, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes,
+                           n_items, views, counts, block_off, keys, vals, cap, overflow);
+}
+void launch_pairs_depth_ordered(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, const uint32_t* sval,
+                                uint32_t n_items, const ViewParams* views, uint32_t tile_size, const uint32_t* counts,
+                                uint32_t* block_sums, uint32_t* n_pairs, uint32_t* keys, uint32_t* vals, uint32_t cap,
+                                uint32_t* overflow, hipStream_t s) {
+    if (!n_items) return;
+    const dim3 gr(item_blocks(n_items)), bl(256);
+    hipLaunchKernelGGL(k_sorted_sums, gr, bl, 0, s, sval, n_items, counts, block_sums);
+    launch_spine_scan(block_sums, item_blocks(n_items), n_pairs, overflow, s);  // the same total P
+    if (pow2(tile_size))
+        hipLaunchKernelGGL(k_pairs_sorted<true>, gr, bl, 0, s, rec, scenes, n_scenes, sval, n_items, views, counts,
+                           block_sums, keys, vals, cap, overflow);
+    else
+        hipLaunchKernelGGL(k_pairs_sorted<false>, gr, bl, 0, s, rec, scenes, n_scenes, sval, n_items, views, counts,
+                           block_sums, keys, vals, cap, overflow);
 }
 // Tiles heaviest first: the compositing grids take their tiles in this order,
 // so the few very dense tiles (tens of times the median pair count) start
@@ -535,12 +724,28 @@ bool launch_tile_order_scenes(const uint32_t* ranges, uint32_t n_tiles, const ui
     return true;
 }
 
-void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
-                   const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s) {
+static uint32_t range_blocks(uint32_t cap) {
     uint32_t blocks = (cap + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, s, keys, n_ptr, cap, ranges, ktile_gtile, n_ktiles);
+    return blocks;
+}
+void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
+                   const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s) {
+    hipLaunchKernelGGL(k_ranges<uint64_t>, dim3(range_blocks(cap)), dim3(256), 0, s, keys, n_ptr, cap, ranges,
+                       ktile_gtile, n_ktiles);
+}
+void launch_ranges32(const uint32_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
+                     const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s) {
+    hipLaunchKernelGGL(k_ranges<uint32_t>, dim3(range_blocks(cap)), dim3(256), 0, s, keys, n_ptr, cap, ranges,
+                       ktile_gtile, n_ktiles);
+}
+void launch_full_keys(const uint32_t* ktile, const uint32_t* vals, const uint32_t* n_ptr, uint32_t cap,
+                      const float4* circ, const ViewParams* views, const uint32_t* ktile_gtile, uint32_t n_ktiles,
+                      const uint32_t* tile_view, uint64_t* out, uint32_t n_host, hipStream_t s) {
+    if (!n_host) return;
+    hipLaunchKernelGGL(k_full_keys, dim3((n_host + 255) / 256), dim3(256), 0, s, ktile, vals, n_ptr, cap, circ, views,
+                       ktile_gtile, n_ktiles, tile_view, out);
 }
 
 }  // namespace rmesh

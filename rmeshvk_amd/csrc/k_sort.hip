@@ -1,7 +1,9 @@
-// Stable LSD radix sort of (u64 key, u32 value) pairs, 8-bit digits over the
-// low `sorting_bits` bits of the key — the semantics of rmesh-sort's
+// Stable LSD radix sort of (u64 or u32 key, u32 value) pairs, 8-bit digits
+// over the low `sorting_bits` bits of the key — the semantics of rmesh-sort's
 // DeviceRadixSort (crates/rmesh-sort/src/drs.rs:23-26,180-279): bits above
-// sorting_bits are ignored and equal masked keys keep their input order.
+// sorting_bits are ignored and equal masked keys keep their input order.  The
+// passes may also start above bit 0 (input already ordered by the bits below:
+// LSD sorting over the remaining digits only, still stable).
 //
 // MI355X design (not the reference's upsweep/scan/downsweep port):
 //  * fixed grid of G blocks, each owning one contiguous chunk of the input
@@ -41,8 +43,8 @@ __device__ __forceinline__ void chunk_of(uint32_t n, uint32_t G, uint32_t b, uin
 // DB: digit width (8: 256 buckets, 6 passes over 48 key bits; 4: 16 buckets,
 // twice the passes with a quarter of the ballots per key each); chosen by the
 // autotuner.
-template <int DB>
-__global__ __launch_bounds__(kSortThreads) void k_rs_upsweep(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr,
+template <int DB, typename K>
+__global__ __launch_bounds__(kSortThreads) void k_rs_upsweep(const K* __restrict__ keys, const uint32_t* __restrict__ n_ptr,
                                                              uint32_t cap, uint32_t shift, uint32_t* __restrict__ hist) {
     constexpr uint32_t RADIX = 1u << DB, DMASK = RADIX - 1u;
     __shared__ uint32_t h[kSortWaves][RADIX];
@@ -57,7 +59,7 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_upsweep(const uint64_t* __r
     constexpr int U = 8;
     uint32_t i0 = lo + threadIdx.x;
     for (; i0 + (U - 1) * kSortThreads < hi; i0 += U * kSortThreads) {
-        uint64_t kk[U];
+        K kk[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) kk[u] = keys[i0 + u * kSortThreads];
 #pragma unroll
@@ -94,14 +96,14 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, ui
     if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
-template <int DB>
-__global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                               uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+template <int DB, typename K>
+__global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                               K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                                const uint32_t* __restrict__ n_ptr, uint32_t cap, uint32_t shift,
                                                                const uint32_t* __restrict__ hist, const uint32_t* __restrict__ totals) {
     constexpr uint32_t RADIX = 1u << DB, DMASK = RADIX - 1u;
     static_assert(RADIX <= (uint32_t)kSortThreads, "one thread per digit");
-    __shared__ uint64_t s_keys[kTileKeys];
+    __shared__ K s_keys[kTileKeys];
     __shared__ uint32_t s_vals[kTileKeys];
     __shared__ uint32_t s_wcnt[kSortWaves][RADIX];
     __shared__ uint32_t s_loc[RADIX];
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
     for (uint32_t base = lo; base < hi; base += kTileKeys) {
         for (int i = tid; i < kSortWaves * (int)RADIX; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
-        uint64_t key[kRounds];
+        K key[kRounds];
         uint32_t val[kRounds];
         uint32_t dr[kRounds];  // digit | rank << 9 ; digit 256 marks an empty slot
         // all of the tile's loads first (one memory latency per tile, not one
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
         for (int j = 0; j < kRounds; ++j) {
             uint32_t i = base + w * kWaveKeys + j * kWave + lane;
             bool valid = i < hi;
-            key[j] = valid ? kin[i] : 0ull;
+            key[j] = valid ? kin[i] : K(0);
             val[j] = valid ? vin[i] : 0u;
         }
         // Phase 1: per-wave stable ranking, 64 keys per round.
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
         // Phase 4: coalesced scatter, runs per digit.
         uint32_t tile_n = min((uint32_t)kTileKeys, hi - base);
         for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
-            uint64_t k = s_keys[i];
+            const K k = s_keys[i];
             uint32_t d = (uint32_t)(k >> shift) & DMASK;
             uint32_t gp = s_glob[d] + (i - s_loc[d]);
             kout[gp] = k;
@@ -204,9 +206,11 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_downsweep(const uint64_t* _
 }
 
 // Blocks of the downsweep that are resident at once on the current device
-// (CUs x blocks per CU at its LDS and register use), cached per device.  A
-// grid of at most that many blocks runs every block of a pass in one round:
-// no second, partly filled round of blocks at the end of each pass.
+// (CUs x blocks per CU at its LDS and register use, for the key width K),
+// cached per device.  A grid of at most that many blocks runs every block of
+// a pass in one round: no second, partly filled round of blocks at the end of
+// each pass.
+template <typename K>
 static uint32_t resident_sort_blocks() {
     static std::mutex mu;
     static uint32_t cached[64] = {};
@@ -216,9 +220,9 @@ static uint32_t resident_sort_blocks() {
     if (!cached[dev]) {
         int cus = 0, b8 = 0, b4 = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b8, reinterpret_cast<const void*>(k_rs_downsweep<8>),
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b8, reinterpret_cast<const void*>(k_rs_downsweep<8, K>),
                                                          kSortThreads, 0) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b4, reinterpret_cast<const void*>(k_rs_downsweep<4>),
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b4, reinterpret_cast<const void*>(k_rs_downsweep<4, K>),
                                                          kSortThreads, 0) != hipSuccess ||
             cus <= 0 || b8 <= 0 || b4 <= 0) {
             cached[dev] = 2048u;
@@ -229,42 +233,51 @@ static uint32_t resident_sort_blocks() {
     return cached[dev];
 }
 
-uint32_t sort_grid(uint32_t cap) {
+template <typename K>
+static uint32_t sort_grid(uint32_t cap) {
     uint32_t g = (cap + kTileKeys - 1) / kTileKeys;
-    const uint32_t r = resident_sort_blocks();
+    const uint32_t r = resident_sort_blocks<K>();
     if (g > r) g = r;
     if (g > 2048) g = 2048;
     if (g < 1) g = 1;
     return g;
 }
 
-// Sorts the pairs in (ka, va); returns true when the result ended in (kb, vb).
-template <int DB>
-static bool radix_passes(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
-                         uint32_t sorting_bits, uint32_t* hist, hipStream_t s) {
+// Sorts the pairs in (ka, va) on key bits [first_bit, end_bit); returns true
+// when the result ended in (kb, vb).
+template <int DB, typename K>
+static bool radix_passes(K* ka, uint32_t* va, K* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
+                         uint32_t first_bit, uint32_t end_bit, uint32_t* hist, hipStream_t s) {
     constexpr uint32_t RADIX = 1u << DB;
-    uint32_t G = sort_grid(cap);
+    uint32_t G = sort_grid<K>(cap);
     bool in_b = false;
-    for (uint32_t shift = 0; shift < sorting_bits; shift += DB) {
-        const uint64_t* kin = in_b ? kb : ka;
+    for (uint32_t shift = first_bit; shift < end_bit; shift += DB) {
+        const K* kin = in_b ? kb : ka;
         const uint32_t* vin = in_b ? vb : va;
-        uint64_t* kout = in_b ? ka : kb;
+        K* kout = in_b ? ka : kb;
         uint32_t* vout = in_b ? va : vb;
-        hipLaunchKernelGGL(k_rs_upsweep<DB>, dim3(G), dim3(kSortThreads), 0, s, kin, n_ptr, cap, shift, hist);
+        hipLaunchKernelGGL((k_rs_upsweep<DB, K>), dim3(G), dim3(kSortThreads), 0, s, kin, n_ptr, cap, shift, hist);
         hipLaunchKernelGGL(k_rs_scan, dim3(RADIX), dim3(256), 0, s, hist, G, hist + (uint64_t)RADIX * G);
-        hipLaunchKernelGGL(k_rs_downsweep<DB>, dim3(G), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_ptr, cap, shift,
-                           hist, hist + (uint64_t)RADIX * G);
+        hipLaunchKernelGGL((k_rs_downsweep<DB, K>), dim3(G), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n_ptr, cap,
+                           shift, hist, hist + (uint64_t)RADIX * G);
         in_b = !in_b;
     }
     return in_b;
 }
 
 bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
-                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s, uint32_t digit_bits) {
-    if (digit_bits == 4) return radix_passes<4>(ka, va, kb, vb, n_ptr, cap, sorting_bits, hist, s);
-    return radix_passes<8>(ka, va, kb, vb, n_ptr, cap, sorting_bits, hist, s);
+                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s, uint32_t digit_bits, uint32_t first_bit) {
+    if (digit_bits == 4) return radix_passes<4>(ka, va, kb, vb, n_ptr, cap, first_bit, sorting_bits, hist, s);
+    return radix_passes<8>(ka, va, kb, vb, n_ptr, cap, first_bit, sorting_bits, hist, s);
 }
 
-size_t radix_hist_words(uint32_t cap) { return 256ull * sort_grid(cap) + 256; }
+bool launch_radix_sort32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
+                         uint32_t* hist, hipStream_t s, uint32_t digit_bits, uint32_t end_bit) {
+    if (digit_bits == 4) return radix_passes<4>(ka, va, kb, vb, n_ptr, cap, 0u, end_bit, hist, s);
+    return radix_passes<8>(ka, va, kb, vb, n_ptr, cap, 0u, end_bit, hist, s);
+}
+
+size_t radix_hist_words(uint32_t cap) { return 256ull * sort_grid<uint64_t>(cap) + 256; }
+size_t radix_hist_words32(uint32_t cap) { return 256ull * sort_grid<uint32_t>(cap) + 256; }
 
 }  // namespace rmesh

@@ -128,6 +128,12 @@ struct rmesh_renderer {
     uint32_t *vals_a = nullptr, *vals_b = nullptr;
     uint32_t cap = 0;
     uint32_t* hist = nullptr;
+    // depth sort of the (tet, view) items before pair generation: keys, item
+    // indices (ping-pong), its histogram, the item count on the device, the
+    // block sums of the depth-ordered pair scan
+    uint32_t *dkey_a = nullptr, *dkey_b = nullptr, *ditem_a = nullptr, *ditem_b = nullptr;
+    uint32_t *ihist = nullptr, *d_nitems = nullptr, *block_sums2 = nullptr;
+    bool pairs_depth_ordered = false;  // the last pair generation emitted items in depth order
     uint32_t* ranges = nullptr;
     uint32_t* tile_order = nullptr;  // compositing workgroup -> global tile, heaviest first
     bool sorted_in_b = false;
@@ -176,6 +182,16 @@ struct rmesh_renderer {
     uint32_t gtiles(uint32_t) const { return n_gt; }
     uint32_t sorting_bits(uint32_t) const { return rmesh_sorting_bits(n_kt); }
     bool multi() const { return sc.size() > 1; }
+    // Pairs generated in depth order of their items, then sorted on the tile
+    // bits only.  Not when one view's sentinel key (tile 0xFFFFFFFF) masks to
+    // the last real tile (tile count 2^(sorting bits - 32)): there the sentinels
+    // share a masked tile with real pairs and the full (tile, depth) sort
+    // orders them.
+    bool depth_first() const {
+        if (!dkey_a) return false;
+        const uint32_t tb = sorting_bits(0) - 32u;
+        return !(n_views == 1 && tb < 32u && n_kt == (1u << tb));
+    }
 };
 
 extern "C" uint32_t rmesh_sorting_bits(uint32_t num_tiles) {
@@ -345,7 +361,7 @@ rmesh_status ensure_capacity(rmesh_renderer* h, uint64_t need) {
     HIP_TRY(dalloc(&h->keys_b, c));
     HIP_TRY(dalloc(&h->vals_a, c));
     HIP_TRY(dalloc(&h->vals_b, c));
-    HIP_TRY(dalloc(&h->hist, radix_hist_words((uint32_t)c)));
+    HIP_TRY(dalloc(&h->hist, std::max(radix_hist_words((uint32_t)c), radix_hist_words32((uint32_t)c))));
     h->cap = (uint32_t)c;
     drop_graph(h);
     return RMESH_OK;
@@ -389,7 +405,10 @@ rmesh_status stage_binning_front(rmesh_renderer* h, uint32_t V, bool capture) {
     }
     launch_fill_u32(h->vis_count, 0u, 1, s);
     launch_fill_u32(h->overflow, 0u, 1, s);
-    launch_project(h->rec, h->d_scenes, ns, h->n_items, h->d_views, h->counts, h->block_sums, h->vis_count, s);
+    const bool df = h->depth_first();
+    launch_project(h->rec, reinterpret_cast<const float4*>(h->circ), h->d_scenes, ns, h->n_items, h->d_views, h->TS,
+                   h->counts, h->block_sums, h->vis_count, df ? h->dkey_a : nullptr, df ? h->ditem_a : nullptr,
+                   h->d_nitems, s);
     launch_spine_scan(h->block_sums, item_blocks(h->n_items), h->n_pairs, h->overflow, s);
     if (!capture) {
         uint32_t P = 0;
@@ -403,17 +422,45 @@ rmesh_status stage_binning_front(rmesh_renderer* h, uint32_t V, bool capture) {
     return RMESH_OK;
 }
 
-void launch_pair_gen(rmesh_renderer* h, hipStream_t s) {
+uint32_t* keys32(uint64_t* k) { return reinterpret_cast<uint32_t*>(k); }
+
+// Pairs in item order (the reference's pre-sort order), offsets from the
+// item-order scan of the binning front.
+void launch_pair_gen_item_order(rmesh_renderer* h, hipStream_t s) {
     launch_pairs(h->rec, reinterpret_cast<const float4*>(h->circ), h->d_scenes, (uint32_t)h->sc.size(), h->n_items,
-                 h->d_views, h->counts, h->block_sums, h->keys_a, h->vals_a, h->cap, h->overflow, s);
+                 h->d_views, h->TS, h->counts, h->block_sums, h->keys_a, h->vals_a, h->cap, h->overflow, s);
+    h->pairs_depth_ordered = false;
 }
+// Pair generation.  Depth-first: the items (M V 32-bit keys) are sorted by
+// depth key, and their pairs emitted in that order, so the pair sort that
+// follows needs only the tile bits (2 passes of 8 bits instead of 6).
+void launch_pair_gen(rmesh_renderer* h, hipStream_t s) {
+    if (!h->depth_first()) {
+        launch_pair_gen_item_order(h, s);
+        return;
+    }
+    const bool in_b = launch_radix_sort32(h->dkey_a, h->ditem_a, h->dkey_b, h->ditem_b, h->d_nitems,
+                                          (uint32_t)h->item_cap, h->ihist, s, h->sort_db);
+    launch_pairs_depth_ordered(h->rec, h->d_scenes, (uint32_t)h->sc.size(), in_b ? h->ditem_b : h->ditem_a,
+                               (uint32_t)h->n_items, h->d_views, h->TS, h->counts, h->block_sums2, h->n_pairs,
+                               keys32(h->keys_a), h->vals_a, h->cap, h->overflow, s);
+    h->pairs_depth_ordered = true;
+}
+// Depth-ordered pairs carry 32-bit keys (the key tile) and are sorted on the
+// tile bits; item-ordered pairs carry the 64-bit (tile, depth) keys.
 void launch_sort_stage(rmesh_renderer* h, hipStream_t s) {
-    h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
-                                       h->sorting_bits(0), h->hist, s, h->sort_db);
+    if (h->pairs_depth_ordered)
+        h->sorted_in_b = launch_radix_sort32(keys32(h->keys_a), h->vals_a, keys32(h->keys_b), h->vals_b, h->n_pairs,
+                                             h->cap, h->hist, s, h->sort_db, h->sorting_bits(0) - 32u);
+    else
+        h->sorted_in_b = launch_radix_sort(h->keys_a, h->vals_a, h->keys_b, h->vals_b, h->n_pairs, h->cap,
+                                           h->sorting_bits(0), h->hist, s, h->sort_db);
 }
 void launch_ranges_stage(rmesh_renderer* h, hipStream_t s) {
     launch_fill_u32(h->ranges, 0u, 2ull * h->n_gt, s);
-    launch_ranges(h->sorted_in_b ? h->keys_b : h->keys_a, h->n_pairs, h->cap, h->ranges, h->d_ktile, h->n_kt, s);
+    uint64_t* sk = h->sorted_in_b ? h->keys_b : h->keys_a;
+    if (h->pairs_depth_ordered) launch_ranges32(keys32(sk), h->n_pairs, h->cap, h->ranges, h->d_ktile, h->n_kt, s);
+    else launch_ranges(sk, h->n_pairs, h->cap, h->ranges, h->d_ktile, h->n_kt, s);
     if (h->scene_order != RMESH_SCENE_ORDER_BY_SCENE ||
         !launch_tile_order_scenes(h->ranges, h->n_gt, h->d_tile_view, h->d_views, (uint32_t)h->sc.size(),
                                   h->tile_order, s))
@@ -629,7 +676,8 @@ void free_all(rmesh_renderer* h) {
                     h->aux, h->dl, h->gt, h->loss_partial, h->loss_out, h->err_pe, h->err_se, h->tet_err,
                     h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd,
                     h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn, h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin,
-                    h->d_scenes, h->d_tile_view, h->d_ktile, h->scene_clock, h->tet_g};
+                    h->d_scenes, h->d_tile_view, h->d_ktile, h->scene_clock, h->tet_g, h->dkey_a, h->dkey_b,
+                    h->ditem_a, h->ditem_b, h->ihist, h->d_nitems, h->block_sums2};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -806,6 +854,15 @@ static rmesh_status create_impl(const rmesh_scene_desc* descs, uint32_t n_scenes
     CREATE_TRY(dalloc(&h->d_ktile, h->tile_cap + h->view_cap));
     CREATE_TRY(dalloc(&h->counts, h->item_cap));
     CREATE_TRY(dalloc(&h->block_sums, item_blocks(h->item_cap) + 1));
+    if (h->item_cap < (1ull << 27)) {  // item indices fit the depth sort's values (item << 5 | count)
+        CREATE_TRY(dalloc(&h->dkey_a, h->item_cap));
+        CREATE_TRY(dalloc(&h->dkey_b, h->item_cap));
+        CREATE_TRY(dalloc(&h->ditem_a, h->item_cap));
+        CREATE_TRY(dalloc(&h->ditem_b, h->item_cap));
+        CREATE_TRY(dalloc(&h->ihist, radix_hist_words32((uint32_t)h->item_cap)));
+        CREATE_TRY(dalloc(&h->d_nitems, 1));
+        CREATE_TRY(dalloc(&h->block_sums2, item_blocks(h->item_cap) + 1));
+    }
     CREATE_TRY(dalloc(&h->vis_count, 1));
     CREATE_TRY(dalloc(&h->n_pairs, 1));
     CREATE_TRY(dalloc(&h->overflow, 1));
@@ -1215,7 +1272,8 @@ extern "C" rmesh_status rmesh_device_bytes(rmesh_renderer* h, uint64_t* out) {
                           h->loss_out, h->err_pe, h->err_se, h->tet_err, h->tet_cnt, h->tile_order, h->ivr,
                           h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd, h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn,
                           h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin, h->d_scenes, h->d_tile_view, h->d_ktile,
-                          h->tet_g};
+                          h->tet_g, h->dkey_a, h->dkey_b, h->ditem_a, h->ditem_b, h->ihist, h->d_nitems,
+                          h->block_sums2};
     uint64_t total = 0;
     for (const void* p : ptrs) {
         if (!p) continue;
@@ -1712,7 +1770,7 @@ extern "C" rmesh_status rmesh_debug_pairs(rmesh_renderer* h, const float* cam, c
     if (st != RMESH_OK) return st;
     st = stage_binning_front(h, 1, false);
     if (st != RMESH_OK) return st;
-    launch_pair_gen(h, h->stream);
+    launch_pair_gen_item_order(h, h->stream);
     uint32_t P = 0;
     HIP_TRY(hipMemcpyAsync(&P, h->n_pairs, 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1734,7 +1792,16 @@ extern "C" rmesh_status rmesh_debug_sorted(rmesh_renderer* h, uint64_t* keys, ui
     HIP_TRY(hipMemcpy(&P, h->n_pairs, 4, hipMemcpyDeviceToHost));
     *n_pairs = P;
     uint64_t n = std::min<uint64_t>(P, cap);
-    if (keys && n) HIP_TRY(hipMemcpy(keys, h->sorted_in_b ? h->keys_b : h->keys_a, n * 8, hipMemcpyDeviceToHost));
+    uint64_t* sk = h->sorted_in_b ? h->keys_b : h->keys_a;
+    if (keys && n && h->pairs_depth_ordered) {
+        // 32-bit key tiles: rebuild the 64-bit (tile, depth) keys in the free key buffer
+        uint64_t* full = h->sorted_in_b ? h->keys_a : h->keys_b;
+        launch_full_keys(keys32(sk), sorted_vals(h), h->n_pairs, h->cap, reinterpret_cast<const float4*>(h->circ),
+                         h->d_views, h->d_ktile, h->n_kt, h->d_tile_view, full, P, h->stream);
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        sk = full;
+    }
+    if (keys && n) HIP_TRY(hipMemcpy(keys, sk, n * 8, hipMemcpyDeviceToHost));
     if (vals && n) HIP_TRY(hipMemcpy(vals, sorted_vals(h), n * 4, hipMemcpyDeviceToHost));
     uint64_t nr = std::min<uint64_t>(ranges_cap, 2ull * h->gtiles(h->last_views));
     if (ranges && nr) HIP_TRY(hipMemcpy(ranges, h->ranges, nr * 4, hipMemcpyDeviceToHost));

@@ -18,15 +18,33 @@ void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 void launch_atomic_probe(float* p, uint32_t lines, uint32_t blocks, uint32_t per_thread, hipStream_t s);
 void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t s);
 uint32_t item_blocks(uint64_t n_items);  // workgroups of the (tet, view) item kernels
-void launch_project(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
-                    const ViewParams* views, uint32_t* counts, uint32_t* block_sums, uint32_t* vis_count, hipStream_t s);
+// tile_size: the views' (one per renderer); a power of two takes the division-free path.
+// dkey / ditem (optional, with n_items_out): the depth sort's keys and item indices.
+void launch_project(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes,
+                    uint64_t n_items, const ViewParams* views, uint32_t tile_size, uint32_t* counts,
+                    uint32_t* block_sums, uint32_t* vis_count, uint32_t* dkey, uint32_t* ditem,
+                    uint32_t* n_items_out, hipStream_t s);
 void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s);
 void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
-                  const ViewParams* views, const uint32_t* counts, const uint32_t* block_off, uint64_t* keys,
-                  uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s);
+                  const ViewParams* views, uint32_t tile_size, const uint32_t* counts, const uint32_t* block_off,
+                  uint64_t* keys, uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s);
+// Depth-ordered pair generation from the items' depth-sorted values sval
+// (item << 5 | count, n_items < 2^27): block sums into block_sums, the spine
+// scan (total to n_pairs), the pairs in that order with 32-bit keys (the key
+// tile).
+void launch_pairs_depth_ordered(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, const uint32_t* sval,
+                                uint32_t n_items, const ViewParams* views, uint32_t tile_size, const uint32_t* counts,
+                                uint32_t* block_sums, uint32_t* n_pairs, uint32_t* keys, uint32_t* vals, uint32_t cap,
+                                uint32_t* overflow, hipStream_t s);
+// 64-bit keys (key tile << 32 | depth key) of sorted 32-bit-key pairs, for inspection
+void launch_full_keys(const uint32_t* ktile, const uint32_t* vals, const uint32_t* n_ptr, uint32_t cap,
+                      const float4* circ, const ViewParams* views, const uint32_t* ktile_gtile, uint32_t n_ktiles,
+                      const uint32_t* tile_view, uint64_t* out, uint32_t n_host, hipStream_t s);
 // ranges indexed by global tile; ktile_gtile maps every key tile to its global tile (0xFFFFFFFF: sentinel)
 void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
                    const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s);
+void launch_ranges32(const uint32_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
+                     const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s);
 // Permutation of the global tiles, heaviest (most pairs) first.
 void launch_tile_order(const uint32_t* ranges, uint32_t n_tiles, uint32_t* order, hipStream_t s);
 // Scene-major heaviest-first order (scenes by ascending pair count); false
@@ -37,10 +55,17 @@ bool launch_tile_order_scenes(const uint32_t* ranges, uint32_t n_tiles, const ui
 void launch_stamp_start(uint64_t* p, uint32_t n, hipStream_t s);
 
 // sort (k_sort.hip)
-// digit_bits: 8 (default) or 4; sorting_bits must be a multiple of it
+// digit_bits: 8 (default) or 4; sorting_bits and first_bit must be multiples
+// of it.  Passes over key bits [first_bit, sorting_bits): input already in
+// order of the bits below first_bit stays so among equal upper bits.
 bool launch_radix_sort(uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
-                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s, uint32_t digit_bits = 8);
+                       uint32_t sorting_bits, uint32_t* hist, hipStream_t s, uint32_t digit_bits = 8,
+                       uint32_t first_bit = 0);
 size_t radix_hist_words(uint32_t cap);
+// u32 keys, bits [0, end_bit)
+bool launch_radix_sort32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, const uint32_t* n_ptr, uint32_t cap,
+                         uint32_t* hist, hipStream_t s, uint32_t digit_bits = 8, uint32_t end_bit = 32);
+size_t radix_hist_words32(uint32_t cap);
 
 // compositing (k_composite.hip)
 struct FwdArgs {

@@ -90,6 +90,22 @@ def test_tile_sizes(ts, config_a):
     assert np.abs(img - ref["rgba"]).max() < IMG_TOL
 
 
+@pytest.mark.parametrize("w", [128, 136])
+def test_full_sort_and_depth_first_layouts(w):
+    """128x128 at tile 8 is 256 key tiles in one view: the sentinel key masks
+    to the last real tile, and the renderer keeps the full (tile, depth) pair
+    sort; 136x128 (272 tiles) takes depth-first pair generation (items sorted
+    by depth, pairs sorted on the tile bits only).  Both give the oracle's
+    lists bit for bit."""
+    rng = np.random.default_rng(99)
+    s = make_scene(cluster_points(4000, rng, 3, 0.25), rng)
+    h = 128
+    cam = fibonacci_cameras(2, 3.0, 50.0, w, h)[1]
+    r = _renderer(s, w, h, tile_size=8)
+    ref, img = _check_binning(r, s, cam, w, h, 8)
+    assert np.abs(img - ref["rgba"]).max() < IMG_TOL
+
+
 def test_config_a_forward_backward(config_a):
     scene, cam, w, h = config_a
     r = _renderer(scene, w, h)
