@@ -92,7 +92,10 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ rec, uint32_
 struct RowScan {
     float top_lo, top_hi;
 
-    // x range of the edge crossings of line y (1e10 / -1e10: none)
+    // x range of the edge crossings of line y (1e10 / -1e10: none).  Branch
+    // free: every edge's crossing is computed and kept only where the edge
+    // spans the line (across a wave of 64 (tet, view) items nearly every
+    // edge spans it somewhere, so a branch would rarely skip the division).
     __device__ __forceinline__ static void line(const float proj[4][2], float y, float& lo, float& hi) {
         constexpr int EI[6] = {0, 0, 0, 1, 1, 2};
         constexpr int EJ[6] = {1, 2, 3, 2, 3, 3};
@@ -102,12 +105,11 @@ struct RowScan {
         for (int e = 0; e < 6; ++e) {
             const float yi = proj[EI[e]][1], yj = proj[EJ[e]][1];
             const float xi = proj[EI[e]][0], xj = proj[EJ[e]][0];
-            if ((yi <= y && yj > y) || (yj <= y && yi > y)) {
-                const float tt = (y - yi) / (yj - yi);
-                const float x = xi + tt * (xj - xi);
-                lo = fminf(lo, x);
-                hi = fmaxf(hi, x);
-            }
+            const bool cross = (yi <= y && yj > y) || (yj <= y && yi > y);
+            const float tt = (y - yi) / (yj - yi);
+            const float x = xi + tt * (xj - xi);
+            lo = cross ? fminf(lo, x) : lo;
+            hi = cross ? fmaxf(hi, x) : hi;
         }
     }
     __device__ __forceinline__ void start(const float proj[4][2], uint32_t ty0, float T) {
@@ -242,9 +244,17 @@ __device__ __forceinline__ uint32_t project_count(const V3 v[4], float dens, con
 }
 
 // (tet, view) of a binning item: scene by binary search, then tet-major
-// inside the scene.
+// inside the scene.  One scene: its range is read at a uniform address
+// (scalar loads), not through the lane's search result.
 __device__ __forceinline__ void item_tet_view(const SceneRange* __restrict__ sc, uint32_t n_scenes, uint64_t item,
                                               uint32_t& t, uint32_t& vv) {
+    if (n_scenes == 1) {
+        const uint32_t V = sc[0].V;
+        const uint64_t local = item - sc[0].item_base;
+        t = sc[0].tet_base + (uint32_t)(local / V);
+        vv = sc[0].view_base + (uint32_t)(local % V);
+        return;
+    }
     const SceneRange& r = sc[scene_of_item(sc, n_scenes, item)];
     const uint64_t local = item - r.item_base;
     t = r.tet_base + (uint32_t)(local / r.V);
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec,
                                                  const SceneRange* __restrict__ scenes, uint32_t n_scenes,
                                                  uint64_t n_items, const ViewParams* __restrict__ views,
                                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ block_sums,
-                                                 uint32_t* __restrict__ vis_count, uint32_t* __restrict__ dkey,
+                                                 uint32_t* __restrict__ block_vis, uint32_t* __restrict__ dkey,
                                                  uint32_t* __restrict__ ditem, uint32_t* __restrict__ n_items_out) {
     __shared__ uint32_t red[256 / kWave];
     __shared__ uint32_t s_vis;
@@ -314,8 +324,10 @@ __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec,
     uint32_t total;
     (void)block_excl_scan(c, red, total);  // ends with a barrier: s_vis is complete
     if (threadIdx.x == 0) {
+        // per-block counts, summed by the spine scan: one global atomic per
+        // block on a single address serialised in L2 and held the kernel up
         block_sums[blockIdx.x] = total;
-        if (s_vis) atomicAdd(&vis_count[0], s_vis);
+        block_vis[blockIdx.x] = s_vis;
     }
 }
 
@@ -324,12 +336,16 @@ __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ rec,
 // writes the grand total (number of pairs P) to *total.  The running total is
 // kept in 64 bits: past 2^32 - 16 pairs the offsets would wrap, so *total
 // saturates at 0xFFFFFFFF (the host refuses that count) and the overflow flag
-// is raised (a captured step then leaves the parameters untouched).
+// is raised (a captured step then leaves the parameters untouched).  With
+// aux set, also the plain sum of aux[0, n) into *aux_total.
 constexpr uint32_t kSpinePer = 16;
 __global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums, uint32_t n, uint32_t* __restrict__ total,
-                                                     uint32_t* __restrict__ overflow) {
+                                                     uint32_t* __restrict__ overflow, const uint32_t* __restrict__ aux,
+                                                     uint32_t* __restrict__ aux_total) {
     __shared__ uint32_t red[1024 / kWave];
     __shared__ unsigned long long s_round;  // the round's total in 64 bits
+    __shared__ uint32_t s_aux;
+    if (threadIdx.x == 0) s_aux = 0u;
     uint64_t carry = 0;
     for (uint32_t base = 0; base < n; base += kSpinePer * 1024u) {
         const uint32_t i0 = base + kSpinePer * threadIdx.x;
@@ -352,7 +368,22 @@ __global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums
 #pragma unroll
         for (uint32_t k = 0; k < kSpinePer; ++k) loc += x[k];
         if (threadIdx.x == 0) s_round = 0ull;
-        __syncthreads();
+        __syncthreads();  // also publishes s_aux = 0 before the first adds
+        if (aux) {
+            uint32_t a = 0;
+            if (i0 + kSpinePer <= n) {
+                const uint4* p = reinterpret_cast<const uint4*>(aux + i0);
+#pragma unroll
+                for (uint32_t q = 0; q < kSpinePer / 4; ++q) {
+                    const uint4 v = p[q];
+                    a += (v.x + v.y) + (v.z + v.w);
+                }
+            } else {
+                for (uint32_t k = 0; k < kSpinePer; ++k) a += i0 + k < n ? aux[i0 + k] : 0u;
+            }
+            a = wave_sum_u32(a);
+            if (__lane_id() == 0 && a) atomicAdd(&s_aux, a);
+        }
         atomicAdd(&s_round, (unsigned long long)loc);
         uint32_t tot;
         uint32_t ex = block_excl_scan((uint32_t)loc, red, tot) + (uint32_t)carry;  // ends with a barrier
@@ -368,6 +399,7 @@ __global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums
         const bool wrap = carry > 0xFFFFFFF0ull;
         *total = wrap ? 0xFFFFFFFFu : (uint32_t)carry;
         if (wrap) *overflow = 1u;
+        if (aux) *aux_total = s_aux;  // the rounds' barriers ordered every wave's add before this read
     }
 }
 
@@ -578,18 +610,19 @@ uint32_t item_blocks(uint64_t n_items) { return (uint32_t)((n_items + 255) / 256
 static bool pow2(uint32_t ts) { return ts && !(ts & (ts - 1u)); }
 void launch_project(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes,
                     uint64_t n_items, const ViewParams* views, uint32_t tile_size, uint32_t* counts,
-                    uint32_t* block_sums, uint32_t* vis_count, uint32_t* dkey, uint32_t* ditem,
+                    uint32_t* block_sums, uint32_t* block_vis, uint32_t* dkey, uint32_t* ditem,
                     uint32_t* n_items_out, hipStream_t s) {
     if (!n_items) return;
     if (pow2(tile_size))
         hipLaunchKernelGGL(k_project<true>, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes,
-                           n_items, views, counts, block_sums, vis_count, dkey, ditem, n_items_out);
+                           n_items, views, counts, block_sums, block_vis, dkey, ditem, n_items_out);
     else
         hipLaunchKernelGGL(k_project<false>, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes,
-                           n_items, views, counts, block_sums, vis_count, dkey, ditem, n_items_out);
+                           n_items, views, counts, block_sums, block_vis, dkey, ditem, n_items_out);
 }
-void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s) {
-    hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total, overflow);
+void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s,
+                       const uint32_t* aux, uint32_t* aux_total) {
+    hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total, overflow, aux, aux_total);
 }
 void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
                   const ViewParams* views, uint32_t tile_size, const uint32_t* counts, const uint32_t* block_off,

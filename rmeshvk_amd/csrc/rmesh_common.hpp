@@ -216,6 +216,12 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // Inclusive scan of a u32 across the wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     uint32_t l = __lane_id();

@@ -19,12 +19,15 @@ void launch_atomic_probe(float* p, uint32_t lines, uint32_t blocks, uint32_t per
 void launch_copy_probe(const float4* src, float4* dst, uint64_t n16, hipStream_t s);
 uint32_t item_blocks(uint64_t n_items);  // workgroups of the (tet, view) item kernels
 // tile_size: the views' (one per renderer); a power of two takes the division-free path.
+// block_vis: per block, the visible items of view 0 (their sum is the instance count).
 // dkey / ditem (optional, with n_items_out): the depth sort's keys and item indices.
 void launch_project(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes,
                     uint64_t n_items, const ViewParams* views, uint32_t tile_size, uint32_t* counts,
-                    uint32_t* block_sums, uint32_t* vis_count, uint32_t* dkey, uint32_t* ditem,
+                    uint32_t* block_sums, uint32_t* block_vis, uint32_t* dkey, uint32_t* ditem,
                     uint32_t* n_items_out, hipStream_t s);
-void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s);
+// Exclusive scan of sums[0, n) in place, total to *total; with aux, the sum of aux[0, n) to *aux_total.
+void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s,
+                       const uint32_t* aux = nullptr, uint32_t* aux_total = nullptr);
 void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
                   const ViewParams* views, uint32_t tile_size, const uint32_t* counts, const uint32_t* block_off,
                   uint64_t* keys, uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s);
