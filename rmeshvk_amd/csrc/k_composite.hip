@@ -573,6 +573,18 @@ __device__ __forceinline__ void batch_layout(BatchLDS<TS, B, BWD>& L, int nb, ch
     if (m && lane == 63 - __clzll((long long)m)) atomicMax(&L.nbp, ((cexc + cc) << 8) | (uint32_t)(k + 1));
 }
 
+// Tets to stage for the next batch.  Tets staged past the capacity cut are
+// dropped and staged again with the following batch (their fetch, face
+// setup and spans are redone), so after a batch that did not fit whole the
+// next one stages the kept count plus half again (at least 4 more), and the
+// full B again after one that fitted.  Batches only partition the list: every
+// pixel still composites the same tets in the same order.
+template <int B>
+__device__ __forceinline__ int next_batch(uint32_t nbp, int nb, uint32_t left) {
+    const uint32_t want = (int)nbp == nb ? (uint32_t)B : min((uint32_t)B, nbp + max(nbp >> 1, 4u));
+    return (int)min(want, left);
+}
+
 struct ChunkRef {
     int k, row, col;
     uint32_t slot0, cnt;
@@ -851,7 +863,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         const uint32_t nchunks = L.nbp >> 8;
         // prefetch the next batch's tet records while this one is processed
         const uint32_t next = done + nbp;
-        const int nb_next = (int)min<uint32_t>(B, count - next);
+        const int nb_next = next_batch<B>(nbp, nb, count - next);
         if (C::PF == 1 && tid < nb_next) fetch_tet<false>(f, sorted_vals[re - 1 - next - tid], rec, nullptr, col_v);
         if (C::PF == 3 && tid < 4 * nb_next)
             fetch_quad<false>(qf, sorted_vals[re - 1 - next - (tid >> 2)], tid & 3, rec, nullptr, col_v);
@@ -1397,7 +1409,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         const uint32_t nbp = L.nbp & 0xFFu;
         const uint32_t nchunks = L.nbp >> 8;
         const uint32_t next = done + nbp;
-        const int nb_next = (int)min<uint32_t>(B, count - next);
+        const int nb_next = next_batch<B>(nbp, nb, count - next);
         // CK: log T at the end of the (at most two) groups this batch spans,
         // positions [count - next, count - 1 - done] in compositing order
         float ck_hi = 0.f, ck_lo = 0.f;
