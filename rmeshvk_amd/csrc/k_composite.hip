@@ -1006,6 +1006,8 @@ struct BwdCfg {
     static constexpr int PF = (PF_ == 3 && 4 * B_ > NT) ? 1 : PF_;  // as FwdCfg::PF
     static constexpr int SCAN = SCAN_;
     static constexpr int WPE = WPE_ > 0 ? WPE_ : 1;
+    static constexpr int PF_ARG = PF_;
+    static constexpr int WPE_ARG = WPE_;
 };
 
 // Segmented inclusive sum across the wave; lanes with equal `key` form
@@ -1271,11 +1273,13 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
     __shared__ float s_dc[3][TS * TS];
     __shared__ float s_acc[B][kTerms];
     __shared__ uint32_t s_ftid[B];  // tet ids of the batch whose terms s_acc holds
-    __shared__ unsigned int s_mask[TS * TS];  // covered tets of the batch per pixel (B <= 32)
+    // covered tets of the batch per pixel, one bit per tet
+    using MaskT = typename std::conditional<(B > 32), unsigned long long, unsigned int>::type;
+    __shared__ MaskT s_mask[TS * TS];
     __shared__ chunk_t<R> s_chunk[chunk_capacity(CAP, R, B)];
     static_assert(CAP <= 2048 && TS <= 32 && R <= 31, "chunk table packing");
     __shared__ float s_loss[NW];
-    static_assert(B <= 32, "pixel masks hold 32 tets");
+    static_assert(B <= 64, "pixel masks hold 64 tets");
     __shared__ uint32_t s_start[R == 1 ? CAP / 32 : 1];  // R = 1: record slots that start a row segment
     static_assert(R != 1 || (CAP % 32 == 0 && CAP / 32 <= NT), "segment start mask");
     static_assert(CAP >= TS * TS, "one tet's pixels of a tile must fit a batch");
@@ -1456,7 +1460,7 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
                         s_r[2][slot] = t_min;
                         s_r[3][slot] = t_max;
                         meta = (uint32_t)fmin | ((uint32_t)fmax << 2) | (1u << 4);
-                        atomicOr(&s_mask[p], 1u << k);
+                        atomicOr(&s_mask[p], (MaskT)1 << k);
                     }
                 }
                 s_meta[slot] = (uint8_t)meta;
@@ -1476,10 +1480,10 @@ __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE)))
         // phase B (pixel-sequential, back to front over this pixel's covered
         // tets): undo and the d(log T) chain (backward_tiled_compute.wgsl:419-451)
         if (active) {
-            unsigned int m = s_mask[tid];
+            MaskT m = s_mask[tid];
             while (m) {
-                const uint32_t k = (uint32_t)__ffs((int)m) - 1u;
-                m &= m - 1u;
+                const uint32_t k = B > 32 ? (uint32_t)__ffsll((long long)m) - 1u : (uint32_t)__ffs((int)m) - 1u;
+                m &= m - (MaskT)1;
                 if constexpr (CK) {
                     const uint32_t q = count - 1u - done - k;  // position in compositing order
                     if (q >= stop) continue;
@@ -1652,12 +1656,17 @@ bool launch_forward(const FwdArgs& a, uint32_t tile_size, int variant, hipStream
 
 template <class C>
 static void bwd_launch(const BwdArgs& a, hipStream_t s) {
-    if (a.ck)
-        hipLaunchKernelGGL((k_backward<C, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
-                           a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
-                           a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.tet_g, a.d_colors, a.ck, a.fin,
-                           a.scene_done);
-    else
+    if (a.ck) {
+        if constexpr (C::B > 32) {
+            // the checkpointed adjoint's batches span at most two checkpoint groups (B <= 32)
+            bwd_launch<BwdCfg<C::TS, 32, C::CAP, C::R, C::PF_ARG, C::SCAN, C::WPE_ARG>>(a, s);
+        } else {
+            hipLaunchKernelGGL((k_backward<C, true>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
+                               a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
+                               a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.tet_g, a.d_colors, a.ck, a.fin,
+                               a.scene_done);
+        }
+    } else
         hipLaunchKernelGGL((k_backward<C, false>), dim3(a.num_gtiles), dim3(C::NT), 0, s, a.rec, a.idx, a.colors,
                            a.sorted_vals, a.ranges, a.views, a.tile_view, a.tile_order,
                            a.rgba, a.dl_img, a.gt, a.loss_type, a.loss_partial, a.tet_g, a.d_colors, nullptr,
@@ -1675,6 +1684,9 @@ static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
         case 5: bwd_launch<BwdCfg<TS, 32, 640, 1, 3, 3, 4>>(a, s); return true;
         // 31.5 KB of LDS and 96 VGPRs at tile 16: 5 workgroups (20 waves) per CU
         case 6: bwd_launch<BwdCfg<TS, 32, 544, 1, 3, 3, 5>>(a, s); return true;
+        // more tets per batch where they are small (the checkpointed mode runs these with B 32)
+        case 7: bwd_launch<BwdCfg<TS, 48, 576, 1, 3, 3, 4>>(a, s); return true;
+        case 8: bwd_launch<BwdCfg<TS, 64, 512, 1, 3, 3, 4>>(a, s); return true;
         default: return false;
     }
 }

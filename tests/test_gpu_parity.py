@@ -185,7 +185,7 @@ def _variant_grid():
     from rmeshvk_amd import RMeshRenderer
 
     nf, nb = RMeshRenderer.num_variants()
-    grid = [(ts, v, v % nb) for ts in (8, 12, 16) for v in range(max(nf, nb)) if v < nf]
+    grid = [(ts, v % nf, v % nb) for ts in (8, 12, 16) for v in range(max(nf, nb))]
     return grid + [(32, 0, 0), (32, 1, 1)]  # tile 32: two geometries each
 
 
