@@ -10,7 +10,9 @@
   (tests/test_oracle_grad.py), while the undo mode is not;
 - the batched, graph-replayed train step in checkpoint mode equals the sum of
   per-view checkpointed steps;
-- mode switches: a backward needs a forward in its own mode.
+- mode switches: a backward needs a forward in its own mode;
+- the adjoint variants with more than 32 tets per batch, which run this mode
+  with 32.
 """
 
 import math
@@ -84,6 +86,23 @@ def test_checkpointed_backward_matches_oracle(make, ts):
     dl = np.ascontiguousarray(np.random.default_rng(ts).normal(0, 1e-2, (h, w, 4)).astype(np.float32))
     g = r.backward(dl)
     ref, fw, go = _oracle_ck(scene, cam, w, h, ts, dl)
+    assert np.abs(img - ref["rgba"]).max() <= 1e-4
+    for k in KEYS:
+        assert _rel(g[k], go[k]) <= GRAD_TOL, (k, _rel(g[k], go[k]))
+
+
+@pytest.mark.parametrize("bv", [7, 8])
+def test_checkpointed_with_wide_batch_variants(bv):
+    """The adjoint variants with more than 32 tets per batch run the
+    checkpointed mode with 32 (a batch spans at most two checkpoint groups)."""
+    scene, cam, w, h = _saturating()
+    r = _renderer(scene, w, h, tile_size=16)
+    r.set_tuning(16, 0, bv)
+    r.set_adjoint_mode("checkpoint")
+    img = r.forward_tiled(*cam)
+    dl = np.ascontiguousarray(np.random.default_rng(bv).normal(0, 1e-2, (h, w, 4)).astype(np.float32))
+    g = r.backward(dl)
+    ref, fw, go = _oracle_ck(scene, cam, w, h, 16, dl)
     assert np.abs(img - ref["rgba"]).max() <= 1e-4
     for k in KEYS:
         assert _rel(g[k], go[k]) <= GRAD_TOL, (k, _rel(g[k], go[k]))
