@@ -1687,6 +1687,8 @@ static bool bwd_variant(const BwdArgs& a, int v, hipStream_t s) {
         // more tets per batch where they are small (the checkpointed mode runs these with B 32)
         case 7: bwd_launch<BwdCfg<TS, 48, 576, 1, 3, 3, 4>>(a, s); return true;
         case 8: bwd_launch<BwdCfg<TS, 64, 512, 1, 3, 3, 4>>(a, s); return true;
+        case 9: bwd_launch<BwdCfg<TS, 40, 576, 1, 3, 3, 4>>(a, s); return true;
+        case 10: bwd_launch<BwdCfg<TS, 48, 512, 1, 3, 3, 4>>(a, s); return true;
         default: return false;
     }
 }

@@ -91,7 +91,7 @@ struct FwdArgs {
 };
 // Batch-geometry variants of the compositing kernels (rmesh_autotune picks one).
 constexpr int kFwdVariants = 8;
-constexpr int kBwdVariants = 9;
+constexpr int kBwdVariants = 11;
 // tile 32 compiles fewer geometries (a batch must hold one tet's 1024 pixels)
 inline int fwd_variants(uint32_t tile_size) { return tile_size == 32 ? 2 : kFwdVariants; }
 inline int bwd_variants(uint32_t tile_size) { return tile_size == 32 ? 2 : kBwdVariants; }
