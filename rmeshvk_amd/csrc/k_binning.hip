@@ -403,6 +403,79 @@ __global__ __launch_bounds__(1024) void k_spine_scan(uint32_t* __restrict__ sums
     }
 }
 
+// Two-level form of the spine scan for many block sums: each workgroup scans
+// kSpineTile sums in place (local offsets) and leaves its exact total (64
+// bits) and its aux sum in part / aux_part; one workgroup turns the totals
+// into offsets (and the grand totals, saturated as above); a third pass adds
+// each workgroup's offset.  Three short launches instead of one workgroup
+// walking every sum in rounds.
+constexpr uint32_t kSpineTile = 4096;
+__global__ __launch_bounds__(1024) void k_spine_local(uint32_t* __restrict__ sums, uint32_t n,
+                                                      unsigned long long* __restrict__ part,
+                                                      const uint32_t* __restrict__ aux,
+                                                      uint32_t* __restrict__ aux_part) {
+    __shared__ uint32_t red[1024 / kWave];
+    __shared__ unsigned long long s_tot;
+    __shared__ uint32_t s_aux;
+    if (threadIdx.x == 0) {
+        s_tot = 0ull;
+        s_aux = 0u;
+    }
+    const uint32_t i0 = blockIdx.x * kSpineTile + 4u * threadIdx.x;
+    uint32_t x[4], a = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) x[k] = i0 + k < n ? sums[i0 + k] : 0u;
+    if (aux) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a += i0 + k < n ? aux[i0 + k] : 0u;
+    }
+    __syncthreads();  // s_tot, s_aux cleared
+    atomicAdd(&s_tot, ((unsigned long long)x[0] + x[1]) + ((unsigned long long)x[2] + x[3]));
+    if (aux) {
+        a = wave_sum_u32(a);
+        if (__lane_id() == 0 && a) atomicAdd(&s_aux, a);
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan((x[0] + x[1]) + (x[2] + x[3]), red, tot);  // ends with a barrier
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (i0 + k < n) sums[i0 + k] = ex;
+        ex += x[k];
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s_tot;
+        if (aux) aux_part[blockIdx.x] = s_aux;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_spine_top(unsigned long long* __restrict__ part, uint32_t nb,
+                                                  uint32_t* __restrict__ total, uint32_t* __restrict__ overflow,
+                                                  const uint32_t* __restrict__ aux_part,
+                                                  uint32_t* __restrict__ aux_total) {
+    if (threadIdx.x != 0) return;
+    unsigned long long carry = 0ull;
+    uint32_t a = 0u;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const unsigned long long p = part[b];
+        part[b] = carry;
+        carry += p;
+        if (aux_part) a += aux_part[b];
+    }
+    const bool wrap = carry > 0xFFFFFFF0ull;
+    *total = wrap ? 0xFFFFFFFFu : (uint32_t)carry;
+    if (wrap) *overflow = 1u;
+    if (aux_part) *aux_total = a;
+}
+
+__global__ __launch_bounds__(1024) void k_spine_add(uint32_t* __restrict__ sums, uint32_t n,
+                                                    const unsigned long long* __restrict__ part) {
+    const uint32_t add = (uint32_t)part[blockIdx.x];
+    const uint32_t i0 = blockIdx.x * kSpineTile + 4u * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i0 + k < n) sums[i0 + k] += add;
+}
+
 // K7 (+ K6 sentinel padding) for one item: its max_write pairs at
 // [start, start + max_write).  64-bit keys: (key tile << 32) | ~bits(depth);
 // 32-bit keys: the key tile alone (pairs already in depth order).
@@ -620,8 +693,17 @@ void launch_project(const float4* rec, const float4* circ, const SceneRange* sce
         hipLaunchKernelGGL(k_project<false>, dim3(item_blocks(n_items)), dim3(256), 0, s, rec, circ, scenes, n_scenes,
                            n_items, views, counts, block_sums, block_vis, dkey, ditem, n_items_out);
 }
+uint32_t spine_parts(uint32_t n) { return (n + kSpineTile - 1) / kSpineTile + 1; }
 void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s,
-                       const uint32_t* aux, uint32_t* aux_total) {
+                       const uint32_t* aux, uint32_t* aux_total, unsigned long long* part, uint32_t* aux_part) {
+    if (part && n > kSpineTile) {
+        const uint32_t nb = (n + kSpineTile - 1) / kSpineTile;
+        hipLaunchKernelGGL(k_spine_local, dim3(nb), dim3(1024), 0, s, sums, n, part, aux, aux ? aux_part : nullptr);
+        hipLaunchKernelGGL(k_spine_top, dim3(1), dim3(64), 0, s, part, nb, total, overflow, aux ? aux_part : nullptr,
+                           aux_total);
+        hipLaunchKernelGGL(k_spine_add, dim3(nb), dim3(1024), 0, s, sums, n, part);
+        return;
+    }
     hipLaunchKernelGGL(k_spine_scan, dim3(1), dim3(1024), 0, s, sums, n, total, overflow, aux, aux_total);
 }
 void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
@@ -638,11 +720,12 @@ void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scene
 void launch_pairs_depth_ordered(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, const uint32_t* sval,
                                 uint32_t n_items, const ViewParams* views, uint32_t tile_size, const uint32_t* counts,
                                 uint32_t* block_sums, uint32_t* n_pairs, uint32_t* keys, uint32_t* vals, uint32_t cap,
-                                uint32_t* overflow, hipStream_t s) {
+                                uint32_t* overflow, unsigned long long* spine_part, hipStream_t s) {
     if (!n_items) return;
     const dim3 gr(item_blocks(n_items)), bl(256);
     hipLaunchKernelGGL(k_sorted_sums, gr, bl, 0, s, sval, n_items, counts, block_sums);
-    launch_spine_scan(block_sums, item_blocks(n_items), n_pairs, overflow, s);  // the same total P
+    launch_spine_scan(block_sums, item_blocks(n_items), n_pairs, overflow, s, nullptr, nullptr, spine_part,
+                      nullptr);  // the same total P
     if (pow2(tile_size))
         hipLaunchKernelGGL(k_pairs_sorted<true>, gr, bl, 0, s, rec, scenes, n_scenes, sval, n_items, views, counts,
                            block_sums, keys, vals, cap, overflow);

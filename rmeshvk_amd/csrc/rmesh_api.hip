@@ -124,6 +124,8 @@ struct rmesh_renderer {
     ViewParams* d_views = nullptr;
     uint32_t *counts = nullptr, *block_sums = nullptr, *vis_count = nullptr, *n_pairs = nullptr, *overflow = nullptr;
     uint32_t* block_vis = nullptr;  // per k_project block: visible items of view 0
+    unsigned long long* spine_part = nullptr;  // two-level spine scan: per-workgroup totals
+    uint32_t* spine_aux = nullptr;             // and instance-count partials
     uint32_t* d_step = nullptr;
     uint64_t *keys_a = nullptr, *keys_b = nullptr;
     uint32_t *vals_a = nullptr, *vals_b = nullptr;
@@ -409,7 +411,8 @@ rmesh_status stage_binning_front(rmesh_renderer* h, uint32_t V, bool capture) {
     launch_project(h->rec, reinterpret_cast<const float4*>(h->circ), h->d_scenes, ns, h->n_items, h->d_views, h->TS,
                    h->counts, h->block_sums, h->block_vis, df ? h->dkey_a : nullptr, df ? h->ditem_a : nullptr,
                    h->d_nitems, s);
-    launch_spine_scan(h->block_sums, item_blocks(h->n_items), h->n_pairs, h->overflow, s, h->block_vis, h->vis_count);
+    launch_spine_scan(h->block_sums, item_blocks(h->n_items), h->n_pairs, h->overflow, s, h->block_vis, h->vis_count,
+                      h->spine_part, h->spine_aux);
     if (!capture) {
         uint32_t P = 0;
         HIP_TRY(hipMemcpyAsync(&P, h->n_pairs, 4, hipMemcpyDeviceToHost, s));
@@ -443,7 +446,7 @@ void launch_pair_gen(rmesh_renderer* h, hipStream_t s) {
                                           (uint32_t)h->item_cap, h->ihist, s, h->sort_db);
     launch_pairs_depth_ordered(h->rec, h->d_scenes, (uint32_t)h->sc.size(), in_b ? h->ditem_b : h->ditem_a,
                                (uint32_t)h->n_items, h->d_views, h->TS, h->counts, h->block_sums2, h->n_pairs,
-                               keys32(h->keys_a), h->vals_a, h->cap, h->overflow, s);
+                               keys32(h->keys_a), h->vals_a, h->cap, h->overflow, h->spine_part, s);
     h->pairs_depth_ordered = true;
 }
 // Depth-ordered pairs carry 32-bit keys (the key tile) and are sorted on the
@@ -677,7 +680,8 @@ void free_all(rmesh_renderer* h) {
                     h->tet_cnt, h->tile_order, h->ivr, h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd,
                     h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn, h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin,
                     h->d_scenes, h->d_tile_view, h->d_ktile, h->scene_clock, h->tet_g, h->dkey_a, h->dkey_b,
-                    h->ditem_a, h->ditem_b, h->ihist, h->d_nitems, h->block_sums2, h->block_vis};
+                    h->ditem_a, h->ditem_b, h->ihist, h->d_nitems, h->block_sums2, h->block_vis, h->spine_part,
+                    h->spine_aux};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -855,6 +859,8 @@ static rmesh_status create_impl(const rmesh_scene_desc* descs, uint32_t n_scenes
     CREATE_TRY(dalloc(&h->counts, h->item_cap));
     CREATE_TRY(dalloc(&h->block_sums, item_blocks(h->item_cap) + 1));
     CREATE_TRY(dalloc(&h->block_vis, item_blocks(h->item_cap) + 1));
+    CREATE_TRY(dalloc(&h->spine_part, spine_parts(item_blocks(h->item_cap) + 1)));
+    CREATE_TRY(dalloc(&h->spine_aux, spine_parts(item_blocks(h->item_cap) + 1)));
     if (h->item_cap < (1ull << 27)) {  // item indices fit the depth sort's values (item << 5 | count)
         CREATE_TRY(dalloc(&h->dkey_a, h->item_cap));
         CREATE_TRY(dalloc(&h->dkey_b, h->item_cap));
@@ -1274,7 +1280,7 @@ extern "C" rmesh_status rmesh_device_bytes(rmesh_renderer* h, uint64_t* out) {
                           h->ivl_xyzd, h->ivl_dist, h->ivl_dxyzd, h->ivl_ddist, h->ivl_dgrad, h->ivl_dvn,
                           h->ivl_tv, h->v2t_off, h->v2t, h->ck, h->fin, h->d_scenes, h->d_tile_view, h->d_ktile,
                           h->tet_g, h->dkey_a, h->dkey_b, h->ditem_a, h->ditem_b, h->ihist, h->d_nitems,
-                          h->block_sums2, h->block_vis};
+                          h->block_sums2, h->block_vis, h->spine_part, h->spine_aux};
     uint64_t total = 0;
     for (const void* p : ptrs) {
         if (!p) continue;

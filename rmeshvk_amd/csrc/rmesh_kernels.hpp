@@ -26,8 +26,11 @@ void launch_project(const float4* rec, const float4* circ, const SceneRange* sce
                     uint32_t* block_sums, uint32_t* block_vis, uint32_t* dkey, uint32_t* ditem,
                     uint32_t* n_items_out, hipStream_t s);
 // Exclusive scan of sums[0, n) in place, total to *total; with aux, the sum of aux[0, n) to *aux_total.
+// part (spine_parts(n) entries) and aux_part (the same count): scratch of the two-level form.
+uint32_t spine_parts(uint32_t n);
 void launch_spine_scan(uint32_t* sums, uint32_t n, uint32_t* total, uint32_t* overflow, hipStream_t s,
-                       const uint32_t* aux = nullptr, uint32_t* aux_total = nullptr);
+                       const uint32_t* aux = nullptr, uint32_t* aux_total = nullptr,
+                       unsigned long long* part = nullptr, uint32_t* aux_part = nullptr);
 void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scenes, uint32_t n_scenes, uint64_t n_items,
                   const ViewParams* views, uint32_t tile_size, const uint32_t* counts, const uint32_t* block_off,
                   uint64_t* keys, uint32_t* vals, uint32_t cap, uint32_t* overflow, hipStream_t s);
@@ -38,7 +41,7 @@ void launch_pairs(const float4* rec, const float4* circ, const SceneRange* scene
 void launch_pairs_depth_ordered(const float4* rec, const SceneRange* scenes, uint32_t n_scenes, const uint32_t* sval,
                                 uint32_t n_items, const ViewParams* views, uint32_t tile_size, const uint32_t* counts,
                                 uint32_t* block_sums, uint32_t* n_pairs, uint32_t* keys, uint32_t* vals, uint32_t cap,
-                                uint32_t* overflow, hipStream_t s);
+                                uint32_t* overflow, unsigned long long* spine_part, hipStream_t s);
 // 64-bit keys (key tile << 32 | depth key) of sorted 32-bit-key pairs, for inspection
 void launch_full_keys(const uint32_t* ktile, const uint32_t* vals, const uint32_t* n_ptr, uint32_t cap,
                       const float4* circ, const ViewParams* views, const uint32_t* ktile_gtile, uint32_t n_ktiles,
