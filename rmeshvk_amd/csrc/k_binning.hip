@@ -658,6 +658,37 @@ __global__ void k_ranges(const KT* __restrict__ keys, const uint32_t* __restrict
     }
 }
 
+// k_ranges over 32-bit keys, four consecutive keys per thread (one 16-byte
+// load; the key before them comes from one more load), same per-key rule.
+__global__ void k_ranges32x4(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr, uint32_t cap,
+                             uint32_t* __restrict__ ranges, const uint32_t* __restrict__ ktile_gtile,
+                             uint32_t n_ktiles) {
+    const uint32_t n = min(*n_ptr, cap);
+    const uint32_t stride = 4u * gridDim.x * blockDim.x;
+    for (uint32_t i0 = 4u * (blockIdx.x * blockDim.x + threadIdx.x); i0 < n; i0 += stride) {
+        // the key buffer holds 2 cap words (it is the 64-bit keys' buffer): the 16-byte load stays inside it
+        const uint4 q = *reinterpret_cast<const uint4*>(keys + i0);
+        const uint32_t kk[4] = {q.x, q.y, q.z, q.w};
+        uint32_t prev = i0 > 0 ? range_tile(keys[i0 - 1], ktile_gtile, n_ktiles) : 0xFFFFFFFFu;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t i = i0 + (uint32_t)u;
+            if (i >= n) break;
+            const uint32_t tile = range_tile(kk[u], ktile_gtile, n_ktiles);
+            if (tile != 0xFFFFFFFFu) {
+                if (i == 0) {
+                    ranges[2ull * tile] = i;
+                } else if (tile != prev) {
+                    ranges[2ull * tile] = i;
+                    if (prev != 0xFFFFFFFFu) ranges[2ull * prev + 1] = i;
+                }
+                if (i == n - 1) ranges[2ull * tile + 1] = i + 1;
+            }
+            prev = tile;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -853,7 +884,7 @@ void launch_ranges(const uint64_t* keys, const uint32_t* n_ptr, uint32_t cap, ui
 }
 void launch_ranges32(const uint32_t* keys, const uint32_t* n_ptr, uint32_t cap, uint32_t* ranges,
                      const uint32_t* ktile_gtile, uint32_t n_ktiles, hipStream_t s) {
-    hipLaunchKernelGGL(k_ranges<uint32_t>, dim3(range_blocks(cap)), dim3(256), 0, s, keys, n_ptr, cap, ranges,
+    hipLaunchKernelGGL(k_ranges32x4, dim3(range_blocks((cap + 3) / 4)), dim3(256), 0, s, keys, n_ptr, cap, ranges,
                        ktile_gtile, n_ktiles);
 }
 void launch_full_keys(const uint32_t* ktile, const uint32_t* vals, const uint32_t* n_ptr, uint32_t cap,
